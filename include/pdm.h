@@ -1,0 +1,157 @@
+/*
+ * pdm.h -- C ABI of the MI355X (gfx950) linear bond-based peridynamics library.
+ *
+ * This is the drop-in boundary for the hot path of the reference solver:
+ * the pdSolve::InitialProblem / SolveDynamic / SolveStatic drivers and the
+ * pdDataManager calls they make.  Each entry point below names the reference
+ * interface it replaces (file:line).  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative PDM_E* code on error;
+ *    pdm_last_error() then returns a message.  (The reference calls exit(0) or
+ *    throws on the same conditions, pdDataManager.h:400-419, pdDataManager.cpp:980-992.)
+ *  - 3-vectors are node-major AoS (x,y,z of node 0, node 1, ...), the reference's
+ *    _rowOfNode/displStep layout (pdDataManager.cpp:1085); node ids are the
+ *    reference's lattice ids (x fastest, pdDataManager.cpp:846-909).
+ *  - Host buffers are caller-owned and copied in/out.  All device memory is
+ *    owned by the handle.  One HIP stream per handle; calls on one handle must
+ *    be serialised by the caller, distinct handles are independent.
+ */
+#ifndef PDM_H
+#define PDM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDM_OK 0
+#define PDM_E_ARG -1        /* bad argument / null pointer */
+#define PDM_E_IO -2         /* input file missing or unreadable */
+#define PDM_E_PARSE -3      /* required keyword missing, value missing, bad value */
+#define PDM_E_MODEL -4      /* node outside every material space, unsupported mode */
+#define PDM_E_HIP -5        /* HIP runtime error */
+#define PDM_E_STATE -6      /* call out of order (e.g. step before init) */
+#define PDM_E_LIMIT -7      /* a size limit of this build exceeded */
+#define PDM_E_CONVERGE -8   /* static solve did not reach the tolerance */
+
+/* precision of the bond-force gather and of CG matvecs */
+#define PDM_FP64 0          /* reference arithmetic in FP64 */
+#define PDM_FP32C 1         /* FP32 bond force, two-float family sums, FP64 state */
+
+typedef struct pdm_handle pdm_handle;
+
+typedef struct {
+    int precision;          /* PDM_FP64 or PDM_FP32C */
+    int device;             /* HIP device ordinal */
+    int use_graph;          /* capture each chunk of RK4 steps in a hipGraph (default 1) */
+    int graph_chunk;        /* steps per captured graph (default 50) */
+    int energies;           /* compute KE/EE every step (reference behaviour, default 1) */
+    int nodes_per_block;    /* force-kernel tile: families per 256-lane workgroup (0 = autotuned) */
+    int verbose;
+} pdm_options;
+
+typedef struct {
+    int iterations;         /* inner CG iterations, all refinement sweeps */
+    int refinements;        /* outer FP64 residual / refinement sweeps */
+    double rel_residual;    /* final ||b - K u|| / ||b|| in FP64 */
+    double seconds;         /* wall time of the solve */
+} pdm_static_stats;
+
+/* Fill *opt with defaults. */
+int pdm_options_default(pdm_options* opt);
+
+/* Parse a pds_input.txt deck (CRLF or LF) and create a handle.
+ * Replaces pdDataManager::ReadInputFile (pdDataManager.cpp:4-758) with
+ * ReadKeyword/FindKeyword (pdDataManager.h:270-421) and SetWkDir's input
+ * check (pdDataManager3.cpp:1503-1541). */
+int pdm_create(const char* input_path, const pdm_options* opt, pdm_handle** out);
+
+/* Same, from deck text in memory. */
+int pdm_create_from_text(const char* deck_text, const pdm_options* opt, pdm_handle** out);
+
+/* Release the handle and its device memory (pdDataManager::~pdDataManager, pdDataManager3.cpp:7-18). */
+int pdm_destroy(pdm_handle* h);
+
+/* Message of the last failed call on h (or of the last failed pdm_create if h is NULL). */
+const char* pdm_last_error(const pdm_handle* h);
+
+/* The InitialProblem sequence on the GPU (pdSolve.cpp:10-22): InitNode
+ * (pdDataManager.cpp:813-1098), InitBond (pdDataManager2.cpp:12-70, cell-binned
+ * GPU search, bond set identical to the reference's), InitMaterial
+ * (pdDataManager.cpp:1297-1338), InitBdryEffectFactor (pdDataManager.cpp:1340-1473),
+ * plus per-entry force coefficients and the initial state. */
+int pdm_init(pdm_handle* h);
+
+/* Sizes (pdDataManager::GetNumNode / GetNumBond, pdDataManager.h:107-108). */
+int64_t pdm_num_nodes(const pdm_handle* h);
+int64_t pdm_num_bonds(const pdm_handle* h);     /* unique bonds */
+int64_t pdm_num_entries(const pdm_handle* h);   /* directed family entries = 2 x bonds */
+int pdm_max_family(const pdm_handle* h);        /* _numMaxFamily, pdDataManager.cpp:1345-1387 */
+
+/* Node data: X (3N, may be NULL), material index (N, may be NULL),
+ * boundary-condition index (N, -1 = none; may be NULL).  pdNode getters, pdNode.h:27-142. */
+int pdm_get_nodes(const pdm_handle* h, double* X, int32_t* mat, int32_t* bc);
+
+/* Family CSR in reference order: row_ptr (N+1), col (E, ascending within a row,
+ * the order InitBond builds families in), cut (E: 1 = bond crosses a slit, the
+ * pdBond::_break state it starts with).  Any pointer may be NULL. */
+int pdm_get_csr(const pdm_handle* h, int64_t* row_ptr, int32_t* col, uint8_t* cut);
+
+/* Boundary-effect factor per node (pdNode::GetBdryEffFactor). */
+int pdm_get_bdryeff(const pdm_handle* h, double* be);
+
+/* Stable time step (pdDataManager::CalcStableTimeStep, pdDataManager2.cpp:6-10) and min bond length. */
+int pdm_get_timestep(const pdm_handle* h, double* dt, double* min_bond_length);
+
+/* Family force sum F_i = sum_j f_ij for a displacement field u (3N host), every
+ * node's displacement taken from u: the CCI family loop of CalcBondBasedMotionRK4
+ * (pdDataManager.cpp:1795-1864) with CalcBondForceCCI (pdDataManager2.cpp:1293-1351).
+ * Runs the handle's precision.  elast (N, may be NULL) receives the per-node energy density. */
+int pdm_compute_forces(pdm_handle* h, const double* u, double* F, double* elast);
+
+/* Set / get the dynamic state U, V (3N each, host; NULL skips). */
+int pdm_set_state(pdm_handle* h, const double* U, const double* V);
+int pdm_get_state(const pdm_handle* h, double* U, double* V);
+
+/* Advance nsteps steps of pdSolve::SolveDynamic's loop (pdSolve.cpp:55-120):
+ * t += dt, InitDynamicSolve (pdDataManager2.cpp:410-439), RK4
+ * (pdDataManager.cpp:1994-2134) with UpdateNodeStatusWithBC each stage
+ * (pdDataManager.cpp:2497-2602), shortened final step.  Stops early (returning
+ * the count in *done) when total_step or total_time is reached. */
+int pdm_step_rk4(pdm_handle* h, int nsteps, int* done);
+
+/* Current time, step count, kinetic and elastic energy of the last step
+ * (GetKinetTotal / GetElastStep, pdDataManager3.cpp:1415-1428). */
+int pdm_get_clock(const pdm_handle* h, double* t, int* step, double* dt_next);
+int pdm_get_energy(const pdm_handle* h, double* ke, double* ee);
+
+/* The whole dynamic driver with the reference's files in out_dir:
+ * pds_output.dat lines, pds_tecplot_<step>.dat at step 1 / every dump / last
+ * step (pdSolve.cpp:24-130, 708-732). */
+int pdm_solve_dynamic(pdm_handle* h, const char* out_dir);
+
+/* Static equilibrium K u = b on the tangent of the same bond operator, matrix-free
+ * Jacobi-preconditioned CG with FP64 residual refinement to rel_tol.  Replaces the
+ * dense SolveStatic/AssembleEquation/Gauss path (pdSolve.cpp:132-673), which is
+ * broken in the reference.  On success U holds the solution and V = 0. */
+int pdm_solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_stats* stats);
+
+/* Apply the tangent operator y = K u (3N host each), in the handle's precision. */
+int pdm_apply_tangent(pdm_handle* h, const double* u, double* y);
+
+/* Tecplot dump, same line format as pdSolve::WriteTecplot / pdNode::Print
+ * (pdSolve.cpp:708-732, pdNode.cpp:418-433). */
+int pdm_write_tecplot(const pdm_handle* h, const char* path, int step);
+
+/* Time of the last kernel launches, from HIP events on the handle's stream:
+ * average duration in ms of the dominant (bond-force stage) kernel over the last
+ * pdm_step_rk4 call, and the number of launches it averaged. */
+int pdm_last_kernel_timing(const pdm_handle* h, double* avg_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

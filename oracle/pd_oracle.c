@@ -1,0 +1,874 @@
+/*
+ * pd_oracle.c -- CPU oracle (checker) for the linear bond-based PD hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py.  Never linked into or called by the product.
+ *
+ * A plain-C FP64 restatement of the reference's algorithms, written to keep
+ * the reference's floating-point operation order (left-to-right products,
+ * pow() where the reference calls pow(), no contraction: build with
+ * -ffp-contract=off).  Node loops may run under OpenMP: every per-node
+ * accumulation stays sequential over that node's family, so results are
+ * identical for any thread count.
+ *
+ * Restated reference functions and what pins each one (tests/test_oracle.py):
+ *   lattice      pdDataManager.cpp:846-909            closed-form id <-> (l,m,n) and coordinates
+ *   regions      pdDataManager.cpp:913-1053           hand-built cases: last space wins, tolerance edges
+ *   pair test    pdDataManager3.cpp:637-686,1573-1704 brute-force geometric distance point->cube on
+ *                                                     random pairs; interior family 250 / 460;
+ *                                                     reference bond totals (2,000-node bar: 168,392)
+ *   bond search  pdDataManager2.cpp:12-70             O(N^2) reference loop == cell list, symmetry,
+ *                                                     ascending families
+ *   max family   pdDataManager.cpp:1345-1387          equals the interior family count (250 / 460)
+ *   c, K, dt     pdDataManager.cpp:1314, pdMaterial.cpp:75-88, pdDataManager2.cpp:6-10
+ *                                                     closed forms 12E/(pi d^4), sqrt(K/rho)
+ *   beta         pdDataManager3.cpp:1801-1831         closed form, piecewise breakpoints
+ *   bdryEff      pdDataManager.cpp:1340-1473, pdDataManager2.cpp:1353-1485
+ *                                                     type-one closed form; full families -> 1;
+ *                                                     energy branch vs an independent numpy sum
+ *   CCI force    pdDataManager2.cpp:1293-1351         2-node closed form, eta=0 -> 0, rigid
+ *                                                     translation -> 0, antisymmetry, sum F = 0
+ *   RK4 + BC     pdDataManager.cpp:1767-2134,2497-2602, pdDataManager2.cpp:410-439
+ *                                                     2-node closed-form recurrence written
+ *                                                     independently in the test; uniform velocity
+ *                                                     advances U by 0.75 V dt per step
+ *   time factor  pdBdryConditionDispl.cpp:15-27, pdBdryConditionVeloc.cpp:15-18
+ *                                                     closed form
+ *   driver loop  pdSolve.cpp:55-120                   step count / shortened last step closed form
+ *   slit cut     (no reference implementation; axis form follows the commented test at
+ *                pdDataManager.cpp:1246-1290)        brute-force segment/rectangle test in numpy
+ *   tangent K u  pdDataManager2.cpp:1763-1852 (operator form)
+ *                                                     finite difference of the force (eta -> 0)
+ */
+#include "pd_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MEGA DBL_MAX
+#define TINY DBL_MIN
+
+/* The reference squares with pow(x, 2.0).  glibc's pow is not correctly rounded
+ * for y = 2 (about 1 in 1,200 random doubles differ from x*x by one ulp), so the
+ * oracle can evaluate those calls either with glibc pow (mode 0, literal) or as
+ * the correctly rounded square x*x (mode 1), which is what IEEE gives the device. */
+static int g_square_exact = 0;
+void orc_set_square_mode(int exact) { g_square_exact = exact; }
+static double sq(double x) { return g_square_exact ? x * x : pow(x, 2.0); }
+
+struct orc_state {
+    orc_problem p;
+    double pi;
+    double tol;
+    int64_t n;
+    double* X;     /* 3N */
+    int* mat;      /* N */
+    int* bc;       /* N, -1 = none */
+    int64_t* rp;   /* N+1 */
+    int32_t* col;  /* E */
+    uint8_t* cut;  /* E */
+    double* be;    /* N bdryEff factor */
+    double* spring;/* per material */
+    double min_bond;
+    double max_c;
+    int max_family;
+    double* trac;  /* 3N traction body-force density (extension) */
+    /* dynamic state */
+    double *U, *V, *Vb, *elast;
+    double t, dt_cur, dt_stable, ke, ee;
+    int steps_done, last_step;
+};
+
+/* ------------------------------------------------------------------ */
+/* geometry helpers: pdDataManager3.cpp:1663-1704                       */
+static double shortest_between(double x, double lo, double hi, double tol)
+{
+    if (x < lo - tol) return fabs(x - lo);
+    else if (x > hi + tol) return fabs(x - hi);
+    return 0.0;
+}
+
+static double longest_between(double x, double lo, double hi, double tol)
+{
+    if (x < lo - tol) return fabs(x - hi);
+    else if (x > hi + tol) return fabs(x - lo);
+    if (x > 0.5 * (hi + lo)) return fabs(x - lo);
+    return fabs(x - hi);
+}
+
+/* pdDataManager3.cpp:1573-1661, with positions X+U (U = 0 at bond time). */
+static void dist_to_face(int norm, double x_face, const double* pi_, const double* pj_, double dx,
+                         double* dmin, double* dmax)
+{
+    double tol = 1e-4 * dx;
+    int n1, n2, n3;
+    if (norm == 1) { n1 = 1; n2 = n3 = 0; }
+    else if (norm == 2) { n1 = n3 = 0; n2 = 1; }
+    else { n1 = n2 = 0; n3 = 1; }
+    double x1i = pi_[0], x2i = pi_[1], x3i = pi_[2];
+    double x1j = pj_[0], x2j = pj_[1], x3j = pj_[2];
+    if (fabs((x1i - x1j) * n1 + (x2i - x2j) * n2 + (x3i - x3j) * n3) < tol) {
+        *dmin = MEGA; *dmax = TINY; return;
+    }
+    double x1lo = x1j - 0.5 * dx, x1hi = x1j + 0.5 * dx;
+    double x2lo = x2j - 0.5 * dx, x2hi = x2j + 0.5 * dx;
+    double x3lo = x3j - 0.5 * dx, x3hi = x3j + 0.5 * dx;
+    double x_i = x1i * n1 + x2i * n2 + x3i * n3;
+    double x_j = x1j * n1 + x2j * n2 + x3j * n3;
+    double fac = (x_face - x_i) / (x_j - x_i);
+    double x1int = x1i + fac * (x1j - x1i);
+    double x2int = x2i + fac * (x2j - x2i);
+    double x3int = x3i + fac * (x3j - x3i);
+    if (x1int > x1lo - tol && x1int < x1hi + tol && x2int > x2lo - tol && x2int < x2hi + tol &&
+        x3int > x3lo - tol && x3int < x3hi + tol) {
+        double x1f = (1 - n1) * x1i + n1 * x_face;
+        double x2f = (1 - n2) * x2i + n2 * x_face;
+        double x3f = (1 - n3) * x3i + n3 * x_face;
+        double x1s = (1 - n1) * shortest_between(x1f, x1lo, x1hi, tol);
+        double x2s = (1 - n2) * shortest_between(x2f, x2lo, x2hi, tol);
+        double x3s = (1 - n3) * shortest_between(x3f, x3lo, x3hi, tol);
+        double x1l = (1 - n1) * longest_between(x1f, x1lo, x1hi, tol);
+        double x2l = (1 - n2) * longest_between(x2f, x2lo, x2hi, tol);
+        double x3l = (1 - n3) * longest_between(x3f, x3lo, x3hi, tol);
+        double ff_s = sqrt(x1s * x1s + x2s * x2s + x3s * x3s);
+        double ff_l = sqrt(x1l * x1l + x2l * x2l + x3l * x3l);
+        double itf = sqrt((x1f - x1i) * (x1f - x1i) + (x2f - x2i) * (x2f - x2i) + (x3f - x3i) * (x3f - x3i));
+        *dmin = sqrt(ff_s * ff_s + itf * itf);
+        *dmax = sqrt(ff_l * ff_l + itf * itf);
+        return;
+    }
+    *dmin = MEGA; *dmax = TINY;
+}
+
+/* pdDataManager3.cpp:637-686 */
+static void dist_to_cube(const double* xi, const double* xj, double dx, double* mn, double* mx)
+{
+    double a = MEGA, b = TINY, t0, t1;
+    double lo[3] = {xj[0] - 0.5 * dx, xj[1] - 0.5 * dx, xj[2] - 0.5 * dx};
+    double hi[3] = {xj[0] + 0.5 * dx, xj[1] + 0.5 * dx, xj[2] + 0.5 * dx};
+    for (int d = 0; d < 3; ++d) {
+        dist_to_face(d + 1, lo[d], xi, xj, dx, &t0, &t1);
+        a = (t0 < a) ? t0 : a; b = (b < t1) ? t1 : b;
+        dist_to_face(d + 1, hi[d], xi, xj, dx, &t0, &t1);
+        a = (t0 < a) ? t0 : a; b = (b < t1) ? t1 : b;
+    }
+    *mn = a; *mx = b;
+}
+
+int orc_pair_bonded(const orc_problem* p, const double* xi, const double* xj, double* dmin, double* dmax)
+{
+    /* pdDataManager2.cpp:33-45: prefilter then dmin < delta + 1e-4 dx */
+    double dx = p->dx;
+    double d = sqrt(sq(xj[0] - xi[0]) + sq(xj[1] - xi[1]) + sq(xj[2] - xi[2]));
+    if (p->horizon < d - 1.5 * dx) { *dmin = MEGA; *dmax = TINY; return 0; }
+    dist_to_cube(xi, xj, dx, dmin, dmax);
+    return *dmin < p->horizon + 1.0e-4 * dx;
+}
+
+/* Slit cut test (extension).  Type 1 keeps the reference's commented axis-plane
+ * test (pdDataManager.cpp:1246-1290); type 2 is an oriented rectangle. */
+static void slit_frame(const orc_slit* s, double* n, double* t, double* w)
+{
+    double nn = sqrt(s->p[3] * s->p[3] + s->p[4] * s->p[4] + s->p[5] * s->p[5]);
+    n[0] = s->p[3] / nn; n[1] = s->p[4] / nn; n[2] = s->p[5] / nn;
+    double tn = s->p[6] * n[0] + s->p[7] * n[1] + s->p[8] * n[2];
+    double tt[3] = {s->p[6] - tn * n[0], s->p[7] - tn * n[1], s->p[8] - tn * n[2]};
+    double tl = sqrt(tt[0] * tt[0] + tt[1] * tt[1] + tt[2] * tt[2]);
+    t[0] = tt[0] / tl; t[1] = tt[1] / tl; t[2] = tt[2] / tl;
+    w[0] = n[1] * t[2] - n[2] * t[1];
+    w[1] = n[2] * t[0] - n[0] * t[2];
+    w[2] = n[0] * t[1] - n[1] * t[0];
+}
+
+static int slit_cuts(const orc_slit* s, const double* xi, const double* xj)
+{
+    if (s->type == 1) {
+        const double* b = s->p;
+        int d = -1;
+        for (int k = 0; k < 3; ++k) if (b[2 * k] == b[2 * k + 1]) { d = k; break; }
+        if (d < 0) return 0;
+        double xp = b[2 * d];
+        if (!((xi[d] - xp) * (xj[d] - xp) < 0.)) return 0;
+        for (int o = 0; o < 3; ++o) {
+            if (o == d) continue;
+            double q = xi[o] + (xj[o] - xi[o]) * (xp - xi[d]) / (xj[d] - xi[d]);
+            if (!((q - b[2 * o]) * (q - b[2 * o + 1]) <= 0.)) return 0;
+        }
+        return 1;
+    }
+    double n[3], t[3], w[3];
+    slit_frame(s, n, t, w);
+    const double* c = s->p;
+    double si = n[0] * (xi[0] - c[0]) + n[1] * (xi[1] - c[1]) + n[2] * (xi[2] - c[2]);
+    double sj = n[0] * (xj[0] - c[0]) + n[1] * (xj[1] - c[1]) + n[2] * (xj[2] - c[2]);
+    if (!(si * sj < 0.)) return 0;
+    double a = si / (si - sj);
+    double h[3] = {xi[0] + a * (xj[0] - xi[0]) - c[0], xi[1] + a * (xj[1] - xi[1]) - c[1],
+                   xi[2] + a * (xj[2] - xi[2]) - c[2]};
+    double ht = h[0] * t[0] + h[1] * t[1] + h[2] * t[2];
+    double hw = h[0] * w[0] + h[1] * w[1] + h[2] * w[2];
+    return fabs(ht) <= s->p[9] && fabs(hw) <= s->p[10];
+}
+
+int orc_pair_cut(const orc_problem* p, const double* xi, const double* xj)
+{
+    for (int k = 0; k < p->nslit; ++k)
+        if (slit_cuts(&p->slit[k], xi, xj)) return 1;
+    return 0;
+}
+
+/* pdDataManager3.cpp:1801-1831 */
+double orc_beta(double dist, double delta, double dx, int cci_on, int cci_mdf)
+{
+    double inner = delta - 0.5 * dx;
+    double outer = delta + 0.5 * dx;
+    if (cci_on && !cci_mdf) outer = delta;
+    if (dist <= inner) return 1.0;
+    else if (dist > inner && dist <= outer) return (delta + 0.5 * dx - dist) / dx;
+    return 0.0;
+}
+
+/* pdDataManager2.cpp:1293-1351 (array overload used by the RK4 path) */
+void orc_bond_force_cci(const orc_problem* p, const double* xi, const double* eta, double ecrit,
+                        double spring, double bij, double effWeight, double* f, double* elast)
+{
+    double xi1 = xi[0], xi2 = xi[1], xi3 = xi[2];
+    double e1 = eta[0], e2 = eta[1], e3 = eta[2];
+    double r = sqrt(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
+    double pp = sqrt(sq(xi1 + e1) + sq(xi2 + e2) + sq(xi3 + e3));
+    double dxe = effWeight * p->dx / 2.0;
+    double beta = orc_beta(r, p->horizon, dxe, p->cci_on, p->cci_mdf);
+    double u = pp - r;
+    double st = u / r;
+    if (u <= 0.0 || (u > 0.0 && st <= ecrit)) {
+        if (p->cci_on) {
+            double vol = p->dx * p->dx * p->dx;
+            f[0] = bij * vol * beta * spring * st * (xi1 + e1) / pp;
+            f[1] = bij * vol * beta * spring * st * (xi2 + e2) / pp;
+            f[2] = bij * vol * beta * spring * st * (xi3 + e3) / pp;
+        } else {
+            f[0] = bij * beta * spring * st * (xi1 + e1) / pp;
+            f[1] = bij * beta * spring * st * (xi2 + e2) / pp;
+            f[2] = bij * beta * spring * st * (xi3 + e3) / pp;
+        }
+        *elast = bij * 0.5 * (0.5 * beta * (spring / r) * u * u) * pow(0.5 * effWeight, 3);
+    } else {
+        f[0] = f[1] = f[2] = 0.0;
+        *elast = 0.0;
+    }
+}
+
+/* pdBdryConditionDispl.cpp:15-27 (types 1,3,4 share it); pdBdryConditionVeloc.cpp:15-18 */
+double orc_time_factor(const orc_bc* bc, double t)
+{
+    if (bc->type == 2) return (t <= bc->t_end) ? 1.0 : 0.0;
+    if (bc->load == 1.0) return (t <= bc->t_end) ? t / bc->t_end : 1.0;
+    return 1.0;
+}
+
+/* ------------------------------------------------------------------ */
+static uint64_t splitmix64(uint64_t* s)
+{
+    uint64_t z = (*s += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+static int in_space(const orc_space* sp, const double* x, double tol)
+{
+    const double* b = sp->b;
+    if (sp->geo == 1)
+        return x[0] >= b[0] - tol && x[0] <= b[1] + tol && x[1] >= b[2] - tol && x[1] <= b[3] + tol &&
+               x[2] >= b[4] - tol && x[2] <= b[5] + tol;
+    if (sp->geo == 2) {
+        double dist = sqrt((x[0] - b[1]) * (x[0] - b[1]) + (x[1] - b[2]) * (x[1] - b[2]));
+        return dist <= b[0] + tol && x[2] >= b[3] - tol && x[2] <= b[4] + tol;
+    }
+    return 0;
+}
+
+orc_state* orc_create(const orc_problem* p)
+{
+    orc_state* s = (orc_state*)calloc(1, sizeof(orc_state));
+    s->p = *p;
+    s->pi = acos(-1.0);
+    s->tol = 1.0e-4 * p->dx;
+    return s;
+}
+
+void orc_destroy(orc_state* s)
+{
+    if (!s) return;
+    free(s->X); free(s->mat); free(s->bc); free(s->rp); free(s->col); free(s->cut);
+    free(s->be); free(s->spring); free(s->trac);
+    free(s->U); free(s->V); free(s->Vb); free(s->elast);
+    free(s);
+}
+
+/* InitNode, pdDataManager.cpp:813-1098 (lattice branch), plus the seeded jitter extension */
+static int init_nodes(orc_state* s)
+{
+    const orc_problem* p = &s->p;
+    int64_t cap = (int64_t)p->nx * p->ny * p->nz;
+    s->X = (double*)malloc(sizeof(double) * 3 * (cap > 0 ? cap : 1));
+    int64_t id = 0;
+    double tol = s->tol;
+    for (int n = 1; n != p->nz + 1; ++n)
+        for (int m = 1; m != p->ny + 1; ++m)
+            for (int l = 1; l != p->nx + 1; ++l) {
+                double x[3];
+                x[0] = (l - (p->nx + 1) * 0.5) * p->dx + p->cx;
+                x[1] = (m - (p->ny + 1) * 0.5) * p->dx + p->cy;
+                x[2] = (n - (p->nz + 1) * 0.5) * p->dx + p->cz;
+                int del = 0;
+                for (int d = 0; d < p->nds; ++d)
+                    if (in_space(&p->ds[d], x, tol)) del = 1;
+                if (!del) {
+                    memcpy(&s->X[3 * id], x, sizeof x);
+                    ++id;
+                }
+            }
+    s->n = id;
+    if (p->jitter != 0.0) {
+        uint64_t st = p->seed;
+        for (int64_t i = 0; i < s->n; ++i)
+            for (int c = 0; c < 3; ++c) {
+                double u01 = (double)(splitmix64(&st) >> 11) * 0x1.0p-53;
+                s->X[3 * i + c] += (2.0 * u01 - 1.0) * p->jitter * p->dx;
+            }
+    }
+    s->mat = (int*)malloc(sizeof(int) * (s->n ? s->n : 1));
+    s->bc = (int*)malloc(sizeof(int) * (s->n ? s->n : 1));
+    for (int64_t i = 0; i < s->n; ++i) {
+        const double* x = &s->X[3 * i];
+        int m = -1;
+        /* every material space overwrites; a node outside a later space loses its
+         * material (pdDataManager.cpp:918-968), so effectively the last space decides */
+        for (int k = 0; k < p->nms; ++k) m = in_space(&p->ms[k], x, tol) ? p->ms[k].ref : -1;
+        if (p->nms == 0 || m < 0) return -3; /* node not in any material space (fatal) */
+        s->mat[i] = m;
+        int b = -1;
+        for (int k = 0; k < p->nbs; ++k)
+            if (in_space(&p->bs[k], x, tol)) b = p->bs[k].ref;
+        s->bc[i] = b;
+    }
+    return 0;
+}
+
+typedef struct { int64_t cnt, cap; int32_t* v; } ivec;
+static void ivec_push(ivec* a, int32_t x)
+{
+    if (a->cnt == a->cap) { a->cap = a->cap ? 2 * a->cap : 64; a->v = (int32_t*)realloc(a->v, sizeof(int32_t) * a->cap); }
+    a->v[a->cnt++] = x;
+}
+static int cmp_i32(const void* a, const void* b)
+{
+    int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+    return (x > y) - (x < y);
+}
+
+/* InitBond, pdDataManager2.cpp:12-70 */
+static int init_bonds(orc_state* s, int method)
+{
+    const orc_problem* p = &s->p;
+    int64_t n = s->n;
+    ivec* fam = (ivec*)calloc(n ? n : 1, sizeof(ivec));
+    double minb = p->horizon;
+    if (method == 0) {
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = i + 1; j < n; ++j) {
+                double dmin, dmax;
+                if (orc_pair_bonded(p, &s->X[3 * i], &s->X[3 * j], &dmin, &dmax)) {
+                    const double* a = &s->X[3 * i]; const double* b = &s->X[3 * j];
+                    double d = sqrt(sq(b[0] - a[0]) + sq(b[1] - a[1]) + sq(b[2] - a[2]));
+                    minb = (d < minb) ? d : minb;
+                    ivec_push(&fam[i], (int32_t)j);
+                    ivec_push(&fam[j], (int32_t)i);
+                }
+            }
+    } else {
+        /* cell list with edge >= delta + 1.5 dx: every pair the reference can accept lies
+         * in the 27 neighbouring cells; each pair is tested as (lower id, higher id) */
+        double lo[3] = {MEGA, MEGA, MEGA}, hi[3] = {-MEGA, -MEGA, -MEGA};
+        for (int64_t i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) {
+                double v = s->X[3 * i + c];
+                lo[c] = v < lo[c] ? v : lo[c]; hi[c] = v > hi[c] ? v : hi[c];
+            }
+        double h = p->horizon + 1.5 * p->dx;
+        int64_t nc[3];
+        for (int c = 0; c < 3; ++c) nc[c] = (n ? (int64_t)floor((hi[c] - lo[c]) / h) : 0) + 1;
+        int64_t ncell = nc[0] * nc[1] * nc[2];
+        int64_t* head = (int64_t*)malloc(sizeof(int64_t) * (ncell + 1));
+        int64_t* cid = (int64_t*)malloc(sizeof(int64_t) * (n ? n : 1));
+        int64_t* order = (int64_t*)malloc(sizeof(int64_t) * (n ? n : 1));
+        memset(head, 0, sizeof(int64_t) * (ncell + 1));
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t q[3];
+            for (int c = 0; c < 3; ++c) {
+                q[c] = (int64_t)floor((s->X[3 * i + c] - lo[c]) / h);
+                if (q[c] >= nc[c]) q[c] = nc[c] - 1;
+            }
+            cid[i] = q[0] + nc[0] * (q[1] + nc[1] * q[2]);
+            head[cid[i] + 1]++;
+        }
+        for (int64_t c = 0; c < ncell; ++c) head[c + 1] += head[c];
+        int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (ncell + 1));
+        memcpy(fill, head, sizeof(int64_t) * (ncell + 1));
+        for (int64_t i = 0; i < n; ++i) order[fill[cid[i]]++] = i;
+        free(fill);
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t c0 = cid[i];
+            int64_t q0 = c0 % nc[0], q1 = (c0 / nc[0]) % nc[1], q2 = c0 / (nc[0] * nc[1]);
+            for (int64_t a = q2 - 1; a <= q2 + 1; ++a) {
+                if (a < 0 || a >= nc[2]) continue;
+                for (int64_t b = q1 - 1; b <= q1 + 1; ++b) {
+                    if (b < 0 || b >= nc[1]) continue;
+                    for (int64_t c = q0 - 1; c <= q0 + 1; ++c) {
+                        if (c < 0 || c >= nc[0]) continue;
+                        int64_t cc = c + nc[0] * (b + nc[1] * a);
+                        for (int64_t k = head[cc]; k < head[cc + 1]; ++k) {
+                            int64_t j = order[k];
+                            if (j <= i) continue;
+                            double dmin, dmax;
+                            if (orc_pair_bonded(p, &s->X[3 * i], &s->X[3 * j], &dmin, &dmax)) {
+                                const double* xa = &s->X[3 * i]; const double* xb = &s->X[3 * j];
+                                double d = sqrt(sq(xb[0] - xa[0]) + sq(xb[1] - xa[1]) + sq(xb[2] - xa[2]));
+                                minb = (d < minb) ? d : minb;
+                                ivec_push(&fam[i], (int32_t)j);
+                                ivec_push(&fam[j], (int32_t)i);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        for (int64_t i = 0; i < n; ++i) qsort(fam[i].v, fam[i].cnt, sizeof(int32_t), cmp_i32);
+        free(head); free(cid); free(order);
+    }
+    s->min_bond = minb;
+    s->rp = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
+    s->rp[0] = 0;
+    for (int64_t i = 0; i < n; ++i) s->rp[i + 1] = s->rp[i] + fam[i].cnt;
+    int64_t E = s->rp[n];
+    s->col = (int32_t*)malloc(sizeof(int32_t) * (E ? E : 1));
+    s->cut = (uint8_t*)malloc(E ? E : 1);
+    for (int64_t i = 0; i < n; ++i) {
+        if (fam[i].cnt) memcpy(&s->col[s->rp[i]], fam[i].v, sizeof(int32_t) * fam[i].cnt);
+        free(fam[i].v);
+    }
+    free(fam);
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
+            int64_t j = s->col[e];
+            int64_t a = i < j ? i : j, b = i < j ? j : i;
+            s->cut[e] = (uint8_t)orc_pair_cut(p, &s->X[3 * a], &s->X[3 * b]);
+        }
+    return 0;
+}
+
+/* InitMaterial, pdDataManager.cpp:1297-1338 */
+static void init_material(orc_state* s)
+{
+    const orc_problem* p = &s->p;
+    s->spring = (double*)malloc(sizeof(double) * (p->nmat ? p->nmat : 1));
+    s->max_c = 0.0;
+    for (int m = 0; m < p->nmat; ++m) {
+        double nu = (p->mat[m].type == 1) ? 0.25 : p->mat[m].nu;
+        double K = p->mat[m].E / (3.0 * (1.0 - 2.0 * nu));
+        s->spring[m] = (p->mat[m].type == 1) ? 18 * K / (s->pi * pow(p->horizon, 4.0)) : 0.0;
+        double c = sqrt(K / p->mat[m].rho);
+        s->max_c = (s->max_c < c) ? c : s->max_c;
+    }
+}
+
+double orc_spring(const orc_state* s, int m) { return s->spring[m]; }
+
+/* CalcBdryEffFactor, pdDataManager2.cpp:1353-1477 */
+static double bdryeff_energy(const orc_state* s, int64_t i)
+{
+    const orc_problem* p = &s->p;
+    double e11 = 1.0e-4;
+    const orc_mat* mi = &p->mat[s->mat[i]];
+    double ecrit = mi->ecrit, spring = s->spring[s->mat[i]];
+    double eps11 = e11, eps22 = e11, eps33 = e11;
+    double eps12 = 0.0, eps13 = 0.0, eps21 = 0.0, eps23 = 0.0, eps31 = 0.0, eps32 = 0.0;
+    double vol = pow(p->dx, 3.0);
+    double sum = 0.0;
+    const double* xi_ = &s->X[3 * i];
+    for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
+        if (s->cut[e]) continue;
+        const double* xj = &s->X[3 * (int64_t)s->col[e]];
+        double xi1 = xj[0] - xi_[0], xi2 = xj[1] - xi_[1], xi3 = xj[2] - xi_[2];
+        double eta1 = eps11 * xi1 + eps12 * xi2 + eps13 * xi3;
+        double eta2 = eps21 * xi1 + eps22 * xi2 + eps23 * xi3;
+        double eta3 = eps31 * xi1 + eps32 * xi2 + eps33 * xi3;
+        double r = sqrt(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
+        double p1 = xi1 + eta1, p2 = xi2 + eta2, p3 = xi3 + eta3;
+        double pp = sqrt(p1 * p1 + p2 * p2 + p3 * p3);
+        double u = pp - r;
+        double st = u / r;
+        double el;
+        if (u <= 0 || (u > 0 && st <= ecrit)) el = 0.5 * (spring / r) * u * u;
+        else el = 0.0;
+        double beta = orc_beta(r, p->horizon, p->dx, p->cci_on, p->cci_mdf);
+        double elj = el * beta * vol;
+        sum += 0.5 * elj;
+    }
+    double p_inf = 3.0 * mi->E * e11 * e11;
+    return p_inf / sum;
+}
+
+/* CalcBdryEffFactorTypeOne, pdDataManager2.cpp:1479-1485 */
+static double bdryeff_type_one(const orc_state* s, double h, int m)
+{
+    const orc_problem* p = &s->p;
+    double full = 3.0 * p->mat[m].E;
+    double part = full - 0.5 * s->spring[m] * s->pi *
+                             (pow(p->horizon, 4) / 4.0 - h * pow(p->horizon, 3) / 3.0 + pow(h, 4) / 12.0);
+    return full / part;
+}
+
+/* InitBdryEffectFactor, pdDataManager.cpp:1340-1473 */
+static void init_bdryeff(orc_state* s)
+{
+    const orc_problem* p = &s->p;
+    int64_t n = s->n;
+    s->be = (double*)malloc(sizeof(double) * (n ? n : 1));
+    for (int64_t i = 0; i < n; ++i) s->be[i] = 1.0;
+    /* max family from a virtual lattice around the origin (pdDataManager.cpp:1345-1387) */
+    int vd = 2 * ((int)(p->horizon / p->dx) + 1);
+    double o[3] = {0.0, 0.0, 0.0};
+    int maxf = 0;
+    for (int nn = 1; nn != vd + 1; ++nn)
+        for (int m = 1; m != vd + 1; ++m)
+            for (int l = 1; l != vd + 1; ++l) {
+                double xj[3] = {(l - vd * 0.5) * p->dx, (m - vd * 0.5) * p->dx, (nn - vd * 0.5) * p->dx};
+                double dmin, dmax;
+                dist_to_cube(o, xj, p->dx, &dmin, &dmax);
+                if (dmin < p->horizon + s->tol) ++maxf;
+            }
+    s->max_family = maxf;
+    if (!p->bdryeff_on) return;
+    double x1L = -p->nx * 0.5 * p->dx + p->cx, x1H = p->nx * 0.5 * p->dx + p->cx;
+    double x2L = -p->ny * 0.5 * p->dx + p->cy, x2H = p->ny * 0.5 * p->dx + p->cy;
+    double x3L = -p->nz * 0.5 * p->dx + p->cz, x3H = p->nz * 0.5 * p->dx + p->cz;
+    double ns = ((int)(p->horizon / p->dx)) * p->dx;
+    for (int64_t i = 0; i < n; ++i) {
+        double x1 = s->X[3 * i], x2 = s->X[3 * i + 1], x3 = s->X[3 * i + 2];
+        int m = s->mat[i];
+        if (s->rp[i + 1] - s->rp[i] == maxf) { s->be[i] = 1.0; continue; }
+        if (x1 >= x1H - ns && x1 <= x1H && x2 >= x2L + ns && x2 <= x2H - ns && x3 >= x3L + ns && x3 <= x3H - ns)
+            s->be[i] = bdryeff_type_one(s, x1H - x1, m);
+        else if (x1 >= x1L && x1 <= x1L + ns && x2 >= x2L + ns && x2 <= x2H - ns && x3 >= x3L + ns && x3 <= x3H - ns)
+            s->be[i] = bdryeff_type_one(s, x1 - x1L, m);
+        else if (x2 >= x2H - ns && x2 <= x2H && x1 >= x1L + ns && x1 <= x1H - ns && x3 >= x3L + ns && x3 <= x3H - ns)
+            s->be[i] = bdryeff_type_one(s, x2H - x2, m);
+        else if (x2 >= x2L && x2 <= x2L + ns && x1 >= x1L + ns && x1 <= x1H - ns && x3 >= x3L + ns && x3 <= x3H - ns)
+            s->be[i] = bdryeff_type_one(s, x2 - x2L, m);
+        else if (x3 >= x3H - ns && x3 <= x3H && x1 >= x1L + ns && x1 <= x1H - ns && x2 >= x2L + ns && x2 <= x2H - ns)
+            s->be[i] = bdryeff_type_one(s, x3H - x3, m);
+        else if (x3 >= x3L && x3 <= x3L + ns && x1 >= x1L + ns && x1 <= x1H - ns && x2 >= x2L + ns && x2 <= x2H - ns)
+            s->be[i] = bdryeff_type_one(s, x3 - x3L, m);
+        else
+            s->be[i] = bdryeff_energy(s, i);
+    }
+}
+
+/* Traction as body force (extension): a TRACTION bc's value is a total force,
+ * spread evenly over the nodes carrying that bc, as a force density per volume. */
+static void init_traction(orc_state* s)
+{
+    const orc_problem* p = &s->p;
+    int64_t n = s->n;
+    s->trac = (double*)calloc(3 * (n ? n : 1), sizeof(double));
+    if (!p->traction_body) return;
+    int64_t cnt[ORC_MAX_BC] = {0};
+    for (int64_t i = 0; i < n; ++i) if (s->bc[i] >= 0) cnt[s->bc[i]]++;
+    double vol = pow(p->dx, 3.0);
+    for (int64_t i = 0; i < n; ++i) {
+        int b = s->bc[i];
+        if (b < 0 || p->bc[b].type != 4) continue;
+        for (int c = 0; c < 3; ++c)
+            if (p->bc[b].f[c] == 1) s->trac[3 * i + c] = p->bc[b].v[c] / (double)cnt[b] / vol;
+    }
+}
+
+int orc_init(orc_state* s, int method, int nthreads)
+{
+    (void)nthreads;
+    int rc = init_nodes(s);
+    if (rc) return rc;
+    rc = init_bonds(s, method);
+    if (rc) return rc;
+    init_material(s);
+    init_bdryeff(s);
+    init_traction(s);
+    int64_t n = s->n;
+    s->U = (double*)calloc(3 * (n ? n : 1), sizeof(double));
+    s->V = (double*)calloc(3 * (n ? n : 1), sizeof(double));
+    s->Vb = (double*)calloc(3 * (n ? n : 1), sizeof(double));
+    s->elast = (double*)calloc((n ? n : 1), sizeof(double));
+    s->dt_stable = s->p.fixed_dt_on ? s->p.fixed_dt : s->p.safety * s->min_bond / s->max_c;
+    s->dt_cur = s->dt_stable;
+    s->t = 0.0;
+    s->steps_done = 0;
+    s->last_step = s->p.total_step;
+    return 0;
+}
+
+int64_t orc_num_nodes(const orc_state* s) { return s->n; }
+int64_t orc_num_entries(const orc_state* s) { return s->rp ? s->rp[s->n] : 0; }
+int orc_num_max_family(const orc_state* s) { return s->max_family; }
+double orc_min_bond_length(const orc_state* s) { return s->min_bond; }
+double orc_stable_dt(const orc_state* s) { return s->dt_stable; }
+void orc_get_nodes(const orc_state* s, double* X, int* mat, int* bc)
+{
+    if (X) memcpy(X, s->X, sizeof(double) * 3 * s->n);
+    if (mat) memcpy(mat, s->mat, sizeof(int) * s->n);
+    if (bc) memcpy(bc, s->bc, sizeof(int) * s->n);
+}
+void orc_get_csr(const orc_state* s, int64_t* rp, int32_t* col, uint8_t* cut)
+{
+    if (rp) memcpy(rp, s->rp, sizeof(int64_t) * (s->n + 1));
+    if (col) memcpy(col, s->col, sizeof(int32_t) * s->rp[s->n]);
+    if (cut) memcpy(cut, s->cut, s->rp[s->n]);
+}
+void orc_get_bdryeff(const orc_state* s, double* be) { memcpy(be, s->be, sizeof(double) * s->n); }
+void orc_set_bdryeff(orc_state* s, const double* be) { memcpy(s->be, be, sizeof(double) * s->n); }
+
+/* The CCI branch of the family loop of CalcBondBasedMotionRK4 (pdDataManager.cpp:1795-1864),
+ * for node i with the displacement of every node read from u. */
+static void family_force(const orc_state* s, int64_t i, const double* u, double* F, double* el)
+{
+    const orc_problem* p = &s->p;
+    const orc_mat* mi = &p->mat[s->mat[i]];
+    double acc[3] = {0.0, 0.0, 0.0}, ea = 0.0;
+    for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
+        if (s->cut[e]) continue;
+        int64_t j = s->col[e];
+        const orc_mat* mj = &p->mat[s->mat[j]];
+        double xi[3] = {s->X[3 * j] - s->X[3 * i], s->X[3 * j + 1] - s->X[3 * i + 1], s->X[3 * j + 2] - s->X[3 * i + 2]};
+        double eta[3] = {u[3 * j] - u[3 * i], u[3 * j + 1] - u[3 * i + 1], u[3 * j + 2] - u[3 * i + 2]};
+        double ecrit = (mj->ecrit < mi->ecrit) ? mj->ecrit : mi->ecrit;
+        double ci = s->spring[s->mat[i]], cj = s->spring[s->mat[j]];
+        double spring = (cj < ci) ? cj : ci;
+        double bij = 0.5 * (s->be[i] + s->be[j]);
+        double f[3], el1;
+        orc_bond_force_cci(p, xi, eta, ecrit, spring, bij, 2.0, f, &el1);
+        acc[0] += f[0]; acc[1] += f[1]; acc[2] += f[2];
+        ea += el1;
+    }
+    F[0] = acc[0]; F[1] = acc[1]; F[2] = acc[2];
+    if (el) *el = ea;
+}
+
+void orc_forces(const orc_state* s, const double* u, double* F, double* elast, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 256) if (nthreads != 1)
+#endif
+    for (int64_t i = 0; i < s->n; ++i) family_force(s, i, u, &F[3 * i], elast ? &elast[i] : NULL);
+}
+
+/* Tangent (eta -> 0) of the same CCI bond force: (K u)_i = sum_j w_ij xi xi^T (u_i - u_j) / r^3
+ * with w_ij = bdryEff_ij * dx^3 * beta * c_ij, i.e. the operator of CalcDampingCoeff
+ * (pdDataManager2.cpp:1763-1852) with the force's beta and bdryEff weights. */
+void orc_apply_tangent(const orc_state* s, const double* u, double* y, int nthreads)
+{
+    const orc_problem* p = &s->p;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 256) if (nthreads != 1)
+#endif
+    for (int64_t i = 0; i < s->n; ++i) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
+            if (s->cut[e]) continue;
+            int64_t j = s->col[e];
+            double xi[3] = {s->X[3 * j] - s->X[3 * i], s->X[3 * j + 1] - s->X[3 * i + 1], s->X[3 * j + 2] - s->X[3 * i + 2]};
+            double r = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+            double ci = s->spring[s->mat[i]], cj = s->spring[s->mat[j]];
+            double c = (cj < ci) ? cj : ci;
+            double bij = 0.5 * (s->be[i] + s->be[j]);
+            double beta = orc_beta(r, p->horizon, p->dx, p->cci_on, p->cci_mdf);
+            double w = bij * (p->dx * p->dx * p->dx) * beta * c;
+            double d[3] = {u[3 * i] - u[3 * j], u[3 * i + 1] - u[3 * j + 1], u[3 * i + 2] - u[3 * j + 2]};
+            double proj = (xi[0] * d[0] + xi[1] * d[1] + xi[2] * d[2]) * w / (r * r * r);
+            acc[0] += proj * xi[0]; acc[1] += proj * xi[1]; acc[2] += proj * xi[2];
+        }
+        y[3 * i] = acc[0]; y[3 * i + 1] = acc[1]; y[3 * i + 2] = acc[2];
+    }
+}
+
+/* A node is a "BC node" (pdNode::GetBC() non-null) when it lies in a boundary space.  With the
+ * traction-as-body-force extension on, TRACTION nodes stay free nodes: in the reference a type-4
+ * node keeps its stale U through all four stages with no overwrite, an update whose amplification
+ * has determinant 1 + 0.875 (w dt)^2 > 1, i.e. unconditionally unstable. */
+static int bc_node(const orc_state* s, int64_t i)
+{
+    int b = s->bc[i];
+    if (b < 0) return 0;
+    if (s->p.traction_body && s->p.bc[b].type == 4) return 0;
+    return 1;
+}
+
+/* UpdateNodeStatusWithBC, pdDataManager.cpp:2497-2602 */
+static void apply_bc(orc_state* s, int64_t i, double t)
+{
+    int b = s->bc[i];
+    if (b < 0) return;
+    const orc_bc* bc = &s->p.bc[b];
+    double fac = orc_time_factor(bc, t);
+    double* U = &s->U[3 * i];
+    double* Vb = &s->Vb[3 * i];
+    const double* X = &s->X[3 * i];
+    for (int c = 0; c < 3; ++c) {
+        if (bc->f[c] != 1) continue;
+        if (bc->type == 1) { U[c] = bc->v[c] * fac; Vb[c] = 0.0; }
+        else if (bc->type == 2) { Vb[c] = bc->v[c] * fac; }
+        else if (bc->type == 3) {
+            U[c] = (bc->v[3 * c] * X[0] + bc->v[3 * c + 1] * X[1] + bc->v[3 * c + 2] * X[2]) * fac;
+            Vb[c] = 0.0;
+        }
+    }
+}
+
+/* One stage of CalcBondBasedMotionRK4 (pdDataManager.cpp:1767-1992): Vbuf accumulation per bond,
+ * BC overwrite, k = Vbuf * dt_s.  trial: displStep (3N). */
+static void rk4_stage(orc_state* s, double dts, double t, const double* trial, double* k, int nthreads)
+{
+    const orc_problem* p = &s->p;
+    int64_t n = s->n;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 256) if (nthreads != 1)
+#endif
+    for (int64_t i = 0; i < n; ++i) {
+        const orc_mat* mi = &p->mat[s->mat[i]];
+        double denst = mi->rho;
+        double bf[3] = {mi->g[0] * denst, mi->g[1] * denst, mi->g[2] * denst};
+        const double* ui = bc_node(s, i) ? &s->U[3 * i] : &trial[3 * i];
+        for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
+            if (s->cut[e]) continue;
+            int64_t j = s->col[e];
+            const orc_mat* mj = &p->mat[s->mat[j]];
+            double xi[3] = {s->X[3 * j] - s->X[3 * i], s->X[3 * j + 1] - s->X[3 * i + 1], s->X[3 * j + 2] - s->X[3 * i + 2]};
+            const double* uj = bc_node(s, j) ? &s->U[3 * j] : &trial[3 * j];
+            double eta[3] = {uj[0] - ui[0], uj[1] - ui[1], uj[2] - ui[2]};
+            double ecrit = (mj->ecrit < mi->ecrit) ? mj->ecrit : mi->ecrit;
+            double ci = s->spring[s->mat[i]], cj = s->spring[s->mat[j]];
+            double spring = (cj < ci) ? cj : ci;
+            double bij = 0.5 * (s->be[i] + s->be[j]);
+            double f[3], el;
+            orc_bond_force_cci(p, xi, eta, ecrit, spring, bij, 2.0, f, &el);
+            s->elast[i] += el;
+            s->Vb[3 * i] += (f[0] + bf[0]) * dts / denst;
+            s->Vb[3 * i + 1] += (f[1] + bf[1]) * dts / denst;
+            s->Vb[3 * i + 2] += (f[2] + bf[2]) * dts / denst;
+        }
+        /* traction body force (extension), once per node */
+        if (p->traction_body && s->bc[i] >= 0 && p->bc[s->bc[i]].type == 4) {
+            double fac = orc_time_factor(&p->bc[s->bc[i]], t);
+            for (int c = 0; c < 3; ++c) s->Vb[3 * i + c] += s->trac[3 * i + c] * fac * dts / denst;
+        }
+    }
+    /* the BC overwrite and k for all nodes happen after the whole family loop */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) if (nthreads != 1)
+#endif
+    for (int64_t i = 0; i < n; ++i) {
+        apply_bc(s, i, t);
+        k[3 * i] = s->Vb[3 * i] * dts;
+        k[3 * i + 1] = s->Vb[3 * i + 1] * dts;
+        k[3 * i + 2] = s->Vb[3 * i + 2] * dts;
+    }
+}
+
+/* RK4, pdDataManager.cpp:1994-2134 (CCI: CalcIntgPointDisplRK4 is a no-op) */
+static void rk4(orc_state* s, double dt, double t, int nthreads)
+{
+    const orc_problem* p = &s->p;
+    int64_t n = s->n, n3 = 3 * n;
+    double* u1 = (double*)malloc(sizeof(double) * n3);
+    double* u234 = (double*)malloc(sizeof(double) * n3);
+    double* k1 = (double*)malloc(sizeof(double) * n3);
+    double* k24 = (double*)malloc(sizeof(double) * n3);
+    double* k3 = (double*)malloc(sizeof(double) * n3);
+    double dtHalf = 0.5 * dt, dt6 = 1.0 / 6.0;
+    memcpy(u1, s->U, sizeof(double) * n3);
+    rk4_stage(s, dtHalf, t, u1, k1, nthreads);
+    for (int64_t i = 0; i < n3; ++i) u234[i] = u1[i] + 0.5 * k1[i];
+    rk4_stage(s, dtHalf, t, u234, k24, nthreads);
+    for (int64_t i = 0; i < n3; ++i) u234[i] = u1[i] + 0.5 * k24[i];
+    rk4_stage(s, dt, t, u234, k3, nthreads);
+    for (int64_t i = 0; i < n3; ++i) { u234[i] = u1[i] + k3[i]; k3[i] += k24[i]; }
+    rk4_stage(s, dt, t, u234, k24, nthreads);
+    for (int64_t i = 0; i < n; ++i) {
+        double denst = p->mat[s->mat[i]].rho;
+        for (int c = 0; c < 3; ++c) {
+            int64_t r = 3 * i + c;
+            if (bc_node(s, i))
+                s->U[r] = s->U[r] + dt6 * (k1[r] + k24[r] + 2.0 * k3[r]);
+            else
+                s->U[r] = s->U[r] + dt6 * (k1[r] + k24[r] + 2.0 * k3[r]) - dt * p->damping * s->V[r] * dt / denst;
+        }
+        for (int c = 0; c < 3; ++c) s->Vb[3 * i + c] += -p->damping * s->V[3 * i + c] * dt / denst;
+        for (int c = 0; c < 3; ++c) s->V[3 * i + c] = s->Vb[3 * i + c];
+    }
+    double vol = pow(p->dx, 3.0);
+    for (int64_t i = 0; i < n; ++i) {
+        double v1 = s->V[3 * i], v2 = s->V[3 * i + 1], v3 = s->V[3 * i + 2];
+        double velsq = v1 * v1 + v2 * v2 + v3 * v3;
+        double denst = p->mat[s->mat[i]].rho;
+        s->ee += vol * s->elast[i];
+        s->ke += 0.5 * denst * velsq * vol;
+    }
+    free(u1); free(u234); free(k1); free(k24); free(k3);
+}
+
+/* pdSolve::SolveDynamic loop body (pdSolve.cpp:55-120), nsteps iterations from the current one */
+int orc_run_dynamic(orc_state* s, int nsteps, int nthreads)
+{
+    const orc_problem* p = &s->p;
+    for (int it = 0; it < nsteps; ++it) {
+        int iter = s->steps_done + 1;
+        if (iter > p->total_step) return 1;
+        s->t += s->dt_cur;
+        if (!(s->t <= p->total_time)) { s->t -= s->dt_cur; return 1; }
+        /* InitDynamicSolve, pdDataManager2.cpp:410-439 */
+        s->ee = 0.0; s->ke = 0.0;
+        memset(s->elast, 0, sizeof(double) * s->n);
+        memcpy(s->Vb, s->V, sizeof(double) * 3 * s->n);
+        rk4(s, s->dt_cur, s->t, nthreads);
+        s->steps_done = iter;
+        if (s->t + s->dt_stable > p->total_time && s->t != p->total_time) {
+            s->last_step = iter + 1;
+            s->dt_cur = p->total_time - s->t;
+        }
+    }
+    return 0;
+}
+
+void orc_get_state(const orc_state* s, double* U, double* V)
+{
+    if (U) memcpy(U, s->U, sizeof(double) * 3 * s->n);
+    if (V) memcpy(V, s->V, sizeof(double) * 3 * s->n);
+}
+void orc_set_state(orc_state* s, const double* U, const double* V)
+{
+    if (U) memcpy(s->U, U, sizeof(double) * 3 * s->n);
+    if (V) memcpy(s->V, V, sizeof(double) * 3 * s->n);
+}
+double orc_time(const orc_state* s) { return s->t; }
+double orc_dt_current(const orc_state* s) { return s->dt_cur; }
+double orc_ke(const orc_state* s) { return s->ke; }
+double orc_ee(const orc_state* s) { return s->ee; }
+int orc_steps_done(const orc_state* s) { return s->steps_done; }

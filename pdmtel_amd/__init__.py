@@ -1,0 +1,225 @@
+"""pdmtel_amd -- MI355X-native linear bond-based peridynamics (Python host over the C ABI).
+
+The product is the HIP library ``libpdm.so`` (C ABI in include/pdm.h).  This
+module is a thin ctypes binding with the reference's driver vocabulary
+(pdSolve::InitialProblem / SolveDynamic / SolveStatic, pdDataManager getters).
+There is no CPU fallback: importing the binding without the built library
+raises, and every call goes to the GPU library.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+from .deck import bar_spec, deck_text, mid_slit  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpdm.so")
+
+PDM_FP64 = 0
+PDM_FP32C = 1
+
+ERRORS = {
+    -1: "PDM_E_ARG", -2: "PDM_E_IO", -3: "PDM_E_PARSE", -4: "PDM_E_MODEL", -5: "PDM_E_HIP",
+    -6: "PDM_E_STATE", -7: "PDM_E_LIMIT", -8: "PDM_E_CONVERGE",
+}
+
+
+class PdmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Options(C.Structure):
+    _fields_ = [("precision", C.c_int), ("device", C.c_int), ("use_graph", C.c_int), ("graph_chunk", C.c_int),
+                ("energies", C.c_int), ("nodes_per_block", C.c_int), ("verbose", C.c_int)]
+
+
+class StaticStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("refinements", C.c_int), ("rel_residual", C.c_double),
+                ("seconds", C.c_double)]
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Load libpdm.so (built in-tree by __graft_entry__.build() / `make -C pdmtel_amd`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `make -C pdmtel_amd` or __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, i32p, i64p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    L.pdm_options_default.argtypes = [C.POINTER(Options)]
+    L.pdm_create.argtypes = [C.c_char_p, C.POINTER(Options), C.POINTER(vp)]
+    L.pdm_create_from_text.argtypes = [C.c_char_p, C.POINTER(Options), C.POINTER(vp)]
+    L.pdm_destroy.argtypes = [vp]
+    L.pdm_last_error.restype = C.c_char_p
+    L.pdm_last_error.argtypes = [vp]
+    L.pdm_init.argtypes = [vp]
+    for name in ("pdm_num_nodes", "pdm_num_bonds", "pdm_num_entries"):
+        getattr(L, name).restype = C.c_int64
+        getattr(L, name).argtypes = [vp]
+    L.pdm_max_family.argtypes = [vp]
+    L.pdm_get_nodes.argtypes = [vp, dp, i32p, i32p]
+    L.pdm_get_csr.argtypes = [vp, i64p, i32p, C.POINTER(C.c_uint8)]
+    L.pdm_get_bdryeff.argtypes = [vp, dp]
+    L.pdm_get_timestep.argtypes = [vp, dp, dp]
+    L.pdm_compute_forces.argtypes = [vp, dp, dp, dp]
+    L.pdm_apply_tangent.argtypes = [vp, dp, dp]
+    L.pdm_set_state.argtypes = [vp, dp, dp]
+    L.pdm_get_state.argtypes = [vp, dp, dp]
+    L.pdm_step_rk4.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    L.pdm_get_clock.argtypes = [vp, dp, C.POINTER(C.c_int), dp]
+    L.pdm_get_energy.argtypes = [vp, dp, dp]
+    L.pdm_solve_dynamic.argtypes = [vp, C.c_char_p]
+    L.pdm_solve_static.argtypes = [vp, C.c_double, C.c_int, C.POINTER(StaticStats)]
+    L.pdm_write_tecplot.argtypes = [vp, C.c_char_p, C.c_int]
+    L.pdm_last_kernel_timing.argtypes = [vp, dp, i64p]
+    _lib = L
+    return L
+
+
+def options(precision: int = PDM_FP64, **kw) -> Options:
+    o = Options()
+    lib().pdm_options_default(C.byref(o))
+    o.precision = precision
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Problem:
+    """One specimen: pdm_create + pdm_init (the InitialProblem sequence, pdSolve.cpp:10-22)."""
+
+    def __init__(self, deck: str, precision: int = PDM_FP64, from_text: bool = True, init: bool = True, **opts):
+        self.L = lib()
+        self.h = C.c_void_p()
+        o = options(precision, **opts)
+        if from_text:
+            rc = self.L.pdm_create_from_text(deck.encode(), C.byref(o), C.byref(self.h))
+        else:
+            rc = self.L.pdm_create(deck.encode(), C.byref(o), C.byref(self.h))
+        if rc != 0:
+            raise PdmError(rc, self.L.pdm_last_error(None).decode())
+        self.precision = precision
+        if init:
+            self.init()
+
+    def _chk(self, rc: int) -> None:
+        if rc != 0:
+            raise PdmError(rc, self.L.pdm_last_error(self.h).decode())
+
+    def init(self) -> None:
+        self._chk(self.L.pdm_init(self.h))
+        self.n = int(self.L.pdm_num_nodes(self.h))
+        self.E = int(self.L.pdm_num_entries(self.h))
+
+    def close(self) -> None:
+        if getattr(self, "h", None) and self.h.value:
+            self.L.pdm_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def max_family(self) -> int:
+        return int(self.L.pdm_max_family(self.h))
+
+    def nodes(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        X = np.zeros(3 * self.n)
+        mat = np.zeros(self.n, dtype=np.int32)
+        bc = np.zeros(self.n, dtype=np.int32)
+        self._chk(self.L.pdm_get_nodes(self.h, _dp(X), mat.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       bc.ctypes.data_as(C.POINTER(C.c_int32))))
+        return X.reshape(-1, 3), mat, bc
+
+    def csr(self):
+        rp = np.zeros(self.n + 1, dtype=np.int64)
+        col = np.zeros(max(self.E, 1), dtype=np.int32)
+        cut = np.zeros(max(self.E, 1), dtype=np.uint8)
+        self._chk(self.L.pdm_get_csr(self.h, rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     col.ctypes.data_as(C.POINTER(C.c_int32)),
+                                     cut.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return rp, col[: self.E], cut[: self.E]
+
+    def bdryeff(self) -> np.ndarray:
+        be = np.zeros(self.n)
+        self._chk(self.L.pdm_get_bdryeff(self.h, _dp(be)))
+        return be
+
+    def timestep(self) -> Tuple[float, float]:
+        dt, mb = C.c_double(), C.c_double()
+        self._chk(self.L.pdm_get_timestep(self.h, C.byref(dt), C.byref(mb)))
+        return dt.value, mb.value
+
+    def forces(self, u: np.ndarray, with_elast: bool = False):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        F = np.zeros(3 * self.n)
+        el = np.zeros(self.n)
+        self._chk(self.L.pdm_compute_forces(self.h, _dp(u), _dp(F), _dp(el)))
+        return (F.reshape(-1, 3), el) if with_elast else F.reshape(-1, 3)
+
+    def tangent(self, u: np.ndarray) -> np.ndarray:
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        y = np.zeros(3 * self.n)
+        self._chk(self.L.pdm_apply_tangent(self.h, _dp(u), _dp(y)))
+        return y.reshape(-1, 3)
+
+    def set_state(self, U: Optional[np.ndarray], V: Optional[np.ndarray]) -> None:
+        Ua = np.ascontiguousarray(U, dtype=np.float64).reshape(-1) if U is not None else None
+        Va = np.ascontiguousarray(V, dtype=np.float64).reshape(-1) if V is not None else None
+        self._chk(self.L.pdm_set_state(self.h, _dp(Ua) if Ua is not None else None,
+                                       _dp(Va) if Va is not None else None))
+
+    def state(self) -> Tuple[np.ndarray, np.ndarray]:
+        U = np.zeros(3 * self.n)
+        V = np.zeros(3 * self.n)
+        self._chk(self.L.pdm_get_state(self.h, _dp(U), _dp(V)))
+        return U.reshape(-1, 3), V.reshape(-1, 3)
+
+    def step(self, nsteps: int) -> int:
+        done = C.c_int()
+        self._chk(self.L.pdm_step_rk4(self.h, int(nsteps), C.byref(done)))
+        return done.value
+
+    def clock(self) -> Tuple[float, int, float]:
+        t, s, dt = C.c_double(), C.c_int(), C.c_double()
+        self._chk(self.L.pdm_get_clock(self.h, C.byref(t), C.byref(s), C.byref(dt)))
+        return t.value, s.value, dt.value
+
+    def energy(self) -> Tuple[float, float]:
+        ke, ee = C.c_double(), C.c_double()
+        self._chk(self.L.pdm_get_energy(self.h, C.byref(ke), C.byref(ee)))
+        return ke.value, ee.value
+
+    def kernel_timing(self) -> Tuple[float, int]:
+        ms, n = C.c_double(), C.c_int64()
+        self._chk(self.L.pdm_last_kernel_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def solve_dynamic(self, out_dir: str) -> None:
+        self._chk(self.L.pdm_solve_dynamic(self.h, out_dir.encode()))
+
+    def solve_static(self, rel_tol: float = 1e-10, max_iterations: int = 100000) -> StaticStats:
+        st = StaticStats()
+        self._chk(self.L.pdm_solve_static(self.h, rel_tol, max_iterations, C.byref(st)))
+        return st
+
+    def write_tecplot(self, path: str, step: int = 0) -> None:
+        self._chk(self.L.pdm_write_tecplot(self.h, path.encode(), int(step)))

@@ -1,0 +1,652 @@
+// pdm_api.cpp -- the C ABI (include/pdm.h): handle lifecycle, InitialProblem on the GPU,
+// the RK4 driver with hipGraph-captured steps, state transfer and output writers.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+
+#include "pdm_internal.h"
+
+using namespace pdm;
+
+namespace {
+
+thread_local std::string g_create_err;
+
+#define HCHK(x)                                                                    \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                    \
+            h->err = std::string("HIP error ") + hipGetErrorString(e_) + " at " #x; \
+            return PDM_E_HIP;                                                      \
+        }                                                                          \
+    } while (0)
+
+int fail(const pdm_handle* h, int code, const std::string& m)
+{
+    const_cast<pdm_handle*>(h)->err = m;
+    return code;
+}
+
+bool read_file(const char* path, std::string& out)
+{
+    std::ifstream in(path, std::ios::in | std::ios::binary);
+    if (!in) return false;
+    std::ostringstream ss;
+    ss << in.rdbuf();
+    out = ss.str();
+    return true;
+}
+
+int create_common(const std::string& text, const pdm_options* opt, pdm_handle** out)
+{
+    if (!out) {
+        g_create_err = "null output handle pointer";
+        return PDM_E_ARG;
+    }
+    *out = nullptr;
+    pdm_handle* h = new pdm_handle();
+    if (opt) h->opt = *opt;
+    else pdm_options_default(&h->opt);
+    if (h->opt.precision != PDM_FP64 && h->opt.precision != PDM_FP32C) {
+        g_create_err = "unknown precision";
+        delete h;
+        return PDM_E_ARG;
+    }
+    if (h->opt.graph_chunk <= 0) h->opt.graph_chunk = 50;
+    std::string err;
+    int rc = parse_deck(text, h->deck, err);
+    if (rc) {
+        g_create_err = err;
+        delete h;
+        return rc;
+    }
+    const Deck& d = h->deck;
+    if (!d.cci_on) {
+        g_create_err = d.fgi_on ? "fixed Gaussian integration (fgi_on) is not supported by this build"
+                                : "adaptive integration (ai_on) is not supported by this build";
+        delete h;
+        return PDM_E_MODEL;
+    }
+    if (d.mats.size() > (size_t)kMaxMat || d.bcs.size() > (size_t)kMaxBC || d.slits.size() > (size_t)kMaxSlit) {
+        g_create_err = "too many materials / boundary conditions / slits";
+        delete h;
+        return PDM_E_LIMIT;
+    }
+    for (const auto& m : d.mats)
+        if (m.type != 1) {
+            g_create_err = "state-based materials are not supported (the reference's state-based path is commented out)";
+            delete h;
+            return PDM_E_MODEL;
+        }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= h->opt.device) {
+        g_create_err = "no HIP device " + std::to_string(h->opt.device);
+        delete h;
+        return PDM_E_HIP;
+    }
+    h->dev = h->opt.device;
+    if (hipSetDevice(h->dev) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+        g_create_err = "HIP stream/event creation failed";
+        delete h;
+        return PDM_E_HIP;
+    }
+    *out = h;
+    return PDM_OK;
+}
+
+template <class T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int require_init(const pdm_handle* h)
+{
+    if (!h) return PDM_E_ARG;
+    if (!h->inited) return fail(h, PDM_E_STATE, "pdm_init has not completed on this handle");
+    return PDM_OK;
+}
+
+StageArgs stage_args(pdm_handle* h)
+{
+    StageArgs a{};
+    a.row_ptr = h->rp;
+    a.col = h->col;
+    a.coef64 = h->coef64;
+    a.coef32 = h->coef32;
+    a.mat = h->matd;
+    a.bc = h->bcd;
+    a.nb = h->nb;
+    a.U = h->U;
+    a.V = h->V;
+    a.Vb = h->Vb;
+    a.k1 = h->k1;
+    a.k23 = h->k23;
+    a.elast = h->elast;
+    a.part = h->part;
+    a.steps = h->steps;
+    a.n = h->n;
+    a.energies = h->opt.energies ? 1 : 0;
+    a.xcd_remap = 1;
+    fill_stage_tables(h, a);
+    return a;
+}
+
+// Upload state U, V (3N AoS) and the gather records' displacement (rec[0].u = U).
+int upload_state(pdm_handle* h, const double* U, const double* V)
+{
+    int64_t n = h->n;
+    std::vector<d4> buf(n);
+    if (U) {
+        for (int64_t i = 0; i < n; ++i) buf[i] = d4{U[3 * i], U[3 * i + 1], U[3 * i + 2], 0.0};
+        HCHK(hipMemcpy(h->U, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
+        std::vector<NodeRec> rec(n);
+        for (int64_t i = 0; i < n; ++i) {
+            rec[i].x = d4{h->X[3 * i], h->X[3 * i + 1], h->X[3 * i + 2], h->deck.mats[h->mat[i]].ecrit};
+            rec[i].u = buf[i];
+        }
+        HCHK(hipMemcpy(h->rec[0], rec.data(), sizeof(NodeRec) * n, hipMemcpyHostToDevice));
+    }
+    if (V) {
+        for (int64_t i = 0; i < n; ++i) buf[i] = d4{V[3 * i], V[3 * i + 1], V[3 * i + 2], 0.0};
+        HCHK(hipMemcpy(h->V, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
+    }
+    return PDM_OK;
+}
+
+int ensure_chunk(pdm_handle* h, int cap)
+{
+    if (h->chunk_cap >= cap) return PDM_OK;
+    dfree(h->steps);
+    dfree(h->energy);
+    HCHK(hipMalloc(&h->steps, sizeof(StepParam) * cap));
+    HCHK(hipMalloc(&h->energy, sizeof(double) * 2 * cap));
+    h->chunk_cap = cap;
+    if (h->graph_exec) {
+        (void)hipGraphExecDestroy(h->graph_exec);
+        h->graph_exec = nullptr;
+        h->graph_steps = 0;
+    }
+    return PDM_OK;
+}
+
+// Enqueue `count` steps whose parameters sit in h->steps[0..count).
+int enqueue_steps(pdm_handle* h, int count)
+{
+    StageArgs a = stage_args(h);
+    int64_t nblk = stage_blocks(h->n);
+    for (int k = 0; k < count; ++k) {
+        for (int st = 0; st < 4; ++st) {
+            a.rin = h->rec[st & 1];
+            a.rout = h->rec[(st + 1) & 1];
+            HCHK(launch_stage(st, h->opt.precision, a, k, h->stream));
+        }
+        if (a.energies) HCHK(launch_energy_final(a, nblk, h->energy + 2 * k, h->stream));
+    }
+    return PDM_OK;
+}
+
+int run_chunk(pdm_handle* h, const std::vector<StepParam>& sp)
+{
+    int count = (int)sp.size();
+    if (count == 0) return PDM_OK;
+    int rc = ensure_chunk(h, std::max(count, h->opt.graph_chunk));
+    if (rc) return rc;
+    HCHK(hipMemcpyAsync(h->steps, sp.data(), sizeof(StepParam) * count, hipMemcpyHostToDevice, h->stream));
+    bool graph = h->opt.use_graph && count == h->opt.graph_chunk;
+    if (graph && !h->graph_exec) {
+        hipGraph_t g;
+        HCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_steps(h, count);
+        hipError_t ce = hipStreamEndCapture(h->stream, &g);
+        if (rc) return rc;
+        HCHK(ce);
+        HCHK(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+        HCHK(hipGraphDestroy(g));
+        h->graph_steps = count;
+    }
+    HCHK(hipEventRecord(h->ev0, h->stream));
+    if (graph) HCHK(hipGraphLaunch(h->graph_exec, h->stream));
+    else {
+        rc = enqueue_steps(h, count);
+        if (rc) return rc;
+    }
+    HCHK(hipEventRecord(h->ev1, h->stream));
+    HCHK(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_kernel_ms += ms;
+    h->last_kernel_launches += 4LL * count;
+    if (h->opt.energies) {
+        double e[2];
+        HCHK(hipMemcpy(e, h->energy + 2 * (count - 1), sizeof e, hipMemcpyDeviceToHost));
+        h->ke = e[0];
+        h->ee = e[1];
+    }
+    return PDM_OK;
+}
+
+// The SolveDynamic loop bookkeeping (pdSolve.cpp:55-120) for up to `want` steps.
+void plan_steps(pdm_handle* h, int want, std::vector<StepParam>& sp)
+{
+    const Deck& d = h->deck;
+    sp.clear();
+    for (int k = 0; k < want; ++k) {
+        int iter = h->step + 1;
+        if (iter > d.total_step) break;
+        double t = h->t + h->dt_cur;
+        if (!(t <= d.total_time)) break;
+        h->t = t;
+        sp.push_back(StepParam{t, h->dt_cur});
+        h->step = iter;
+        if (h->t + h->dt_stable > d.total_time && h->t != d.total_time) {
+            h->last_step = iter + 1;
+            h->dt_cur = d.total_time - h->t;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdm_options_default(pdm_options* opt)
+{
+    if (!opt) return PDM_E_ARG;
+    std::memset(opt, 0, sizeof(*opt));
+    opt->precision = PDM_FP64;
+    opt->device = 0;
+    opt->use_graph = 1;
+    opt->graph_chunk = 50;
+    opt->energies = 1;
+    opt->nodes_per_block = 0;
+    opt->verbose = 0;
+    return PDM_OK;
+}
+
+int pdm_create(const char* input_path, const pdm_options* opt, pdm_handle** out)
+{
+    if (!input_path) {
+        g_create_err = "null input path";
+        return PDM_E_ARG;
+    }
+    std::string text;
+    if (!read_file(input_path, text)) {
+        g_create_err = std::string("SetWkDir: input file not exits! (") + input_path + ")";
+        return PDM_E_IO;
+    }
+    return create_common(text, opt, out);
+}
+
+int pdm_create_from_text(const char* deck_text, const pdm_options* opt, pdm_handle** out)
+{
+    if (!deck_text) {
+        g_create_err = "null deck text";
+        return PDM_E_ARG;
+    }
+    return create_common(std::string(deck_text), opt, out);
+}
+
+int pdm_destroy(pdm_handle* h)
+{
+    if (!h) return PDM_E_ARG;
+    (void)hipSetDevice(h->dev);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    for (auto& r : h->rec) dfree(r);
+    dfree(h->Xd); dfree(h->matd); dfree(h->bcd); dfree(h->nb); dfree(h->col); dfree(h->rp); dfree(h->cut);
+    dfree(h->coef64); dfree(h->coef32); dfree(h->be);
+    dfree(h->U); dfree(h->V); dfree(h->Vb); dfree(h->k1); dfree(h->k23);
+    dfree(h->elast); dfree(h->part); dfree(h->energy); dfree(h->steps); dfree(h->scratchF);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return PDM_OK;
+}
+
+const char* pdm_last_error(const pdm_handle* h)
+{
+    if (!h) return g_create_err.c_str();
+    return h->err.c_str();
+}
+
+int pdm_init(pdm_handle* h)
+{
+    if (!h) return PDM_E_ARG;
+    if (h->inited) return fail(h, PDM_E_STATE, "pdm_init called twice");
+    HCHK(hipSetDevice(h->dev));
+    int rc = build_nodes(h);
+    if (rc) return rc;
+    init_materials(h);
+    // traction density of each TRACTION bc (extension): total force / (nodes carrying it) / dx^3
+    const Deck& d = h->deck;
+    std::memset(h->trac_density, 0, sizeof(h->trac_density));
+    if (d.traction_body) {
+        std::vector<int64_t> cnt(d.bcs.size(), 0);
+        for (int64_t i = 0; i < h->n; ++i)
+            if (h->bc[i] >= 0) cnt[h->bc[i]]++;
+        double vol = std::pow(d.dx, 3.0);
+        for (size_t b = 0; b < d.bcs.size(); ++b)
+            if (d.bcs[b].type == 4 && cnt[b] > 0)
+                for (int c = 0; c < 3; ++c)
+                    if (d.bcs[b].f[c] == 1) h->trac_density[b][c] = d.bcs[b].v[c] / (double)cnt[b] / vol;
+    }
+    rc = build_bonds(h);
+    if (rc) return rc;
+    rc = build_bdryeff_and_coef(h);
+    if (rc) return rc;
+    int64_t n = std::max<int64_t>(h->n, 1);
+    for (d4** p : {&h->U, &h->V, &h->Vb, &h->k1, &h->k23}) {
+        HCHK(hipMalloc(p, sizeof(d4) * n));
+        HCHK(hipMemsetAsync(*p, 0, sizeof(d4) * n, h->stream));
+    }
+    HCHK(hipMalloc(&h->elast, sizeof(double) * n));
+    HCHK(hipMemsetAsync(h->elast, 0, sizeof(double) * n, h->stream));
+    h->part_blocks = stage_blocks(h->n);
+    HCHK(hipMalloc(&h->part, sizeof(double) * 2 * std::max<int64_t>(h->part_blocks, 1)));
+    HCHK(hipMemsetAsync(h->part, 0, sizeof(double) * 2 * std::max<int64_t>(h->part_blocks, 1), h->stream));
+    HCHK(hipStreamSynchronize(h->stream));
+    // CalcStableTimeStep (pdDataManager2.cpp:6-10) and the driver's clock
+    h->dt_stable = d.fixed_dt_on ? d.fixed_dt : d.safety * h->min_bond / h->max_c;
+    h->dt_cur = h->dt_stable;
+    h->t = 0.0;
+    h->step = 0;
+    h->last_step = d.total_step;
+    h->inited = true;
+    return PDM_OK;
+}
+
+int64_t pdm_num_nodes(const pdm_handle* h) { return h ? h->n : -1; }
+int64_t pdm_num_entries(const pdm_handle* h) { return h ? h->E : -1; }
+int64_t pdm_num_bonds(const pdm_handle* h) { return h ? h->E / 2 : -1; }
+int pdm_max_family(const pdm_handle* h) { return h ? h->max_family : -1; }
+
+int pdm_get_nodes(const pdm_handle* h, double* X, int32_t* mat, int32_t* bc)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (X) std::memcpy(X, h->X.data(), sizeof(double) * 3 * h->n);
+    if (mat) std::memcpy(mat, h->mat.data(), sizeof(int32_t) * h->n);
+    if (bc) std::memcpy(bc, h->bc.data(), sizeof(int32_t) * h->n);
+    return PDM_OK;
+}
+
+int pdm_get_csr(const pdm_handle* h, int64_t* row_ptr, int32_t* col, uint8_t* cut)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    pdm_handle* hh = const_cast<pdm_handle*>(h);
+    if (row_ptr) { pdm_handle* h = hh; HCHK(hipMemcpy(row_ptr, h->rp, sizeof(int64_t) * (h->n + 1), hipMemcpyDeviceToHost)); }
+    if (col && h->E) { pdm_handle* h = hh; HCHK(hipMemcpy(col, h->col, sizeof(int32_t) * h->E, hipMemcpyDeviceToHost)); }
+    if (cut && h->E) { pdm_handle* h = hh; HCHK(hipMemcpy(cut, h->cut, h->E, hipMemcpyDeviceToHost)); }
+    return PDM_OK;
+}
+
+int pdm_get_bdryeff(const pdm_handle* h, double* be)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (!be) return PDM_E_ARG;
+    pdm_handle* hh = const_cast<pdm_handle*>(h);
+    {
+        pdm_handle* h = hh;
+        HCHK(hipMemcpy(be, h->be, sizeof(double) * h->n, hipMemcpyDeviceToHost));
+    }
+    return PDM_OK;
+}
+
+int pdm_get_timestep(const pdm_handle* h, double* dt, double* min_bond_length)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (dt) *dt = h->dt_stable;
+    if (min_bond_length) *min_bond_length = h->min_bond;
+    return PDM_OK;
+}
+
+int pdm_compute_forces(pdm_handle* h, const double* u, double* F, double* elast)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (!u || !F) return fail(h, PDM_E_ARG, "null u or F");
+    int64_t n = h->n;
+    if (n == 0) return PDM_OK;
+    std::vector<NodeRec> rec(n);
+    for (int64_t i = 0; i < n; ++i) {
+        rec[i].x = d4{h->X[3 * i], h->X[3 * i + 1], h->X[3 * i + 2], h->deck.mats[h->mat[i]].ecrit};
+        rec[i].u = d4{u[3 * i], u[3 * i + 1], u[3 * i + 2], 0.0};
+    }
+    NodeRec* tmp = nullptr;
+    double *Fd = nullptr, *ed = nullptr;
+    HCHK(hipMalloc(&tmp, sizeof(NodeRec) * n));
+    HCHK(hipMalloc(&Fd, sizeof(double) * 3 * n));
+    HCHK(hipMalloc(&ed, sizeof(double) * n));
+    HCHK(hipMemcpy(tmp, rec.data(), sizeof(NodeRec) * n, hipMemcpyHostToDevice));
+    ForceArgs a{};
+    a.rin = tmp;
+    a.row_ptr = h->rp;
+    a.col = h->col;
+    a.coef64 = h->coef64;
+    a.coef32 = h->coef32;
+    a.mat = h->matd;
+    a.F = Fd;
+    a.elast = ed;
+    a.n = n;
+    a.inv_vol_cube = 1.0 / (h->deck.dx * h->deck.dx * h->deck.dx);
+    HCHK(launch_forces(h->opt.precision, a, h->stream));
+    HCHK(hipStreamSynchronize(h->stream));
+    HCHK(hipMemcpy(F, Fd, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    if (elast) HCHK(hipMemcpy(elast, ed, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HCHK(hipFree(tmp));
+    HCHK(hipFree(Fd));
+    HCHK(hipFree(ed));
+    return PDM_OK;
+}
+
+int pdm_apply_tangent(pdm_handle* h, const double* u, double* y)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (!u || !y) return fail(h, PDM_E_ARG, "null u or y");
+    int64_t n = h->n;
+    if (n == 0) return PDM_OK;
+    double *ud = nullptr, *yd = nullptr;
+    HCHK(hipMalloc(&ud, sizeof(double) * 3 * n));
+    HCHK(hipMalloc(&yd, sizeof(double) * 3 * n));
+    HCHK(hipMemcpy(ud, u, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    ForceArgs a{};
+    a.rin = h->rec[0];
+    a.row_ptr = h->rp;
+    a.col = h->col;
+    a.coef64 = h->coef64;
+    a.coef32 = h->coef32;
+    a.mat = h->matd;
+    a.n = n;
+    HCHK(launch_tangent(h->opt.precision, a, ud, yd, h->stream));
+    HCHK(hipStreamSynchronize(h->stream));
+    HCHK(hipMemcpy(y, yd, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    HCHK(hipFree(ud));
+    HCHK(hipFree(yd));
+    return PDM_OK;
+}
+
+int pdm_set_state(pdm_handle* h, const double* U, const double* V)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    HCHK(hipStreamSynchronize(h->stream));
+    return upload_state(h, U, V);
+}
+
+int pdm_get_state(const pdm_handle* h, double* U, double* V)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    pdm_handle* hh = const_cast<pdm_handle*>(h);
+    int64_t n = h->n;
+    std::vector<d4> buf(n);
+    {
+        pdm_handle* h = hh;
+        HCHK(hipStreamSynchronize(h->stream));
+        if (U) {
+            HCHK(hipMemcpy(buf.data(), h->U, sizeof(d4) * n, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) { U[3 * i] = buf[i].x; U[3 * i + 1] = buf[i].y; U[3 * i + 2] = buf[i].z; }
+        }
+        if (V) {
+            HCHK(hipMemcpy(buf.data(), h->V, sizeof(d4) * n, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; ++i) { V[3 * i] = buf[i].x; V[3 * i + 1] = buf[i].y; V[3 * i + 2] = buf[i].z; }
+        }
+    }
+    return PDM_OK;
+}
+
+int pdm_step_rk4(pdm_handle* h, int nsteps, int* done)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(h, PDM_E_ARG, "negative step count");
+    HCHK(hipSetDevice(h->dev));
+    h->last_kernel_ms = 0.0;
+    h->last_kernel_launches = 0;
+    int total = 0;
+    std::vector<StepParam> sp;
+    while (total < nsteps) {
+        int want = std::min(nsteps - total, h->opt.graph_chunk);
+        plan_steps(h, want, sp);
+        if (sp.empty()) break;
+        rc = run_chunk(h, sp);
+        if (rc) return rc;
+        total += (int)sp.size();
+        if ((int)sp.size() < want) break;
+    }
+    if (done) *done = total;
+    return PDM_OK;
+}
+
+int pdm_get_clock(const pdm_handle* h, double* t, int* step, double* dt_next)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (t) *t = h->t;
+    if (step) *step = h->step;
+    if (dt_next) *dt_next = h->dt_cur;
+    return PDM_OK;
+}
+
+int pdm_get_energy(const pdm_handle* h, double* ke, double* ee)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (ke) *ke = h->ke;
+    if (ee) *ee = h->ee;
+    return PDM_OK;
+}
+
+int pdm_last_kernel_timing(const pdm_handle* h, double* avg_ms, int64_t* launches)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (avg_ms) *avg_ms = h->last_kernel_launches ? h->last_kernel_ms / (double)h->last_kernel_launches : 0.0;
+    if (launches) *launches = h->last_kernel_launches;
+    return PDM_OK;
+}
+
+int pdm_write_tecplot(const pdm_handle* h, const char* path, int step)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    (void)step;
+    if (!path) return fail(h, PDM_E_ARG, "null path");
+    int64_t n = h->n;
+    std::vector<double> U(3 * n), V(3 * n), be(n);
+    rc = pdm_get_state(h, U.data(), V.data());
+    if (rc) return rc;
+    rc = pdm_get_bdryeff(h, be.data());
+    if (rc) return rc;
+    std::ofstream out(path, std::ios::out);
+    if (!out) return fail(h, PDM_E_IO, std::string("cannot write ") + path);
+    const Deck& d = h->deck;
+    // pdSolve::WriteTecplot (pdSolve.cpp:708-732), pdNode::Print (pdNode.cpp:418-433)
+    out << "Tecplot formatting output mode\n";
+    out.setf(std::ios::scientific);
+    out.precision(5);
+    out << "VARIABLES = \n ID X1 X2 X3 U1 U2 U3 V1 V2 V3 MatID BCID bdryEff DmgFrac \n"
+        << "ZONE I=" << d.nx << ", J=" << d.ny << ", K=" << d.nz << ", F=POINT\n\n";
+    for (int64_t i = 0; i < n; ++i) {
+        out << i << "\t" << h->X[3 * i] << "\t" << h->X[3 * i + 1] << "\t" << h->X[3 * i + 2] << "\t" << U[3 * i]
+            << "\t" << U[3 * i + 1] << "\t" << U[3 * i + 2] << "\t" << V[3 * i] << "\t" << V[3 * i + 1] << "\t"
+            << V[3 * i + 2] << "\t" << h->mat[i];
+        out << "\t" << h->bc[i];
+        out << "\t" << be[i] << "\t" << 0.0 << std::endl;
+    }
+    return out.good() ? PDM_OK : fail(h, PDM_E_IO, std::string("write failed: ") + path);
+}
+
+int pdm_solve_dynamic(pdm_handle* h, const char* out_dir)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    std::string dir = out_dir ? std::string(out_dir) : std::string(".");
+    if (!dir.empty() && dir.back() != '/') dir += '/';
+    std::ofstream outf((dir + "pds_output.dat").c_str(), std::ios::app);
+    if (!outf) return fail(h, PDM_E_IO, "cannot open pds_output.dat in " + dir);
+    outf.setf(std::ios::scientific);
+    outf.precision(5);
+    std::ofstream stepf((dir + "pds_steps.dat").c_str(), std::ios::out);
+    stepf << "Data at given node for each time step output mode\n\n";
+    stepf << "Step Time ID X1 X2 X3 U1 U2 U3 V1 V2 V3 MatID BCID bdryEff DmgFrac \n";
+    outf << std::endl << "** Dynamic solution: starting from cycle 1 **" << std::endl;
+    const Deck& d = h->deck;
+    int dump = std::max(1, d.dump);
+    std::vector<StepParam> sp;
+    std::vector<double> en;
+    // break chunks at steps that write a dump (1, every dump, last)
+    for (;;) {
+        int next = h->step + 1;
+        int want = 1;
+        if (next != 1) {
+            int to_dump = dump - (h->step % dump);
+            want = std::min(to_dump, h->opt.graph_chunk);
+        }
+        int before = h->step;
+        plan_steps(h, want, sp);
+        if (sp.empty()) break;
+        rc = run_chunk(h, sp);
+        if (rc) return rc;
+        int cnt = (int)sp.size();
+        if (h->opt.energies) {
+            en.resize(2 * cnt);
+            HCHK(hipMemcpy(en.data(), h->energy, sizeof(double) * 2 * cnt, hipMemcpyDeviceToHost));
+        }
+        for (int k = 0; k < cnt; ++k) {
+            int iter = before + k + 1;
+            outf << iter << ": dt= " << sp[k].dt << " t= " << sp[k].t << " KE= " << (h->opt.energies ? en[2 * k] : 0.0)
+                 << " EE= " << (h->opt.energies ? en[2 * k + 1] : 0.0) << std::endl;
+        }
+        int iter = h->step;
+        if (iter == 1 || iter == h->last_step || iter % dump == 0) {
+            std::string path = dir + "pds_tecplot_" + std::to_string(iter) + ".dat";
+            rc = pdm_write_tecplot(h, path.c_str(), iter);
+            if (rc) return rc;
+        }
+    }
+    return PDM_OK;
+}
+
+int pdm_solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_stats* stats)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    (void)rel_tol;
+    (void)max_iterations;
+    (void)stats;
+    return fail(h, PDM_E_MODEL, "static solve not available yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+"""Writer for ``pds_input.txt`` decks from a plain-dict problem spec.
+
+The keyword/value-line format and every keyword name follow the reference
+parser (pdDataManager.cpp:4-758, pdDataManager.h:270-421): a keyword on its
+own line, the values on the next line, whitespace separated.  Keywords the
+reference does not know (``grid_jitter``, ``number_of_slits``/``slit_N``,
+``traction_as_body_force``) are extensions; the reference's search-by-keyword
+parser skips lines it is not asked for, so a deck stays readable by it.
+
+Spec keys (all lengths in metres, moduli in Pa):
+  title, grid_dimension (nx,ny,nz), grid_spacing, grid_center (3),
+  horizon, total_time, total_step, step_dump_frequency, safety_factor,
+  fixed_time_step (None = stable dt),
+  materials: [{type: 1, E, sigy, ecrit, rho, gravity: (3)}],
+  bcs: [{type, flags: (3), value: (3 or 9), t_end, load}],
+  material_spaces / deletion_spaces / boundary_spaces:
+      [{geo: 1, ref, box: (x1lo,x1hi,x2lo,x2hi,x3lo,x3hi)} |
+       {geo: 2, ref, cyl: (r, cx, cy, x3lo, x3hi)}]   (ref is 1-based, as in the deck)
+  damping, bdryeff_on, cci: (on, modified) | None, fgi: (on, intg_point) | None,
+  support_domain_size (None = 3 dx),
+  jitter (fraction of dx), seed, slits: [{type: 1, box: (6)} | {type: 2, c, n, t, half_t, half_s}],
+  traction_body (0/1)
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Any, Dict, List
+
+import numpy as np
+
+
+def _fmt(v: Any) -> str:
+    if isinstance(v, bool):
+        return "1" if v else "0"
+    if isinstance(v, (int, np.integer)):
+        return str(int(v))
+    if isinstance(v, str):
+        return v
+    return repr(float(v))
+
+
+def _line(*vals: Any) -> str:
+    return " " + " ".join(_fmt(v) for v in vals)
+
+
+def deck_text(spec: Dict[str, Any], crlf: bool = False) -> str:
+    """Render a spec dict as pds_input.txt text (LF by default, CRLF on request)."""
+    out: List[str] = []
+
+    def kw(name: str, *vals: Any) -> None:
+        out.append(name)
+        out.append(_line(*vals))
+
+    out.append("// generated deck")
+    kw("problem_title", spec.get("title", "pdm_problem"))
+    kw("grid_center", *spec.get("grid_center", (0.0, 0.0, 0.0)))
+    kw("grid_dimension", *[int(v) for v in spec["grid_dimension"]])
+    kw("grid_spacing", spec["grid_spacing"])
+    if spec.get("horizon") is not None:
+        kw("horizon", spec["horizon"])
+    kw("total_time", spec["total_time"])
+    kw("total_step", int(spec["total_step"]))
+    kw("step_dump_frequency", int(spec.get("step_dump_frequency", 1000000)))
+    if spec.get("safety_factor") is not None:
+        kw("safety_factor", spec["safety_factor"])
+    if spec.get("fixed_time_step") is not None:
+        kw("fixed_time_step", spec["fixed_time_step"])
+    mats = spec["materials"]
+    kw("number_of_materials", len(mats))
+    for i, m in enumerate(mats, 1):
+        kw(f"material_{i}", int(m.get("type", 1)), m["E"], m.get("sigy", 1e30), m["ecrit"], m["rho"])
+        g = m.get("gravity")
+        if g is not None:
+            kw(f"gravity_{i}", *g)
+    bcs = spec.get("bcs", [])
+    kw("number_of_boundary_conditions", len(bcs))
+    for i, b in enumerate(bcs, 1):
+        vals = list(b["value"])
+        kw(f"boundary_condition_{i}", int(b["type"]), *[int(f) for f in b["flags"]], *vals, b["t_end"], b["load"])
+
+    def spaces(prefix: str, count_kw: str, lst: List[Dict[str, Any]], with_ref: bool) -> None:
+        kw(count_kw, len(lst))
+        for i, s in enumerate(lst, 1):
+            geo = int(s.get("geo", 1))
+            ref = [int(s["ref"])] if with_ref else []
+            body = s["box"] if geo == 1 else s["cyl"]
+            kw(f"{prefix}{i}", geo, *ref, *body)
+
+    spaces("material_space_", "number_of_material_spaces", spec["material_spaces"], True)
+    if spec.get("deletion_spaces"):
+        spaces("deletion_space_", "number_of_deletion_spaces", spec["deletion_spaces"], False)
+    spaces("boundary_space_", "number_of_boundary_spaces", spec.get("boundary_spaces", []), True)
+    kw("damping_coeff", spec.get("damping", 0.0))
+    kw("bdryEff_on", int(spec.get("bdryeff_on", 1)))
+    cci = spec.get("cci", (1, 1))
+    if cci is not None and cci[0]:
+        kw("cci_on", 1, int(cci[1]))
+    else:
+        kw("cci_on", 0, 0)
+        fgi = spec.get("fgi")
+        if fgi is None or not fgi[0]:
+            raise ValueError("spec needs cci or fgi")
+        kw("fgi_on", 1)
+        kw("intg_point", int(fgi[1]))
+        if spec.get("support_domain_size") is not None:
+            kw("support_domain_size", spec["support_domain_size"])
+    if spec.get("jitter"):
+        kw("grid_jitter", spec["jitter"], int(spec.get("seed", 0)))
+    slits = spec.get("slits", [])
+    if slits:
+        kw("number_of_slits", len(slits))
+        for i, s in enumerate(slits, 1):
+            if int(s["type"]) == 1:
+                kw(f"slit_{i}", 1, *s["box"])
+            else:
+                kw(f"slit_{i}", 2, *s["c"], *s["n"], *s["t"], s["half_t"], s["half_s"])
+    if spec.get("traction_body"):
+        kw("traction_as_body_force", 1)
+    text = "\n".join(out) + "\n"
+    if crlf:
+        text = text.replace("\n", "\r\n")
+    return text
+
+
+def bar_spec(nx: int, ny: int, nz: int, dx: float = 1e-3, delta_over_dx: float = 3.015,
+             E: float = 72e9, rho: float = 2440.0, steps: int = 200, safety: float = 0.8,
+             traction: float = 1e6, jitter: float = 0.0, seed: int = 0, slits=None,
+             cci_modified: int = 1, bdryeff_on: int = 1) -> Dict[str, Any]:
+    """Uniaxial bar: x=0 face fixed (DISPL, step), +x traction on the x=L layer.
+
+    The traction is a TRACTION bc (type 4) whose value is the total force
+    sigma * (ny dx) * (nz dx); it acts only with ``traction_body`` on (the
+    reference treats type 4 as a no-op).  Node centres sit at (i+0.5) dx.
+    """
+    L = nx * dx
+    area = (ny * dx) * (nz * dx)
+    big = 1e3
+    spec: Dict[str, Any] = dict(
+        title=f"bar_{nx}x{ny}x{nz}",
+        grid_dimension=(nx, ny, nz),
+        grid_spacing=dx,
+        grid_center=(0.5 * nx * dx, 0.5 * ny * dx, 0.5 * nz * dx),
+        horizon=delta_over_dx * dx,
+        total_time=1.0,
+        total_step=steps,
+        step_dump_frequency=10 ** 9,
+        safety_factor=safety,
+        materials=[dict(type=1, E=E, sigy=1e30, ecrit=1e10, rho=rho)],
+        bcs=[dict(type=1, flags=(1, 1, 1), value=(0.0, 0.0, 0.0), t_end=1.0, load=2),
+             dict(type=4, flags=(1, 0, 0), value=(traction * area, 0.0, 0.0), t_end=1.0, load=2)],
+        material_spaces=[dict(geo=1, ref=1, box=(-big, big, -big, big, -big, big))],
+        boundary_spaces=[dict(geo=1, ref=1, box=(-big, 1.0 * dx, -big, big, -big, big)),
+                         dict(geo=1, ref=2, box=(L - 1.0 * dx, big, -big, big, -big, big))],
+        damping=0.0,
+        bdryeff_on=bdryeff_on,
+        cci=(1, cci_modified),
+        jitter=jitter,
+        seed=seed,
+        slits=list(slits or []),
+        traction_body=1,
+    )
+    return spec
+
+
+def mid_slit(spec: Dict[str, Any], angle_deg: float, axis_t=(0.0, 0.0, 1.0)) -> Dict[str, Any]:
+    """A full-section planar slit through the bar centre, normal in the x-y plane at angle_deg to x."""
+    nx, ny, nz = spec["grid_dimension"]
+    dx = spec["grid_spacing"]
+    a = math.radians(angle_deg)
+    c = spec["grid_center"]
+    big = 10.0 * max(nx, ny, nz) * dx
+    return dict(type=2, c=tuple(c), n=(math.cos(a), math.sin(a), 0.0), t=tuple(axis_t), half_t=big, half_s=big)

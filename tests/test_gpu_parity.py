@@ -1,0 +1,138 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle.
+
+Bar: bit-exact for the bond set (row_ptr, col, cut), bdryEff and node data;
+forces within 1e-12 normwise (max |dF| / max |F|) in FP64 with the oracle's
+pow(x, 2.0) evaluated as the correctly rounded square; RK4 trajectories within
+1e-11 normwise after a few steps (only the family summation order differs).
+"""
+
+import numpy as np
+import pytest
+
+import pdmtel_amd as pm
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "bar_12x6x6": dict(nx=12, ny=6, nz=6),
+    "bar_20x10x10": dict(nx=20, ny=10, nz=10),
+    "bar_jitter": dict(nx=16, ny=8, nz=8, jitter=0.01, seed=0),
+    "bar_delta4": dict(nx=14, ny=10, nz=10, delta_over_dx=4.015, jitter=0.01, seed=3),
+    "bar_cci_original": dict(nx=12, ny=8, nz=8, cci_modified=0),
+}
+
+
+def make(case, **over):
+    kw = dict(CASES[case])
+    kw.update(over)
+    return pm.bar_spec(kw.pop("nx"), kw.pop("ny"), kw.pop("nz"), **kw)
+
+
+def pair(spec, precision=pm.PDM_FP64, **opts):
+    prob = pm.Problem(pm.deck_text(spec), precision=precision, **opts)
+    orc = pyoracle.Oracle(spec, bond_method=1)
+    return prob, orc
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_bond_set_bit_exact(case):
+    spec = make(case)
+    prob, orc = pair(spec)
+    print(f"[bond set] {case} n={prob.n}", flush=True)
+    rp, col, cut = prob.csr()
+    orp, ocol, ocut = orc.csr()
+    assert prob.n == orc.n
+    assert np.array_equal(rp, orp)
+    assert np.array_equal(col, ocol)
+    assert np.array_equal(cut, ocut)
+    assert prob.max_family == orc.max_family
+    X, mat, bc = prob.nodes()
+    Xo, mato, bco = orc.nodes()
+    assert np.array_equal(X, Xo) and np.array_equal(mat, mato) and np.array_equal(bc, bco)
+    assert np.array_equal(prob.bdryeff(), orc.bdryeff())
+
+
+def test_bond_set_with_slits():
+    spec = make("bar_jitter")
+    spec["slits"] = [pm.mid_slit(spec, 30.0),
+                     dict(type=1, box=(0.0, 1.0, 0.004, 0.004, 0.0, 1.0))]
+    prob, orc = pair(spec)
+    rp, col, cut = prob.csr()
+    orp, ocol, ocut = orc.csr()
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+    assert np.array_equal(cut, ocut)
+    assert cut.sum() > 0
+    assert np.array_equal(prob.bdryeff(), orc.bdryeff())
+
+
+@pytest.mark.parametrize("case", ["bar_12x6x6", "bar_jitter", "bar_delta4", "bar_cci_original"])
+def test_forces_fp64(case):
+    spec = make(case)
+    prob, orc = pair(spec)
+    pyoracle.set_square_mode(True)
+    try:
+        rng = np.random.default_rng(7)
+        X, _, _ = prob.nodes()
+        for name, u in (("random", rng.uniform(-1e-6, 1e-6, size=(prob.n, 3))),
+                        ("affine", X @ np.array([[1.4e-5, 0, 0], [0, -3.5e-6, 0], [0, 2e-6, -3.5e-6]]).T)):
+            F, el = prob.forces(u, with_elast=True)
+            Fo, elo = orc.forces(u, with_elast=True)
+            err = np.abs(F - Fo).max() / np.abs(Fo).max()
+            print(f"[forces fp64] {case} {name} err={err:.3e}", flush=True)
+            assert err <= 1e-12, (name, err)
+            eerr = np.abs(el - elo).max() / np.abs(elo).max()
+            assert eerr <= 1e-12, (name, eerr)
+    finally:
+        pyoracle.set_square_mode(False)
+
+
+@pytest.mark.parametrize("case", ["bar_12x6x6", "bar_jitter"])
+def test_rk4_fp64(case):
+    spec = make(case, steps=40)
+    prob, orc = pair(spec, use_graph=0)
+    pyoracle.set_square_mode(True)
+    try:
+        dt, mb = prob.timestep()
+        assert abs(dt - orc.dt) <= 4e-16 * orc.dt
+        for chunk in (1, 3, 6):
+            prob.step(chunk)
+            orc.run(chunk)
+            U, V = prob.state()
+            Uo, Vo = orc.state()
+            eu = np.abs(U - Uo).max() / np.abs(Uo).max()
+            ev = np.abs(V - Vo).max() / np.abs(Vo).max()
+            t, step, _ = prob.clock()
+            print(f"[rk4 fp64] {case} step={step} eU={eu:.3e} eV={ev:.3e}", flush=True)
+            assert step == orc.steps_done and t == orc.t
+            assert eu <= 1e-11 and ev <= 1e-11
+        ke, ee = prob.energy()
+        assert abs(ke - orc.ke) <= 1e-11 * abs(orc.ke)
+        assert abs(ee - orc.ee) <= 1e-11 * abs(orc.ee)
+    finally:
+        pyoracle.set_square_mode(False)
+
+
+def test_rk4_graph_matches_eager():
+    spec = make("bar_jitter", steps=40)
+    a = pm.Problem(pm.deck_text(spec), use_graph=1, graph_chunk=5)
+    b = pm.Problem(pm.deck_text(spec), use_graph=0)
+    for _ in range(3):
+        a.step(5)
+        b.step(5)
+        Ua, Va = a.state()
+        Ub, Vb = b.state()
+        assert np.array_equal(Ua, Ub) and np.array_equal(Va, Vb)
+
+
+def test_rk4_fp32c_error():
+    spec = make("bar_jitter", steps=40)
+    p64 = pm.Problem(pm.deck_text(spec), precision=pm.PDM_FP64)
+    p32 = pm.Problem(pm.deck_text(spec), precision=pm.PDM_FP32C)
+    p64.step(40)
+    p32.step(40)
+    U64, _ = p64.state()
+    U32, _ = p32.state()
+    err = np.abs(U32 - U64).max() / np.abs(U64).max()
+    print(f"[rk4 fp32c] max rel displacement error vs fp64 after 40 steps: {err:.3e}", flush=True)
+    assert err <= 1e-6
