@@ -60,10 +60,26 @@ typedef struct {
     double seconds;         /* wall time of the solve */
 } pdm_static_stats;
 
+/* Parsed deck values (pdDataManager getters, pdDataManager.h:104-142). */
+typedef struct {
+    int grid_dimension[3];
+    double grid_spacing, horizon, grid_center[3];
+    double total_time;
+    int total_step, step_dump_frequency;
+    double safety_factor, fixed_time_step;   /* fixed_time_step < 0: stable dt */
+    int num_materials, num_bcs, num_material_spaces, num_deletion_spaces, num_boundary_spaces, num_slits;
+    double damping_coeff;
+    int bdryeff_on, cci_on, cci_modified, fgi_on, ai_on, mls_on, intg_point;
+    double support_domain_size, tolerance;
+    double jitter;
+    long long jitter_seed;
+    int traction_body;
+} pdm_problem_info;
+
 /* Fill *opt with defaults. */
 int pdm_options_default(pdm_options* opt);
 
-/* Parse a pds_input.txt deck (CRLF or LF) and create a handle.
+/* Parse a pds_input.txt deck (CRLF or LF) and create a handle; no GPU work.
  * Replaces pdDataManager::ReadInputFile (pdDataManager.cpp:4-758) with
  * ReadKeyword/FindKeyword (pdDataManager.h:270-421) and SetWkDir's input
  * check (pdDataManager3.cpp:1503-1541). */
@@ -72,13 +88,16 @@ int pdm_create(const char* input_path, const pdm_options* opt, pdm_handle** out)
 /* Same, from deck text in memory. */
 int pdm_create_from_text(const char* deck_text, const pdm_options* opt, pdm_handle** out);
 
+/* Parsed deck values of a created handle (before or after pdm_init). */
+int pdm_get_problem_info(const pdm_handle* h, pdm_problem_info* info);
+
 /* Release the handle and its device memory (pdDataManager::~pdDataManager, pdDataManager3.cpp:7-18). */
 int pdm_destroy(pdm_handle* h);
 
 /* Message of the last failed call on h (or of the last failed pdm_create if h is NULL). */
 const char* pdm_last_error(const pdm_handle* h);
 
-/* The InitialProblem sequence on the GPU (pdSolve.cpp:10-22): InitNode
+/* Select the device, then the InitialProblem sequence on the GPU (pdSolve.cpp:10-22): InitNode
  * (pdDataManager.cpp:813-1098), InitBond (pdDataManager2.cpp:12-70, cell-binned
  * GPU search, bond set identical to the reference's), InitMaterial
  * (pdDataManager.cpp:1297-1338), InitBdryEffectFactor (pdDataManager.cpp:1340-1473),

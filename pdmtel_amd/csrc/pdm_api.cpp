@@ -1,6 +1,7 @@
 // pdm_api.cpp -- the C ABI (include/pdm.h): handle lifecycle, InitialProblem on the GPU,
 // the RK4 driver with hipGraph-captured steps, state transfer and output writers.
 #include <algorithm>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -15,13 +16,13 @@ namespace {
 
 thread_local std::string g_create_err;
 
-#define HCHK(x)                                                                    \
-    do {                                                                           \
-        hipError_t e_ = (x);                                                       \
-        if (e_ != hipSuccess) {                                                    \
-            h->err = std::string("HIP error ") + hipGetErrorString(e_) + " at " #x; \
-            return PDM_E_HIP;                                                      \
-        }                                                                          \
+#define HCHK(x)                                                                                        \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess) {                                                                        \
+            const_cast<pdm_handle*>(h)->err = std::string("HIP error ") + hipGetErrorString(e_) + " at " #x; \
+            return PDM_E_HIP;                                                                          \
+        }                                                                                              \
     } while (0)
 
 int fail(const pdm_handle* h, int code, const std::string& m)
@@ -81,19 +82,6 @@ int create_common(const std::string& text, const pdm_options* opt, pdm_handle** 
             delete h;
             return PDM_E_MODEL;
         }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= h->opt.device) {
-        g_create_err = "no HIP device " + std::to_string(h->opt.device);
-        delete h;
-        return PDM_E_HIP;
-    }
-    h->dev = h->opt.device;
-    if (hipSetDevice(h->dev) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
-        g_create_err = "HIP stream/event creation failed";
-        delete h;
-        return PDM_E_HIP;
-    }
     *out = h;
     return PDM_OK;
 }
@@ -115,12 +103,9 @@ int require_init(const pdm_handle* h)
 StageArgs stage_args(pdm_handle* h)
 {
     StageArgs a{};
-    a.row_ptr = h->rp;
-    a.col = h->col;
-    a.coef64 = h->coef64;
-    a.coef32 = h->coef32;
-    a.mat = h->matd;
-    a.bc = h->bcd;
+    a.tv = tile_view(h);
+    a.mat = h->mati;
+    a.bc = h->bci;
     a.nb = h->nb;
     a.U = h->U;
     a.V = h->V;
@@ -130,30 +115,45 @@ StageArgs stage_args(pdm_handle* h)
     a.elast = h->elast;
     a.part = h->part;
     a.steps = h->steps;
-    a.n = h->n;
     a.energies = h->opt.energies ? 1 : 0;
-    a.xcd_remap = 1;
     fill_stage_tables(h, a);
     return a;
 }
 
-// Upload state U, V (3N AoS) and the gather records' displacement (rec[0].u = U).
+// Host AoS (reference order) <-> device d4 (internal order).
+void to_internal(const pdm_handle* h, const double* src, std::vector<d4>& dst)
+{
+    int64_t n = h->n;
+    dst.resize(n);
+    for (int64_t a = 0; a < n; ++a) {
+        int64_t i = h->perm[a];
+        dst[a] = d4{src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.0};
+    }
+}
+
+void from_internal(const pdm_handle* h, const std::vector<d4>& src, double* dst)
+{
+    int64_t n = h->n;
+    for (int64_t a = 0; a < n; ++a) {
+        int64_t i = h->perm[a];
+        dst[3 * i] = src[a].x;
+        dst[3 * i + 1] = src[a].y;
+        dst[3 * i + 2] = src[a].z;
+    }
+}
+
+// Upload state U, V (3N AoS, reference order); the next stage 1 gathers U.
 int upload_state(pdm_handle* h, const double* U, const double* V)
 {
     int64_t n = h->n;
-    std::vector<d4> buf(n);
+    std::vector<d4> buf;
     if (U) {
-        for (int64_t i = 0; i < n; ++i) buf[i] = d4{U[3 * i], U[3 * i + 1], U[3 * i + 2], 0.0};
+        to_internal(h, U, buf);
         HCHK(hipMemcpy(h->U, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
-        std::vector<NodeRec> rec(n);
-        for (int64_t i = 0; i < n; ++i) {
-            rec[i].x = d4{h->X[3 * i], h->X[3 * i + 1], h->X[3 * i + 2], h->deck.mats[h->mat[i]].ecrit};
-            rec[i].u = buf[i];
-        }
-        HCHK(hipMemcpy(h->rec[0], rec.data(), sizeof(NodeRec) * n, hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(h->u[0], buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
     }
     if (V) {
-        for (int64_t i = 0; i < n; ++i) buf[i] = d4{V[3 * i], V[3 * i + 1], V[3 * i + 2], 0.0};
+        to_internal(h, V, buf);
         HCHK(hipMemcpy(h->V, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
     }
     return PDM_OK;
@@ -179,14 +179,13 @@ int ensure_chunk(pdm_handle* h, int cap)
 int enqueue_steps(pdm_handle* h, int count)
 {
     StageArgs a = stage_args(h);
-    int64_t nblk = stage_blocks(h->n);
     for (int k = 0; k < count; ++k) {
         for (int st = 0; st < 4; ++st) {
-            a.rin = h->rec[st & 1];
-            a.rout = h->rec[(st + 1) & 1];
+            a.uin = h->u[st & 1];
+            a.uout = h->u[(st + 1) & 1];
             HCHK(launch_stage(st, h->opt.precision, a, k, h->stream));
         }
-        if (a.energies) HCHK(launch_energy_final(a, nblk, h->energy + 2 * k, h->stream));
+        if (a.energies) HCHK(launch_energy_final(h->part, h->ntl, h->part2, h->energy + 2 * k, h->stream));
     }
     return PDM_OK;
 }
@@ -298,15 +297,56 @@ int pdm_destroy(pdm_handle* h)
     (void)hipSetDevice(h->dev);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-    for (auto& r : h->rec) dfree(r);
-    dfree(h->Xd); dfree(h->matd); dfree(h->bcd); dfree(h->nb); dfree(h->col); dfree(h->rp); dfree(h->cut);
-    dfree(h->coef64); dfree(h->coef32); dfree(h->be);
+    dfree(h->Xd); dfree(h->rp); dfree(h->col); dfree(h->cut); dfree(h->be); dfree(h->matd);
+    dfree(h->permd); dfree(h->Xi); dfree(h->u[0]); dfree(h->u[1]); dfree(h->rpi); dfree(h->slot);
+    dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->mati); dfree(h->bci); dfree(h->nb);
+    dfree(h->tiles); dfree(h->tile_start); dfree(h->halo_ptr); dfree(h->halo);
     dfree(h->U); dfree(h->V); dfree(h->Vb); dfree(h->k1); dfree(h->k23);
-    dfree(h->elast); dfree(h->part); dfree(h->energy); dfree(h->steps); dfree(h->scratchF);
+    dfree(h->elast); dfree(h->part); dfree(h->part2); dfree(h->energy); dfree(h->steps);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
+    return PDM_OK;
+}
+
+int pdm_get_problem_info(const pdm_handle* h, pdm_problem_info* info)
+{
+    if (!h || !info) return PDM_E_ARG;
+    const Deck& d = h->deck;
+    std::memset(info, 0, sizeof(*info));
+    info->grid_dimension[0] = d.nx;
+    info->grid_dimension[1] = d.ny;
+    info->grid_dimension[2] = d.nz;
+    info->grid_spacing = d.dx;
+    info->horizon = d.horizon;
+    info->grid_center[0] = d.cx;
+    info->grid_center[1] = d.cy;
+    info->grid_center[2] = d.cz;
+    info->total_time = d.total_time;
+    info->total_step = d.total_step;
+    info->step_dump_frequency = d.dump;
+    info->safety_factor = d.safety;
+    info->fixed_time_step = d.fixed_dt_on ? d.fixed_dt : -1.0;
+    info->num_materials = (int)d.mats.size();
+    info->num_bcs = (int)d.bcs.size();
+    info->num_material_spaces = (int)d.mat_spaces.size();
+    info->num_deletion_spaces = (int)d.del_spaces.size();
+    info->num_boundary_spaces = (int)d.bc_spaces.size();
+    info->num_slits = (int)d.slits.size();
+    info->damping_coeff = d.damping;
+    info->bdryeff_on = d.bdryeff_on;
+    info->cci_on = d.cci_on;
+    info->cci_modified = d.cci_mdf;
+    info->fgi_on = d.fgi_on;
+    info->ai_on = d.ai_on;
+    info->mls_on = d.mls_on;
+    info->intg_point = d.intg_point;
+    info->support_domain_size = d.rw;
+    info->tolerance = d.tol;
+    info->jitter = d.jitter;
+    info->jitter_seed = (long long)d.seed;
+    info->traction_body = d.traction_body;
     return PDM_OK;
 }
 
@@ -320,7 +360,16 @@ int pdm_init(pdm_handle* h)
 {
     if (!h) return PDM_E_ARG;
     if (h->inited) return fail(h, PDM_E_STATE, "pdm_init called twice");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= h->opt.device || h->opt.device < 0)
+        return fail(h, PDM_E_HIP, "no HIP device " + std::to_string(h->opt.device));
+    h->dev = h->opt.device;
     HCHK(hipSetDevice(h->dev));
+    if (!h->stream) {
+        HCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        HCHK(hipEventCreate(&h->ev0));
+        HCHK(hipEventCreate(&h->ev1));
+    }
     int rc = build_nodes(h);
     if (rc) return rc;
     init_materials(h);
@@ -339,8 +388,14 @@ int pdm_init(pdm_handle* h)
     }
     rc = build_bonds(h);
     if (rc) return rc;
-    rc = build_bdryeff_and_coef(h);
+    rc = build_tiles(h);
     if (rc) return rc;
+    if (h->opt.verbose)
+        fprintf(stderr, "pdm_init: nodes=%lld entries=%lld tiles=%lld (edge %dx%dx%d dx) tile_max=%d halo_max=%d "
+                        "halo_mean=%.1f lds=%zu B\n",
+                (long long)h->n, (long long)h->E, (long long)h->ntl, h->tile_edge[0], h->tile_edge[1],
+                h->tile_edge[2], h->tile_max, h->halo_max, h->ntl ? (double)h->halo_total / (double)h->ntl : 0.0,
+                stage_lds_bytes(tile_view(h), !h->uniform_ecrit));
     int64_t n = std::max<int64_t>(h->n, 1);
     for (d4** p : {&h->U, &h->V, &h->Vb, &h->k1, &h->k23}) {
         HCHK(hipMalloc(p, sizeof(d4) * n));
@@ -348,9 +403,9 @@ int pdm_init(pdm_handle* h)
     }
     HCHK(hipMalloc(&h->elast, sizeof(double) * n));
     HCHK(hipMemsetAsync(h->elast, 0, sizeof(double) * n, h->stream));
-    h->part_blocks = stage_blocks(h->n);
-    HCHK(hipMalloc(&h->part, sizeof(double) * 2 * std::max<int64_t>(h->part_blocks, 1)));
-    HCHK(hipMemsetAsync(h->part, 0, sizeof(double) * 2 * std::max<int64_t>(h->part_blocks, 1), h->stream));
+    HCHK(hipMalloc(&h->part, sizeof(double) * 2 * std::max<int64_t>(h->ntl, 1)));
+    HCHK(hipMemsetAsync(h->part, 0, sizeof(double) * 2 * std::max<int64_t>(h->ntl, 1), h->stream));
+    HCHK(hipMalloc(&h->part2, sizeof(double) * 2 * energy_part2_size(h->ntl)));
     HCHK(hipStreamSynchronize(h->stream));
     // CalcStableTimeStep (pdDataManager2.cpp:6-10) and the driver's clock
     h->dt_stable = d.fixed_dt_on ? d.fixed_dt : d.safety * h->min_bond / h->max_c;
@@ -381,10 +436,9 @@ int pdm_get_csr(const pdm_handle* h, int64_t* row_ptr, int32_t* col, uint8_t* cu
 {
     int rc = require_init(h);
     if (rc) return rc;
-    pdm_handle* hh = const_cast<pdm_handle*>(h);
-    if (row_ptr) { pdm_handle* h = hh; HCHK(hipMemcpy(row_ptr, h->rp, sizeof(int64_t) * (h->n + 1), hipMemcpyDeviceToHost)); }
-    if (col && h->E) { pdm_handle* h = hh; HCHK(hipMemcpy(col, h->col, sizeof(int32_t) * h->E, hipMemcpyDeviceToHost)); }
-    if (cut && h->E) { pdm_handle* h = hh; HCHK(hipMemcpy(cut, h->cut, h->E, hipMemcpyDeviceToHost)); }
+    if (row_ptr) HCHK(hipMemcpy(row_ptr, h->rp, sizeof(int64_t) * (h->n + 1), hipMemcpyDeviceToHost));
+    if (col && h->E) HCHK(hipMemcpy(col, h->col, sizeof(int32_t) * h->E, hipMemcpyDeviceToHost));
+    if (cut && h->E) HCHK(hipMemcpy(cut, h->cut, h->E, hipMemcpyDeviceToHost));
     return PDM_OK;
 }
 
@@ -393,11 +447,7 @@ int pdm_get_bdryeff(const pdm_handle* h, double* be)
     int rc = require_init(h);
     if (rc) return rc;
     if (!be) return PDM_E_ARG;
-    pdm_handle* hh = const_cast<pdm_handle*>(h);
-    {
-        pdm_handle* h = hh;
-        HCHK(hipMemcpy(be, h->be, sizeof(double) * h->n, hipMemcpyDeviceToHost));
-    }
+    HCHK(hipMemcpy(be, h->be, sizeof(double) * h->n, hipMemcpyDeviceToHost));
     return PDM_OK;
 }
 
@@ -417,33 +467,24 @@ int pdm_compute_forces(pdm_handle* h, const double* u, double* F, double* elast)
     if (!u || !F) return fail(h, PDM_E_ARG, "null u or F");
     int64_t n = h->n;
     if (n == 0) return PDM_OK;
-    std::vector<NodeRec> rec(n);
-    for (int64_t i = 0; i < n; ++i) {
-        rec[i].x = d4{h->X[3 * i], h->X[3 * i + 1], h->X[3 * i + 2], h->deck.mats[h->mat[i]].ecrit};
-        rec[i].u = d4{u[3 * i], u[3 * i + 1], u[3 * i + 2], 0.0};
-    }
-    NodeRec* tmp = nullptr;
-    double *Fd = nullptr, *ed = nullptr;
-    HCHK(hipMalloc(&tmp, sizeof(NodeRec) * n));
-    HCHK(hipMalloc(&Fd, sizeof(double) * 3 * n));
+    std::vector<d4> buf;
+    to_internal(h, u, buf);
+    d4 *ud = nullptr, *Fd = nullptr;
+    double* ed = nullptr;
+    HCHK(hipMalloc(&ud, sizeof(d4) * n));
+    HCHK(hipMalloc(&Fd, sizeof(d4) * n));
     HCHK(hipMalloc(&ed, sizeof(double) * n));
-    HCHK(hipMemcpy(tmp, rec.data(), sizeof(NodeRec) * n, hipMemcpyHostToDevice));
-    ForceArgs a{};
-    a.rin = tmp;
-    a.row_ptr = h->rp;
-    a.col = h->col;
-    a.coef64 = h->coef64;
-    a.coef32 = h->coef32;
-    a.mat = h->matd;
-    a.F = Fd;
-    a.elast = ed;
-    a.n = n;
-    a.inv_vol_cube = 1.0 / (h->deck.dx * h->deck.dx * h->deck.dx);
-    HCHK(launch_forces(h->opt.precision, a, h->stream));
+    HCHK(hipMemcpy(ud, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
+    HCHK(launch_forces(h->opt.precision, tile_view(h), ud, Fd, ed, h->stream));
     HCHK(hipStreamSynchronize(h->stream));
-    HCHK(hipMemcpy(F, Fd, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
-    if (elast) HCHK(hipMemcpy(elast, ed, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HCHK(hipFree(tmp));
+    HCHK(hipMemcpy(buf.data(), Fd, sizeof(d4) * n, hipMemcpyDeviceToHost));
+    from_internal(h, buf, F);
+    if (elast) {
+        std::vector<double> e(n);
+        HCHK(hipMemcpy(e.data(), ed, sizeof(double) * n, hipMemcpyDeviceToHost));
+        for (int64_t a = 0; a < n; ++a) elast[h->perm[a]] = e[a];
+    }
+    HCHK(hipFree(ud));
     HCHK(hipFree(Fd));
     HCHK(hipFree(ed));
     return PDM_OK;
@@ -456,21 +497,16 @@ int pdm_apply_tangent(pdm_handle* h, const double* u, double* y)
     if (!u || !y) return fail(h, PDM_E_ARG, "null u or y");
     int64_t n = h->n;
     if (n == 0) return PDM_OK;
-    double *ud = nullptr, *yd = nullptr;
-    HCHK(hipMalloc(&ud, sizeof(double) * 3 * n));
-    HCHK(hipMalloc(&yd, sizeof(double) * 3 * n));
-    HCHK(hipMemcpy(ud, u, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-    ForceArgs a{};
-    a.rin = h->rec[0];
-    a.row_ptr = h->rp;
-    a.col = h->col;
-    a.coef64 = h->coef64;
-    a.coef32 = h->coef32;
-    a.mat = h->matd;
-    a.n = n;
-    HCHK(launch_tangent(h->opt.precision, a, ud, yd, h->stream));
+    std::vector<d4> buf;
+    to_internal(h, u, buf);
+    d4 *ud = nullptr, *yd = nullptr;
+    HCHK(hipMalloc(&ud, sizeof(d4) * n));
+    HCHK(hipMalloc(&yd, sizeof(d4) * n));
+    HCHK(hipMemcpy(ud, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
+    HCHK(launch_tangent(h->opt.precision, tile_view(h), ud, yd, h->stream));
     HCHK(hipStreamSynchronize(h->stream));
-    HCHK(hipMemcpy(y, yd, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    HCHK(hipMemcpy(buf.data(), yd, sizeof(d4) * n, hipMemcpyDeviceToHost));
+    from_internal(h, buf, y);
     HCHK(hipFree(ud));
     HCHK(hipFree(yd));
     return PDM_OK;
@@ -488,20 +524,16 @@ int pdm_get_state(const pdm_handle* h, double* U, double* V)
 {
     int rc = require_init(h);
     if (rc) return rc;
-    pdm_handle* hh = const_cast<pdm_handle*>(h);
     int64_t n = h->n;
     std::vector<d4> buf(n);
-    {
-        pdm_handle* h = hh;
-        HCHK(hipStreamSynchronize(h->stream));
-        if (U) {
-            HCHK(hipMemcpy(buf.data(), h->U, sizeof(d4) * n, hipMemcpyDeviceToHost));
-            for (int64_t i = 0; i < n; ++i) { U[3 * i] = buf[i].x; U[3 * i + 1] = buf[i].y; U[3 * i + 2] = buf[i].z; }
-        }
-        if (V) {
-            HCHK(hipMemcpy(buf.data(), h->V, sizeof(d4) * n, hipMemcpyDeviceToHost));
-            for (int64_t i = 0; i < n; ++i) { V[3 * i] = buf[i].x; V[3 * i + 1] = buf[i].y; V[3 * i + 2] = buf[i].z; }
-        }
+    HCHK(hipStreamSynchronize(h->stream));
+    if (U) {
+        HCHK(hipMemcpy(buf.data(), h->U, sizeof(d4) * n, hipMemcpyDeviceToHost));
+        from_internal(h, buf, U);
+    }
+    if (V) {
+        HCHK(hipMemcpy(buf.data(), h->V, sizeof(d4) * n, hipMemcpyDeviceToHost));
+        from_internal(h, buf, V);
     }
     return PDM_OK;
 }

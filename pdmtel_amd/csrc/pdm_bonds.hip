@@ -71,6 +71,8 @@ __global__ void k_scan_add(int64_t* __restrict__ out, const int64_t* __restrict_
     if (idx < n) out[idx] += offs[idx / kScanTile];
 }
 
+}  // namespace
+
 // out[i] = sum_{k<i} in[k] for i in [0, n]; out has n+1 entries.
 int scan_exclusive(pdm_handle* h, const int64_t* in, int64_t* out, int64_t n)
 {
@@ -100,6 +102,8 @@ int scan_exclusive(pdm_handle* h, const int64_t* in, int64_t* out, int64_t n)
     HIPCHK(hipFree(sums_scan));
     return PDM_OK;
 }
+
+namespace {
 
 __device__ __forceinline__ int64_t cell_of(const BondArgs& a, double x, int c)
 {
@@ -270,61 +274,6 @@ __global__ void k_bdryeff_energy(BdryArgs a)
     }
     double p_inf = 3.0 * a.E[m] * e11 * e11;
     a.be[i] = p_inf / sum;
-}
-
-struct CoefArgs {
-    const d4* X;
-    const int64_t* row_ptr;
-    const int32_t* col;
-    const uint8_t* cut;
-    const int32_t* mat;
-    const double* be;
-    double* coef64;
-    float* coef32;
-    int32_t* nb;
-    NodeRec* rec0;
-    NodeRec* rec1;
-    int64_t n;
-    double horizon, dx, vcube;
-    bool original_cci;
-    double spring[kMaxMat], ecrit[kMaxMat];
-};
-
-// coef = bdryEff_ij * dx^3 * beta * c_ij, the leading factors of CalcBondForceCCI's
-// force expression (pdDataManager2.cpp:1329-1331) in the same association order.
-__global__ void k_coef(CoefArgs a)
-{
-    int64_t i = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    int lane = threadIdx.x % 64;
-    if (i >= a.n) return;
-    d4 xi = a.X[i];
-    double bi = a.be[i];
-    double ci = a.spring[a.mat[i]];
-    int cnt = 0;
-    for (int64_t e = a.row_ptr[i] + lane; e < a.row_ptr[i + 1]; e += 64) {
-        int32_t j = a.col[e];
-        d4 xj = a.X[j];
-        double x1 = xj.x - xi.x, x2 = xj.y - xi.y, x3 = xj.z - xi.z;
-        double r = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
-        double cj = a.spring[a.mat[j]];
-        double spring = (cj < ci) ? cj : ci;
-        double bij = 0.5 * (bi + a.be[j]);
-        double beta = g_beta(r, a.horizon, a.dx, a.original_cci);
-        double c = a.cut[e] ? 0.0 : bij * a.vcube * beta * spring;
-        a.coef64[e] = c;
-        a.coef32[e] = (float)c;
-        cnt += a.cut[e] ? 0 : 1;
-    }
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    if (lane == 0) {
-        a.nb[i] = cnt;
-        NodeRec r;
-        r.x = xi;
-        r.x.w = a.ecrit[a.mat[i]];
-        r.u.x = r.u.y = r.u.z = r.u.w = 0.0;
-        a.rec0[i] = r;
-        a.rec1[i] = r;
-    }
 }
 
 }  // namespace
@@ -505,9 +454,7 @@ int build_bonds(pdm_handle* h)
     HIPCHK(hipMemcpyAsync(h->be, beh.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(branch_d, branch.data(), n, hipMemcpyHostToDevice, s));
     HIPCHK(hipMalloc(&h->matd, sizeof(int32_t) * std::max<int64_t>(n, 1)));
-    HIPCHK(hipMalloc(&h->bcd, sizeof(int32_t) * std::max<int64_t>(n, 1)));
     HIPCHK(hipMemcpyAsync(h->matd, h->mat.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(h->bcd, h->bc.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
     BdryArgs ba{};
     ba.X = h->Xd;
     ba.row_ptr = h->rp;
@@ -532,44 +479,6 @@ int build_bonds(pdm_handle* h)
     }
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipFree(branch_d));
-    return PDM_OK;
-}
-
-int build_bdryeff_and_coef(pdm_handle* h)
-{
-    const Deck& d = h->deck;
-    hipStream_t s = h->stream;
-    int64_t n = h->n, E = h->E;
-    HIPCHK(hipMalloc(&h->coef64, sizeof(double) * std::max<int64_t>(E, 1)));
-    HIPCHK(hipMalloc(&h->coef32, sizeof(float) * std::max<int64_t>(E, 1)));
-    HIPCHK(hipMalloc(&h->nb, sizeof(int32_t) * std::max<int64_t>(n, 1)));
-    for (int b = 0; b < 2; ++b) HIPCHK(hipMalloc(&h->rec[b], sizeof(NodeRec) * std::max<int64_t>(n, 1)));
-    CoefArgs ca{};
-    ca.X = h->Xd;
-    ca.row_ptr = h->rp;
-    ca.col = h->col;
-    ca.cut = h->cut;
-    ca.mat = h->matd;
-    ca.be = h->be;
-    ca.coef64 = h->coef64;
-    ca.coef32 = h->coef32;
-    ca.nb = h->nb;
-    ca.rec0 = h->rec[0];
-    ca.rec1 = h->rec[1];
-    ca.n = n;
-    ca.horizon = d.horizon;
-    ca.dx = d.dx;
-    ca.vcube = d.dx * d.dx * d.dx;
-    ca.original_cci = d.cci_on && !d.cci_mdf;
-    for (size_t m = 0; m < d.mats.size() && m < (size_t)kMaxMat; ++m) {
-        ca.spring[m] = h->spring[m];
-        ca.ecrit[m] = d.mats[m].ecrit;
-    }
-    if (n) {
-        k_coef<<<dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s>>>(ca);
-        HIPCHK(hipGetLastError());
-    }
-    HIPCHK(hipStreamSynchronize(s));
     return PDM_OK;
 }
 

@@ -1,15 +1,23 @@
 // pdm_internal.h -- handle layout, device records and kernel entry points.
 //
-// HBM layout (one specimen, N nodes, E directed family entries):
-//   rec[2][N]  NodeRec, 64 B: reference position X (+ ecrit of the node's
-//              material in .w) and the displacement every family gather reads
-//              this stage (ping-pong: stage s reads rec[s&1], writes rec[(s+1)&1]).
-//   row_ptr    int64 N+1, col int32 E (ascending within a row), cut uint8 E
-//   coef64     FP64 E: bdryEff_ij * dx^3 * beta(r_ij) * c_ij, 0 for cut bonds
-//   coef32     FP32 E: the same, rounded once (FP32-compensated path)
-//   U,V,Vb,k1,k23  d4 per node (FP64 state; .w unused)
-// The node order is the reference's lattice order (x fastest), so a family's
-// neighbours sit in a few contiguous id runs and gathers hit L2.
+// HBM layout of one specimen (N nodes, E directed family entries, T tiles):
+//
+//  Reference order (built once, kept for pdm_get_csr / bdryEff output):
+//    Xd d4[N], rp int64[N+1], col int32[E] (ascending per row), cut uint8[E], be FP64[N]
+//
+//  Internal (tile-major) order, read every RK4 stage:
+//    Nodes are binned into spatial tiles (edge >= the family reach along each
+//    axis is not required: the neighbourhood spans R tiles per axis) and renumbered
+//    tile by tile.  For tile t the union of its nodes' families (and the nodes
+//    themselves) is the halo list halo[halo_ptr[t] .. halo_ptr[t+1]), sorted by
+//    internal id; each family entry stores its neighbour's position in that list
+//    as a uint16 slot.  A workgroup stages the halo's X and u (FP64) in LDS once,
+//    then every family gather is an LDS read.
+//    Xi d4[N]      position, .w = ecrit of the node's material
+//    u[2] d4[N]    displacement gathered by the next stage (ping-pong)
+//    rpi int64[N+1], slot uint16[E], coef64 FP64[E] / coef32 FP32[E]
+//      coef = bdryEff_ij * dx^3 * beta(r_ij) * c_ij, 0 for a cut bond
+//    U, V, Vb, k1, k23 d4[N]  FP64 RK4 state; mat, bc, nb int32[N]
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,15 +34,13 @@ namespace pdm {
 constexpr int kMaxMat = 8;
 constexpr int kMaxBC = 16;
 constexpr int kMaxSlit = 8;
-constexpr int kMaxFamily = 1024;   // row-sort limit (families up to ~delta = 5.5 dx)
+constexpr int kMaxFamily = 1024;     // row-sort limit (families up to ~delta = 5.5 dx)
+constexpr int kMaxTileNodes = 512;   // nodes per tile
+constexpr int kMaxFlags = 8192;      // nodes in a tile's neighbourhood (halo build)
+constexpr int kMaxHalo = 2560;       // halo entries staged in LDS: 7 x 8 B x 2560 + 32 B x 512 < 160 KB
 
 struct alignas(32) d4 {
     double x, y, z, w;
-};
-
-struct alignas(64) NodeRec {
-    d4 x;   // position, .w = ecrit of the node's material
-    d4 u;   // displacement seen by family gathers this stage
 };
 
 struct MatDev {
@@ -62,14 +68,33 @@ struct StepParam {
     double t, dt;
 };
 
-// Arguments of the RK4 stage kernels.
-struct StageArgs {
-    const NodeRec* rin;
-    NodeRec* rout;
-    const int64_t* row_ptr;
-    const int32_t* col;
+// Tile-gather inputs shared by every tiled kernel.
+struct TileView {
+    const int32_t* tiles;      // launch index -> tile id
+    const int64_t* tile_start; // tile id -> first internal node (size ntiles + 1)
+    const int64_t* halo_ptr;   // launch index -> halo list start (size ntl + 1)
+    const int32_t* halo;       // internal ids
+    const uint16_t* self_slot; // internal node -> its slot in its own tile's halo
+    const int64_t* rpi;
+    const uint16_t* slot;
     const double* coef64;
     const float* coef32;
+    const d4* X;
+    int64_t n;
+    int64_t ntl;               // launched (non-empty) tiles
+    int halo_max;              // largest halo (sizes the LDS staging area)
+    int tile_max;              // most nodes in a tile (sizes the LDS result area)
+    int xcd_remap;
+    int uniform_ecrit;         // all materials share ecrit: no per-neighbour ecrit in LDS
+    double ecrit0;             // that common ecrit
+    double inv_vol_cube;       // 1 / (dx*dx*dx)
+};
+
+// Arguments of the RK4 stage kernels.
+struct StageArgs {
+    TileView tv;
+    const d4* uin;
+    d4* uout;
     const int32_t* mat;
     const int32_t* bc;
     const int32_t* nb;      // unbroken family entries per node
@@ -79,32 +104,14 @@ struct StageArgs {
     d4* k1;
     d4* k23;
     double* elast;          // per-node elastic energy density, summed over the 4 stages
-    double* part;           // per-block partial KE / EE (stage 4)
+    double* part;           // per-tile partial KE / EE (stage 4)
     const StepParam* steps; // per-step t, dt
-    int64_t n;
     double damping;
     double vol;             // pow(dx, 3): node volume for energies
-    double inv_vol_cube;    // 1 / (dx*dx*dx): coef -> bdryEff * beta * c for energies
     int nmat, nbc;
     int energies;
-    int xcd_remap;
     MatDev mats[kMaxMat];
     BCDev bcs[kMaxBC];
-};
-
-// Arguments of the standalone force evaluation (pdm_compute_forces / tangent).
-struct ForceArgs {
-    const NodeRec* rin;
-    const int64_t* row_ptr;
-    const int32_t* col;
-    const double* coef64;
-    const float* coef32;
-    const int32_t* mat;
-    double* F;      // 3N AoS output
-    double* elast;  // N or null
-    int64_t n;
-    double inv_vol_cube;
-    MatDev mats[kMaxMat];
 };
 
 // Bond-build parameters.
@@ -130,31 +137,49 @@ struct pdm_handle {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    // host copies
+    // host copies (reference order)
     int64_t n = 0, E = 0;
     std::vector<double> X;          // 3N AoS
     std::vector<int32_t> mat, bc;
+    std::vector<int32_t> perm;      // internal -> reference id
     std::vector<double> spring, sound;
     double min_bond = 0, max_c = 0, dt_stable = 0;
     int max_family = 0;
     double trac_density[pdm::kMaxBC][3] = {};
 
-    // device
-    pdm::NodeRec* rec[2] = {nullptr, nullptr};
+    // device, reference order
     pdm::d4* Xd = nullptr;
-    int32_t *matd = nullptr, *bcd = nullptr, *nb = nullptr, *col = nullptr;
     int64_t* rp = nullptr;
+    int32_t* col = nullptr;
     uint8_t* cut = nullptr;
+    double* be = nullptr;
+    int32_t* matd = nullptr;
+
+    // device, internal (tile-major) order
+    int32_t* permd = nullptr;
+    pdm::d4* Xi = nullptr;
+    pdm::d4* u[2] = {nullptr, nullptr};
+    int64_t* rpi = nullptr;
+    uint16_t* slot = nullptr;
+    uint16_t* self_slot = nullptr;
     double* coef64 = nullptr;
     float* coef32 = nullptr;
-    double* be = nullptr;
+    int32_t *mati = nullptr, *bci = nullptr, *nb = nullptr;
+    int32_t* tiles = nullptr;
+    int64_t* tile_start = nullptr;
+    int64_t* halo_ptr = nullptr;
+    int32_t* halo = nullptr;
+    int64_t ntiles = 0, ntl = 0, halo_total = 0;
+    int halo_max = 0, tile_max = 0;
+    int tile_edge[3] = {4, 4, 4};   // tile edge in grid spacings
+    int uniform_ecrit = 1;
+
     pdm::d4 *U = nullptr, *V = nullptr, *Vb = nullptr, *k1 = nullptr, *k23 = nullptr;
     double* elast = nullptr;
-    double* part = nullptr;        // per-block partial energies
+    double* part = nullptr;        // per-tile partial energies
+    double* part2 = nullptr;       // second-level partials
     double* energy = nullptr;      // per-step KE, EE (chunk)
     pdm::StepParam* steps = nullptr;
-    double* scratchF = nullptr;    // 3N FP64 for standalone forces
-    int64_t part_blocks = 0;
     int chunk_cap = 0;
 
     // dynamic clock (mirrors pdSolve::SolveDynamic's loop variables)
@@ -167,7 +192,6 @@ struct pdm_handle {
     // graph cache
     hipGraphExec_t graph_exec = nullptr;
     int graph_steps = 0;
-    int graph_rec_parity = -1;
 };
 
 namespace pdm {
@@ -181,15 +205,20 @@ double time_factor(const BoundaryCondition& b, double t);
 bool is_bc_node(const Deck& d, int bcid);
 void fill_stage_tables(const pdm_handle* h, StageArgs& a);
 
-// device setup (pdm_bonds.hip)
+// reference-order family search and boundary-effect factors (pdm_bonds.hip)
 int build_bonds(pdm_handle* h);
-int build_bdryeff_and_coef(pdm_handle* h);
+int scan_exclusive(pdm_handle* h, const int64_t* in, int64_t* out, int64_t n);
+
+// tiling, internal CSR, halos, coefficients (pdm_tiles.hip)
+int build_tiles(pdm_handle* h);
+TileView tile_view(const pdm_handle* h);
 
 // kernels (pdm_kernels.hip)
 hipError_t launch_stage(int stage, int precision, const StageArgs& a, int step_idx, hipStream_t s);
-hipError_t launch_energy_final(const StageArgs& a, int64_t nblocks, double* out2, hipStream_t s);
-hipError_t launch_forces(int precision, const ForceArgs& a, hipStream_t s);
-hipError_t launch_tangent(int precision, const ForceArgs& a, const double* u3, double* y3, hipStream_t s);
-int64_t stage_blocks(int64_t n);
+hipError_t launch_energy_final(const double* part, int64_t nparts, double* part2, double* out2, hipStream_t s);
+hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, double* elast, hipStream_t s);
+hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s);
+size_t stage_lds_bytes(const TileView& tv, bool with_ecrit);
+int64_t energy_part2_size(int64_t nparts);
 
 }  // namespace pdm

@@ -202,7 +202,6 @@ void fill_stage_tables(const pdm_handle* h, StageArgs& a)
     }
     a.damping = d.damping;
     a.vol = std::pow(d.dx, 3.0);
-    a.inv_vol_cube = 1.0 / (d.dx * d.dx * d.dx);
 }
 
 }  // namespace pdm

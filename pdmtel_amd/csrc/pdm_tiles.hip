@@ -1,0 +1,515 @@
+// pdm_tiles.hip -- spatial tiling of the family CSR for LDS-staged gathers.
+//
+// Nodes are binned into tiles of tile_edge x dx along each axis and renumbered
+// tile by tile (within a tile: reference id order).  Each row of the internal
+// CSR is the reference family mapped to internal ids and sorted.  For every
+// tile a halo list holds the internal ids of the tile's nodes and all their
+// neighbours, sorted; a family entry stores the uint16 position of its
+// neighbour in that list.  The halo of a tile lies in the (2R+1)^3 block of
+// tiles around it, R = ceil(reach / edge) per axis, reach = delta + dx/2 plus
+// a margin: an accepted pair has dmin < delta + 1e-4 dx, and dmin is at least
+// the axis distance minus half a cell minus the face tolerance.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pdm_geom.h"
+#include "pdm_internal.h"
+
+namespace pdm {
+namespace {
+
+#define HIPCHK(x)                                                                  \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                    \
+            h->err = std::string("HIP error ") + hipGetErrorString(e_) + " at " #x; \
+            return PDM_E_HIP;                                                      \
+        }                                                                          \
+    } while (0)
+
+constexpr int kMaxNbr = 343;  // (2R+1)^3 for R <= 3
+
+struct TileGrid {
+    int64_t nt[3];
+    int R[3];
+};
+
+// Internal node arrays from the reference-order ones.
+__global__ void k_permute_nodes(const int32_t* __restrict__ perm, const d4* __restrict__ Xd,
+                                const int32_t* __restrict__ mat, const int32_t* __restrict__ bc_ref,
+                                const double* __restrict__ be_ref, const double* __restrict__ ecrit_of_mat, int64_t n,
+                                d4* __restrict__ Xi, int32_t* __restrict__ mati, int32_t* __restrict__ bci,
+                                double* __restrict__ bei)
+{
+    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    int32_t i = perm[a];
+    d4 x = Xd[i];
+    int m = mat[i];
+    x.w = ecrit_of_mat[m];
+    Xi[a] = x;
+    mati[a] = m;
+    bci[a] = bc_ref[i];
+    bei[a] = be_ref[i];
+}
+
+// packed[e'] = internal col << 1 | cut, rows in internal order.
+__global__ void k_rows_internal(const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm,
+                                const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                                const uint8_t* __restrict__ cut, const int64_t* __restrict__ rpi, int64_t n,
+                                uint32_t* __restrict__ packed)
+{
+    int64_t a = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    int lane = threadIdx.x % 64;
+    if (a >= n) return;
+    int32_t i = perm[a];
+    int64_t b = rp[i], len = rp[i + 1] - b, o = rpi[a];
+    for (int64_t k = lane; k < len; k += 64)
+        packed[o + k] = ((uint32_t)iperm[col[b + k]] << 1) | (uint32_t)cut[b + k];
+}
+
+__global__ void k_sort_rows_u32(const int64_t* __restrict__ rpi, uint32_t* __restrict__ v, int64_t n)
+{
+    __shared__ uint32_t sh[kMaxFamily];
+    int64_t a = blockIdx.x;
+    if (a >= n) return;
+    int64_t b = rpi[a];
+    int len = (int)(rpi[a + 1] - b);
+    int P = 1;
+    while (P < len) P <<= 1;
+    for (int t = threadIdx.x; t < P; t += blockDim.x) sh[t] = (t < len) ? v[b + t] : 0xFFFFFFFFu;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int jj = k >> 1; jj > 0; jj >>= 1) {
+            for (int t = threadIdx.x; t < P; t += blockDim.x) {
+                int p = t ^ jj;
+                if (p > t) {
+                    bool up = ((t & k) == 0);
+                    uint32_t va = sh[t], vb = sh[p];
+                    if ((va > vb) == up) {
+                        sh[t] = vb;
+                        sh[p] = va;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int t = threadIdx.x; t < len; t += blockDim.x) v[b + t] = sh[t];
+}
+
+struct HaloArgs {
+    const int32_t* tiles;
+    const int64_t* tile_start;
+    const int32_t* tileid;   // internal node -> tile id
+    const int64_t* rpi;
+    const uint32_t* packed;
+    int64_t ntl;
+    TileGrid g;
+    // pass 1
+    int64_t* halo_len;
+    // pass 2
+    const int64_t* halo_ptr;
+    int32_t* halo;
+    uint16_t* slot;
+    uint16_t* self_slot;
+    int* error;
+};
+
+// One workgroup per tile.  WRITE=false: halo length; WRITE=true: halo list and slots.
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_halo(HaloArgs a)
+{
+    __shared__ int32_t nb_tile[kMaxNbr];
+    __shared__ int32_t nb_off[kMaxNbr + 1];
+    __shared__ int16_t grid_to_nb[kMaxNbr];
+    __shared__ uint8_t flag[kMaxFlags];
+    __shared__ uint16_t pre[kMaxFlags];
+    __shared__ int32_t tsum[256];
+    __shared__ int bad;
+    const int64_t k = blockIdx.x;
+    const int32_t tid = a.tiles[k];
+    const int64_t ntx = a.g.nt[0], nty = a.g.nt[1], ntz = a.g.nt[2];
+    const int64_t tx = tid % ntx, ty = (tid / ntx) % nty, tz = tid / (ntx * nty);
+    const int Rx = a.g.R[0], Ry = a.g.R[1], Rz = a.g.R[2];
+    const int gx = 2 * Rx + 1, gy = 2 * Ry + 1, gz = 2 * Rz + 1;
+    if (threadIdx.x == 0) {
+        bad = 0;
+        int cnt = 0;
+        int32_t off = 0;
+        for (int dz = -Rz; dz <= Rz; ++dz)
+            for (int dy = -Ry; dy <= Ry; ++dy)
+                for (int dx = -Rx; dx <= Rx; ++dx) {
+                    int g = (dx + Rx) + gx * ((dy + Ry) + gy * (dz + Rz));
+                    int64_t x = tx + dx, y = ty + dy, z = tz + dz;
+                    if (x < 0 || y < 0 || z < 0 || x >= ntx || y >= nty || z >= ntz) {
+                        grid_to_nb[g] = -1;
+                        continue;
+                    }
+                    int32_t t2 = (int32_t)(x + ntx * (y + nty * z));
+                    grid_to_nb[g] = (int16_t)cnt;
+                    nb_tile[cnt] = t2;
+                    nb_off[cnt] = off;
+                    off += (int32_t)(a.tile_start[t2 + 1] - a.tile_start[t2]);
+                    ++cnt;
+                }
+        nb_off[cnt] = off;
+        if (off > kMaxFlags) bad = 1;
+        tsum[0] = cnt;
+    }
+    __syncthreads();
+    if (bad) {
+        if (threadIdx.x == 0) atomicExch(a.error, 1);
+        return;
+    }
+    const int nnb = tsum[0];
+    const int F = nb_off[nnb];
+    __syncthreads();
+    for (int f = threadIdx.x; f < F; f += blockDim.x) flag[f] = 0;
+    __syncthreads();
+    const int64_t a0 = a.tile_start[tid], a1 = a.tile_start[tid + 1];
+    const int own = grid_to_nb[Rx + gx * (Ry + gy * Rz)];
+    for (int64_t q = a0 + threadIdx.x; q < a1; q += blockDim.x) flag[nb_off[own] + (int)(q - a0)] = 1;
+    const int64_t e0 = a.rpi[a0], e1 = a.rpi[a1];
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        int32_t c = (int32_t)(a.packed[e] >> 1);
+        int32_t tj = a.tileid[c];
+        int64_t jx = tj % ntx, jy = (tj / ntx) % nty, jz = tj / (ntx * nty);
+        int64_t dx = jx - tx, dy = jy - ty, dz = jz - tz;
+        if (dx < -Rx || dx > Rx || dy < -Ry || dy > Ry || dz < -Rz || dz > Rz) {
+            atomicExch(a.error, 2);
+            continue;
+        }
+        int g = (int)((dx + Rx) + gx * ((dy + Ry) + gy * (dz + Rz)));
+        int nbi = grid_to_nb[g];
+        flag[nb_off[nbi] + (int)(c - a.tile_start[tj])] = 1;
+    }
+    __syncthreads();
+    // block exclusive scan of the flags
+    const int per = (F + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int f0 = threadIdx.x * per;
+    int loc = 0;
+    for (int f = f0; f < f0 + per && f < F; ++f) loc += flag[f];
+    tsum[threadIdx.x] = loc;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+        int add = (threadIdx.x >= (unsigned)off) ? tsum[threadIdx.x - off] : 0;
+        __syncthreads();
+        tsum[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int run = tsum[threadIdx.x] - loc;
+    for (int f = f0; f < f0 + per && f < F; ++f) {
+        pre[f] = (uint16_t)run;
+        run += flag[f];
+    }
+    const int H = tsum[blockDim.x - 1];
+    __syncthreads();
+    if (!WRITE) {
+        if (threadIdx.x == 0) a.halo_len[k] = H;
+        return;
+    }
+    const int64_t hp = a.halo_ptr[k];
+    for (int nbi = 0; nbi < nnb; ++nbi) {
+        int64_t s0 = a.tile_start[nb_tile[nbi]];
+        int cnt = nb_off[nbi + 1] - nb_off[nbi];
+        for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+            int f = nb_off[nbi] + q;
+            if (flag[f]) a.halo[hp + pre[f]] = (int32_t)(s0 + q);
+        }
+    }
+    for (int64_t q = a0 + threadIdx.x; q < a1; q += blockDim.x) a.self_slot[q] = pre[nb_off[own] + (int)(q - a0)];
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        int32_t c = (int32_t)(a.packed[e] >> 1);
+        int32_t tj = a.tileid[c];
+        int64_t jx = tj % ntx, jy = (tj / ntx) % nty, jz = tj / (ntx * nty);
+        int g = (int)((jx - tx + Rx) + gx * ((jy - ty + Ry) + gy * (jz - tz + Rz)));
+        if (g < 0 || g >= gx * gy * gz) continue;
+        int nbi = grid_to_nb[g];
+        a.slot[e] = pre[nb_off[nbi] + (int)(c - a.tile_start[tj])];
+    }
+}
+
+struct CoefArgs {
+    const d4* Xi;
+    const int64_t* rpi;
+    const uint32_t* packed;
+    const int32_t* mati;
+    const double* bei;
+    double* coef64;
+    float* coef32;
+    int32_t* nb;
+    int64_t n;
+    double horizon, dx, vcube;
+    bool original_cci;
+    double spring[kMaxMat];
+};
+
+// coef = bdryEff_ij * dx^3 * beta * c_ij: the leading factors of CalcBondForceCCI's force
+// expression (pdDataManager2.cpp:1329-1331) in the same association order.
+__global__ void k_coef(CoefArgs a)
+{
+    int64_t i = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    int lane = threadIdx.x % 64;
+    if (i >= a.n) return;
+    d4 xi = a.Xi[i];
+    double bi = a.bei[i];
+    double ci = a.spring[a.mati[i]];
+    int cnt = 0;
+    for (int64_t e = a.rpi[i] + lane; e < a.rpi[i + 1]; e += 64) {
+        uint32_t pk = a.packed[e];
+        int32_t j = (int32_t)(pk >> 1);
+        bool cut = (pk & 1u) != 0;
+        d4 xj = a.Xi[j];
+        double x1 = xj.x - xi.x, x2 = xj.y - xi.y, x3 = xj.z - xi.z;
+        double r = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+        double cj = a.spring[a.mati[j]];
+        double spring = (cj < ci) ? cj : ci;
+        double bij = 0.5 * (bi + a.bei[j]);
+        double beta = g_beta(r, a.horizon, a.dx, a.original_cci);
+        double c = cut ? 0.0 : bij * a.vcube * beta * spring;
+        a.coef64[e] = c;
+        a.coef32[e] = (float)c;
+        cnt += cut ? 0 : 1;
+    }
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    if (lane == 0) a.nb[i] = cnt;
+}
+
+}  // namespace
+
+TileView tile_view(const pdm_handle* h)
+{
+    TileView v{};
+    v.tiles = h->tiles;
+    v.tile_start = h->tile_start;
+    v.halo_ptr = h->halo_ptr;
+    v.halo = h->halo;
+    v.self_slot = h->self_slot;
+    v.rpi = h->rpi;
+    v.slot = h->slot;
+    v.coef64 = h->coef64;
+    v.coef32 = h->coef32;
+    v.X = h->Xi;
+    v.n = h->n;
+    v.ntl = h->ntl;
+    v.halo_max = h->halo_max;
+    v.tile_max = h->tile_max;
+    v.xcd_remap = 1;
+    v.uniform_ecrit = h->uniform_ecrit;
+    v.ecrit0 = h->deck.mats.empty() ? 0.0 : h->deck.mats[0].ecrit;
+    v.inv_vol_cube = 1.0 / (h->deck.dx * h->deck.dx * h->deck.dx);
+    return v;
+}
+
+int build_tiles(pdm_handle* h)
+{
+    const Deck& d = h->deck;
+    hipStream_t s = h->stream;
+    const int64_t n = h->n, E = h->E;
+    // tile grid over the node bounding box; bins start half a spacing below the lowest node
+    double lo[3], hi[3];
+    for (int c = 0; c < 3; ++c) { lo[c] = 1e300; hi[c] = -1e300; }
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], h->X[3 * i + c]);
+            hi[c] = std::max(hi[c], h->X[3 * i + c]);
+        }
+    TileGrid g{};
+    double edge[3];
+    const double reach = d.horizon + 0.5 * d.dx + 0.01 * d.dx;
+    for (int c = 0; c < 3; ++c) {
+        if (n == 0) { lo[c] = hi[c] = 0.0; }
+        lo[c] -= 0.5 * d.dx;
+        edge[c] = h->tile_edge[c] * d.dx;
+        g.nt[c] = (int64_t)std::floor((hi[c] - lo[c]) / edge[c]) + 1;
+        g.R[c] = (int)std::ceil(reach / edge[c]);
+        if (g.R[c] > 3) {
+            h->err = "tile edge too small for the horizon";
+            return PDM_E_LIMIT;
+        }
+    }
+    const int64_t ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+    if ((double)g.nt[0] * (double)g.nt[1] * (double)g.nt[2] > 2.0e9) {
+        h->err = "tile grid too large";
+        return PDM_E_LIMIT;
+    }
+    // counting sort of nodes by tile (stable: reference id order within a tile)
+    std::vector<int32_t> tid_ref(n);
+    std::vector<int64_t> tstart(ntiles + 1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t q[3];
+        for (int c = 0; c < 3; ++c) {
+            q[c] = (int64_t)std::floor((h->X[3 * i + c] - lo[c]) / edge[c]);
+            q[c] = std::min(std::max<int64_t>(q[c], 0), g.nt[c] - 1);
+        }
+        tid_ref[i] = (int32_t)(q[0] + g.nt[0] * (q[1] + g.nt[1] * q[2]));
+        tstart[tid_ref[i] + 1]++;
+    }
+    for (int64_t t = 0; t < ntiles; ++t) tstart[t + 1] += tstart[t];
+    std::vector<int64_t> cursor(tstart.begin(), tstart.end() - 1);
+    h->perm.assign(n, 0);
+    std::vector<int32_t> iperm(n), tileid(n);
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t a = cursor[tid_ref[i]]++;
+        h->perm[a] = (int32_t)i;
+        iperm[i] = (int32_t)a;
+        tileid[a] = tid_ref[i];
+    }
+    std::vector<int32_t> tiles;
+    int tmax = 0;
+    for (int64_t t = 0; t < ntiles; ++t) {
+        int64_t cnt = tstart[t + 1] - tstart[t];
+        if (cnt > 0) tiles.push_back((int32_t)t);
+        tmax = std::max<int>(tmax, (int)cnt);
+    }
+    if (tmax > kMaxTileNodes) {
+        h->err = "a tile holds more than " + std::to_string(kMaxTileNodes) + " nodes";
+        return PDM_E_LIMIT;
+    }
+    h->ntiles = ntiles;
+    h->ntl = (int64_t)tiles.size();
+    h->tile_max = tmax;
+    // internal row pointers
+    std::vector<int64_t> rph(n + 1), rpih(n + 1, 0);
+    HIPCHK(hipMemcpy(rph.data(), h->rp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+    for (int64_t a = 0; a < n; ++a) rpih[a + 1] = rpih[a] + (rph[h->perm[a] + 1] - rph[h->perm[a]]);
+    // uniform ecrit across materials?
+    h->uniform_ecrit = 1;
+    for (size_t m = 1; m < d.mats.size(); ++m)
+        if (d.mats[m].ecrit != d.mats[0].ecrit) h->uniform_ecrit = 0;
+
+    int32_t *ipermd = nullptr, *tileidd = nullptr, *bcref = nullptr;
+    double *ecm = nullptr, *bei = nullptr;
+    uint32_t* packed = nullptr;
+    int64_t* halo_len = nullptr;
+    int* err = nullptr;
+    const int64_t n1 = std::max<int64_t>(n, 1), E1 = std::max<int64_t>(E, 1);
+    HIPCHK(hipMalloc(&h->permd, sizeof(int32_t) * n1));
+    HIPCHK(hipMalloc(&ipermd, sizeof(int32_t) * n1));
+    HIPCHK(hipMalloc(&tileidd, sizeof(int32_t) * n1));
+    HIPCHK(hipMalloc(&bcref, sizeof(int32_t) * n1));
+    HIPCHK(hipMalloc(&ecm, sizeof(double) * kMaxMat));
+    HIPCHK(hipMalloc(&bei, sizeof(double) * n1));
+    HIPCHK(hipMalloc(&h->Xi, sizeof(d4) * n1));
+    HIPCHK(hipMalloc(&h->mati, sizeof(int32_t) * n1));
+    HIPCHK(hipMalloc(&h->bci, sizeof(int32_t) * n1));
+    HIPCHK(hipMalloc(&h->nb, sizeof(int32_t) * n1));
+    HIPCHK(hipMalloc(&h->rpi, sizeof(int64_t) * (n + 1)));
+    HIPCHK(hipMalloc(&h->tile_start, sizeof(int64_t) * (ntiles + 1)));
+    HIPCHK(hipMalloc(&h->tiles, sizeof(int32_t) * std::max<int64_t>(h->ntl, 1)));
+    HIPCHK(hipMalloc(&h->halo_ptr, sizeof(int64_t) * (h->ntl + 1)));
+    HIPCHK(hipMalloc(&halo_len, sizeof(int64_t) * std::max<int64_t>(h->ntl, 1)));
+    HIPCHK(hipMalloc(&h->self_slot, sizeof(uint16_t) * n1));
+    HIPCHK(hipMalloc(&packed, sizeof(uint32_t) * E1));
+    HIPCHK(hipMalloc(&h->slot, sizeof(uint16_t) * E1));
+    HIPCHK(hipMalloc(&h->coef64, sizeof(double) * E1));
+    HIPCHK(hipMalloc(&h->coef32, sizeof(float) * E1));
+    HIPCHK(hipMalloc(&err, sizeof(int)));
+    double ech[kMaxMat] = {0};
+    for (size_t m = 0; m < d.mats.size(); ++m) ech[m] = d.mats[m].ecrit;
+    HIPCHK(hipMemcpyAsync(h->permd, h->perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ipermd, iperm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(tileidd, tileid.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(bcref, h->bc.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(ecm, ech, sizeof(ech), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(h->rpi, rpih.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(h->tile_start, tstart.data(), sizeof(int64_t) * (ntiles + 1), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(h->tiles, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(err, 0, sizeof(int), s));
+    if (n) {
+        k_permute_nodes<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(
+            h->permd, h->Xd, h->matd, bcref, h->be, ecm, n, h->Xi, h->mati, h->bci, bei);
+        HIPCHK(hipGetLastError());
+        unsigned nbw = (unsigned)((n + 3) / 4);
+        k_rows_internal<<<dim3(nbw), dim3(256), 0, s>>>(h->permd, ipermd, h->rp, h->col, h->cut, h->rpi, n, packed);
+        HIPCHK(hipGetLastError());
+        k_sort_rows_u32<<<dim3((unsigned)n), dim3(256), 0, s>>>(h->rpi, packed, n);
+        HIPCHK(hipGetLastError());
+    }
+    HaloArgs ha{};
+    ha.tiles = h->tiles;
+    ha.tile_start = h->tile_start;
+    ha.tileid = tileidd;
+    ha.rpi = h->rpi;
+    ha.packed = packed;
+    ha.ntl = h->ntl;
+    ha.g = g;
+    ha.halo_len = halo_len;
+    ha.error = err;
+    if (h->ntl) {
+        k_halo<false><<<dim3((unsigned)h->ntl), dim3(256), 0, s>>>(ha);
+        HIPCHK(hipGetLastError());
+    }
+    int rc = scan_exclusive(h, halo_len, h->halo_ptr, h->ntl);
+    if (rc) return rc;
+    int herr = 0;
+    HIPCHK(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) {
+        h->err = herr == 1 ? "a tile neighbourhood holds more than " + std::to_string(kMaxFlags) + " nodes"
+                           : "a family entry lies outside its tile neighbourhood";
+        return PDM_E_LIMIT;
+    }
+    std::vector<int64_t> hp(h->ntl + 1);
+    HIPCHK(hipMemcpy(hp.data(), h->halo_ptr, sizeof(int64_t) * (h->ntl + 1), hipMemcpyDeviceToHost));
+    h->halo_total = hp[h->ntl];
+    int64_t hmax = 0;
+    for (int64_t k = 0; k < h->ntl; ++k) hmax = std::max(hmax, hp[k + 1] - hp[k]);
+    if (hmax > kMaxHalo) {
+        h->err = "a tile halo holds " + std::to_string(hmax) + " nodes, more than the LDS limit " +
+                 std::to_string(kMaxHalo);
+        return PDM_E_LIMIT;
+    }
+    h->halo_max = (int)hmax;
+    HIPCHK(hipMalloc(&h->halo, sizeof(int32_t) * std::max<int64_t>(h->halo_total, 1)));
+    ha.halo_ptr = h->halo_ptr;
+    ha.halo = h->halo;
+    ha.slot = h->slot;
+    ha.self_slot = h->self_slot;
+    if (h->ntl) {
+        k_halo<true><<<dim3((unsigned)h->ntl), dim3(256), 0, s>>>(ha);
+        HIPCHK(hipGetLastError());
+    }
+    CoefArgs ca{};
+    ca.Xi = h->Xi;
+    ca.rpi = h->rpi;
+    ca.packed = packed;
+    ca.mati = h->mati;
+    ca.bei = bei;
+    ca.coef64 = h->coef64;
+    ca.coef32 = h->coef32;
+    ca.nb = h->nb;
+    ca.n = n;
+    ca.horizon = d.horizon;
+    ca.dx = d.dx;
+    ca.vcube = d.dx * d.dx * d.dx;
+    ca.original_cci = d.cci_on && !d.cci_mdf;
+    for (size_t m = 0; m < d.mats.size() && m < (size_t)kMaxMat; ++m) ca.spring[m] = h->spring[m];
+    if (n) {
+        k_coef<<<dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s>>>(ca);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) {
+        h->err = "a family entry lies outside its tile neighbourhood";
+        return PDM_E_LIMIT;
+    }
+    for (int b = 0; b < 2; ++b) {
+        HIPCHK(hipMalloc(&h->u[b], sizeof(d4) * n1));
+        HIPCHK(hipMemsetAsync(h->u[b], 0, sizeof(d4) * n1, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipFree(ipermd));
+    HIPCHK(hipFree(tileidd));
+    HIPCHK(hipFree(bcref));
+    HIPCHK(hipFree(ecm));
+    HIPCHK(hipFree(bei));
+    HIPCHK(hipFree(packed));
+    HIPCHK(hipFree(halo_len));
+    HIPCHK(hipFree(err));
+    return PDM_OK;
+}
+
+}  // namespace pdm
