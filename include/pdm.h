@@ -39,7 +39,9 @@ extern "C" {
 
 /* precision of the bond-force gather and of CG matvecs */
 #define PDM_FP64 0          /* reference arithmetic in FP64 */
-#define PDM_FP32C 1         /* FP32 bond force, two-float family sums, FP64 state */
+#define PDM_FP32C 1         /* FP32 bond force (cancellation-free stretch), family sums in an FP64
+                               accumulator, FP64 state */
+#define PDM_FP32C2 2        /* as PDM_FP32C with two-float (hi, lo) FP32 family sums */
 
 typedef struct pdm_handle pdm_handle;
 
@@ -49,7 +51,9 @@ typedef struct {
     int use_graph;          /* capture each chunk of RK4 steps in a hipGraph (default 1) */
     int graph_chunk;        /* steps per captured graph (default 50) */
     int energies;           /* compute KE/EE every step (reference behaviour, default 1) */
-    int nodes_per_block;    /* force-kernel tile: families per 256-lane workgroup (0 = autotuned) */
+    int tile_edge;          /* tile edge in grid spacings, 2..8 (0 = autotuned at pdm_init) */
+    int waves_per_tile;     /* wavefronts per tile workgroup, 4 or 8 (0 = autotuned) */
+    int families_per_wave;  /* families a wavefront sums at once, 1, 2 or 4 (0 = autotuned) */
     int verbose;
 } pdm_options;
 

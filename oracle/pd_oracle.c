@@ -857,6 +857,42 @@ int orc_run_dynamic(orc_state* s, int nsteps, int nthreads)
     return 0;
 }
 
+/* Static system of the build's static solve (the intent of pdSolve::AssembleEquation,
+ * pdSolve.cpp:195-363, on the tangent of the CCI force): per component a free mask
+ * (1 free, 0 prescribed), prescribed values uc and loads b.  DISPL flagged components
+ * take value * factor(T), VELOC value * T, DISPLGRAD (G X) * factor(T), T = total_time;
+ * b = body force once per unbroken bond + traction body force (extension). */
+void orc_static_system(const orc_state* s, double* mask, double* uc, double* b)
+{
+    const orc_problem* p = &s->p;
+    double T = p->total_time;
+    for (int64_t i = 0; i < s->n; ++i) {
+        int64_t nb = 0;
+        for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) nb += s->cut[e] ? 0 : 1;
+        const orc_mat* m = &p->mat[s->mat[i]];
+        for (int c = 0; c < 3; ++c) {
+            mask[3 * i + c] = 1.0;
+            uc[3 * i + c] = 0.0;
+            b[3 * i + c] = (double)nb * (m->g[c] * m->rho);
+        }
+        int bi = s->bc[i];
+        if (bi < 0) continue;
+        const orc_bc* bc = &p->bc[bi];
+        double fac = orc_time_factor(bc, T);
+        for (int c = 0; c < 3; ++c) {
+            if (bc->type == 4) b[3 * i + c] += s->trac[3 * i + c] * fac;
+            if (bc->f[c] != 1 || !bc_node(s, i)) continue;
+            if (bc->type == 1) { mask[3 * i + c] = 0.0; uc[3 * i + c] = bc->v[c] * fac; }
+            else if (bc->type == 2) { mask[3 * i + c] = 0.0; uc[3 * i + c] = bc->v[c] * T; }
+            else if (bc->type == 3) {
+                mask[3 * i + c] = 0.0;
+                uc[3 * i + c] = (bc->v[3 * c] * s->X[3 * i] + bc->v[3 * c + 1] * s->X[3 * i + 1] +
+                                 bc->v[3 * c + 2] * s->X[3 * i + 2]) * fac;
+            }
+        }
+    }
+}
+
 void orc_get_state(const orc_state* s, double* U, double* V)
 {
     if (U) memcpy(U, s->U, sizeof(double) * 3 * s->n);

@@ -97,6 +97,8 @@ void orc_set_bdryeff(orc_state* s, const double* be);
 void orc_forces(const orc_state* s, const double* u, double* F, double* elast, int nthreads);
 /* Linearised (tangent at eta=0) bond operator y = K u, same weights as the force. */
 void orc_apply_tangent(const orc_state* s, const double* u, double* y, int nthreads);
+/* Static system: free mask, prescribed values, loads (3N each). */
+void orc_static_system(const orc_state* s, double* mask, double* uc, double* b);
 /* nsteps of pdSolve::SolveDynamic's loop (t advanced before RK4, shortened last step). */
 int orc_run_dynamic(orc_state* s, int nsteps, int nthreads);
 void orc_get_state(const orc_state* s, double* U, double* V);

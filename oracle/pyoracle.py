@@ -105,6 +105,7 @@ def lib() -> C.CDLL:
         L.orc_bond_force_cci.argtypes = [C.POINTER(Problem), dp, dp, C.c_double, C.c_double, C.c_double,
                                          C.c_double, dp, dp]
         L.orc_set_square_mode.argtypes = [C.c_int]
+        L.orc_static_system.argtypes = [vp, dp, dp, dp]
         L.orc_time_factor.restype = C.c_double
         L.orc_time_factor.argtypes = [C.POINTER(_BC), C.c_double]
         _lib = L
@@ -264,6 +265,33 @@ class Oracle:
         U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1)
         V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
         self.L.orc_set_state(self.h, _dp(U), _dp(V))
+
+    def static_system(self):
+        m, uc, b = np.zeros(3 * self.n), np.zeros(3 * self.n), np.zeros(3 * self.n)
+        self.L.orc_static_system(self.h, _dp(m), _dp(uc), _dp(b))
+        return m.reshape(-1, 3), uc.reshape(-1, 3), b.reshape(-1, 3)
+
+    def dense_tangent(self) -> np.ndarray:
+        """K assembled column by column from the matrix-free tangent (small problems only)."""
+        n3 = 3 * self.n
+        K = np.zeros((n3, n3))
+        e = np.zeros(n3)
+        for k in range(n3):
+            e[k] = 1.0
+            K[:, k] = self.tangent(e).reshape(-1)
+            e[k] = 0.0
+        return K
+
+    def solve_static_dense(self):
+        """Static solution by a dense direct solve of the masked system (small problems only)."""
+        m, uc, b = self.static_system()
+        K = self.dense_tangent()
+        free = m.reshape(-1) > 0
+        ucf = uc.reshape(-1)
+        rhs = b.reshape(-1) - K @ ucf
+        u = ucf.copy()
+        u[free] = np.linalg.solve(K[np.ix_(free, free)], rhs[free])
+        return u.reshape(-1, 3)
 
     @property
     def max_family(self) -> int:

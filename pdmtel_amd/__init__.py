@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "libpdm.so")
 
 PDM_FP64 = 0
 PDM_FP32C = 1
+PDM_FP32C2 = 2
 
 ERRORS = {
     -1: "PDM_E_ARG", -2: "PDM_E_IO", -3: "PDM_E_PARSE", -4: "PDM_E_MODEL", -5: "PDM_E_HIP",
@@ -37,7 +38,8 @@ class PdmError(RuntimeError):
 
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int), ("device", C.c_int), ("use_graph", C.c_int), ("graph_chunk", C.c_int),
-                ("energies", C.c_int), ("nodes_per_block", C.c_int), ("verbose", C.c_int)]
+                ("energies", C.c_int), ("tile_edge", C.c_int), ("waves_per_tile", C.c_int),
+                ("families_per_wave", C.c_int), ("verbose", C.c_int)]
 
 
 class StaticStats(C.Structure):

@@ -51,7 +51,7 @@ int create_common(const std::string& text, const pdm_options* opt, pdm_handle** 
     pdm_handle* h = new pdm_handle();
     if (opt) h->opt = *opt;
     else pdm_options_default(&h->opt);
-    if (h->opt.precision != PDM_FP64 && h->opt.precision != PDM_FP32C) {
+    if (h->opt.precision != PDM_FP64 && h->opt.precision != PDM_FP32C && h->opt.precision != PDM_FP32C2) {
         g_create_err = "unknown precision";
         delete h;
         return PDM_E_ARG;
@@ -263,7 +263,9 @@ int pdm_options_default(pdm_options* opt)
     opt->use_graph = 1;
     opt->graph_chunk = 50;
     opt->energies = 1;
-    opt->nodes_per_block = 0;
+    opt->tile_edge = 0;
+    opt->waves_per_tile = 0;
+    opt->families_per_wave = 0;
     opt->verbose = 0;
     return PDM_OK;
 }
@@ -388,6 +390,19 @@ int pdm_init(pdm_handle* h)
     }
     rc = build_bonds(h);
     if (rc) return rc;
+    if (h->opt.tile_edge) {
+        if (h->opt.tile_edge < 2 || h->opt.tile_edge > 7) return fail(h, PDM_E_ARG, "tile_edge must be 2..7");
+        h->tile_edge[0] = h->tile_edge[1] = h->tile_edge[2] = h->opt.tile_edge;
+    }
+    if (h->opt.waves_per_tile) {
+        if (h->opt.waves_per_tile != 4 && h->opt.waves_per_tile != 8) return fail(h, PDM_E_ARG, "waves_per_tile must be 4 or 8");
+        h->waves = h->opt.waves_per_tile;
+    }
+    if (h->opt.families_per_wave) {
+        int f = h->opt.families_per_wave;
+        if (f != 1 && f != 2 && f != 4) return fail(h, PDM_E_ARG, "families_per_wave must be 1, 2 or 4");
+        h->fam_per_wave = f;
+    }
     rc = build_tiles(h);
     if (rc) return rc;
     if (h->opt.verbose)
@@ -675,10 +690,8 @@ int pdm_solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_stat
 {
     int rc = require_init(h);
     if (rc) return rc;
-    (void)rel_tol;
-    (void)max_iterations;
-    (void)stats;
-    return fail(h, PDM_E_MODEL, "static solve not available yet");
+    HCHK(hipSetDevice(h->dev));
+    return solve_static(h, rel_tol, max_iterations, stats);
 }
 
 }  // extern "C"

@@ -35,7 +35,8 @@ constexpr int kMaxMat = 8;
 constexpr int kMaxBC = 16;
 constexpr int kMaxSlit = 8;
 constexpr int kMaxFamily = 1024;     // row-sort limit (families up to ~delta = 5.5 dx)
-constexpr int kMaxTileNodes = 512;   // nodes per tile
+constexpr int kMaxTileNodes = 343;   // nodes per tile (tile edge up to 7 dx)
+constexpr size_t kLdsBytes = 160 * 1024;  // LDS per CU on gfx950
 constexpr int kMaxFlags = 8192;      // nodes in a tile's neighbourhood (halo build)
 constexpr int kMaxHalo = 2560;       // halo entries staged in LDS: 7 x 8 B x 2560 + 32 B x 512 < 160 KB
 
@@ -85,6 +86,8 @@ struct TileView {
     int halo_max;              // largest halo (sizes the LDS staging area)
     int tile_max;              // most nodes in a tile (sizes the LDS result area)
     int xcd_remap;
+    int waves;                 // wavefronts per tile workgroup (4 or 8)
+    int fam_per_wave;          // families a wavefront sums at once (1, 2 or 4)
     int uniform_ecrit;         // all materials share ecrit: no per-neighbour ecrit in LDS
     double ecrit0;             // that common ecrit
     double inv_vol_cube;       // 1 / (dx*dx*dx)
@@ -172,6 +175,8 @@ struct pdm_handle {
     int64_t ntiles = 0, ntl = 0, halo_total = 0;
     int halo_max = 0, tile_max = 0;
     int tile_edge[3] = {4, 4, 4};   // tile edge in grid spacings
+    int waves = 8;                  // wavefronts per tile workgroup
+    int fam_per_wave = 2;           // families a wavefront sums at once
     int uniform_ecrit = 1;
 
     pdm::d4 *U = nullptr, *V = nullptr, *Vb = nullptr, *k1 = nullptr, *k23 = nullptr;
@@ -218,6 +223,10 @@ hipError_t launch_stage(int stage, int precision, const StageArgs& a, int step_i
 hipError_t launch_energy_final(const double* part, int64_t nparts, double* part2, double* out2, hipStream_t s);
 hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, double* elast, hipStream_t s);
 hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s);
+hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s);
+
+// static solve (pdm_cg.cpp / pdm_cg.hip)
+int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_stats* stats);
 size_t stage_lds_bytes(const TileView& tv, bool with_ecrit);
 int64_t energy_part2_size(int64_t nparts);
 

@@ -1,27 +1,31 @@
 // pdm_kernels.hip -- LDS-staged bond-force gathers fused with the RK4 stage update.
 //
-// One workgroup per tile.  Phase 1 stages the tile's halo (positions and this
-// stage's displacements, FP64, SoA) and its families' row starts and own slots
-// into LDS with coalesced loads.  Phase 2: each 64-lane wavefront takes one
-// family at a time; every lane keeps kUnroll entries (a uint16 LDS slot and a
-// coefficient each, streamed from HBM) in flight, and the first chunk of the
-// wavefront's next family is loaded while the current one is computed.  The
-// neighbour's data comes from LDS; the family sum is reduced across the
-// wavefront with xor shuffles into an LDS result slot.  Phase 3: one thread per
-// tile node applies the stage update (velocity buffer, boundary conditions, k,
-// next stage's displacement) on contiguous state arrays.  One launch is one
-// stage of RK4 (pdDataManager.cpp:1994-2134); a step is four.
+// One workgroup per tile.  Phase 1 stages the tile's halo as 48-byte records
+// (position and this stage's displacement, FP64) plus the families' row
+// starts and own slots into LDS with coalesced loads; one extra record far
+// outside the body pads partial chunks.  Phase 2: each 64-lane wavefront works
+// on NPW families at a time, 64/NPW lanes per family; every lane keeps kUnroll
+// entries (a uint16 LDS slot and a coefficient each, streamed from HBM) in
+// flight, and the first chunk of the next round is loaded while the current
+// one is computed.  The neighbour's record is three ds_read_b128.  Family sums
+// are reduced inside the lane group with DPP moves (quad_perm, row_half_mirror,
+// row_mirror) and the gfx950 permlane16/32 swaps, never through LDS.  Phase 3:
+// one thread per tile node applies the stage update (velocity buffer, boundary
+// conditions, k, next stage's displacement) on contiguous state arrays.  One
+// launch is one stage of RK4 (pdDataManager.cpp:1994-2134); a step is four.
 //
 // Per-bond arithmetic:
-//   PDM_FP64:  CalcBondForceCCI's FP64 expression (pdDataManager2.cpp:1293-1351):
-//              r = |xi|, p = |xi+eta|, s = (p - r)/r, f = ((coef*s)*(xi+eta))/p, with
-//              coef = bdryEff_ij * dx^3 * beta * c_ij precomputed in the same association
-//              order, so each bond force equals the reference's when pow(x, 2.0) is the
-//              correctly rounded square.
-//   PDM_FP32C: xi and eta formed in FP64 and rounded to FP32; the stretch from the
-//              cancellation-free form s = (2 xi.eta + eta.eta) / ((p + r) r); FP32
-//              coefficients; family sums kept as two-float (hi, lo) pairs, also across
-//              lanes; the stage update in FP64.
+//   PDM_FP64:   CalcBondForceCCI's FP64 expression (pdDataManager2.cpp:1293-1351):
+//               r = |xi|, p = |xi+eta|, s = (p - r)/r, f = ((coef*s)*(xi+eta))/p, with
+//               coef = bdryEff_ij * dx^3 * beta * c_ij precomputed in the same association
+//               order, so each bond force equals the reference's when pow(x, 2.0) is the
+//               correctly rounded square.
+//   PDM_FP32C:  xi and eta formed in FP64 and rounded to FP32; the stretch from the
+//               cancellation-free form s = (2 xi.eta + eta.eta) / ((p + r) r); FP32
+//               coefficients; each bond force added to an FP64 family accumulator (an FP64
+//               add is full rate on CDNA4, cheaper than a two-float TwoSum).
+//   PDM_FP32C2: the same bond force with two-float (hi, lo) FP32 family sums, also across
+//               lanes.  The stage update is FP64 in every mode.
 #include <cmath>
 
 #include "pdm_internal.h"
@@ -30,7 +34,7 @@ namespace pdm {
 namespace {
 
 constexpr int kUnroll = 4;              // entries per lane in flight
-constexpr int kChunk = 64 * kUnroll;    // entries per wavefront chunk
+constexpr double kFar = 1.0e3;          // coordinate of the padding record (m)
 
 __device__ __forceinline__ int64_t launch_index(int remap)
 {
@@ -42,9 +46,64 @@ __device__ __forceinline__ int64_t launch_index(int remap)
     return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
-__device__ __forceinline__ double wave_sum(double v)
+// ------------------------------------------------------------ cross-lane sums
+// DPP controls (CDNA): quad_perm [1,0,3,2] = xor 1, quad_perm [2,3,0,1] = xor 2,
+// row_half_mirror pairs the two quads of each 8-lane half, row_mirror the two halves
+// of each 16-lane row.  Applied in that order they leave the row sum in every lane.
+constexpr int kDppXor1 = 0xB1;
+constexpr int kDppXor2 = 0x4E;
+constexpr int kDppHalfMirror = 0x141;
+constexpr int kDppMirror = 0x140;
+
+template <int CTRL>
+__device__ __forceinline__ float dppf(float v)
 {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dppd(double v)
+{
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// v from lane i ^ 16 (SWAP16) or i ^ 32, via the gfx950 permlane swaps.
+template <bool SWAP16>
+__device__ __forceinline__ int perm_xor(int v)
+{
+    auto p = SWAP16 ? __builtin_amdgcn_permlane16_swap(v, v, false, false)
+                    : __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    // with both operands v: p[0] holds the even (lower) half's values, p[1] the odd (upper)'s
+    const int lane = threadIdx.x & 63;
+    const bool upper = SWAP16 ? ((lane >> 4) & 1) : (lane >> 5);
+    return upper ? p[0] : p[1];
+}
+
+template <bool SWAP16>
+__device__ __forceinline__ double perm_xord(double v)
+{
+    return __hiloint2double(perm_xor<SWAP16>(__double2hiint(v)), perm_xor<SWAP16>(__double2loint(v)));
+}
+
+template <bool SWAP16>
+__device__ __forceinline__ float perm_xorf(float v)
+{
+    return __int_as_float(perm_xor<SWAP16>(__float_as_int(v)));
+}
+
+// Sum over aligned groups of LPN lanes (16, 32 or 64); every lane gets its group's sum.
+template <int LPN>
+__device__ __forceinline__ double group_sum(double v)
+{
+    v += dppd<kDppXor1>(v);
+    v += dppd<kDppXor2>(v);
+    v += dppd<kDppHalfMirror>(v);
+    v += dppd<kDppMirror>(v);
+    if (LPN >= 32) v += perm_xord<true>(v);
+    if (LPN >= 64) v += perm_xord<false>(v);
     return v;
 }
 
@@ -55,28 +114,66 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
     e = (a - (s - bp)) + (b - bp);
 }
 
-// LDS layout of a tile: SoA halo arrays of H doubles, then per-node results
-// (4 doubles), row starts (nt + 1 int64) and own slots (nt ints).
+// Two-float group sum: (h, l) pairs combined with TwoSum at every butterfly step.
+template <int CTRL>
+__device__ __forceinline__ void tf_step_dpp(float& h, float& l)
+{
+    float oh = dppf<CTRL>(h), ol = dppf<CTRL>(l), t, e;
+    two_sum(h, oh, t, e);
+    h = t;
+    l = l + ol + e;
+}
+
+template <bool SWAP16>
+__device__ __forceinline__ void tf_step_perm(float& h, float& l)
+{
+    float oh = perm_xorf<SWAP16>(h), ol = perm_xorf<SWAP16>(l), t, e;
+    two_sum(h, oh, t, e);
+    h = t;
+    l = l + ol + e;
+}
+
+template <int LPN>
+__device__ __forceinline__ void group_sum_tf(float& h, float& l)
+{
+    tf_step_dpp<kDppXor1>(h, l);
+    tf_step_dpp<kDppXor2>(h, l);
+    tf_step_dpp<kDppHalfMirror>(h, l);
+    tf_step_dpp<kDppMirror>(h, l);
+    if (LPN >= 32) tf_step_perm<true>(h, l);
+    if (LPN >= 64) tf_step_perm<false>(h, l);
+}
+
+__device__ __forceinline__ double wave_sum(double v) { return group_sum<64>(v); }
+
+// ------------------------------------------------------------ LDS tile layout
+// Halo record: x, y, z, ux, uy, uz.
+struct alignas(16) Rec {
+    double x, y, z, u, v, w;
+};
+
+// H + 1 records (the last one is the padding record), optional per-record ecrit,
+// per-node results (4 doubles), row starts (nt + 1 int64), own slots (nt ints).
 struct Lds {
-    double *x, *y, *z, *ux, *uy, *uz, *ec;
+    Rec* rec;
+    double* ec;
     double* res;
     int64_t* row;
     int* self;
+    int pad;  // slot of the padding record
 };
 
 __device__ __forceinline__ Lds lds_view(double* base, int H, int tile_max, bool with_ecrit)
 {
     Lds l;
-    l.x = base;
-    l.y = base + H;
-    l.z = base + 2 * H;
-    l.ux = base + 3 * H;
-    l.uy = base + 4 * H;
-    l.uz = base + 5 * H;
-    l.ec = with_ecrit ? base + 6 * H : nullptr;
-    l.res = base + (with_ecrit ? 7 : 6) * H;
+    l.rec = reinterpret_cast<Rec*>(base);
+    double* p = base + 6 * (H + 1);
+    l.ec = with_ecrit ? p : nullptr;
+    if (with_ecrit) p += H + 1;
+    l.res = p;
     l.row = reinterpret_cast<int64_t*>(l.res + 4 * tile_max);
     l.self = reinterpret_cast<int*>(l.row + tile_max + 1);
+    l.pad = H;
     return l;
 }
 
@@ -87,180 +184,338 @@ __device__ __forceinline__ void stage_tile(const TileView& tv, const d4* __restr
         int32_t id = tv.halo[h0 + q];
         d4 x = tv.X[id];
         d4 v = u[id];
-        l.x[q] = x.x;
-        l.y[q] = x.y;
-        l.z[q] = x.z;
-        l.ux[q] = v.x;
-        l.uy[q] = v.y;
-        l.uz[q] = v.z;
+        l.rec[q] = Rec{x.x, x.y, x.z, v.x, v.y, v.z};
         if (with_ecrit) l.ec[q] = x.w;
+    }
+    if (threadIdx.x == 0) {
+        l.rec[H] = Rec{kFar, kFar, kFar, 0.0, 0.0, 0.0};
+        if (with_ecrit) l.ec[H] = 1.0;
     }
     for (int q = threadIdx.x; q <= nt; q += blockDim.x) l.row[q] = tv.rpi[a0 + q];
     for (int q = threadIdx.x; q < nt; q += blockDim.x) l.self[q] = tv.self_slot[a0 + q];
-}
-
-template <int PREC>
-struct Chunk;
-template <>
-struct Chunk<PDM_FP64> {
-    int s[kUnroll];
-    double c[kUnroll];
-};
-template <>
-struct Chunk<PDM_FP32C> {
-    int s[kUnroll];
-    float c[kUnroll];
-};
-
-template <int PREC>
-__device__ __forceinline__ void load_chunk(const TileView& tv, int64_t base, int64_t e1, int lane, Chunk<PREC>& ch)
-{
-#pragma unroll
-    for (int k = 0; k < kUnroll; ++k) {
-        const int64_t e = base + lane + 64 * k;
-        const bool v = e < e1;
-        ch.s[k] = v ? (int)__builtin_nontemporal_load(&tv.slot[e]) : -1;
-        if constexpr (PREC == PDM_FP64) ch.c[k] = v ? __builtin_nontemporal_load(&tv.coef64[e]) : 0.0;
-        else ch.c[k] = v ? __builtin_nontemporal_load(&tv.coef32[e]) : 0.0f;
-    }
 }
 
 struct Self {
     double x, y, z, u, v, w, ecrit;
 };
 
-// Per-lane accumulators: FP64 sums, or two-float (hi, lo) pairs.
-template <int PREC>
-struct Acc;
-template <>
-struct Acc<PDM_FP64> {
-    double f0, f1, f2, el;
-    __device__ void zero() { f0 = f1 = f2 = el = 0.0; }
-};
-template <>
-struct Acc<PDM_FP32C> {
-    float h0, h1, h2, l0, l1, l2, el;
-    __device__ void zero() { h0 = h1 = h2 = l0 = l1 = l2 = el = 0.0f; }
-};
+// ------------------------------------------------------------ per-bond operations
+// Each operation: the coefficient type it streams, a per-lane accumulator, the bond
+// update, the lane-group reduction and the store of (fx, fy, fz, energy density).
+// Energy: acc.el sums coef * s * u; the bond energy bdryEff * beta * c * u^2 / (4 r)
+// is that times 1 / (4 dx^3), applied once per family.
 
-// One bond, reference FP64 arithmetic.
-template <bool ENERGY, bool UE>
-__device__ __forceinline__ void bond(const Lds& l, int s, double c, const Self& me, double ivc, Acc<PDM_FP64>& a)
-{
-    double xi1 = l.x[s] - me.x, xi2 = l.y[s] - me.y, xi3 = l.z[s] - me.z;
-    double et1 = l.ux[s] - me.u, et2 = l.uy[s] - me.v, et3 = l.uz[s] - me.w;
-    double r = sqrt(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
-    double a1 = xi1 + et1, a2 = xi2 + et2, a3 = xi3 + et3;
-    double p = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
-    double u = p - r;
-    double st = u / r;
-    double ecrit = me.ecrit;
-    if (!UE) {
-        double ej = l.ec[s];
-        ecrit = (ej < me.ecrit) ? ej : me.ecrit;
-    }
-    if (u <= 0.0 || (u > 0.0 && st <= ecrit)) {
-        double cs = c * st;
+struct OpF64 {  // PDM_FP64, reference arithmetic
+    using Coef = double;
+    struct Acc {
+        double f0, f1, f2, el;
+    };
+    template <bool ENERGY, bool UE>
+    __device__ static void bond(const Lds& l, int s, double c, const Self& me, Acc& a)
+    {
+        const Rec r = l.rec[s];
+        double xi1 = r.x - me.x, xi2 = r.y - me.y, xi3 = r.z - me.z;
+        double et1 = r.u - me.u, et2 = r.v - me.v, et3 = r.w - me.w;
+        double rr = sqrt(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
+        double a1 = xi1 + et1, a2 = xi2 + et2, a3 = xi3 + et3;
+        double p = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+        double u = p - rr;
+        double st = u / rr;
+        double ecrit = me.ecrit;
+        if (!UE) {
+            double ej = l.ec[s];
+            ecrit = (ej < me.ecrit) ? ej : me.ecrit;
+        }
+        const bool ok = u <= 0.0 || (u > 0.0 && st <= ecrit);
+        const double cs = ok ? c * st : 0.0;
         a.f0 += (cs * a1) / p;
         a.f1 += (cs * a2) / p;
         a.f2 += (cs * a3) / p;
-        if (ENERGY) a.el += 0.25 * (c * ivc) * st * u;
+        if (ENERGY) a.el += cs * u;
     }
-}
+    template <int LPN>
+    __device__ static void reduce(Acc& a)
+    {
+        a.f0 = group_sum<LPN>(a.f0);
+        a.f1 = group_sum<LPN>(a.f1);
+        a.f2 = group_sum<LPN>(a.f2);
+        a.el = group_sum<LPN>(a.el);
+    }
+    __device__ static void store(const Acc& a, double ivc4, double* out)
+    {
+        out[0] = a.f0;
+        out[1] = a.f1;
+        out[2] = a.f2;
+        out[3] = a.el * ivc4;
+    }
+};
 
-// One bond, FP32 with the cancellation-free stretch and two-float sums.
+struct Bond32 {
+    float g, a1, a2, a3, e;  // force factor, xi + eta, energy term coef * s * u
+};
+
+// FP32 bond force from FP64 records: cancellation-free stretch.
 template <bool ENERGY, bool UE>
-__device__ __forceinline__ void bond(const Lds& l, int s, float c, const Self& me, double ivc, Acc<PDM_FP32C>& a)
+__device__ __forceinline__ Bond32 bond32(const Lds& l, int s, float c, const Self& me)
 {
-    float x1 = (float)(l.x[s] - me.x), x2 = (float)(l.y[s] - me.y), x3 = (float)(l.z[s] - me.z);
-    float e1 = (float)(l.ux[s] - me.u), e2 = (float)(l.uy[s] - me.v), e3 = (float)(l.uz[s] - me.w);
-    float r2 = x1 * x1 + x2 * x2 + x3 * x3;
-    float q = 2.0f * (x1 * e1 + x2 * e2 + x3 * e3) + (e1 * e1 + e2 * e2 + e3 * e3);
-    float rinv = __builtin_amdgcn_rsqf(r2);
-    float r = r2 * rinv;
-    float p = __builtin_amdgcn_sqrtf(r2 + q);
-    float u = q * __builtin_amdgcn_rcpf(p + r);
-    float st = u * rinv;
+    const Rec r = l.rec[s];
+    const float x1 = (float)(r.x - me.x), x2 = (float)(r.y - me.y), x3 = (float)(r.z - me.z);
+    const float e1 = (float)(r.u - me.u), e2 = (float)(r.v - me.v), e3 = (float)(r.w - me.w);
+    const float r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
+    const float xe = __builtin_fmaf(x3, e3, __builtin_fmaf(x2, e2, x1 * e1));
+    const float ee = __builtin_fmaf(e3, e3, __builtin_fmaf(e2, e2, e1 * e1));
+    const float q = __builtin_fmaf(2.0f, xe, ee);  // |xi+eta|^2 - |xi|^2
+    const float p2 = r2 + q;
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float pinv = __builtin_amdgcn_rsqf(p2);
+    const float rr = r2 * rinv;
+    const float p = p2 * pinv;
+    const float st = q * __builtin_amdgcn_rcpf((p + rr) * rr);  // (p - r) / r without cancellation
+    const float u = st * rr;
     float ecrit = (float)me.ecrit;
     if (!UE) {
         float ej = (float)l.ec[s];
         ecrit = (ej < ecrit) ? ej : ecrit;
     }
-    if (u <= 0.0f || st <= ecrit) {
-        float g = c * st * __builtin_amdgcn_rcpf(p);
-        float t, err;
-        two_sum(a.h0, g * (x1 + e1), t, err); a.h0 = t; a.l0 += err;
-        two_sum(a.h1, g * (x2 + e2), t, err); a.h1 = t; a.l1 += err;
-        two_sum(a.h2, g * (x3 + e3), t, err); a.h2 = t; a.l2 += err;
-        if (ENERGY) a.el += 0.25f * (c * (float)ivc) * st * u;
-    }
+    const bool ok = u <= 0.0f || st <= ecrit;
+    const float cs = ok ? c * st : 0.0f;
+    Bond32 b;
+    b.g = cs * pinv;
+    b.a1 = x1 + e1;
+    b.a2 = x2 + e2;
+    b.a3 = x3 + e3;
+    b.e = ENERGY ? cs * u : 0.0f;
+    return b;
 }
 
-__device__ __forceinline__ void reduce_store(Acc<PDM_FP64>& a, int lane, double* out)
-{
-    double f0 = wave_sum(a.f0), f1 = wave_sum(a.f1), f2 = wave_sum(a.f2), el = wave_sum(a.el);
-    if (lane == 0) {
-        out[0] = f0;
-        out[1] = f1;
-        out[2] = f2;
-        out[3] = el;
+struct OpF32 {  // PDM_FP32C: FP32 bond force, FP64 family accumulator
+    using Coef = float;
+    struct Acc {
+        double f0, f1, f2;
+        float el;
+    };
+    template <bool ENERGY, bool UE>
+    __device__ static void bond(const Lds& l, int s, float c, const Self& me, Acc& a)
+    {
+        const Bond32 b = bond32<ENERGY, UE>(l, s, c, me);
+        a.f0 += (double)(b.g * b.a1);
+        a.f1 += (double)(b.g * b.a2);
+        a.f2 += (double)(b.g * b.a3);
+        if (ENERGY) a.el += b.e;
     }
-}
+    template <int LPN>
+    __device__ static void reduce(Acc& a)
+    {
+        a.f0 = group_sum<LPN>(a.f0);
+        a.f1 = group_sum<LPN>(a.f1);
+        a.f2 = group_sum<LPN>(a.f2);
+        a.el = (float)group_sum<LPN>((double)a.el);
+    }
+    __device__ static void store(const Acc& a, double ivc4, double* out)
+    {
+        out[0] = a.f0;
+        out[1] = a.f1;
+        out[2] = a.f2;
+        out[3] = (double)a.el * ivc4;
+    }
+};
 
-__device__ __forceinline__ void reduce_store(Acc<PDM_FP32C>& a, int lane, double* out)
-{
-    for (int off = 32; off > 0; off >>= 1) {
-        float o0 = __shfl_xor(a.h0, off, 64), o1 = __shfl_xor(a.h1, off, 64), o2 = __shfl_xor(a.h2, off, 64);
-        float m0 = __shfl_xor(a.l0, off, 64), m1 = __shfl_xor(a.l1, off, 64), m2 = __shfl_xor(a.l2, off, 64);
+struct OpF32TF {  // PDM_FP32C2: FP32 bond force, two-float family sums
+    using Coef = float;
+    struct Acc {
+        float h0, h1, h2, l0, l1, l2, el;
+    };
+    template <bool ENERGY, bool UE>
+    __device__ static void bond(const Lds& l, int s, float c, const Self& me, Acc& a)
+    {
+        const Bond32 b = bond32<ENERGY, UE>(l, s, c, me);
         float t, err;
-        two_sum(a.h0, o0, t, err); a.h0 = t; a.l0 = a.l0 + m0 + err;
-        two_sum(a.h1, o1, t, err); a.h1 = t; a.l1 = a.l1 + m1 + err;
-        two_sum(a.h2, o2, t, err); a.h2 = t; a.l2 = a.l2 + m2 + err;
-        a.el += __shfl_xor(a.el, off, 64);
+        two_sum(a.h0, b.g * b.a1, t, err); a.h0 = t; a.l0 += err;
+        two_sum(a.h1, b.g * b.a2, t, err); a.h1 = t; a.l1 += err;
+        two_sum(a.h2, b.g * b.a3, t, err); a.h2 = t; a.l2 += err;
+        if (ENERGY) a.el += b.e;
     }
-    if (lane == 0) {
+    template <int LPN>
+    __device__ static void reduce(Acc& a)
+    {
+        group_sum_tf<LPN>(a.h0, a.l0);
+        group_sum_tf<LPN>(a.h1, a.l1);
+        group_sum_tf<LPN>(a.h2, a.l2);
+        a.el = (float)group_sum<LPN>((double)a.el);
+    }
+    __device__ static void store(const Acc& a, double ivc4, double* out)
+    {
         out[0] = (double)a.h0 + (double)a.l0;
         out[1] = (double)a.h1 + (double)a.l1;
         out[2] = (double)a.h2 + (double)a.l2;
-        out[3] = (double)a.el;
+        out[3] = (double)a.el * ivc4;
+    }
+};
+
+// Tangent of the bond force at eta = 0: y_i = sum_j coef_ij xi (xi . (u_i - u_j)) / r^3.
+struct OpT64 {
+    using Coef = double;
+    struct Acc {
+        double f0, f1, f2, el;
+    };
+    template <bool ENERGY, bool UE>
+    __device__ static void bond(const Lds& l, int s, double c, const Self& me, Acc& a)
+    {
+        const Rec r = l.rec[s];
+        double x1 = r.x - me.x, x2 = r.y - me.y, x3 = r.z - me.z;
+        double d0 = me.u - r.u, d1 = me.v - r.v, d2 = me.w - r.w;
+        double r2 = x1 * x1 + x2 * x2 + x3 * x3;
+        double proj = (x1 * d0 + x2 * d1 + x3 * d2) * c / (r2 * sqrt(r2));
+        a.f0 += proj * x1;
+        a.f1 += proj * x2;
+        a.f2 += proj * x3;
+    }
+    template <int LPN>
+    __device__ static void reduce(Acc& a)
+    {
+        a.f0 = group_sum<LPN>(a.f0);
+        a.f1 = group_sum<LPN>(a.f1);
+        a.f2 = group_sum<LPN>(a.f2);
+    }
+    __device__ static void store(const Acc& a, double, double* out)
+    {
+        out[0] = a.f0;
+        out[1] = a.f1;
+        out[2] = a.f2;
+        out[3] = 0.0;
+    }
+};
+
+struct OpT32 {
+    using Coef = float;
+    struct Acc {
+        double f0, f1, f2, el;
+    };
+    template <bool ENERGY, bool UE>
+    __device__ static void bond(const Lds& l, int s, float c, const Self& me, Acc& a)
+    {
+        const Rec r = l.rec[s];
+        const float x1 = (float)(r.x - me.x), x2 = (float)(r.y - me.y), x3 = (float)(r.z - me.z);
+        const float d0 = (float)(me.u - r.u), d1 = (float)(me.v - r.v), d2 = (float)(me.w - r.w);
+        const float r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        const float proj = __builtin_fmaf(x3, d2, __builtin_fmaf(x2, d1, x1 * d0)) * c * (rinv * rinv * rinv);
+        a.f0 += (double)(proj * x1);
+        a.f1 += (double)(proj * x2);
+        a.f2 += (double)(proj * x3);
+    }
+    template <int LPN>
+    __device__ static void reduce(Acc& a)
+    {
+        OpT64::reduce<LPN>(reinterpret_cast<OpT64::Acc&>(a));
+    }
+    __device__ static void store(const Acc& a, double, double* out)
+    {
+        out[0] = a.f0;
+        out[1] = a.f1;
+        out[2] = a.f2;
+        out[3] = 0.0;
+    }
+};
+
+// Diagonal of the tangent operator (Jacobi preconditioner): d_c = sum_j coef_ij xi_c^2 / r^3.
+struct OpD64 {
+    using Coef = double;
+    struct Acc {
+        double f0, f1, f2, el;
+    };
+    template <bool ENERGY, bool UE>
+    __device__ static void bond(const Lds& l, int s, double c, const Self& me, Acc& a)
+    {
+        const Rec r = l.rec[s];
+        double x1 = r.x - me.x, x2 = r.y - me.y, x3 = r.z - me.z;
+        double r2 = x1 * x1 + x2 * x2 + x3 * x3;
+        double w = c / (r2 * sqrt(r2));
+        a.f0 += w * x1 * x1;
+        a.f1 += w * x2 * x2;
+        a.f2 += w * x3 * x3;
+    }
+    template <int LPN>
+    __device__ static void reduce(Acc& a)
+    {
+        OpT64::reduce<LPN>(reinterpret_cast<OpT64::Acc&>(a));
+    }
+    __device__ static void store(const Acc& a, double, double* out)
+    {
+        out[0] = a.f0;
+        out[1] = a.f1;
+        out[2] = a.f2;
+        out[3] = 0.0;
+    }
+};
+
+template <class OP>
+struct Chunk {
+    int s[kUnroll];
+    typename OP::Coef c[kUnroll];
+};
+
+// kUnroll entries per lane, stride LPN; entries past the row point at the padding
+// record with coefficient 0.
+template <class OP, int LPN>
+__device__ __forceinline__ void load_chunk(const TileView& tv, int64_t base, int64_t e1, int sl, int pad,
+                                           Chunk<OP>& ch)
+{
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) {
+        const int64_t e = base + sl + LPN * k;
+        const bool v = e < e1;
+        ch.s[k] = v ? (int)__builtin_nontemporal_load(&tv.slot[e]) : pad;
+        if constexpr (sizeof(typename OP::Coef) == 8) ch.c[k] = v ? __builtin_nontemporal_load(&tv.coef64[e]) : 0.0;
+        else ch.c[k] = v ? __builtin_nontemporal_load(&tv.coef32[e]) : 0.0f;
     }
 }
 
-// Family sums of the tile's nodes into l.res; wavefront w takes nodes w, w + NW, ...
-template <int PREC, bool ENERGY, bool UE, int NW>
+// Family sums of the tile's nodes into l.res.  Round r of wavefront w covers nodes
+// (r NW + w) NPW + g for lane groups g = 0..NPW-1; the first chunk of round r + 1
+// is loaded before round r is computed.
+template <class OP, bool ENERGY, bool UE, int NW, int NPW>
 __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, int nt)
 {
+    constexpr int LPN = 64 / NPW;
+    constexpr int CH = LPN * kUnroll;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int loc = wave;
-    if (loc >= nt) return;
-    int64_t e0 = l.row[loc], e1 = l.row[loc + 1];
-    Chunk<PREC> cur;
-    load_chunk<PREC>(tv, e0, e1, lane, cur);
+    const int g = lane / LPN, sl = lane % LPN;
+    const double ivc4 = 0.25 * tv.inv_vol_cube;
+    int first = wave * NPW;
+    if (first >= nt) return;
+    int loc = first + g;
+    int64_t e0 = (loc < nt) ? l.row[loc] : 0, e1 = (loc < nt) ? l.row[loc + 1] : 0;
+    Chunk<OP> cur;
+    load_chunk<OP, LPN>(tv, e0, e1, sl, l.pad, cur);
     for (;;) {
-        const int nloc = loc + NW;
+        const int nfirst = first + NW * NPW;
+        const int nloc = nfirst + g;
         int64_t n0 = 0, n1 = 0;
-        Chunk<PREC> nxt;
-        if (nloc < nt) {
-            n0 = l.row[nloc];
-            n1 = l.row[nloc + 1];
-            load_chunk<PREC>(tv, n0, n1, lane, nxt);
+        Chunk<OP> nxt;
+        if (nfirst < nt) {
+            if (nloc < nt) {
+                n0 = l.row[nloc];
+                n1 = l.row[nloc + 1];
+            }
+            load_chunk<OP, LPN>(tv, n0, n1, sl, l.pad, nxt);
         }
-        const int ss = l.self[loc];
-        Self me{l.x[ss], l.y[ss], l.z[ss], l.ux[ss], l.uy[ss], l.uz[ss], UE ? tv.ecrit0 : l.ec[ss]};
-        Acc<PREC> acc;
-        acc.zero();
+        const int ss = (loc < nt) ? l.self[loc] : l.pad;
+        const Rec ri = l.rec[ss];
+        const Self me{ri.x, ri.y, ri.z, ri.u, ri.v, ri.w, UE ? tv.ecrit0 : l.ec[ss]};
+        typename OP::Acc acc{};
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k)
-            if (cur.s[k] >= 0) bond<ENERGY, UE>(l, cur.s[k], cur.c[k], me, tv.inv_vol_cube, acc);
-        for (int64_t base = e0 + kChunk; base < e1; base += kChunk) {
-            Chunk<PREC> more;
-            load_chunk<PREC>(tv, base, e1, lane, more);
+        for (int k = 0; k < kUnroll; ++k) OP::template bond<ENERGY, UE>(l, cur.s[k], cur.c[k], me, acc);
+        for (int64_t base = e0 + CH; base < e1; base += CH) {
+            Chunk<OP> more;
+            load_chunk<OP, LPN>(tv, base, e1, sl, l.pad, more);
 #pragma unroll
-            for (int k = 0; k < kUnroll; ++k)
-                if (more.s[k] >= 0) bond<ENERGY, UE>(l, more.s[k], more.c[k], me, tv.inv_vol_cube, acc);
+            for (int k = 0; k < kUnroll; ++k) OP::template bond<ENERGY, UE>(l, more.s[k], more.c[k], me, acc);
         }
-        reduce_store(acc, lane, l.res + 4 * loc);
-        if (nloc >= nt) break;
+        OP::template reduce<LPN>(acc);
+        if (sl == 0 && loc < nt) OP::store(acc, ivc4, l.res + 4 * loc);
+        if (nfirst >= nt) break;
+        first = nfirst;
         loc = nloc;
         e0 = n0;
         e1 = n1;
@@ -275,8 +530,26 @@ __device__ __forceinline__ double time_factor_dev(const BCDev& b, double t)
     return 1.0;
 }
 
+template <int PREC>
+struct OpFor;
+template <>
+struct OpFor<PDM_FP64> {
+    using Force = OpF64;
+    using Tangent = OpT64;
+};
+template <>
+struct OpFor<PDM_FP32C> {
+    using Force = OpF32;
+    using Tangent = OpT32;
+};
+template <>
+struct OpFor<PDM_FP32C2> {
+    using Force = OpF32TF;
+    using Tangent = OpT32;
+};
+
 // One RK4 stage (STAGE 0..3) for one tile.  Reads a.uin, writes a.uout.
-template <int STAGE, int PREC, bool ENERGY, bool UE, int NW>
+template <int STAGE, int PREC, bool ENERGY, bool UE, int NW, int NPW>
 __global__ __launch_bounds__(64 * NW) void k_stage(StageArgs a, int step_idx)
 {
     extern __shared__ double L[];
@@ -292,7 +565,7 @@ __global__ __launch_bounds__(64 * NW) void k_stage(StageArgs a, int step_idx)
     const Lds l = lds_view(L, H, tv.tile_max, !UE);
     stage_tile(tv, a.uin, h0, H, a0, nt, l, !UE);
     __syncthreads();
-    tile_families<PREC, ENERGY, UE, NW>(tv, l, nt);
+    tile_families<typename OpFor<PREC>::Force, ENERGY, UE, NW, NPW>(tv, l, nt);
     __syncthreads();
 
     const StepParam sp = a.steps[step_idx];
@@ -431,33 +704,10 @@ __global__ __launch_bounds__(256) void k_reduce_pairs(const double* __restrict__
     }
 }
 
-template <int PREC, bool UE, int NW>
-__global__ __launch_bounds__(64 * NW) void k_forces(TileView tv, const d4* __restrict__ u, d4* __restrict__ F,
+// Family sums (force or tangent) for every node, no state update: y = (fx, fy, fz), energy.
+template <class OP, bool UE, int NW, int NPW>
+__global__ __launch_bounds__(64 * NW) void k_gather(TileView tv, const d4* __restrict__ u, d4* __restrict__ y,
                                                     double* __restrict__ elast)
-{
-    extern __shared__ double L[];
-    const int64_t k = blockIdx.x;
-    if (k >= tv.ntl) return;
-    const int32_t tid = tv.tiles[k];
-    const int64_t a0 = tv.tile_start[tid];
-    const int nt = (int)(tv.tile_start[tid + 1] - a0);
-    const int64_t h0 = tv.halo_ptr[k];
-    const int H = (int)(tv.halo_ptr[k + 1] - h0);
-    const Lds l = lds_view(L, H, tv.tile_max, !UE);
-    stage_tile(tv, u, h0, H, a0, nt, l, !UE);
-    __syncthreads();
-    tile_families<PREC, true, UE, NW>(tv, l, nt);
-    __syncthreads();
-    for (int loc = threadIdx.x; loc < nt; loc += 64 * NW) {
-        const double* res = l.res + 4 * loc;
-        F[a0 + loc] = d4{res[0], res[1], res[2], 0.0};
-        if (elast) elast[a0 + loc] = res[3];
-    }
-}
-
-// Tangent of the bond force at eta = 0: y_i = sum_j coef_ij xi (xi . (u_i - u_j)) / r^3.
-template <int PREC, int NW>
-__global__ __launch_bounds__(64 * NW) void k_tangent(TileView tv, const d4* __restrict__ u, d4* __restrict__ y)
 {
     extern __shared__ double L[];
     const int64_t k = launch_index(tv.xcd_remap);
@@ -467,57 +717,16 @@ __global__ __launch_bounds__(64 * NW) void k_tangent(TileView tv, const d4* __re
     const int nt = (int)(tv.tile_start[tid + 1] - a0);
     const int64_t h0 = tv.halo_ptr[k];
     const int H = (int)(tv.halo_ptr[k + 1] - h0);
-    const Lds l = lds_view(L, H, tv.tile_max, false);
-    stage_tile(tv, u, h0, H, a0, nt, l, false);
+    const Lds l = lds_view(L, H, tv.tile_max, !UE);
+    stage_tile(tv, u, h0, H, a0, nt, l, !UE);
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int loc = wave; loc < nt; loc += NW) {
-        const int ss = l.self[loc];
-        const double xi = l.x[ss], yi = l.y[ss], zi = l.z[ss];
-        const double ui = l.ux[ss], vi = l.uy[ss], wi = l.uz[ss];
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        const int64_t e1 = l.row[loc + 1];
-        for (int64_t base = l.row[loc]; base < e1; base += kChunk) {
-            Chunk<PREC> ch;
-            load_chunk<PREC>(tv, base, e1, lane, ch);
-#pragma unroll
-            for (int q = 0; q < kUnroll; ++q) {
-                const int s = ch.s[q];
-                if (s < 0) continue;
-                if constexpr (PREC == PDM_FP64) {
-                    const double c = ch.c[q];
-                    double x1 = l.x[s] - xi, x2 = l.y[s] - yi, x3 = l.z[s] - zi;
-                    double d0 = ui - l.ux[s], d1 = vi - l.uy[s], d2 = wi - l.uz[s];
-                    double r2 = x1 * x1 + x2 * x2 + x3 * x3;
-                    double proj = (x1 * d0 + x2 * d1 + x3 * d2) * c / (r2 * sqrt(r2));
-                    s0 += proj * x1;
-                    s1 += proj * x2;
-                    s2 += proj * x3;
-                } else {
-                    const float c = ch.c[q];
-                    float x1 = (float)(l.x[s] - xi), x2 = (float)(l.y[s] - yi), x3 = (float)(l.z[s] - zi);
-                    float d0 = (float)(ui - l.ux[s]), d1 = (float)(vi - l.uy[s]), d2 = (float)(wi - l.uz[s]);
-                    float r2 = x1 * x1 + x2 * x2 + x3 * x3;
-                    float ri = __builtin_amdgcn_rsqf(r2);
-                    float proj = (x1 * d0 + x2 * d1 + x3 * d2) * c * (ri * ri * ri);
-                    s0 += (double)(proj * x1);
-                    s1 += (double)(proj * x2);
-                    s2 += (double)(proj * x3);
-                }
-            }
-        }
-        s0 = wave_sum(s0);
-        s1 = wave_sum(s1);
-        s2 = wave_sum(s2);
-        if (lane == 0) {
-            l.res[4 * loc] = s0;
-            l.res[4 * loc + 1] = s1;
-            l.res[4 * loc + 2] = s2;
-        }
+    tile_families<OP, true, UE, NW, NPW>(tv, l, nt);
+    __syncthreads();
+    for (int loc = threadIdx.x; loc < nt; loc += 64 * NW) {
+        const double* res = l.res + 4 * loc;
+        y[a0 + loc] = d4{res[0], res[1], res[2], 0.0};
+        if (elast) elast[a0 + loc] = res[3];
     }
-    __syncthreads();
-    for (int loc = threadIdx.x; loc < nt; loc += 64 * NW)
-        y[a0 + loc] = d4{l.res[4 * loc], l.res[4 * loc + 1], l.res[4 * loc + 2], 0.0};
 }
 
 int64_t padded_grid(int64_t ntl) { return (ntl + 7) / 8 * 8; }
@@ -534,66 +743,74 @@ hipError_t set_lds(size_t bytes)
     return e;
 }
 
-constexpr int kWavesDefault = 8;
-
-template <int STAGE, int PREC, bool ENERGY, bool UE>
+template <int STAGE, int PREC, bool ENERGY, bool UE, int NW, int NPW>
 hipError_t launch_stage_t(const StageArgs& a, int step_idx, size_t lds, hipStream_t s)
 {
-    constexpr int NW = kWavesDefault;
-    hipError_t e = set_lds<k_stage<STAGE, PREC, ENERGY, UE, NW>>(lds);
+    hipError_t e = set_lds<k_stage<STAGE, PREC, ENERGY, UE, NW, NPW>>(lds);
     if (e != hipSuccess) return e;
-    k_stage<STAGE, PREC, ENERGY, UE, NW>
+    k_stage<STAGE, PREC, ENERGY, UE, NW, NPW>
         <<<dim3((unsigned)padded_grid(a.tv.ntl)), dim3(64 * NW), lds, s>>>(a, step_idx);
     return hipGetLastError();
 }
 
-template <int STAGE, int PREC>
+template <int STAGE, int PREC, int NW, int NPW>
 hipError_t launch_stage_e(const StageArgs& a, int step_idx, size_t lds, hipStream_t s)
 {
     if (a.energies) {
-        if (a.tv.uniform_ecrit) return launch_stage_t<STAGE, PREC, true, true>(a, step_idx, lds, s);
-        return launch_stage_t<STAGE, PREC, true, false>(a, step_idx, lds, s);
+        if (a.tv.uniform_ecrit) return launch_stage_t<STAGE, PREC, true, true, NW, NPW>(a, step_idx, lds, s);
+        return launch_stage_t<STAGE, PREC, true, false, NW, NPW>(a, step_idx, lds, s);
     }
-    if (a.tv.uniform_ecrit) return launch_stage_t<STAGE, PREC, false, true>(a, step_idx, lds, s);
-    return launch_stage_t<STAGE, PREC, false, false>(a, step_idx, lds, s);
+    if (a.tv.uniform_ecrit) return launch_stage_t<STAGE, PREC, false, true, NW, NPW>(a, step_idx, lds, s);
+    return launch_stage_t<STAGE, PREC, false, false, NW, NPW>(a, step_idx, lds, s);
+}
+
+template <int PREC, int NW, int NPW>
+hipError_t launch_stage_s(int stage, const StageArgs& a, int step_idx, size_t lds, hipStream_t s)
+{
+    switch (stage) {
+    case 0: return launch_stage_e<0, PREC, NW, NPW>(a, step_idx, lds, s);
+    case 1: return launch_stage_e<1, PREC, NW, NPW>(a, step_idx, lds, s);
+    case 2: return launch_stage_e<2, PREC, NW, NPW>(a, step_idx, lds, s);
+    default: return launch_stage_e<3, PREC, NW, NPW>(a, step_idx, lds, s);
+    }
 }
 
 template <int PREC>
 hipError_t launch_stage_p(int stage, const StageArgs& a, int step_idx, size_t lds, hipStream_t s)
 {
-    switch (stage) {
-    case 0: return launch_stage_e<0, PREC>(a, step_idx, lds, s);
-    case 1: return launch_stage_e<1, PREC>(a, step_idx, lds, s);
-    case 2: return launch_stage_e<2, PREC>(a, step_idx, lds, s);
-    default: return launch_stage_e<3, PREC>(a, step_idx, lds, s);
+    const int w = a.tv.waves, f = a.tv.fam_per_wave;
+    if (w == 4) {
+        if (f == 1) return launch_stage_s<PREC, 4, 1>(stage, a, step_idx, lds, s);
+        if (f == 2) return launch_stage_s<PREC, 4, 2>(stage, a, step_idx, lds, s);
+        return launch_stage_s<PREC, 4, 4>(stage, a, step_idx, lds, s);
     }
+    if (f == 1) return launch_stage_s<PREC, 8, 1>(stage, a, step_idx, lds, s);
+    if (f == 2) return launch_stage_s<PREC, 8, 2>(stage, a, step_idx, lds, s);
+    return launch_stage_s<PREC, 8, 4>(stage, a, step_idx, lds, s);
 }
 
-template <int PREC, bool UE>
-hipError_t launch_forces_t(const TileView& tv, const d4* u, d4* F, double* elast, size_t lds, hipStream_t s)
+template <class OP, bool UE>
+hipError_t launch_gather_t(const TileView& tv, const d4* u, d4* y, double* elast, size_t lds, hipStream_t s)
 {
-    constexpr int NW = kWavesDefault;
-    hipError_t e = set_lds<k_forces<PREC, UE, NW>>(lds);
+    constexpr int NW = 8, NPW = 4;
+    hipError_t e = set_lds<k_gather<OP, UE, NW, NPW>>(lds);
     if (e != hipSuccess) return e;
-    k_forces<PREC, UE, NW><<<dim3((unsigned)tv.ntl), dim3(64 * NW), lds, s>>>(tv, u, F, elast);
+    k_gather<OP, UE, NW, NPW><<<dim3((unsigned)padded_grid(tv.ntl)), dim3(64 * NW), lds, s>>>(tv, u, y, elast);
     return hipGetLastError();
 }
 
-template <int PREC>
-hipError_t launch_tangent_t(const TileView& tv, const d4* u, d4* y, size_t lds, hipStream_t s)
+template <class OP>
+hipError_t launch_gather_u(const TileView& tv, const d4* u, d4* y, double* elast, size_t lds, hipStream_t s)
 {
-    constexpr int NW = kWavesDefault;
-    hipError_t e = set_lds<k_tangent<PREC, NW>>(lds);
-    if (e != hipSuccess) return e;
-    k_tangent<PREC, NW><<<dim3((unsigned)padded_grid(tv.ntl)), dim3(64 * NW), lds, s>>>(tv, u, y);
-    return hipGetLastError();
+    if (tv.uniform_ecrit) return launch_gather_t<OP, true>(tv, u, y, elast, lds, s);
+    return launch_gather_t<OP, false>(tv, u, y, elast, lds, s);
 }
 
 }  // namespace
 
 size_t stage_lds_bytes(const TileView& tv, bool with_ecrit)
 {
-    return sizeof(double) * ((size_t)(with_ecrit ? 7 : 6) * tv.halo_max + 4 * (size_t)tv.tile_max) +
+    return sizeof(double) * ((size_t)(with_ecrit ? 7 : 6) * (tv.halo_max + 1) + 4 * (size_t)tv.tile_max) +
            sizeof(int64_t) * ((size_t)tv.tile_max + 1) + sizeof(int) * (size_t)tv.tile_max;
 }
 
@@ -603,8 +820,11 @@ hipError_t launch_stage(int stage, int precision, const StageArgs& a, int step_i
 {
     if (a.tv.ntl == 0) return hipSuccess;
     size_t lds = stage_lds_bytes(a.tv, !a.tv.uniform_ecrit);
-    if (precision == PDM_FP64) return launch_stage_p<PDM_FP64>(stage, a, step_idx, lds, s);
-    return launch_stage_p<PDM_FP32C>(stage, a, step_idx, lds, s);
+    switch (precision) {
+    case PDM_FP64: return launch_stage_p<PDM_FP64>(stage, a, step_idx, lds, s);
+    case PDM_FP32C: return launch_stage_p<PDM_FP32C>(stage, a, step_idx, lds, s);
+    default: return launch_stage_p<PDM_FP32C2>(stage, a, step_idx, lds, s);
+    }
 }
 
 hipError_t launch_energy_final(const double* part, int64_t nparts, double* part2, double* out2, hipStream_t s)
@@ -623,20 +843,25 @@ hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, 
 {
     if (tv.ntl == 0) return hipSuccess;
     size_t lds = stage_lds_bytes(tv, !tv.uniform_ecrit);
-    if (precision == PDM_FP64) {
-        if (tv.uniform_ecrit) return launch_forces_t<PDM_FP64, true>(tv, u, F, elast, lds, s);
-        return launch_forces_t<PDM_FP64, false>(tv, u, F, elast, lds, s);
+    switch (precision) {
+    case PDM_FP64: return launch_gather_u<OpF64>(tv, u, F, elast, lds, s);
+    case PDM_FP32C: return launch_gather_u<OpF32>(tv, u, F, elast, lds, s);
+    default: return launch_gather_u<OpF32TF>(tv, u, F, elast, lds, s);
     }
-    if (tv.uniform_ecrit) return launch_forces_t<PDM_FP32C, true>(tv, u, F, elast, lds, s);
-    return launch_forces_t<PDM_FP32C, false>(tv, u, F, elast, lds, s);
+}
+
+hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s)
+{
+    if (tv.ntl == 0) return hipSuccess;
+    return launch_gather_u<OpD64>(tv, u, y, nullptr, stage_lds_bytes(tv, !tv.uniform_ecrit), s);
 }
 
 hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s)
 {
     if (tv.ntl == 0) return hipSuccess;
-    size_t lds = stage_lds_bytes(tv, false);
-    if (precision == PDM_FP64) return launch_tangent_t<PDM_FP64>(tv, u, y, lds, s);
-    return launch_tangent_t<PDM_FP32C>(tv, u, y, lds, s);
+    size_t lds = stage_lds_bytes(tv, !tv.uniform_ecrit);
+    if (precision == PDM_FP64) return launch_gather_u<OpT64>(tv, u, y, nullptr, lds, s);
+    return launch_gather_u<OpT32>(tv, u, y, nullptr, lds, s);
 }
 
 }  // namespace pdm

@@ -297,6 +297,8 @@ TileView tile_view(const pdm_handle* h)
     v.halo_max = h->halo_max;
     v.tile_max = h->tile_max;
     v.xcd_remap = 1;
+    v.waves = h->waves;
+    v.fam_per_wave = h->fam_per_wave;
     v.uniform_ecrit = h->uniform_ecrit;
     v.ecrit0 = h->deck.mats.empty() ? 0.0 : h->deck.mats[0].ecrit;
     v.inv_vol_cube = 1.0 / (h->deck.dx * h->deck.dx * h->deck.dx);
@@ -462,6 +464,15 @@ int build_tiles(pdm_handle* h)
         return PDM_E_LIMIT;
     }
     h->halo_max = (int)hmax;
+    {
+        TileView tv = tile_view(h);
+        size_t need = stage_lds_bytes(tv, !h->uniform_ecrit);
+        if (need > kLdsBytes) {
+            h->err = "tile LDS footprint " + std::to_string(need) + " B exceeds " + std::to_string(kLdsBytes) +
+                     " B; use a smaller tile_edge";
+            return PDM_E_LIMIT;
+        }
+    }
     HIPCHK(hipMalloc(&h->halo, sizeof(int32_t) * std::max<int64_t>(h->halo_total, 1)));
     ha.halo_ptr = h->halo_ptr;
     ha.halo = h->halo;
