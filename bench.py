@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
-"""Benchmark: bond-updates/s of the RK4 dynamic solve on one MI355X.
+"""Benchmark: bond-updates/s of the RK4 dynamic solve on one MI355X, plus the
+static-solve wall time.
 
-Workload (default): the 200 x 40 x 40 bar (320,000 nodes, dx = 1 mm,
+RK4 workload (default): the 200 x 40 x 40 bar (320,000 nodes, dx = 1 mm,
 delta = 3.015 dx, 73,728,544 family entries), node jitter U(-0.01, 0.01) dx
 from seed 0, E = 72 GPa, rho = 2440, x = 0 face fixed, 1 MPa +x traction on
 the x = L layer, RK4 at 0.8 x the stable dt.  One step = 4 stages, each a
@@ -12,20 +13,35 @@ Both paths run the same K steps after W warm-up steps: the FP64 path
 FP32-compensated rate; the line also carries the FP64 rate and the max
 relative displacement error of FP32C vs FP64 after the K + W steps.
 Inputs are resident in HBM before the timed region (pdm_init builds them
-on the GPU); the timed region is pdm_step_rk4 (hipGraph replays).
+on the GPU); the timed region is pdm_step_rk4 (hipGraph replays of the
+four fused stage kernels), synchronised on both sides.
 
-roofline: dominant kernel = the fused bond-force / RK4-stage kernel; its
-average launch time comes from HIP events on the library's stream around
-the timed region (pdm_last_kernel_timing); algorithmic bytes per launch =
-E x (4 B col + coefficient bytes) + N x 288 B per-node state traffic.
+Static solve: the same bar with one planar edge crack from the y = min face
+to the mid-plane, normal at 30 degrees to x (a crack through the whole
+section would leave the far half unsupported); matrix-free Jacobi-PCG to 1e-10
+relative residual, FP32 matvecs with FP64 refinement (and the FP64 solve for
+the error figure).  Wall seconds of pdm_solve_static.
 
-cpu_baseline: the oracle (faithful serial restatement of the reference's
-RK4 loop) on the same bar for one step, 1 thread, on this host.
+roofline: dominant kernel = the fused bond-force / RK4-stage kernel.  Its
+average launch time comes from HIP events recorded on the library's stream
+around the timed region (pdm_last_kernel_timing).  Algorithmic bytes per
+launch use the per-unit figures of the bond-update unit: 8 B per family
+entry for FP32 coefficients (int32 neighbour + FP32 coefficient; 12 B with
+FP64 coefficients) plus 264 B per node (trial u, X, Vbuf: 72 B, and the
+fused stage update, 8 vectors of 24 B).  The library's own format is smaller
+(uint16 LDS slot + coefficient: 6 B / 10 B per entry); `format_bytes` gives
+that figure.  `traffic` is FETCH_SIZE + WRITE_SIZE per launch from rocprofv3
+counter runs on this workload, scaled by a plain copy kernel's readings, as
+recorded under profiles/ (null for any other workload).
+
+cpu_baseline: the oracle (a serial restatement of the reference's RK4 loop)
+on the same bar, a few steps on 1 thread of this host.
 """
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -33,21 +49,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 288  # per node per stage: NodeRec read 64 + row_ptr 16 + mat/bc/nb 12 + Vb/V r+w 64
-#                  + U r/w 64 + k r/w 64 + next-stage displacement write 32, rounded to 288
+ENTRY_BYTES = {"fp32c": 8, "fp64": 12}        # int32 neighbour + coefficient
+FORMAT_ENTRY_BYTES = {"fp32c": 6, "fp64": 10}  # uint16 slot + coefficient (this library's format)
+NODE_BYTES = 72 + 8 * 24                       # trial u, X, Vbuf + fused stage update (8 x 24 B)
+DEFAULT_DIMS = (200, 40, 40)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_bar200x40x40_traffic.txt")
 
 
-def workload_spec(args):
+def workload_spec(args, steps):
     import pdmtel_amd as pm
 
-    total = args.warmup + args.steps + 10
-    spec = pm.bar_spec(args.nx, args.ny, args.nz, jitter=0.01, seed=0, steps=total)
-    return spec
+    return pm.bar_spec(args.nx, args.ny, args.nz, jitter=0.01, seed=0, steps=steps)
 
 
 def run_precision(args, spec, precision):
-    import numpy as np
-
     import pdmtel_amd as pm
 
     text = pm.deck_text(spec)
@@ -57,7 +72,7 @@ def run_precision(args, spec, precision):
     if args.warmup:
         prob.step(args.warmup)
     t0 = time.perf_counter()
-    done = prob.step(args.steps)
+    done = prob.step(args.steps)  # returns after the stream has drained
     t1 = time.perf_counter()
     assert done == args.steps, f"ran {done} of {args.steps} steps"
     ms_kernel, launches = prob.kernel_timing()
@@ -68,7 +83,31 @@ def run_precision(args, spec, precision):
     return res
 
 
-def cpu_baseline(spec, steps=1):
+def run_static(args):
+    import numpy as np
+
+    import pdmtel_amd as pm
+
+    spec = workload_spec(args, 1)
+    spec["slits"] = [pm.mid_slit(spec, 30.0, edge=True)]
+    text = pm.deck_text(spec)
+    out = {}
+    for name, prec in (("fp32c", pm.PDM_FP32C), ("fp64", pm.PDM_FP64)):
+        prob = pm.Problem(text, precision=prec)
+        t0 = time.perf_counter()
+        st = prob.solve_static(rel_tol=1e-10, max_iterations=200000)
+        wall = time.perf_counter() - t0
+        U, _ = prob.state()
+        out[name] = dict(wall=wall, iterations=st.iterations, refinements=st.refinements,
+                         rel_residual=st.rel_residual, U=U)
+        print(f"# static {name}: {wall:.3f} s, {st.iterations} iterations, {st.refinements} refinements, "
+              f"residual {st.rel_residual:.2e}", file=sys.stderr, flush=True)
+        prob.close()
+    err = float(np.abs(out["fp32c"]["U"] - out["fp64"]["U"]).max() / np.abs(out["fp64"]["U"]).max())
+    return out, err
+
+
+def cpu_baseline(spec, steps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
@@ -76,7 +115,18 @@ def cpu_baseline(spec, steps=1):
     t0 = time.perf_counter()
     o.run(steps)
     dt = time.perf_counter() - t0
-    return dict(value=4.0 * o.E * steps / dt, seconds=dt, E=o.E, n=o.n)
+    return dict(value=4.0 * o.E * steps / dt, seconds=dt, E=o.E, n=o.n, steps=steps)
+
+
+def recorded_traffic(args, prec):
+    """FETCH_SIZE + WRITE_SIZE bytes per stage launch recorded for the default workload, else None."""
+    if (args.nx, args.ny, args.nz) != DEFAULT_DIMS or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        m = re.match(r"^traffic_bytes_per_launch\s+(\w+)\s+([0-9.eE+]+)", line)
+        if m and m.group(1) == prec:
+            return float(m.group(2))
+    return None
 
 
 def calib_copy(nbytes):
@@ -98,18 +148,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--nx", type=int, default=200)
-    ap.add_argument("--ny", type=int, default=40)
-    ap.add_argument("--nz", type=int, default=40)
+    ap.add_argument("--nx", type=int, default=DEFAULT_DIMS[0])
+    ap.add_argument("--ny", type=int, default=DEFAULT_DIMS[1])
+    ap.add_argument("--nz", type=int, default=DEFAULT_DIMS[2])
     ap.add_argument("--chunk", type=int, default=50)
     ap.add_argument("--precision", choices=["both", "fp64", "fp32c"], default="both")
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-static", action="store_true")
     ap.add_argument("--calib-copy", type=int, default=0, help="run only a copy kernel of this many bytes")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
+        print(f"bench.py: this library runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path is not sharded across GPUs)", file=sys.stderr)
         sys.exit(2)
     if args.calib_copy:
         calib_copy(args.calib_copy)
@@ -119,7 +171,7 @@ def main():
 
     import pdmtel_amd as pm
 
-    spec = workload_spec(args)
+    spec = workload_spec(args, args.warmup + args.steps + 10)
     res = {}
     if args.precision in ("both", "fp64"):
         res["fp64"] = run_precision(args, spec, pm.PDM_FP64)
@@ -130,12 +182,12 @@ def main():
         res["fp32c"] = run_precision(args, spec, pm.PDM_FP32C)
         print(f"# fp32c: {res['fp32c']['seconds']:.3f} s for {args.steps} steps, "
               f"kernel {res['fp32c']['kernel_ms']:.4f} ms/launch", file=sys.stderr, flush=True)
-    head = res.get("fp32c") or res["fp64"]
     head_prec = "fp32c" if "fp32c" in res else "fp64"
+    head = res[head_prec]
     E, n = head["E"], head["n"]
     rate = 4.0 * E * args.steps / head["seconds"]
-    coef_bytes = 4 if head_prec == "fp32c" else 8
-    bytes_launch = E * (4 + coef_bytes) + n * NODE_BYTES
+    bytes_launch = E * ENTRY_BYTES[head_prec] + n * NODE_BYTES
+    format_bytes = E * FORMAT_ENTRY_BYTES[head_prec] + n * NODE_BYTES
     kernel_s = head["kernel_ms"] * 1e-3
     achieved = bytes_launch / kernel_s / 1e9 if kernel_s > 0 else 0.0
     err = None
@@ -163,6 +215,7 @@ def main():
             "family_entries": E,
         },
         "fp64_value": (4.0 * E * args.steps / res["fp64"]["seconds"]) if "fp64" in res else None,
+        "fp32c_over_fp64": (res["fp64"]["seconds"] / res["fp32c"]["seconds"]) if len(res) == 2 else None,
         "fp32c_max_rel_disp_err_vs_fp64": err,
         "roofline": {
             "bound": "hbm",
@@ -170,22 +223,40 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": recorded_traffic(args, head_prec),
             "kernel": "k_stage (fused bond-force gather + RK4 stage)",
             "kernel_avg_ms": head["kernel_ms"],
+            "launches": head["launches"],
             "bytes_per_launch": bytes_launch,
+            "format_bytes": format_bytes,
         },
         "cpu_baseline": None,
     }
+    if "fp64" in res and head_prec != "fp64":
+        k64 = res["fp64"]["kernel_ms"] * 1e-3
+        b64 = E * ENTRY_BYTES["fp64"] + n * NODE_BYTES
+        line["roofline_fp64"] = {"achieved": b64 / k64 / 1e9, "frac": b64 / k64 / 1e9 / HBM_PEAK_GBS,
+                                 "kernel_avg_ms": res["fp64"]["kernel_ms"], "bytes_per_launch": b64,
+                                 "traffic": recorded_traffic(args, "fp64")}
+    if not args.skip_static:
+        st, serr = run_static(args)
+        line["static_solve"] = {
+            "workload": f"bar {args.nx}x{args.ny}x{args.nz} jitter 0.01dx seed 0, edge crack to the mid-plane, normal 30 deg to x, "
+                        "Jacobi-PCG to 1e-10 relative residual",
+            "wall_s": st["fp32c"]["wall"], "iterations": st["fp32c"]["iterations"],
+            "refinements": st["fp32c"]["refinements"], "rel_residual": st["fp32c"]["rel_residual"],
+            "fp64_wall_s": st["fp64"]["wall"], "fp64_iterations": st["fp64"]["iterations"],
+            "max_rel_disp_err_vs_fp64": serr,
+        }
     if not args.skip_cpu:
-        cb = cpu_baseline(spec, steps=1)
+        cb = cpu_baseline(spec, steps=args.cpu_steps)
         line["cpu_baseline"] = {
             "value": cb["value"],
             "unit": "bond-updates/s",
             "cores": 1,
             "kind": "port",
             "sample": f"oracle serial restatement of the reference RK4 loop, same {cb['n']}-node bar, "
-                      f"1 RK4 step ({4 * cb['E']} bond-updates) in {cb['seconds']:.1f} s",
+                      f"{cb['steps']} RK4 steps ({4 * cb['E'] * cb['steps']} bond-updates) in {cb['seconds']:.1f} s",
         }
     print(json.dumps(line))
 

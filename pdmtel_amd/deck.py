@@ -163,11 +163,23 @@ def bar_spec(nx: int, ny: int, nz: int, dx: float = 1e-3, delta_over_dx: float =
     return spec
 
 
-def mid_slit(spec: Dict[str, Any], angle_deg: float, axis_t=(0.0, 0.0, 1.0)) -> Dict[str, Any]:
-    """A full-section planar slit through the bar centre, normal in the x-y plane at angle_deg to x."""
+def mid_slit(spec: Dict[str, Any], angle_deg: float, axis_t=(0.0, 0.0, 1.0), edge: bool = False) -> Dict[str, Any]:
+    """A planar slit through the bar centre, normal in the x-y plane at angle_deg to x.
+
+    edge=False: the slit crosses the whole section (the bar is cut in two).
+    edge=True: an edge crack, from the centre line out through the y = min face
+    (half of the section), so the specimen stays connected.
+    """
     nx, ny, nz = spec["grid_dimension"]
     dx = spec["grid_spacing"]
     a = math.radians(angle_deg)
-    c = spec["grid_center"]
+    c = list(spec["grid_center"])
     big = 10.0 * max(nx, ny, nz) * dx
-    return dict(type=2, c=tuple(c), n=(math.cos(a), math.sin(a), 0.0), t=tuple(axis_t), half_t=big, half_s=big)
+    n = (math.cos(a), math.sin(a), 0.0)
+    if edge:
+        # in-plane axis w = n x t = (sin a, -cos a, 0) for t = z: points to the y = min side
+        w = (n[1] * axis_t[2] - n[2] * axis_t[1], n[2] * axis_t[0] - n[0] * axis_t[2], n[0] * axis_t[1] - n[1] * axis_t[0])
+        if w[1] > 0:
+            w = tuple(-x for x in w)
+        c = [c[k] + big * w[k] for k in range(3)]
+    return dict(type=2, c=tuple(c), n=n, t=tuple(axis_t), half_t=big, half_s=big)

@@ -153,3 +153,17 @@ def test_static_solve_matches_dense(precision):
     assert st.rel_residual <= 1e-10
     assert err <= 1e-7
     assert np.all(V == 0.0)
+
+
+def test_static_solve_edge_crack_fp32c():
+    """Static solve of a bar with an edge crack (cut bonds): FP32C refinement vs the dense solve."""
+    spec = make("bar_jitter")
+    spec["slits"] = [pm.mid_slit(spec, 30.0, edge=True)]
+    prob, orc = pair(spec, precision=pm.PDM_FP32C)
+    assert np.array_equal(prob.csr()[2], orc.csr()[2]) and prob.csr()[2].sum() > 0
+    st = prob.solve_static(rel_tol=1e-10)
+    U, _ = prob.state()
+    Ud = orc.solve_static_dense()
+    err = np.abs(U - Ud).max() / np.abs(Ud).max()
+    print(f"[static edge crack] iters={st.iterations} res={st.rel_residual:.2e} err={err:.2e}", flush=True)
+    assert st.rel_residual <= 1e-10 and err <= 1e-7
