@@ -128,3 +128,29 @@ def test_deck_writer_roundtrip_values():
     assert list(inf.grid_dimension) == [5, 4, 3]
     assert inf.total_step == 17 and inf.safety_factor == 0.5
     assert inf.num_bcs == 2 and inf.num_boundary_spaces == 2
+
+
+def _run_cli(wkdir, *args):
+    exe = os.path.join(ROOT, "pdmtel_amd", "pdm_run")
+    return subprocess.run([exe, str(wkdir), *args], capture_output=True, text=True, timeout=60)
+
+
+def test_cli_quit_option_and_errors(tmp_path):
+    """pdm_run parses the deck through the C++ facade; option 3 quits like the reference menu."""
+    import shutil
+
+    shutil.copy(os.path.join(GOLDEN, "pds_input_sample.txt"), tmp_path / "pds_input.txt")
+    r = _run_cli(tmp_path, "3")
+    assert r.returncode == 0, r.stderr
+    assert "user quits without doing any calculation!" in (tmp_path / "pds_output.dat").read_text()
+    r = _run_cli(tmp_path / "missing", "1")
+    assert r.returncode == 1 and "code -2" in r.stderr
+    (tmp_path / "pds_input.txt").write_text(raw_without(open(os.path.join(GOLDEN, "pds_input_sample.txt")).read(), "grid_spacing"))
+    r = _run_cli(tmp_path, "1")
+    assert r.returncode == 1 and "code -3" in r.stderr
+
+
+def raw_without(text, keyword):
+    lines = text.splitlines()
+    i = next(k for k, l in enumerate(lines) if l.strip() == keyword)
+    return "\n".join(lines[:i] + lines[i + 2:])
