@@ -15,7 +15,9 @@
 //    then every family gather is an LDS read.
 //    Xi d4[N]      position, .w = ecrit of the node's material
 //    u[2] d4[N]    displacement gathered by the next stage (ping-pong)
-//    rpi int64[N+1], slot uint16[E], coef64 FP64[E] / coef32 FP32[E]
+//    rpi int64[N+1], slot uint16[Ei], coef64 FP64[Ei] / coef32 FP32[Ei]
+//      rows padded to a multiple of kRowAlign entries (Ei >= E); padding entries
+//      point at the tile's padding record with coef 0
 //      coef = bdryEff_ij * dx^3 * beta(r_ij) * c_ij, 0 for a cut bond
 //    U, V, Vb, k1, k23 d4[N]  FP64 RK4 state; mat, bc, nb int32[N]
 #pragma once
@@ -39,6 +41,7 @@ constexpr int kMaxTileNodes = 343;   // nodes per tile (tile edge up to 7 dx)
 constexpr size_t kLdsBytes = 160 * 1024;  // LDS per CU on gfx950
 constexpr int kMaxFlags = 8192;      // nodes in a tile's neighbourhood (halo build)
 constexpr int kMaxHalo = 2560;       // halo entries staged in LDS: 7 x 8 B x 2560 + 32 B x 512 < 160 KB
+constexpr int kRowAlign = 8;          // internal rows padded to whole blocks of 8 entries
 
 struct alignas(32) d4 {
     double x, y, z, w;
@@ -142,6 +145,7 @@ struct pdm_handle {
 
     // host copies (reference order)
     int64_t n = 0, E = 0;
+    int64_t Ei = 0;                 // internal (padded) family entries
     std::vector<double> X;          // 3N AoS
     std::vector<int32_t> mat, bc;
     std::vector<int32_t> perm;      // internal -> reference id

@@ -3,10 +3,11 @@
 // One workgroup per tile.  Phase 1 stages the tile's halo as 48-byte records
 // (position and this stage's displacement, FP64) plus the families' row
 // starts and own slots into LDS with coalesced loads; one extra record far
-// outside the body pads partial chunks.  Phase 2: each 64-lane wavefront works
-// on NPW families at a time, 64/NPW lanes per family; every lane keeps kUnroll
-// entries (a uint16 LDS slot and a coefficient each, streamed from HBM) in
-// flight, and the first chunk of the next round is loaded while the current
+// outside the body backs the row padding.  Phase 2: each 64-lane wavefront works
+// on NPW families at a time, 64/NPW lanes per family; every lane loads one
+// aligned block of B consecutive entries (uint16 LDS slots and coefficients,
+// streamed from HBM as 16-byte vector loads: B = 8 with FP32 coefficients, 4
+// with FP64), and the first block of the next round is loaded while the current
 // one is computed.  The neighbour's record is three ds_read_b128.  Family sums
 // are reduced inside the lane group with DPP moves (quad_perm, row_half_mirror,
 // row_mirror) and the gfx950 permlane16/32 swaps, never through LDS.  Phase 3:
@@ -22,18 +23,19 @@
 //               correctly rounded square.
 //   PDM_FP32C:  xi and eta formed in FP64 and rounded to FP32; the stretch from the
 //               cancellation-free form s = (2 xi.eta + eta.eta) / ((p + r) r); FP32
-//               coefficients; each bond force added to an FP64 family accumulator (an FP64
-//               add is full rate on CDNA4, cheaper than a two-float TwoSum).
+//               coefficients; the bond forces of a lane's block (8 entries) summed in
+//               FP32, each block sum added to an FP64 family accumulator.
 //   PDM_FP32C2: the same bond force with two-float (hi, lo) FP32 family sums, also across
 //               lanes.  The stage update is FP64 in every mode.
 #include <cmath>
+#include <type_traits>
+#include <utility>
 
 #include "pdm_internal.h"
 
 namespace pdm {
 namespace {
 
-constexpr int kUnroll = 4;              // entries per lane in flight
 constexpr double kFar = 1.0e3;          // coordinate of the padding record (m)
 
 __device__ __forceinline__ int64_t launch_index(int remap)
@@ -277,7 +279,8 @@ __device__ __forceinline__ Bond32 bond32(const Lds& l, int s, float c, const Sel
         float ej = (float)l.ec[s];
         ecrit = (ej < ecrit) ? ej : ecrit;
     }
-    const bool ok = u <= 0.0f || st <= ecrit;
+    // u <= 0 || s <= ecrit, with u = s r and r > 0
+    const bool ok = st <= __builtin_fmaxf(ecrit, 0.0f);
     const float cs = ok ? c * st : 0.0f;
     Bond32 b;
     b.g = cs * pinv;
@@ -288,20 +291,28 @@ __device__ __forceinline__ Bond32 bond32(const Lds& l, int s, float c, const Sel
     return b;
 }
 
-struct OpF32 {  // PDM_FP32C: FP32 bond force, FP64 family accumulator
+struct OpF32 {  // PDM_FP32C: FP32 bond force, FP32 block sums, FP64 family accumulator
     using Coef = float;
     struct Acc {
         double f0, f1, f2;
+        float p0, p1, p2;
         float el;
     };
     template <bool ENERGY, bool UE>
     __device__ static void bond(const Lds& l, int s, float c, const Self& me, Acc& a)
     {
         const Bond32 b = bond32<ENERGY, UE>(l, s, c, me);
-        a.f0 += (double)(b.g * b.a1);
-        a.f1 += (double)(b.g * b.a2);
-        a.f2 += (double)(b.g * b.a3);
+        a.p0 = __builtin_fmaf(b.g, b.a1, a.p0);
+        a.p1 = __builtin_fmaf(b.g, b.a2, a.p1);
+        a.p2 = __builtin_fmaf(b.g, b.a3, a.p2);
         if (ENERGY) a.el += b.e;
+    }
+    __device__ static void flush(Acc& a)
+    {
+        a.f0 += (double)a.p0;
+        a.f1 += (double)a.p1;
+        a.f2 += (double)a.p2;
+        a.p0 = a.p1 = a.p2 = 0.0f;
     }
     template <int LPN>
     __device__ static void reduce(Acc& a)
@@ -390,7 +401,15 @@ struct OpT32 {
     using Coef = float;
     struct Acc {
         double f0, f1, f2, el;
+        float p0, p1, p2;
     };
+    __device__ static void flush(Acc& a)
+    {
+        a.f0 += (double)a.p0;
+        a.f1 += (double)a.p1;
+        a.f2 += (double)a.p2;
+        a.p0 = a.p1 = a.p2 = 0.0f;
+    }
     template <bool ENERGY, bool UE>
     __device__ static void bond(const Lds& l, int s, float c, const Self& me, Acc& a)
     {
@@ -400,14 +419,16 @@ struct OpT32 {
         const float r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
         const float rinv = __builtin_amdgcn_rsqf(r2);
         const float proj = __builtin_fmaf(x3, d2, __builtin_fmaf(x2, d1, x1 * d0)) * c * (rinv * rinv * rinv);
-        a.f0 += (double)(proj * x1);
-        a.f1 += (double)(proj * x2);
-        a.f2 += (double)(proj * x3);
+        a.p0 = __builtin_fmaf(proj, x1, a.p0);
+        a.p1 = __builtin_fmaf(proj, x2, a.p1);
+        a.p2 = __builtin_fmaf(proj, x3, a.p2);
     }
     template <int LPN>
     __device__ static void reduce(Acc& a)
     {
-        OpT64::reduce<LPN>(reinterpret_cast<OpT64::Acc&>(a));
+        a.f0 = group_sum<LPN>(a.f0);
+        a.f1 = group_sum<LPN>(a.f1);
+        a.f2 = group_sum<LPN>(a.f2);
     }
     __device__ static void store(const Acc& a, double, double* out)
     {
@@ -449,25 +470,73 @@ struct OpD64 {
     }
 };
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// Operations without FP32 partial sums have nothing to flush.
+template <class OP, class = void>
+struct HasFlush {
+    static constexpr bool value = false;
+};
+template <class OP>
+struct HasFlush<OP, decltype(OP::flush(std::declval<typename OP::Acc&>()))> {
+    static constexpr bool value = true;
+};
+template <class OP>
+__device__ __forceinline__ void flush(typename OP::Acc& a)
+{
+    if constexpr (HasFlush<OP>::value) OP::flush(a);
+}
+
+// One lane's block: B consecutive family entries.
 template <class OP>
 struct Chunk {
-    int s[kUnroll];
-    typename OP::Coef c[kUnroll];
+    static constexpr int B = sizeof(typename OP::Coef) == 8 ? 4 : 8;
+    int s[B];
+    typename OP::Coef c[B];
 };
 
-// kUnroll entries per lane, stride LPN; entries past the row point at the padding
-// record with coefficient 0.
-template <class OP, int LPN>
-__device__ __forceinline__ void load_chunk(const TileView& tv, int64_t base, int64_t e1, int sl, int pad,
-                                           Chunk<OP>& ch)
+// The block starting at entry e (a multiple of B; rows are whole blocks of
+// kRowAlign entries), or, at or past the row end e1, B references to the
+// padding record with coefficient 0.  Slots: one 16- or 8-byte load; coefficients:
+// two 16-byte loads.
+template <class OP>
+__device__ __forceinline__ void load_block(const TileView& tv, int64_t e, int64_t e1, int pad, Chunk<OP>& ch)
 {
+    constexpr int B = Chunk<OP>::B;
+    static_assert(kRowAlign % B == 0, "rows must be whole blocks");
+    if (e < e1) {
+        if constexpr (B == 8) {
+            const u32x4 sv = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(tv.slot + e));
+            const f32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e));
+            const f32x4 c1 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e + 4));
 #pragma unroll
-    for (int k = 0; k < kUnroll; ++k) {
-        const int64_t e = base + sl + LPN * k;
-        const bool v = e < e1;
-        ch.s[k] = v ? (int)__builtin_nontemporal_load(&tv.slot[e]) : pad;
-        if constexpr (sizeof(typename OP::Coef) == 8) ch.c[k] = v ? __builtin_nontemporal_load(&tv.coef64[e]) : 0.0;
-        else ch.c[k] = v ? __builtin_nontemporal_load(&tv.coef32[e]) : 0.0f;
+            for (int k = 0; k < 4; ++k) {
+                ch.s[2 * k] = (int)(sv[k] & 0xFFFFu);
+                ch.s[2 * k + 1] = (int)(sv[k] >> 16);
+                ch.c[k] = c0[k];
+                ch.c[4 + k] = c1[k];
+            }
+        } else {
+            const u32x2 sv = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(tv.slot + e));
+            const f64x2 c0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(tv.coef64 + e));
+            const f64x2 c1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(tv.coef64 + e + 2));
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                ch.s[2 * k] = (int)(sv[k] & 0xFFFFu);
+                ch.s[2 * k + 1] = (int)(sv[k] >> 16);
+                ch.c[k] = c0[k];
+                ch.c[2 + k] = c1[k];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            ch.s[k] = pad;
+            ch.c[k] = 0;
+        }
     }
 }
 
@@ -478,7 +547,8 @@ template <class OP, bool ENERGY, bool UE, int NW, int NPW>
 __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, int nt)
 {
     constexpr int LPN = 64 / NPW;
-    constexpr int CH = LPN * kUnroll;
+    constexpr int B = Chunk<OP>::B;
+    constexpr int CH = LPN * B;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane / LPN, sl = lane % LPN;
     const double ivc4 = 0.25 * tv.inv_vol_cube;
@@ -487,7 +557,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, 
     int loc = first + g;
     int64_t e0 = (loc < nt) ? l.row[loc] : 0, e1 = (loc < nt) ? l.row[loc + 1] : 0;
     Chunk<OP> cur;
-    load_chunk<OP, LPN>(tv, e0, e1, sl, l.pad, cur);
+    load_block<OP>(tv, e0 + B * sl, e1, l.pad, cur);
     for (;;) {
         const int nfirst = first + NW * NPW;
         const int nloc = nfirst + g;
@@ -498,19 +568,21 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, 
                 n0 = l.row[nloc];
                 n1 = l.row[nloc + 1];
             }
-            load_chunk<OP, LPN>(tv, n0, n1, sl, l.pad, nxt);
+            load_block<OP>(tv, n0 + B * sl, n1, l.pad, nxt);
         }
         const int ss = (loc < nt) ? l.self[loc] : l.pad;
         const Rec ri = l.rec[ss];
         const Self me{ri.x, ri.y, ri.z, ri.u, ri.v, ri.w, UE ? tv.ecrit0 : l.ec[ss]};
         typename OP::Acc acc{};
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k) OP::template bond<ENERGY, UE>(l, cur.s[k], cur.c[k], me, acc);
+        for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, cur.s[k], cur.c[k], me, acc);
+        flush<OP>(acc);
         for (int64_t base = e0 + CH; base < e1; base += CH) {
             Chunk<OP> more;
-            load_chunk<OP, LPN>(tv, base, e1, sl, l.pad, more);
+            load_block<OP>(tv, base + B * sl, e1, l.pad, more);
 #pragma unroll
-            for (int k = 0; k < kUnroll; ++k) OP::template bond<ENERGY, UE>(l, more.s[k], more.c[k], me, acc);
+            for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, more.s[k], more.c[k], me, acc);
+            flush<OP>(acc);
         }
         OP::template reduce<LPN>(acc);
         if (sl == 0 && loc < nt) OP::store(acc, ivc4, l.res + 4 * loc);

@@ -5,7 +5,10 @@
 // CSR is the reference family mapped to internal ids and sorted.  For every
 // tile a halo list holds the internal ids of the tile's nodes and all their
 // neighbours, sorted; a family entry stores the uint16 position of its
-// neighbour in that list.  The halo of a tile lies in the (2R+1)^3 block of
+// neighbour in that list.  Rows are padded to a multiple of kRowAlign entries
+// (padding entries point at the tile's padding record with a zero coefficient) so
+// that a lane can load a whole aligned block of slots and coefficients at once.
+// The halo of a tile lies in the (2R+1)^3 block of
 // tiles around it, R = ceil(reach / edge) per axis, reach = delta + dx/2 plus
 // a margin: an accepted pair has dmin < delta + 1e-4 dx, and dmin is at least
 // the axis distance minus half a cell minus the face tolerance.
@@ -54,7 +57,9 @@ __global__ void k_permute_nodes(const int32_t* __restrict__ perm, const d4* __re
     bei[a] = be_ref[i];
 }
 
-// packed[e'] = internal col << 1 | cut, rows in internal order.
+constexpr uint32_t kPadEntry = 0xFFFFFFFFu;  // row padding (sorts last)
+
+// packed[e'] = internal col << 1 | cut, rows in internal order, padded with kPadEntry.
 __global__ void k_rows_internal(const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm,
                                 const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
                                 const uint8_t* __restrict__ cut, const int64_t* __restrict__ rpi, int64_t n,
@@ -64,9 +69,9 @@ __global__ void k_rows_internal(const int32_t* __restrict__ perm, const int32_t*
     int lane = threadIdx.x % 64;
     if (a >= n) return;
     int32_t i = perm[a];
-    int64_t b = rp[i], len = rp[i + 1] - b, o = rpi[a];
-    for (int64_t k = lane; k < len; k += 64)
-        packed[o + k] = ((uint32_t)iperm[col[b + k]] << 1) | (uint32_t)cut[b + k];
+    int64_t b = rp[i], len = rp[i + 1] - b, o = rpi[a], plen = rpi[a + 1] - o;
+    for (int64_t k = lane; k < plen; k += 64)
+        packed[o + k] = (k < len) ? (((uint32_t)iperm[col[b + k]] << 1) | (uint32_t)cut[b + k]) : kPadEntry;
 }
 
 __global__ void k_sort_rows_u32(const int64_t* __restrict__ rpi, uint32_t* __restrict__ v, int64_t n)
@@ -173,7 +178,9 @@ __global__ __launch_bounds__(256) void k_halo(HaloArgs a)
     for (int64_t q = a0 + threadIdx.x; q < a1; q += blockDim.x) flag[nb_off[own] + (int)(q - a0)] = 1;
     const int64_t e0 = a.rpi[a0], e1 = a.rpi[a1];
     for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-        int32_t c = (int32_t)(a.packed[e] >> 1);
+        const uint32_t pk = a.packed[e];
+        if (pk == kPadEntry) continue;
+        int32_t c = (int32_t)(pk >> 1);
         int32_t tj = a.tileid[c];
         int64_t jx = tj % ntx, jy = (tj / ntx) % nty, jz = tj / (ntx * nty);
         int64_t dx = jx - tx, dy = jy - ty, dz = jz - tz;
@@ -221,7 +228,12 @@ __global__ __launch_bounds__(256) void k_halo(HaloArgs a)
     }
     for (int64_t q = a0 + threadIdx.x; q < a1; q += blockDim.x) a.self_slot[q] = pre[nb_off[own] + (int)(q - a0)];
     for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-        int32_t c = (int32_t)(a.packed[e] >> 1);
+        const uint32_t pk = a.packed[e];
+        if (pk == kPadEntry) {
+            a.slot[e] = (uint16_t)H;  // the padding record
+            continue;
+        }
+        int32_t c = (int32_t)(pk >> 1);
         int32_t tj = a.tileid[c];
         int64_t jx = tj % ntx, jy = (tj / ntx) % nty, jz = tj / (ntx * nty);
         int g = (int)((jx - tx + Rx) + gx * ((jy - ty + Ry) + gy * (jz - tz + Rz)));
@@ -259,6 +271,11 @@ __global__ void k_coef(CoefArgs a)
     int cnt = 0;
     for (int64_t e = a.rpi[i] + lane; e < a.rpi[i + 1]; e += 64) {
         uint32_t pk = a.packed[e];
+        if (pk == kPadEntry) {
+            a.coef64[e] = 0.0;
+            a.coef32[e] = 0.0f;
+            continue;
+        }
         int32_t j = (int32_t)(pk >> 1);
         bool cut = (pk & 1u) != 0;
         d4 xj = a.Xi[j];
@@ -309,7 +326,7 @@ int build_tiles(pdm_handle* h)
 {
     const Deck& d = h->deck;
     hipStream_t s = h->stream;
-    const int64_t n = h->n, E = h->E;
+    const int64_t n = h->n;
     // tile grid over the node bounding box; bins start half a spacing below the lowest node
     double lo[3], hi[3];
     for (int c = 0; c < 3; ++c) { lo[c] = 1e300; hi[c] = -1e300; }
@@ -376,7 +393,12 @@ int build_tiles(pdm_handle* h)
     // internal row pointers
     std::vector<int64_t> rph(n + 1), rpih(n + 1, 0);
     HIPCHK(hipMemcpy(rph.data(), h->rp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
-    for (int64_t a = 0; a < n; ++a) rpih[a + 1] = rpih[a] + (rph[h->perm[a] + 1] - rph[h->perm[a]]);
+    for (int64_t a = 0; a < n; ++a) {
+        const int64_t len = rph[h->perm[a] + 1] - rph[h->perm[a]];
+        rpih[a + 1] = rpih[a] + (len + kRowAlign - 1) / kRowAlign * kRowAlign;
+    }
+    const int64_t Ei = rpih[n];
+    h->Ei = Ei;
     // uniform ecrit across materials?
     h->uniform_ecrit = 1;
     for (size_t m = 1; m < d.mats.size(); ++m)
@@ -387,7 +409,7 @@ int build_tiles(pdm_handle* h)
     uint32_t* packed = nullptr;
     int64_t* halo_len = nullptr;
     int* err = nullptr;
-    const int64_t n1 = std::max<int64_t>(n, 1), E1 = std::max<int64_t>(E, 1);
+    const int64_t n1 = std::max<int64_t>(n, 1), E1 = std::max<int64_t>(Ei, 1);
     HIPCHK(hipMalloc(&h->permd, sizeof(int32_t) * n1));
     HIPCHK(hipMalloc(&ipermd, sizeof(int32_t) * n1));
     HIPCHK(hipMalloc(&tileidd, sizeof(int32_t) * n1));
