@@ -410,7 +410,7 @@ int pdm_init(pdm_handle* h)
                         "halo_mean=%.1f lds=%zu B\n",
                 (long long)h->n, (long long)h->E, (long long)h->ntl, h->tile_edge[0], h->tile_edge[1],
                 h->tile_edge[2], h->tile_max, h->halo_max, h->ntl ? (double)h->halo_total / (double)h->ntl : 0.0,
-                stage_lds_bytes(tile_view(h), !h->uniform_ecrit));
+                stage_lds_bytes(tile_view(h), !h->uniform_ecrit, true));
     int64_t n = std::max<int64_t>(h->n, 1);
     for (d4** p : {&h->U, &h->V, &h->Vb, &h->k1, &h->k23}) {
         HCHK(hipMalloc(p, sizeof(d4) * n));

@@ -231,7 +231,7 @@ hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream
 
 // static solve (pdm_cg.cpp / pdm_cg.hip)
 int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_stats* stats);
-size_t stage_lds_bytes(const TileView& tv, bool with_ecrit);
+size_t stage_lds_bytes(const TileView& tv, bool with_ecrit, bool with_epi);
 int64_t energy_part2_size(int64_t nparts);
 
 }  // namespace pdm

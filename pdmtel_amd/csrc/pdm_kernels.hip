@@ -28,6 +28,7 @@
 //   PDM_FP32C2: the same bond force with two-float (hi, lo) FP32 family sums, also across
 //               lanes.  The stage update is FP64 in every mode.
 #include <cmath>
+#include <cstddef>
 #include <type_traits>
 #include <utility>
 
@@ -154,18 +155,38 @@ struct alignas(16) Rec {
     double x, y, z, u, v, w;
 };
 
+// Stage-update inputs of one tile node, loaded while the halo is staged.
+struct EpiRec {
+    double v[5][3];  // velocity source (V or Vb), U, k1, k23, V: as the stage needs them
+    double el;       // elastic energy density of the earlier stages
+    int m, b, nb, pad;
+};
+
 // H + 1 records (the last one is the padding record), optional per-record ecrit,
-// per-node results (4 doubles), row starts (nt + 1 int64), own slots (nt ints).
+// per-node results (4 doubles), row starts (nt + 1 int64), own slots (nt ints);
+// for the stage kernels also the stage-update inputs and the material / BC tables.
 struct Lds {
     Rec* rec;
     double* ec;
     double* res;
     int64_t* row;
     int* self;
+    EpiRec* epi;
+    MatDev* mats;
+    BCDev* bcs;
     int pad;  // slot of the padding record
 };
 
-__device__ __forceinline__ Lds lds_view(double* base, int H, int tile_max, bool with_ecrit)
+__host__ __device__ inline size_t lds_bytes(int H, int tile_max, bool with_ecrit, bool with_epi)
+{
+    size_t b = sizeof(double) * ((size_t)(with_ecrit ? 7 : 6) * (H + 1) + 4 * (size_t)tile_max) +
+               sizeof(int64_t) * ((size_t)tile_max + 1) + sizeof(int) * (size_t)tile_max;
+    b = (b + 15) / 16 * 16;
+    if (with_epi) b += sizeof(EpiRec) * (size_t)tile_max + sizeof(MatDev) * kMaxMat + sizeof(BCDev) * kMaxBC;
+    return b;
+}
+
+__device__ __forceinline__ Lds lds_view(double* base, int H, int tile_max, bool with_ecrit, bool with_epi = false)
 {
     Lds l;
     l.rec = reinterpret_cast<Rec*>(base);
@@ -175,6 +196,10 @@ __device__ __forceinline__ Lds lds_view(double* base, int H, int tile_max, bool 
     l.res = p;
     l.row = reinterpret_cast<int64_t*>(l.res + 4 * tile_max);
     l.self = reinterpret_cast<int*>(l.row + tile_max + 1);
+    char* q = reinterpret_cast<char*>(base) + lds_bytes(H, tile_max, with_ecrit, false);
+    l.epi = with_epi ? reinterpret_cast<EpiRec*>(q) : nullptr;
+    l.mats = with_epi ? reinterpret_cast<MatDev*>(q + sizeof(EpiRec) * (size_t)tile_max) : nullptr;
+    l.bcs = with_epi ? reinterpret_cast<BCDev*>(reinterpret_cast<char*>(l.mats) + sizeof(MatDev) * kMaxMat) : nullptr;
     l.pad = H;
     return l;
 }
@@ -182,12 +207,19 @@ __device__ __forceinline__ Lds lds_view(double* base, int H, int tile_max, bool 
 __device__ __forceinline__ void stage_tile(const TileView& tv, const d4* __restrict__ u, int64_t h0, int H,
                                            int64_t a0, int nt, const Lds& l, bool with_ecrit)
 {
-    for (int q = threadIdx.x; q < H; q += blockDim.x) {
-        int32_t id = tv.halo[h0 + q];
-        d4 x = tv.X[id];
-        d4 v = u[id];
-        l.rec[q] = Rec{x.x, x.y, x.z, v.x, v.y, v.z};
-        if (with_ecrit) l.ec[q] = x.w;
+    // two records per thread per pass, both id loads before the record loads
+    for (int q0 = threadIdx.x; q0 < H; q0 += 2 * blockDim.x) {
+        const int q1 = q0 + blockDim.x;
+        const int32_t id0 = tv.halo[h0 + q0];
+        const int32_t id1 = (q1 < H) ? tv.halo[h0 + q1] : id0;
+        const d4 x0 = tv.X[id0], v0 = u[id0];
+        const d4 x1 = tv.X[id1], v1 = u[id1];
+        l.rec[q0] = Rec{x0.x, x0.y, x0.z, v0.x, v0.y, v0.z};
+        if (with_ecrit) l.ec[q0] = x0.w;
+        if (q1 < H) {
+            l.rec[q1] = Rec{x1.x, x1.y, x1.z, v1.x, v1.y, v1.z};
+            if (with_ecrit) l.ec[q1] = x1.w;
+        }
     }
     if (threadIdx.x == 0) {
         l.rec[H] = Rec{kFar, kFar, kFar, 0.0, 0.0, 0.0};
@@ -620,6 +652,41 @@ struct OpFor<PDM_FP32C2> {
     using Tangent = OpT32;
 };
 
+// Stage-update inputs of internal node i as stage STAGE needs them.
+template <int STAGE, bool ENERGY>
+struct EpiLoad {
+    d4 v0, v1, v2, v3, v4;
+    double el;
+    int m, b, nb;
+    __device__ __forceinline__ void load(const StageArgs& a, int64_t i)
+    {
+        v0 = (STAGE == 0) ? a.V[i] : a.Vb[i];
+        v1 = a.U[i];
+        if (STAGE == 3) v2 = a.k1[i];
+        if (STAGE >= 2) v3 = a.k23[i];
+        if (STAGE == 3) v4 = a.V[i];
+        if (ENERGY && STAGE > 0) el = a.elast[i];
+        m = a.mat[i];
+        b = a.bc[i];
+        nb = a.nb[i];
+    }
+    __device__ __forceinline__ void store(EpiRec& e) const
+    {
+        const d4* v[5] = {&v0, &v1, &v2, &v3, &v4};
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            if ((q == 2 && STAGE != 3) || (q == 3 && STAGE < 2) || (q == 4 && STAGE != 3)) continue;
+            e.v[q][0] = v[q]->x;
+            e.v[q][1] = v[q]->y;
+            e.v[q][2] = v[q]->z;
+        }
+        if (ENERGY && STAGE > 0) e.el = el;
+        e.m = m;
+        e.b = b;
+        e.nb = nb;
+    }
+};
+
 // One RK4 stage (STAGE 0..3) for one tile.  Reads a.uin, writes a.uout.
 template <int STAGE, int PREC, bool ENERGY, bool UE, int NW, int NPW>
 __global__ __launch_bounds__(64 * NW) void k_stage(StageArgs a, int step_idx)
@@ -634,8 +701,27 @@ __global__ __launch_bounds__(64 * NW) void k_stage(StageArgs a, int step_idx)
     const int nt = (int)(tv.tile_start[tid + 1] - a0);
     const int64_t h0 = tv.halo_ptr[k];
     const int H = (int)(tv.halo_ptr[k + 1] - h0);
-    const Lds l = lds_view(L, H, tv.tile_max, !UE);
+    const Lds l = lds_view(L, H, tv.tile_max, !UE, true);
+    // the stage-update inputs (first node of each thread) are loaded together with the halo
+    EpiLoad<STAGE, ENERGY> pre;
+    const bool has_pre = (int)threadIdx.x < nt;
+    if (has_pre) pre.load(a, a0 + threadIdx.x);
     stage_tile(tv, a.uin, h0, H, a0, nt, l, !UE);
+    {
+        const double* src = reinterpret_cast<const double*>(a.mats);
+        double* dst = reinterpret_cast<double*>(l.mats);
+        constexpr int nw = (int)((sizeof(MatDev) * kMaxMat + sizeof(BCDev) * kMaxBC) / sizeof(double));
+        static_assert(sizeof(MatDev) % 8 == 0 && sizeof(BCDev) % 8 == 0, "tables are double words");
+        static_assert(offsetof(StageArgs, bcs) == offsetof(StageArgs, mats) + sizeof(MatDev) * kMaxMat,
+                      "bcs follow mats");
+        for (int q = threadIdx.x; q < nw; q += blockDim.x) dst[q] = src[q];
+    }
+    if (has_pre) pre.store(l.epi[threadIdx.x]);
+    for (int loc = threadIdx.x + 64 * NW; loc < nt; loc += 64 * NW) {
+        EpiLoad<STAGE, ENERGY> more;
+        more.load(a, a0 + loc);
+        more.store(l.epi[loc]);
+    }
     __syncthreads();
     tile_families<typename OpFor<PREC>::Force, ENERGY, UE, NW, NPW>(tv, l, nt);
     __syncthreads();
@@ -647,30 +733,33 @@ __global__ __launch_bounds__(64 * NW) void k_stage(StageArgs a, int step_idx)
     for (int loc = threadIdx.x; loc < nt; loc += 64 * NW) {
         const int64_t i = a0 + loc;
         const double* res = l.res + 4 * loc;
-        const int m = a.mat[i];
-        const int b = a.bc[i];
-        const double rho = a.mats[m].rho;
-        const double nbi = (double)a.nb[i];
+        const EpiRec& ep = l.epi[loc];
+        const int m = ep.m;
+        const int b = ep.b;
+        const MatDev& md = l.mats[m];
+        const double rho = md.rho;
+        const double nbi = (double)ep.nb;
         double Ft[3];
-        for (int c = 0; c < 3; ++c) Ft[c] = res[c] + nbi * (a.mats[m].g[c] * rho);
+        for (int c = 0; c < 3; ++c) Ft[c] = res[c] + nbi * (md.g[c] * rho);
         const double eli = res[3];
         bool bcnode = false;
         double fac = 1.0;
         if (b >= 0) {
-            const BCDev& bd = a.bcs[b];
+            const BCDev& bd = l.bcs[b];
             bcnode = bd.bcnode != 0;
             fac = time_factor_dev(bd, sp.t);
             if (bd.type == 4)
                 for (int c = 0; c < 3; ++c) Ft[c] += bd.trac[c] * fac;
         }
-        d4 vb = (STAGE == 0) ? a.V[i] : a.Vb[i];
-        d4 U = a.U[i];
-        double* vbp = &vb.x;
-        double* Up = &U.x;
+        double vbp[3], Up[3];
+        for (int c = 0; c < 3; ++c) {
+            vbp[c] = ep.v[0][c];
+            Up[c] = ep.v[1][c];
+        }
         for (int c = 0; c < 3; ++c) vbp[c] += Ft[c] * dts / rho;
         // UpdateNodeStatusWithBC (pdDataManager.cpp:2497-2602)
         if (b >= 0) {
-            const BCDev& bd = a.bcs[b];
+            const BCDev& bd = l.bcs[b];
             const d4 X = tv.X[i];
             for (int c = 0; c < 3; ++c) {
                 if (bd.f[c] != 1) continue;
@@ -684,51 +773,44 @@ __global__ __launch_bounds__(64 * NW) void k_stage(StageArgs a, int step_idx)
         }
         double kk[3];
         for (int c = 0; c < 3; ++c) kk[c] = vbp[c] * dts;
-        d4 nu;
-        double* nup = &nu.x;
-        nu.w = 0.0;
+        double nup[3];
         if (STAGE == 0) {
             a.k1[i] = d4{kk[0], kk[1], kk[2], 0.0};
             for (int c = 0; c < 3; ++c) nup[c] = bcnode ? Up[c] : Up[c] + 0.5 * kk[c];
-            a.Vb[i] = vb;
+            a.Vb[i] = d4{vbp[0], vbp[1], vbp[2], 0.0};
             if (ENERGY) a.elast[i] = eli;
         } else if (STAGE == 1) {
             a.k23[i] = d4{kk[0], kk[1], kk[2], 0.0};
             for (int c = 0; c < 3; ++c) nup[c] = bcnode ? Up[c] : Up[c] + 0.5 * kk[c];
-            a.Vb[i] = vb;
-            if (ENERGY) a.elast[i] += eli;
+            a.Vb[i] = d4{vbp[0], vbp[1], vbp[2], 0.0};
+            if (ENERGY) a.elast[i] = ep.el + eli;
         } else if (STAGE == 2) {
-            d4 k2 = a.k23[i];
-            a.k23[i] = d4{kk[0] + k2.x, kk[1] + k2.y, kk[2] + k2.z, 0.0};  // k3 += k24 (pdDataManager.cpp:2052)
+            const double* k2 = ep.v[3];
+            a.k23[i] = d4{kk[0] + k2[0], kk[1] + k2[1], kk[2] + k2[2], 0.0};  // k3 += k24 (pdDataManager.cpp:2052)
             for (int c = 0; c < 3; ++c) nup[c] = bcnode ? Up[c] : Up[c] + kk[c];
-            a.Vb[i] = vb;
-            if (ENERGY) a.elast[i] += eli;
+            a.Vb[i] = d4{vbp[0], vbp[1], vbp[2], 0.0};
+            if (ENERGY) a.elast[i] = ep.el + eli;
         } else {
             const double dt6 = 1.0 / 6.0;
-            const d4 k1 = a.k1[i], k23 = a.k23[i], V = a.V[i];
-            const double* k1p = &k1.x;
-            const double* k23p = &k23.x;
-            const double* Vp = &V.x;
+            const double* k1p = ep.v[2];
+            const double* k23p = ep.v[3];
+            const double* Vp = ep.v[4];
             for (int c = 0; c < 3; ++c) {
                 if (bcnode) Up[c] = Up[c] + dt6 * (k1p[c] + kk[c] + 2.0 * k23p[c]);
                 else Up[c] = Up[c] + dt6 * (k1p[c] + kk[c] + 2.0 * k23p[c]) - dt * a.damping * Vp[c] * dt / rho;
             }
             for (int c = 0; c < 3; ++c) vbp[c] += -a.damping * Vp[c] * dt / rho;
-            vb.w = 0.0;
-            a.V[i] = vb;
+            a.V[i] = d4{vbp[0], vbp[1], vbp[2], 0.0};
             for (int c = 0; c < 3; ++c) nup[c] = Up[c];
             if (ENERGY) {
-                const double elt = a.elast[i] + eli;
+                const double elt = ep.el + eli;
                 const double velsq = vbp[0] * vbp[0] + vbp[1] * vbp[1] + vbp[2] * vbp[2];
                 ee_i += a.vol * elt;
                 ke_i += 0.5 * rho * velsq * a.vol;
             }
         }
-        if (b >= 0 || STAGE == 3) {
-            U.w = 0.0;
-            a.U[i] = U;
-        }
-        a.uout[i] = nu;
+        if (b >= 0 || STAGE == 3) a.U[i] = d4{Up[0], Up[1], Up[2], 0.0};
+        a.uout[i] = d4{nup[0], nup[1], nup[2], 0.0};
     }
     if (STAGE == 3 && ENERGY) {
         const int wave = threadIdx.x >> 6;
@@ -880,10 +962,9 @@ hipError_t launch_gather_u(const TileView& tv, const d4* u, d4* y, double* elast
 
 }  // namespace
 
-size_t stage_lds_bytes(const TileView& tv, bool with_ecrit)
+size_t stage_lds_bytes(const TileView& tv, bool with_ecrit, bool with_epi)
 {
-    return sizeof(double) * ((size_t)(with_ecrit ? 7 : 6) * (tv.halo_max + 1) + 4 * (size_t)tv.tile_max) +
-           sizeof(int64_t) * ((size_t)tv.tile_max + 1) + sizeof(int) * (size_t)tv.tile_max;
+    return lds_bytes(tv.halo_max, tv.tile_max, with_ecrit, with_epi);
 }
 
 int64_t energy_part2_size(int64_t nparts) { return nparts / 1024 + 2; }
@@ -891,7 +972,7 @@ int64_t energy_part2_size(int64_t nparts) { return nparts / 1024 + 2; }
 hipError_t launch_stage(int stage, int precision, const StageArgs& a, int step_idx, hipStream_t s)
 {
     if (a.tv.ntl == 0) return hipSuccess;
-    size_t lds = stage_lds_bytes(a.tv, !a.tv.uniform_ecrit);
+    size_t lds = stage_lds_bytes(a.tv, !a.tv.uniform_ecrit, true);
     switch (precision) {
     case PDM_FP64: return launch_stage_p<PDM_FP64>(stage, a, step_idx, lds, s);
     case PDM_FP32C: return launch_stage_p<PDM_FP32C>(stage, a, step_idx, lds, s);
@@ -914,7 +995,7 @@ hipError_t launch_energy_final(const double* part, int64_t nparts, double* part2
 hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, double* elast, hipStream_t s)
 {
     if (tv.ntl == 0) return hipSuccess;
-    size_t lds = stage_lds_bytes(tv, !tv.uniform_ecrit);
+    size_t lds = stage_lds_bytes(tv, !tv.uniform_ecrit, false);
     switch (precision) {
     case PDM_FP64: return launch_gather_u<OpF64>(tv, u, F, elast, lds, s);
     case PDM_FP32C: return launch_gather_u<OpF32>(tv, u, F, elast, lds, s);
@@ -925,13 +1006,13 @@ hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, 
 hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s)
 {
     if (tv.ntl == 0) return hipSuccess;
-    return launch_gather_u<OpD64>(tv, u, y, nullptr, stage_lds_bytes(tv, !tv.uniform_ecrit), s);
+    return launch_gather_u<OpD64>(tv, u, y, nullptr, stage_lds_bytes(tv, !tv.uniform_ecrit, false), s);
 }
 
 hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s)
 {
     if (tv.ntl == 0) return hipSuccess;
-    size_t lds = stage_lds_bytes(tv, !tv.uniform_ecrit);
+    size_t lds = stage_lds_bytes(tv, !tv.uniform_ecrit, false);
     if (precision == PDM_FP64) return launch_gather_u<OpT64>(tv, u, y, nullptr, lds, s);
     return launch_gather_u<OpT32>(tv, u, y, nullptr, lds, s);
 }

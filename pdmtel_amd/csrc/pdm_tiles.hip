@@ -488,7 +488,7 @@ int build_tiles(pdm_handle* h)
     h->halo_max = (int)hmax;
     {
         TileView tv = tile_view(h);
-        size_t need = stage_lds_bytes(tv, !h->uniform_ecrit);
+        size_t need = stage_lds_bytes(tv, !h->uniform_ecrit, true);
         if (need > kLdsBytes) {
             h->err = "tile LDS footprint " + std::to_string(need) + " B exceeds " + std::to_string(kLdsBytes) +
                      " B; use a smaller tile_edge";
