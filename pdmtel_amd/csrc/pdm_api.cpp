@@ -302,7 +302,7 @@ int pdm_destroy(pdm_handle* h)
     dfree(h->Xd); dfree(h->rp); dfree(h->col); dfree(h->cut); dfree(h->be); dfree(h->matd);
     dfree(h->permd); dfree(h->Xi); dfree(h->u[0]); dfree(h->u[1]); dfree(h->rpi); dfree(h->slot);
     dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->mati); dfree(h->bci); dfree(h->nb);
-    dfree(h->tiles); dfree(h->tile_start); dfree(h->halo_ptr); dfree(h->halo);
+    dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr); dfree(h->halo);
     dfree(h->U); dfree(h->V); dfree(h->Vb); dfree(h->k1); dfree(h->k23);
     dfree(h->elast); dfree(h->part); dfree(h->part2); dfree(h->energy); dfree(h->steps);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -410,7 +410,7 @@ int pdm_init(pdm_handle* h)
                         "halo_mean=%.1f lds=%zu B\n",
                 (long long)h->n, (long long)h->E, (long long)h->ntl, h->tile_edge[0], h->tile_edge[1],
                 h->tile_edge[2], h->tile_max, h->halo_max, h->ntl ? (double)h->halo_total / (double)h->ntl : 0.0,
-                stage_lds_bytes(tile_view(h), !h->uniform_ecrit, true));
+                stage_lds_bytes(tile_view(h), !h->uniform_ecrit, true, h->opt.precision != PDM_FP64));
     int64_t n = std::max<int64_t>(h->n, 1);
     for (d4** p : {&h->U, &h->V, &h->Vb, &h->k1, &h->k23}) {
         HCHK(hipMalloc(p, sizeof(d4) * n));

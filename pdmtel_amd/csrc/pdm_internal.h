@@ -84,6 +84,7 @@ struct TileView {
     const double* coef64;
     const float* coef32;
     const d4* X;
+    const d4* tref;            // launch index -> tile centre (reference point of FP32 records)
     int64_t n;
     int64_t ntl;               // launched (non-empty) tiles
     int halo_max;              // largest halo (sizes the LDS staging area)
@@ -174,6 +175,7 @@ struct pdm_handle {
     int32_t *mati = nullptr, *bci = nullptr, *nb = nullptr;
     int32_t* tiles = nullptr;
     int64_t* tile_start = nullptr;
+    pdm::d4* tref = nullptr;            // launch index -> tile centre
     int64_t* halo_ptr = nullptr;
     int32_t* halo = nullptr;
     int64_t ntiles = 0, ntl = 0, halo_total = 0;
@@ -231,7 +233,7 @@ hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream
 
 // static solve (pdm_cg.cpp / pdm_cg.hip)
 int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_stats* stats);
-size_t stage_lds_bytes(const TileView& tv, bool with_ecrit, bool with_epi);
+size_t stage_lds_bytes(const TileView& tv, bool with_ecrit, bool with_epi, bool f32);
 int64_t energy_part2_size(int64_t nparts);
 
 }  // namespace pdm

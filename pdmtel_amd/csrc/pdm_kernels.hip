@@ -97,6 +97,29 @@ __device__ __forceinline__ float perm_xorf(float v)
     return __int_as_float(perm_xor<SWAP16>(__float_as_int(v)));
 }
 
+// v + v(lane ^ 16) (SWAP16) or v + v(lane ^ 32): the swap of v with itself leaves one
+// lane's own value in one result and its partner's in the other, so their sum is the
+// pair sum in every lane, with no select.
+template <bool SWAP16>
+__device__ __forceinline__ double pair_sumd(double v)
+{
+    const int hi = __double2hiint(v), lo = __double2loint(v);
+    auto ph = SWAP16 ? __builtin_amdgcn_permlane16_swap(hi, hi, false, false)
+                     : __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    auto pl = SWAP16 ? __builtin_amdgcn_permlane16_swap(lo, lo, false, false)
+                     : __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    return __hiloint2double(ph[0], pl[0]) + __hiloint2double(ph[1], pl[1]);
+}
+
+template <bool SWAP16>
+__device__ __forceinline__ float pair_sumf(float v)
+{
+    const int b = __float_as_int(v);
+    auto p = SWAP16 ? __builtin_amdgcn_permlane16_swap(b, b, false, false)
+                    : __builtin_amdgcn_permlane32_swap(b, b, false, false);
+    return __int_as_float(p[0]) + __int_as_float(p[1]);
+}
+
 // Sum over aligned groups of LPN lanes (16, 32 or 64); every lane gets its group's sum.
 template <int LPN>
 __device__ __forceinline__ double group_sum(double v)
@@ -105,8 +128,20 @@ __device__ __forceinline__ double group_sum(double v)
     v += dppd<kDppXor2>(v);
     v += dppd<kDppHalfMirror>(v);
     v += dppd<kDppMirror>(v);
-    if (LPN >= 32) v += perm_xord<true>(v);
-    if (LPN >= 64) v += perm_xord<false>(v);
+    if (LPN >= 32) v = pair_sumd<true>(v);
+    if (LPN >= 64) v = pair_sumd<false>(v);
+    return v;
+}
+
+template <int LPN>
+__device__ __forceinline__ float group_sumf(float v)
+{
+    v += dppf<kDppXor1>(v);
+    v += dppf<kDppXor2>(v);
+    v += dppf<kDppHalfMirror>(v);
+    v += dppf<kDppMirror>(v);
+    if (LPN >= 32) v = pair_sumf<true>(v);
+    if (LPN >= 64) v = pair_sumf<false>(v);
     return v;
 }
 
@@ -150,10 +185,21 @@ __device__ __forceinline__ void group_sum_tf(float& h, float& l)
 __device__ __forceinline__ double wave_sum(double v) { return group_sum<64>(v); }
 
 // ------------------------------------------------------------ LDS tile layout
-// Halo record: x, y, z, ux, uy, uz.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// Halo record of the FP64 operations: x, y, z, ux, uy, uz.
 struct alignas(16) Rec {
     double x, y, z, u, v, w;
 };
+
+// Halo record of the FP32 operations: position and displacement relative to the
+// tile's reference point (the tile centre, and the displacement of the tile's middle
+// node), rounded to FP32, in two arrays: (x, y, z, ux) and (uy, uz).  Relative values
+// keep the FP32 rounding of xi and eta at the scale of the halo, not of the body.
 
 // Stage-update inputs of one tile node, loaded while the halo is staged.
 struct EpiRec {
@@ -166,8 +212,11 @@ struct EpiRec {
 // per-node results (4 doubles), row starts (nt + 1 int64), own slots (nt ints);
 // for the stage kernels also the stage-update inputs and the material / BC tables.
 struct Lds {
-    Rec* rec;
-    double* ec;
+    Rec* rec;        // FP64 records
+    f32x4* fa;       // FP32 records
+    f32x2* fb;
+    double* ec;      // per-record ecrit (FP64 records)
+    float* fec;      // per-record ecrit (FP32 records)
     double* res;
     int64_t* row;
     int* self;
@@ -177,75 +226,145 @@ struct Lds {
     int pad;  // slot of the padding record
 };
 
-__host__ __device__ inline size_t lds_bytes(int H, int tile_max, bool with_ecrit, bool with_epi)
+__host__ __device__ inline size_t round16(size_t b) { return (b + 15) / 16 * 16; }
+
+__host__ __device__ inline size_t lds_rec_bytes(int H, bool with_ecrit, bool f32)
 {
-    size_t b = sizeof(double) * ((size_t)(with_ecrit ? 7 : 6) * (H + 1) + 4 * (size_t)tile_max) +
-               sizeof(int64_t) * ((size_t)tile_max + 1) + sizeof(int) * (size_t)tile_max;
-    b = (b + 15) / 16 * 16;
+    const size_t n = (size_t)H + 1;
+    if (f32) return round16(n * sizeof(f32x4)) + round16(n * sizeof(f32x2)) + (with_ecrit ? round16(n * 4) : 0);
+    return n * sizeof(Rec) + (with_ecrit ? round16(n * 8) : 0);
+}
+
+__host__ __device__ inline size_t lds_bytes(int H, int tile_max, bool with_ecrit, bool with_epi, bool f32)
+{
+    size_t b = lds_rec_bytes(H, with_ecrit, f32);
+    b += round16(sizeof(double) * 4 * (size_t)tile_max + sizeof(int64_t) * ((size_t)tile_max + 1) +
+                 sizeof(int) * (size_t)tile_max);
     if (with_epi) b += sizeof(EpiRec) * (size_t)tile_max + sizeof(MatDev) * kMaxMat + sizeof(BCDev) * kMaxBC;
     return b;
 }
 
-__device__ __forceinline__ Lds lds_view(double* base, int H, int tile_max, bool with_ecrit, bool with_epi = false)
+__device__ __forceinline__ Lds lds_view(double* base, int H, int tile_max, bool with_ecrit, bool with_epi, bool f32)
 {
-    Lds l;
-    l.rec = reinterpret_cast<Rec*>(base);
-    double* p = base + 6 * (H + 1);
-    l.ec = with_ecrit ? p : nullptr;
-    if (with_ecrit) p += H + 1;
-    l.res = p;
+    Lds l{};
+    char* p = reinterpret_cast<char*>(base);
+    const size_t n = (size_t)H + 1;
+    if (f32) {
+        l.fa = reinterpret_cast<f32x4*>(p);
+        p += round16(n * sizeof(f32x4));
+        l.fb = reinterpret_cast<f32x2*>(p);
+        p += round16(n * sizeof(f32x2));
+        if (with_ecrit) {
+            l.fec = reinterpret_cast<float*>(p);
+            p += round16(n * 4);
+        }
+    } else {
+        l.rec = reinterpret_cast<Rec*>(p);
+        p += n * sizeof(Rec);
+        if (with_ecrit) {
+            l.ec = reinterpret_cast<double*>(p);
+            p += round16(n * 8);
+        }
+    }
+    l.res = reinterpret_cast<double*>(p);
     l.row = reinterpret_cast<int64_t*>(l.res + 4 * tile_max);
     l.self = reinterpret_cast<int*>(l.row + tile_max + 1);
-    char* q = reinterpret_cast<char*>(base) + lds_bytes(H, tile_max, with_ecrit, false);
-    l.epi = with_epi ? reinterpret_cast<EpiRec*>(q) : nullptr;
-    l.mats = with_epi ? reinterpret_cast<MatDev*>(q + sizeof(EpiRec) * (size_t)tile_max) : nullptr;
-    l.bcs = with_epi ? reinterpret_cast<BCDev*>(reinterpret_cast<char*>(l.mats) + sizeof(MatDev) * kMaxMat) : nullptr;
+    p = reinterpret_cast<char*>(base) + lds_bytes(H, tile_max, with_ecrit, false, f32);
+    if (with_epi) {
+        l.epi = reinterpret_cast<EpiRec*>(p);
+        l.mats = reinterpret_cast<MatDev*>(p + sizeof(EpiRec) * (size_t)tile_max);
+        l.bcs = reinterpret_cast<BCDev*>(reinterpret_cast<char*>(l.mats) + sizeof(MatDev) * kMaxMat);
+    }
     l.pad = H;
     return l;
 }
 
-__device__ __forceinline__ void stage_tile(const TileView& tv, const d4* __restrict__ u, int64_t h0, int H,
-                                           int64_t a0, int nt, const Lds& l, bool with_ecrit)
+// Stage the tile's halo records (record q = internal node halo[h0 + q]), the padding
+// record, the row starts and own slots.  Two records per thread per pass, both id
+// loads issued before the record loads.
+template <bool F32>
+__device__ __forceinline__ void stage_tile(const TileView& tv, const d4* __restrict__ u, int64_t k, int64_t h0,
+                                           int H, int64_t a0, int nt, const Lds& l, bool with_ecrit)
 {
-    // two records per thread per pass, both id loads before the record loads
+    d4 xr{}, ur{};
+    if (F32) {
+        xr = tv.tref[k];
+        ur = u[a0 + nt / 2];
+    }
     for (int q0 = threadIdx.x; q0 < H; q0 += 2 * blockDim.x) {
         const int q1 = q0 + blockDim.x;
         const int32_t id0 = tv.halo[h0 + q0];
         const int32_t id1 = (q1 < H) ? tv.halo[h0 + q1] : id0;
         const d4 x0 = tv.X[id0], v0 = u[id0];
         const d4 x1 = tv.X[id1], v1 = u[id1];
-        l.rec[q0] = Rec{x0.x, x0.y, x0.z, v0.x, v0.y, v0.z};
-        if (with_ecrit) l.ec[q0] = x0.w;
-        if (q1 < H) {
-            l.rec[q1] = Rec{x1.x, x1.y, x1.z, v1.x, v1.y, v1.z};
-            if (with_ecrit) l.ec[q1] = x1.w;
+        if (F32) {
+            l.fa[q0] = f32x4{(float)(x0.x - xr.x), (float)(x0.y - xr.y), (float)(x0.z - xr.z), (float)(v0.x - ur.x)};
+            l.fb[q0] = f32x2{(float)(v0.y - ur.y), (float)(v0.z - ur.z)};
+            if (with_ecrit) l.fec[q0] = (float)x0.w;
+            if (q1 < H) {
+                l.fa[q1] = f32x4{(float)(x1.x - xr.x), (float)(x1.y - xr.y), (float)(x1.z - xr.z),
+                                 (float)(v1.x - ur.x)};
+                l.fb[q1] = f32x2{(float)(v1.y - ur.y), (float)(v1.z - ur.z)};
+                if (with_ecrit) l.fec[q1] = (float)x1.w;
+            }
+        } else {
+            l.rec[q0] = Rec{x0.x, x0.y, x0.z, v0.x, v0.y, v0.z};
+            if (with_ecrit) l.ec[q0] = x0.w;
+            if (q1 < H) {
+                l.rec[q1] = Rec{x1.x, x1.y, x1.z, v1.x, v1.y, v1.z};
+                if (with_ecrit) l.ec[q1] = x1.w;
+            }
         }
     }
     if (threadIdx.x == 0) {
-        l.rec[H] = Rec{kFar, kFar, kFar, 0.0, 0.0, 0.0};
-        if (with_ecrit) l.ec[H] = 1.0;
+        if (F32) {
+            l.fa[H] = f32x4{(float)kFar, (float)kFar, (float)kFar, 0.0f};
+            l.fb[H] = f32x2{0.0f, 0.0f};
+            if (with_ecrit) l.fec[H] = 1.0f;
+        } else {
+            l.rec[H] = Rec{kFar, kFar, kFar, 0.0, 0.0, 0.0};
+            if (with_ecrit) l.ec[H] = 1.0;
+        }
     }
     for (int q = threadIdx.x; q <= nt; q += blockDim.x) l.row[q] = tv.rpi[a0 + q];
     for (int q = threadIdx.x; q < nt; q += blockDim.x) l.self[q] = tv.self_slot[a0 + q];
 }
 
+// The family owner's own record.
 struct Self {
     double x, y, z, u, v, w, ecrit;
+    __device__ static Self make(const Lds& l, const TileView& tv, int ss, bool ue)
+    {
+        const Rec r = l.rec[ss];
+        return Self{r.x, r.y, r.z, r.u, r.v, r.w, ue ? tv.ecrit0 : l.ec[ss]};
+    }
+};
+
+struct SelfF {
+    float x, y, z, u, v, w;
+    float ecrit;
+    __device__ static SelfF make(const Lds& l, const TileView& tv, int ss, bool ue)
+    {
+        const f32x4 a = l.fa[ss];
+        const f32x2 b = l.fb[ss];
+        return SelfF{a.x, a.y, a.z, a.w, b.x, b.y, ue ? (float)tv.ecrit0 : l.fec[ss]};
+    }
 };
 
 // ------------------------------------------------------------ per-bond operations
-// Each operation: the coefficient type it streams, a per-lane accumulator, the bond
-// update, the lane-group reduction and the store of (fx, fy, fz, energy density).
-// Energy: acc.el sums coef * s * u; the bond energy bdryEff * beta * c * u^2 / (4 r)
-// is that times 1 / (4 dx^3), applied once per family.
+// Each operation: the coefficient type it streams, its record type (Self / SelfF),
+// a per-lane accumulator, the bond update, the lane-group reduction and the store
+// of (fx, fy, fz, energy density).  Energy: acc.el sums coef * s * u; the bond energy
+// bdryEff * beta * c * u^2 / (4 r) is that times 1 / (4 dx^3), applied once per family.
 
 struct OpF64 {  // PDM_FP64, reference arithmetic
     using Coef = double;
+    using SelfT = Self;
     struct Acc {
         double f0, f1, f2, el;
     };
     template <bool ENERGY, bool UE>
-    __device__ static void bond(const Lds& l, int s, double c, const Self& me, Acc& a)
+    __device__ static void bond(const Lds& l, unsigned s, double c, const Self& me, Acc& a)
     {
         const Rec r = l.rec[s];
         double xi1 = r.x - me.x, xi2 = r.y - me.y, xi3 = r.z - me.z;
@@ -288,50 +407,50 @@ struct Bond32 {
     float g, a1, a2, a3, e;  // force factor, xi + eta, energy term coef * s * u
 };
 
-// FP32 bond force from FP64 records: cancellation-free stretch.
+// FP32 bond force from FP32 relative records, cancellation-free stretch:
+// q = |xi+eta|^2 - |xi|^2 = eta . (2 xi + eta), s = (p - r) / r = q / ((p + r) r).
 template <bool ENERGY, bool UE>
-__device__ __forceinline__ Bond32 bond32(const Lds& l, int s, float c, const Self& me)
+__device__ __forceinline__ Bond32 bond32(const Lds& l, unsigned s, float c, const SelfF& me)
 {
-    const Rec r = l.rec[s];
-    const float x1 = (float)(r.x - me.x), x2 = (float)(r.y - me.y), x3 = (float)(r.z - me.z);
-    const float e1 = (float)(r.u - me.u), e2 = (float)(r.v - me.v), e3 = (float)(r.w - me.w);
+    const f32x4 ra = l.fa[s];
+    const f32x2 rb = l.fb[s];
+    const float x1 = ra.x - me.x, x2 = ra.y - me.y, x3 = ra.z - me.z;
+    const float e1 = ra.w - me.u, e2 = rb.x - me.v, e3 = rb.y - me.w;
+    const float a1 = x1 + e1, a2 = x2 + e2, a3 = x3 + e3;
     const float r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
-    const float xe = __builtin_fmaf(x3, e3, __builtin_fmaf(x2, e2, x1 * e1));
-    const float ee = __builtin_fmaf(e3, e3, __builtin_fmaf(e2, e2, e1 * e1));
-    const float q = __builtin_fmaf(2.0f, xe, ee);  // |xi+eta|^2 - |xi|^2
+    const float q = __builtin_fmaf(e3, x3 + a3, __builtin_fmaf(e2, x2 + a2, e1 * (x1 + a1)));
     const float p2 = r2 + q;
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float pinv = __builtin_amdgcn_rsqf(p2);
     const float rr = r2 * rinv;
     const float p = p2 * pinv;
-    const float st = q * __builtin_amdgcn_rcpf((p + rr) * rr);  // (p - r) / r without cancellation
-    const float u = st * rr;
-    float ecrit = (float)me.ecrit;
+    const float st = q * __builtin_amdgcn_rcpf((p + rr) * rr);
+    // u <= 0 || s <= ecrit, with u = s r and r > 0
+    float ecrit = me.ecrit;
     if (!UE) {
-        float ej = (float)l.ec[s];
+        const float ej = l.fec[s];
         ecrit = (ej < ecrit) ? ej : ecrit;
     }
-    // u <= 0 || s <= ecrit, with u = s r and r > 0
-    const bool ok = st <= __builtin_fmaxf(ecrit, 0.0f);
-    const float cs = ok ? c * st : 0.0f;
+    const float cs = (st <= __builtin_fmaxf(ecrit, 0.0f)) ? c * st : 0.0f;
     Bond32 b;
     b.g = cs * pinv;
-    b.a1 = x1 + e1;
-    b.a2 = x2 + e2;
-    b.a3 = x3 + e3;
-    b.e = ENERGY ? cs * u : 0.0f;
+    b.a1 = a1;
+    b.a2 = a2;
+    b.a3 = a3;
+    b.e = ENERGY ? cs * (st * rr) : 0.0f;
     return b;
 }
 
 struct OpF32 {  // PDM_FP32C: FP32 bond force, FP32 block sums, FP64 family accumulator
     using Coef = float;
+    using SelfT = SelfF;
     struct Acc {
         double f0, f1, f2;
         float p0, p1, p2;
         float el;
     };
     template <bool ENERGY, bool UE>
-    __device__ static void bond(const Lds& l, int s, float c, const Self& me, Acc& a)
+    __device__ static void bond(const Lds& l, unsigned s, float c, const SelfF& me, Acc& a)
     {
         const Bond32 b = bond32<ENERGY, UE>(l, s, c, me);
         a.p0 = __builtin_fmaf(b.g, b.a1, a.p0);
@@ -352,7 +471,7 @@ struct OpF32 {  // PDM_FP32C: FP32 bond force, FP32 block sums, FP64 family accu
         a.f0 = group_sum<LPN>(a.f0);
         a.f1 = group_sum<LPN>(a.f1);
         a.f2 = group_sum<LPN>(a.f2);
-        a.el = (float)group_sum<LPN>((double)a.el);
+        a.el = group_sumf<LPN>(a.el);
     }
     __device__ static void store(const Acc& a, double ivc4, double* out)
     {
@@ -365,11 +484,12 @@ struct OpF32 {  // PDM_FP32C: FP32 bond force, FP32 block sums, FP64 family accu
 
 struct OpF32TF {  // PDM_FP32C2: FP32 bond force, two-float family sums
     using Coef = float;
+    using SelfT = SelfF;
     struct Acc {
         float h0, h1, h2, l0, l1, l2, el;
     };
     template <bool ENERGY, bool UE>
-    __device__ static void bond(const Lds& l, int s, float c, const Self& me, Acc& a)
+    __device__ static void bond(const Lds& l, unsigned s, float c, const SelfF& me, Acc& a)
     {
         const Bond32 b = bond32<ENERGY, UE>(l, s, c, me);
         float t, err;
@@ -384,7 +504,7 @@ struct OpF32TF {  // PDM_FP32C2: FP32 bond force, two-float family sums
         group_sum_tf<LPN>(a.h0, a.l0);
         group_sum_tf<LPN>(a.h1, a.l1);
         group_sum_tf<LPN>(a.h2, a.l2);
-        a.el = (float)group_sum<LPN>((double)a.el);
+        a.el = group_sumf<LPN>(a.el);
     }
     __device__ static void store(const Acc& a, double ivc4, double* out)
     {
@@ -398,11 +518,12 @@ struct OpF32TF {  // PDM_FP32C2: FP32 bond force, two-float family sums
 // Tangent of the bond force at eta = 0: y_i = sum_j coef_ij xi (xi . (u_i - u_j)) / r^3.
 struct OpT64 {
     using Coef = double;
+    using SelfT = Self;
     struct Acc {
         double f0, f1, f2, el;
     };
     template <bool ENERGY, bool UE>
-    __device__ static void bond(const Lds& l, int s, double c, const Self& me, Acc& a)
+    __device__ static void bond(const Lds& l, unsigned s, double c, const Self& me, Acc& a)
     {
         const Rec r = l.rec[s];
         double x1 = r.x - me.x, x2 = r.y - me.y, x3 = r.z - me.z;
@@ -431,6 +552,7 @@ struct OpT64 {
 
 struct OpT32 {
     using Coef = float;
+    using SelfT = SelfF;
     struct Acc {
         double f0, f1, f2, el;
         float p0, p1, p2;
@@ -443,11 +565,12 @@ struct OpT32 {
         a.p0 = a.p1 = a.p2 = 0.0f;
     }
     template <bool ENERGY, bool UE>
-    __device__ static void bond(const Lds& l, int s, float c, const Self& me, Acc& a)
+    __device__ static void bond(const Lds& l, unsigned s, float c, const SelfF& me, Acc& a)
     {
-        const Rec r = l.rec[s];
-        const float x1 = (float)(r.x - me.x), x2 = (float)(r.y - me.y), x3 = (float)(r.z - me.z);
-        const float d0 = (float)(me.u - r.u), d1 = (float)(me.v - r.v), d2 = (float)(me.w - r.w);
+        const f32x4 ra = l.fa[s];
+        const f32x2 rb = l.fb[s];
+        const float x1 = ra.x - me.x, x2 = ra.y - me.y, x3 = ra.z - me.z;
+        const float d0 = me.u - ra.w, d1 = me.v - rb.x, d2 = me.w - rb.y;
         const float r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
         const float rinv = __builtin_amdgcn_rsqf(r2);
         const float proj = __builtin_fmaf(x3, d2, __builtin_fmaf(x2, d1, x1 * d0)) * c * (rinv * rinv * rinv);
@@ -474,11 +597,12 @@ struct OpT32 {
 // Diagonal of the tangent operator (Jacobi preconditioner): d_c = sum_j coef_ij xi_c^2 / r^3.
 struct OpD64 {
     using Coef = double;
+    using SelfT = Self;
     struct Acc {
         double f0, f1, f2, el;
     };
     template <bool ENERGY, bool UE>
-    __device__ static void bond(const Lds& l, int s, double c, const Self& me, Acc& a)
+    __device__ static void bond(const Lds& l, unsigned s, double c, const Self& me, Acc& a)
     {
         const Rec r = l.rec[s];
         double x1 = r.x - me.x, x2 = r.y - me.y, x3 = r.z - me.z;
@@ -502,10 +626,9 @@ struct OpD64 {
     }
 };
 
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x2 __attribute__((ext_vector_type(2)));
+// Whether an operation reads FP32 records.
+template <class OP>
+constexpr bool kF32Rec = std::is_same<typename OP::SelfT, SelfF>::value;
 
 // Operations without FP32 partial sums have nothing to flush.
 template <class OP, class = void>
@@ -526,7 +649,7 @@ __device__ __forceinline__ void flush(typename OP::Acc& a)
 template <class OP>
 struct Chunk {
     static constexpr int B = sizeof(typename OP::Coef) == 8 ? 4 : 8;
-    int s[B];
+    unsigned s[B];
     typename OP::Coef c[B];
 };
 
@@ -546,8 +669,8 @@ __device__ __forceinline__ void load_block(const TileView& tv, int64_t e, int64_
             const f32x4 c1 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e + 4));
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                ch.s[2 * k] = (int)(sv[k] & 0xFFFFu);
-                ch.s[2 * k + 1] = (int)(sv[k] >> 16);
+                ch.s[2 * k] = sv[k] & 0xFFFFu;
+                ch.s[2 * k + 1] = sv[k] >> 16;
                 ch.c[k] = c0[k];
                 ch.c[4 + k] = c1[k];
             }
@@ -557,8 +680,8 @@ __device__ __forceinline__ void load_block(const TileView& tv, int64_t e, int64_
             const f64x2 c1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(tv.coef64 + e + 2));
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
-                ch.s[2 * k] = (int)(sv[k] & 0xFFFFu);
-                ch.s[2 * k + 1] = (int)(sv[k] >> 16);
+                ch.s[2 * k] = sv[k] & 0xFFFFu;
+                ch.s[2 * k + 1] = sv[k] >> 16;
                 ch.c[k] = c0[k];
                 ch.c[2 + k] = c1[k];
             }
@@ -566,21 +689,46 @@ __device__ __forceinline__ void load_block(const TileView& tv, int64_t e, int64_
     } else {
 #pragma unroll
         for (int k = 0; k < B; ++k) {
-            ch.s[k] = pad;
+            ch.s[k] = (unsigned)pad;
             ch.c[k] = 0;
         }
     }
 }
 
+// One round of a lane group: the family of tile node loc (blocks from `cur`, which the
+// caller loaded), with the next round's first block loaded into `nxt` beforehand.
+template <class OP, bool ENERGY, bool UE, int LPN>
+__device__ __forceinline__ void family_round(const TileView& tv, const Lds& l, int nt, int loc, int64_t e0,
+                                             int64_t e1, int sl, const Chunk<OP>& cur, double ivc4)
+{
+    constexpr int B = Chunk<OP>::B;
+    constexpr int CH = LPN * B;
+    const int ss = (loc < nt) ? l.self[loc] : l.pad;
+    const typename OP::SelfT me = OP::SelfT::make(l, tv, ss, UE);
+    typename OP::Acc acc{};
+#pragma unroll
+    for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, cur.s[k], cur.c[k], me, acc);
+    flush<OP>(acc);
+    for (int64_t base = e0 + CH; base < e1; base += CH) {
+        Chunk<OP> more;
+        load_block<OP>(tv, base + B * sl, e1, l.pad, more);
+#pragma unroll
+        for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, more.s[k], more.c[k], me, acc);
+        flush<OP>(acc);
+    }
+    OP::template reduce<LPN>(acc);
+    if (sl == 0 && loc < nt) OP::store(acc, ivc4, l.res + 4 * loc);
+}
+
 // Family sums of the tile's nodes into l.res.  Round r of wavefront w covers nodes
-// (r NW + w) NPW + g for lane groups g = 0..NPW-1; the first chunk of round r + 1
-// is loaded before round r is computed.
+// (r NW + w) NPW + g for lane groups g = 0..NPW-1; the first block of round r + 1
+// is loaded before round r is computed.  Rounds alternate between two block
+// buffers, so no registers are copied between rounds.
 template <class OP, bool ENERGY, bool UE, int NW, int NPW>
 __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, int nt)
 {
     constexpr int LPN = 64 / NPW;
     constexpr int B = Chunk<OP>::B;
-    constexpr int CH = LPN * B;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane / LPN, sl = lane % LPN;
     const double ivc4 = 0.25 * tv.inv_vol_cube;
@@ -588,42 +736,66 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, 
     if (first >= nt) return;
     int loc = first + g;
     int64_t e0 = (loc < nt) ? l.row[loc] : 0, e1 = (loc < nt) ? l.row[loc + 1] : 0;
-    Chunk<OP> cur;
-    load_block<OP>(tv, e0 + B * sl, e1, l.pad, cur);
+    Chunk<OP> c0, c1;
+    load_block<OP>(tv, e0 + B * sl, e1, l.pad, c0);
+    if constexpr (!kF32Rec<OP>) {
+        // FP64 operations: one block buffer and a copy per round (fewer registers)
+        for (;;) {
+            const int nfirst = first + NW * NPW;
+            const int nloc = nfirst + g;
+            int64_t n0 = 0, n1 = 0;
+            if (nfirst < nt) {
+                if (nloc < nt) {
+                    n0 = l.row[nloc];
+                    n1 = l.row[nloc + 1];
+                }
+                load_block<OP>(tv, n0 + B * sl, n1, l.pad, c1);
+            }
+            family_round<OP, ENERGY, UE, LPN>(tv, l, nt, loc, e0, e1, sl, c0, ivc4);
+            if (nfirst >= nt) return;
+            first = nfirst;
+            loc = nloc;
+            e0 = n0;
+            e1 = n1;
+            c0 = c1;
+        }
+    }
     for (;;) {
-        const int nfirst = first + NW * NPW;
-        const int nloc = nfirst + g;
+        // round with c0, prefetch into c1
+        int nfirst = first + NW * NPW;
+        int nloc = nfirst + g;
         int64_t n0 = 0, n1 = 0;
-        Chunk<OP> nxt;
         if (nfirst < nt) {
             if (nloc < nt) {
                 n0 = l.row[nloc];
                 n1 = l.row[nloc + 1];
             }
-            load_block<OP>(tv, n0 + B * sl, n1, l.pad, nxt);
+            load_block<OP>(tv, n0 + B * sl, n1, l.pad, c1);
         }
-        const int ss = (loc < nt) ? l.self[loc] : l.pad;
-        const Rec ri = l.rec[ss];
-        const Self me{ri.x, ri.y, ri.z, ri.u, ri.v, ri.w, UE ? tv.ecrit0 : l.ec[ss]};
-        typename OP::Acc acc{};
-#pragma unroll
-        for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, cur.s[k], cur.c[k], me, acc);
-        flush<OP>(acc);
-        for (int64_t base = e0 + CH; base < e1; base += CH) {
-            Chunk<OP> more;
-            load_block<OP>(tv, base + B * sl, e1, l.pad, more);
-#pragma unroll
-            for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, more.s[k], more.c[k], me, acc);
-            flush<OP>(acc);
-        }
-        OP::template reduce<LPN>(acc);
-        if (sl == 0 && loc < nt) OP::store(acc, ivc4, l.res + 4 * loc);
+        family_round<OP, ENERGY, UE, LPN>(tv, l, nt, loc, e0, e1, sl, c0, ivc4);
         if (nfirst >= nt) break;
         first = nfirst;
         loc = nloc;
         e0 = n0;
         e1 = n1;
-        cur = nxt;
+        // round with c1, prefetch into c0
+        nfirst = first + NW * NPW;
+        nloc = nfirst + g;
+        n0 = 0;
+        n1 = 0;
+        if (nfirst < nt) {
+            if (nloc < nt) {
+                n0 = l.row[nloc];
+                n1 = l.row[nloc + 1];
+            }
+            load_block<OP>(tv, n0 + B * sl, n1, l.pad, c0);
+        }
+        family_round<OP, ENERGY, UE, LPN>(tv, l, nt, loc, e0, e1, sl, c1, ivc4);
+        if (nfirst >= nt) break;
+        first = nfirst;
+        loc = nloc;
+        e0 = n0;
+        e1 = n1;
     }
 }
 
@@ -701,12 +873,13 @@ __global__ __launch_bounds__(64 * NW) void k_stage(StageArgs a, int step_idx)
     const int nt = (int)(tv.tile_start[tid + 1] - a0);
     const int64_t h0 = tv.halo_ptr[k];
     const int H = (int)(tv.halo_ptr[k + 1] - h0);
-    const Lds l = lds_view(L, H, tv.tile_max, !UE, true);
+    constexpr bool F32 = kF32Rec<typename OpFor<PREC>::Force>;
+    const Lds l = lds_view(L, H, tv.tile_max, !UE, true, F32);
     // the stage-update inputs (first node of each thread) are loaded together with the halo
     EpiLoad<STAGE, ENERGY> pre;
     const bool has_pre = (int)threadIdx.x < nt;
     if (has_pre) pre.load(a, a0 + threadIdx.x);
-    stage_tile(tv, a.uin, h0, H, a0, nt, l, !UE);
+    stage_tile<F32>(tv, a.uin, k, h0, H, a0, nt, l, !UE);
     {
         const double* src = reinterpret_cast<const double*>(a.mats);
         double* dst = reinterpret_cast<double*>(l.mats);
@@ -871,8 +1044,9 @@ __global__ __launch_bounds__(64 * NW) void k_gather(TileView tv, const d4* __res
     const int nt = (int)(tv.tile_start[tid + 1] - a0);
     const int64_t h0 = tv.halo_ptr[k];
     const int H = (int)(tv.halo_ptr[k + 1] - h0);
-    const Lds l = lds_view(L, H, tv.tile_max, !UE);
-    stage_tile(tv, u, h0, H, a0, nt, l, !UE);
+    constexpr bool F32 = kF32Rec<OP>;
+    const Lds l = lds_view(L, H, tv.tile_max, !UE, false, F32);
+    stage_tile<F32>(tv, u, k, h0, H, a0, nt, l, !UE);
     __syncthreads();
     tile_families<OP, true, UE, NW, NPW>(tv, l, nt);
     __syncthreads();
@@ -962,9 +1136,9 @@ hipError_t launch_gather_u(const TileView& tv, const d4* u, d4* y, double* elast
 
 }  // namespace
 
-size_t stage_lds_bytes(const TileView& tv, bool with_ecrit, bool with_epi)
+size_t stage_lds_bytes(const TileView& tv, bool with_ecrit, bool with_epi, bool f32)
 {
-    return lds_bytes(tv.halo_max, tv.tile_max, with_ecrit, with_epi);
+    return lds_bytes(tv.halo_max, tv.tile_max, with_ecrit, with_epi, f32);
 }
 
 int64_t energy_part2_size(int64_t nparts) { return nparts / 1024 + 2; }
@@ -972,7 +1146,7 @@ int64_t energy_part2_size(int64_t nparts) { return nparts / 1024 + 2; }
 hipError_t launch_stage(int stage, int precision, const StageArgs& a, int step_idx, hipStream_t s)
 {
     if (a.tv.ntl == 0) return hipSuccess;
-    size_t lds = stage_lds_bytes(a.tv, !a.tv.uniform_ecrit, true);
+    size_t lds = stage_lds_bytes(a.tv, !a.tv.uniform_ecrit, true, precision != PDM_FP64);
     switch (precision) {
     case PDM_FP64: return launch_stage_p<PDM_FP64>(stage, a, step_idx, lds, s);
     case PDM_FP32C: return launch_stage_p<PDM_FP32C>(stage, a, step_idx, lds, s);
@@ -995,7 +1169,7 @@ hipError_t launch_energy_final(const double* part, int64_t nparts, double* part2
 hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, double* elast, hipStream_t s)
 {
     if (tv.ntl == 0) return hipSuccess;
-    size_t lds = stage_lds_bytes(tv, !tv.uniform_ecrit, false);
+    size_t lds = stage_lds_bytes(tv, !tv.uniform_ecrit, false, precision != PDM_FP64);
     switch (precision) {
     case PDM_FP64: return launch_gather_u<OpF64>(tv, u, F, elast, lds, s);
     case PDM_FP32C: return launch_gather_u<OpF32>(tv, u, F, elast, lds, s);
@@ -1006,13 +1180,13 @@ hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, 
 hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s)
 {
     if (tv.ntl == 0) return hipSuccess;
-    return launch_gather_u<OpD64>(tv, u, y, nullptr, stage_lds_bytes(tv, !tv.uniform_ecrit, false), s);
+    return launch_gather_u<OpD64>(tv, u, y, nullptr, stage_lds_bytes(tv, !tv.uniform_ecrit, false, false), s);
 }
 
 hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s)
 {
     if (tv.ntl == 0) return hipSuccess;
-    size_t lds = stage_lds_bytes(tv, !tv.uniform_ecrit, false);
+    size_t lds = stage_lds_bytes(tv, !tv.uniform_ecrit, false, precision != PDM_FP64);
     if (precision == PDM_FP64) return launch_gather_u<OpT64>(tv, u, y, nullptr, lds, s);
     return launch_gather_u<OpT32>(tv, u, y, nullptr, lds, s);
 }

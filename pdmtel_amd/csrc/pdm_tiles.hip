@@ -309,6 +309,7 @@ TileView tile_view(const pdm_handle* h)
     v.coef64 = h->coef64;
     v.coef32 = h->coef32;
     v.X = h->Xi;
+    v.tref = h->tref;
     v.n = h->n;
     v.ntl = h->ntl;
     v.halo_max = h->halo_max;
@@ -423,6 +424,7 @@ int build_tiles(pdm_handle* h)
     HIPCHK(hipMalloc(&h->rpi, sizeof(int64_t) * (n + 1)));
     HIPCHK(hipMalloc(&h->tile_start, sizeof(int64_t) * (ntiles + 1)));
     HIPCHK(hipMalloc(&h->tiles, sizeof(int32_t) * std::max<int64_t>(h->ntl, 1)));
+    HIPCHK(hipMalloc(&h->tref, sizeof(d4) * std::max<int64_t>(h->ntl, 1)));
     HIPCHK(hipMalloc(&h->halo_ptr, sizeof(int64_t) * (h->ntl + 1)));
     HIPCHK(hipMalloc(&halo_len, sizeof(int64_t) * std::max<int64_t>(h->ntl, 1)));
     HIPCHK(hipMalloc(&h->self_slot, sizeof(uint16_t) * n1));
@@ -441,6 +443,16 @@ int build_tiles(pdm_handle* h)
     HIPCHK(hipMemcpyAsync(h->rpi, rpih.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(h->tile_start, tstart.data(), sizeof(int64_t) * (ntiles + 1), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(h->tiles, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice, s));
+    // tile centres: the reference point of the FP32 halo records
+    std::vector<d4> trefh(tiles.size());
+    for (size_t k = 0; k < tiles.size(); ++k) {
+        const int64_t t = tiles[k];
+        const int64_t q[3] = {t % g.nt[0], (t / g.nt[0]) % g.nt[1], t / (g.nt[0] * g.nt[1])};
+        double c[3];
+        for (int a = 0; a < 3; ++a) c[a] = lo[a] + ((double)q[a] + 0.5) * edge[a];
+        trefh[k] = d4{c[0], c[1], c[2], 0.0};
+    }
+    HIPCHK(hipMemcpy(h->tref, trefh.data(), sizeof(d4) * trefh.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemsetAsync(err, 0, sizeof(int), s));
     if (n) {
         k_permute_nodes<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(
@@ -488,7 +500,7 @@ int build_tiles(pdm_handle* h)
     h->halo_max = (int)hmax;
     {
         TileView tv = tile_view(h);
-        size_t need = stage_lds_bytes(tv, !h->uniform_ecrit, true);
+        size_t need = stage_lds_bytes(tv, !h->uniform_ecrit, true, false);  // FP64 records: the larger
         if (need > kLdsBytes) {
             h->err = "tile LDS footprint " + std::to_string(need) + " B exceeds " + std::to_string(kLdsBytes) +
                      " B; use a smaller tile_edge";
