@@ -95,6 +95,7 @@ struct TileView {
     int uniform_ecrit;         // all materials share ecrit: no per-neighbour ecrit in LDS
     double ecrit0;             // that common ecrit
     double inv_vol_cube;       // 1 / (dx*dx*dx)
+    double fscale;             // scale S of the FP32 halo records (fixed-point positions)
 };
 
 // Arguments of the RK4 stage kernels.
@@ -176,6 +177,7 @@ struct pdm_handle {
     int32_t* tiles = nullptr;
     int64_t* tile_start = nullptr;
     pdm::d4* tref = nullptr;            // launch index -> tile centre
+    double fscale = 1.0;                // fixed-point scale of the FP32 halo records
     int64_t* halo_ptr = nullptr;
     int32_t* halo = nullptr;
     int64_t ntiles = 0, ntl = 0, halo_total = 0;

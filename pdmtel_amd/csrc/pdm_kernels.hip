@@ -196,10 +196,12 @@ struct alignas(16) Rec {
     double x, y, z, u, v, w;
 };
 
-// Halo record of the FP32 operations: position and displacement relative to the
-// tile's reference point (the tile centre, and the displacement of the tile's middle
-// node), rounded to FP32, in two arrays: (x, y, z, ux) and (uy, uz).  Relative values
-// keep the FP32 rounding of xi and eta at the scale of the halo, not of the body.
+// Halo record of the FP32 operations, relative to the tile's reference point (the
+// tile centre, and the displacement of the tile's middle node) and scaled by
+// S = tv.fscale (2^30 over the largest reach of a halo): the position as int32 fixed
+// point, so xi = X_j - X_i is an exact integer difference rounded once to FP32, and the
+// displacement as FP32.  Two arrays: (x, y, z as int32 bits, ux) and (uy, uz).  The
+// bond force is scale-free in these units; the energy carries one factor 1/S.
 
 // Stage-update inputs of one tile node, loaded while the halo is staged.
 struct EpiRec {
@@ -213,7 +215,7 @@ struct EpiRec {
 // for the stage kernels also the stage-update inputs and the material / BC tables.
 struct Lds {
     Rec* rec;        // FP64 records
-    f32x4* fa;       // FP32 records
+    u32x4* fa;       // FP32 records
     f32x2* fb;
     double* ec;      // per-record ecrit (FP64 records)
     float* fec;      // per-record ecrit (FP32 records)
@@ -231,7 +233,7 @@ __host__ __device__ inline size_t round16(size_t b) { return (b + 15) / 16 * 16;
 __host__ __device__ inline size_t lds_rec_bytes(int H, bool with_ecrit, bool f32)
 {
     const size_t n = (size_t)H + 1;
-    if (f32) return round16(n * sizeof(f32x4)) + round16(n * sizeof(f32x2)) + (with_ecrit ? round16(n * 4) : 0);
+    if (f32) return round16(n * sizeof(u32x4)) + round16(n * sizeof(f32x2)) + (with_ecrit ? round16(n * 4) : 0);
     return n * sizeof(Rec) + (with_ecrit ? round16(n * 8) : 0);
 }
 
@@ -250,8 +252,8 @@ __device__ __forceinline__ Lds lds_view(double* base, int H, int tile_max, bool 
     char* p = reinterpret_cast<char*>(base);
     const size_t n = (size_t)H + 1;
     if (f32) {
-        l.fa = reinterpret_cast<f32x4*>(p);
-        p += round16(n * sizeof(f32x4));
+        l.fa = reinterpret_cast<u32x4*>(p);
+        p += round16(n * sizeof(u32x4));
         l.fb = reinterpret_cast<f32x2*>(p);
         p += round16(n * sizeof(f32x2));
         if (with_ecrit) {
@@ -279,6 +281,14 @@ __device__ __forceinline__ Lds lds_view(double* base, int H, int tile_max, bool 
     return l;
 }
 
+constexpr unsigned kFarFix = 1u << 30;  // fixed-point coordinate of the padding record
+
+__device__ __forceinline__ u32x4 frec_a(const d4& x, const d4& v, const d4& xr, const d4& ur, double S)
+{
+    return u32x4{(unsigned)__double2int_rn((x.x - xr.x) * S), (unsigned)__double2int_rn((x.y - xr.y) * S),
+                 (unsigned)__double2int_rn((x.z - xr.z) * S), __float_as_uint((float)((v.x - ur.x) * S))};
+}
+
 // Stage the tile's halo records (record q = internal node halo[h0 + q]), the padding
 // record, the row starts and own slots.  Two records per thread per pass, both id
 // loads issued before the record loads.
@@ -287,6 +297,7 @@ __device__ __forceinline__ void stage_tile(const TileView& tv, const d4* __restr
                                            int H, int64_t a0, int nt, const Lds& l, bool with_ecrit)
 {
     d4 xr{}, ur{};
+    const double S = tv.fscale;
     if (F32) {
         xr = tv.tref[k];
         ur = u[a0 + nt / 2];
@@ -298,13 +309,12 @@ __device__ __forceinline__ void stage_tile(const TileView& tv, const d4* __restr
         const d4 x0 = tv.X[id0], v0 = u[id0];
         const d4 x1 = tv.X[id1], v1 = u[id1];
         if (F32) {
-            l.fa[q0] = f32x4{(float)(x0.x - xr.x), (float)(x0.y - xr.y), (float)(x0.z - xr.z), (float)(v0.x - ur.x)};
-            l.fb[q0] = f32x2{(float)(v0.y - ur.y), (float)(v0.z - ur.z)};
+            l.fa[q0] = frec_a(x0, v0, xr, ur, S);
+            l.fb[q0] = f32x2{(float)((v0.y - ur.y) * S), (float)((v0.z - ur.z) * S)};
             if (with_ecrit) l.fec[q0] = (float)x0.w;
             if (q1 < H) {
-                l.fa[q1] = f32x4{(float)(x1.x - xr.x), (float)(x1.y - xr.y), (float)(x1.z - xr.z),
-                                 (float)(v1.x - ur.x)};
-                l.fb[q1] = f32x2{(float)(v1.y - ur.y), (float)(v1.z - ur.z)};
+                l.fa[q1] = frec_a(x1, v1, xr, ur, S);
+                l.fb[q1] = f32x2{(float)((v1.y - ur.y) * S), (float)((v1.z - ur.z) * S)};
                 if (with_ecrit) l.fec[q1] = (float)x1.w;
             }
         } else {
@@ -318,7 +328,7 @@ __device__ __forceinline__ void stage_tile(const TileView& tv, const d4* __restr
     }
     if (threadIdx.x == 0) {
         if (F32) {
-            l.fa[H] = f32x4{(float)kFar, (float)kFar, (float)kFar, 0.0f};
+            l.fa[H] = u32x4{kFarFix, kFarFix, kFarFix, 0u};
             l.fb[H] = f32x2{0.0f, 0.0f};
             if (with_ecrit) l.fec[H] = 1.0f;
         } else {
@@ -341,13 +351,15 @@ struct Self {
 };
 
 struct SelfF {
-    float x, y, z, u, v, w;
+    int x, y, z;
+    float u, v, w;
     float ecrit;
     __device__ static SelfF make(const Lds& l, const TileView& tv, int ss, bool ue)
     {
-        const f32x4 a = l.fa[ss];
+        const u32x4 a = l.fa[ss];
         const f32x2 b = l.fb[ss];
-        return SelfF{a.x, a.y, a.z, a.w, b.x, b.y, ue ? (float)tv.ecrit0 : l.fec[ss]};
+        return SelfF{(int)a.x, (int)a.y, (int)a.z, __uint_as_float(a.w), b.x, b.y,
+                     ue ? (float)tv.ecrit0 : l.fec[ss]};
     }
 };
 
@@ -407,15 +419,15 @@ struct Bond32 {
     float g, a1, a2, a3, e;  // force factor, xi + eta, energy term coef * s * u
 };
 
-// FP32 bond force from FP32 relative records, cancellation-free stretch:
+// FP32 bond force from the scaled FP32 records, cancellation-free stretch:
 // q = |xi+eta|^2 - |xi|^2 = eta . (2 xi + eta), s = (p - r) / r = q / ((p + r) r).
 template <bool ENERGY, bool UE>
 __device__ __forceinline__ Bond32 bond32(const Lds& l, unsigned s, float c, const SelfF& me)
 {
-    const f32x4 ra = l.fa[s];
+    const u32x4 ra = l.fa[s];
     const f32x2 rb = l.fb[s];
-    const float x1 = ra.x - me.x, x2 = ra.y - me.y, x3 = ra.z - me.z;
-    const float e1 = ra.w - me.u, e2 = rb.x - me.v, e3 = rb.y - me.w;
+    const float x1 = (float)((int)ra.x - me.x), x2 = (float)((int)ra.y - me.y), x3 = (float)((int)ra.z - me.z);
+    const float e1 = __uint_as_float(ra.w) - me.u, e2 = rb.x - me.v, e3 = rb.y - me.w;
     const float a1 = x1 + e1, a2 = x2 + e2, a3 = x3 + e3;
     const float r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
     const float q = __builtin_fmaf(e3, x3 + a3, __builtin_fmaf(e2, x2 + a2, e1 * (x1 + a1)));
@@ -567,10 +579,10 @@ struct OpT32 {
     template <bool ENERGY, bool UE>
     __device__ static void bond(const Lds& l, unsigned s, float c, const SelfF& me, Acc& a)
     {
-        const f32x4 ra = l.fa[s];
+        const u32x4 ra = l.fa[s];
         const f32x2 rb = l.fb[s];
-        const float x1 = ra.x - me.x, x2 = ra.y - me.y, x3 = ra.z - me.z;
-        const float d0 = me.u - ra.w, d1 = me.v - rb.x, d2 = me.w - rb.y;
+        const float x1 = (float)((int)ra.x - me.x), x2 = (float)((int)ra.y - me.y), x3 = (float)((int)ra.z - me.z);
+        const float d0 = me.u - __uint_as_float(ra.w), d1 = me.v - rb.x, d2 = me.w - rb.y;
         const float r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
         const float rinv = __builtin_amdgcn_rsqf(r2);
         const float proj = __builtin_fmaf(x3, d2, __builtin_fmaf(x2, d1, x1 * d0)) * c * (rinv * rinv * rinv);
@@ -731,7 +743,8 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, 
     constexpr int B = Chunk<OP>::B;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane / LPN, sl = lane % LPN;
-    const double ivc4 = 0.25 * tv.inv_vol_cube;
+    // FP32 records are scaled by S: the energy sum coef s (s r S) carries one factor S
+    const double ivc4 = 0.25 * tv.inv_vol_cube * (kF32Rec<OP> ? 1.0 / tv.fscale : 1.0);
     int first = wave * NPW;
     if (first >= nt) return;
     int loc = first + g;

@@ -320,6 +320,7 @@ TileView tile_view(const pdm_handle* h)
     v.uniform_ecrit = h->uniform_ecrit;
     v.ecrit0 = h->deck.mats.empty() ? 0.0 : h->deck.mats[0].ecrit;
     v.inv_vol_cube = 1.0 / (h->deck.dx * h->deck.dx * h->deck.dx);
+    v.fscale = h->fscale;
     return v;
 }
 
@@ -351,6 +352,11 @@ int build_tiles(pdm_handle* h)
         }
     }
     const int64_t ntiles = g.nt[0] * g.nt[1] * g.nt[2];
+    // every halo node lies within (R + 1/2) tile edges of its tile's centre along each axis;
+    // the FP32 records hold positions as int32 fixed point with 2^30 covering that reach
+    double reach_c = 0.0;
+    for (int c = 0; c < 3; ++c) reach_c = std::max(reach_c, (g.R[c] + 0.5) * edge[c]);
+    h->fscale = std::ldexp(1.0, 30) / (reach_c + d.dx);
     if ((double)g.nt[0] * (double)g.nt[1] * (double)g.nt[2] > 2.0e9) {
         h->err = "tile grid too large";
         return PDM_E_LIMIT;
