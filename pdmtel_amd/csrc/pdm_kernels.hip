@@ -657,75 +657,118 @@ __device__ __forceinline__ void flush(typename OP::Acc& a)
     if constexpr (HasFlush<OP>::value) OP::flush(a);
 }
 
-// One lane's block: B consecutive family entries.
-template <class OP>
+// One lane's block: B consecutive family entries, slots packed two per word.
+template <class OP, int B>
 struct Chunk {
-    static constexpr int B = sizeof(typename OP::Coef) == 8 ? 4 : 8;
-    unsigned s[B];
+    unsigned sp[B / 2];
     typename OP::Coef c[B];
+    __device__ __forceinline__ unsigned slot(int k) const { return (sp[k >> 1] >> ((k & 1) * 16)) & 0xFFFFu; }
 };
 
-// The block starting at entry e (a multiple of B; rows are whole blocks of
-// kRowAlign entries), or, at or past the row end e1, B references to the
-// padding record with coefficient 0.  Slots: one 16- or 8-byte load; coefficients:
-// two 16-byte loads.
-template <class OP>
-__device__ __forceinline__ void load_block(const TileView& tv, int64_t e, int64_t e1, int pad, Chunk<OP>& ch)
+// Entries [e, e + 8) of a row ending at e1 (rows are whole blocks of kRowAlign = 8
+// entries), or, at or past e1, 8 references to the padding record with coefficient 0.
+template <class OP, int B>
+__device__ __forceinline__ void load8(const TileView& tv, int64_t e, int64_t e1, unsigned pad2, Chunk<OP, B>& ch,
+                                      int o)
 {
-    constexpr int B = Chunk<OP>::B;
-    static_assert(kRowAlign % B == 0, "rows must be whole blocks");
+    static_assert(kRowAlign == 8, "sub-blocks of 8 entries");
     if (e < e1) {
-        if constexpr (B == 8) {
-            const u32x4 sv = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(tv.slot + e));
+        const u32x4 sv = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(tv.slot + e));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) ch.sp[o / 2 + k] = sv[k];
+        if constexpr (sizeof(typename OP::Coef) == 4) {
             const f32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e));
             const f32x4 c1 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e + 4));
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                ch.s[2 * k] = sv[k] & 0xFFFFu;
-                ch.s[2 * k + 1] = sv[k] >> 16;
-                ch.c[k] = c0[k];
-                ch.c[4 + k] = c1[k];
+                ch.c[o + k] = c0[k];
+                ch.c[o + 4 + k] = c1[k];
             }
         } else {
-            const u32x2 sv = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(tv.slot + e));
-            const f64x2 c0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(tv.coef64 + e));
-            const f64x2 c1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(tv.coef64 + e + 2));
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                ch.s[2 * k] = sv[k] & 0xFFFFu;
-                ch.s[2 * k + 1] = sv[k] >> 16;
-                ch.c[k] = c0[k];
-                ch.c[2 + k] = c1[k];
+            for (int k = 0; k < 4; ++k) {
+                const f64x2 cv = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(tv.coef64 + e + 2 * k));
+                ch.c[o + 2 * k] = cv[0];
+                ch.c[o + 2 * k + 1] = cv[1];
             }
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < B; ++k) {
-            ch.s[k] = (unsigned)pad;
-            ch.c[k] = 0;
+        for (int k = 0; k < 4; ++k) ch.sp[o / 2 + k] = pad2;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) ch.c[o + k] = 0;
+    }
+}
+
+// The lane's block of B entries: for B >= 8, sub-blocks of 8 entries at e, e + 8 LPN,
+// e + 16 LPN, ... (e a multiple of 8), so that each load instruction of the lane group
+// covers contiguous memory; for B = 4, the 4 entries at e.  One 16-byte slot load and
+// two 16-byte coefficient loads per 8 entries.
+template <class OP, int B, int LPN>
+__device__ __forceinline__ void load_block(const TileView& tv, int64_t e, int64_t e1, int pad, Chunk<OP, B>& ch)
+{
+    const unsigned pad2 = (unsigned)pad * 0x10001u;
+    if constexpr (B >= 8) {
+#pragma unroll
+        for (int o = 0; o < B; o += 8) load8<OP, B>(tv, e + (int64_t)o * LPN, e1, pad2, ch, o);
+    } else {
+        static_assert(B == 4, "blocks of 4, 8 or 16 entries");
+        if (e < e1) {
+            const u32x2 sv = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(tv.slot + e));
+            ch.sp[0] = sv[0];
+            ch.sp[1] = sv[1];
+            if constexpr (sizeof(typename OP::Coef) == 4) {
+                const f32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e));
+#pragma unroll
+                for (int k = 0; k < 4; ++k) ch.c[k] = c0[k];
+            } else {
+                const f64x2 c0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(tv.coef64 + e));
+                const f64x2 c1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(tv.coef64 + e + 2));
+                ch.c[0] = c0[0];
+                ch.c[1] = c0[1];
+                ch.c[2] = c1[0];
+                ch.c[3] = c1[1];
+            }
+        } else {
+            ch.sp[0] = ch.sp[1] = pad2;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) ch.c[k] = 0;
         }
     }
+}
+
+// Entries per lane block: one block per lane covers 256 entries of a family (an
+// interior family at delta = 3.015 dx holds 250) for the FP32 operations; the FP64
+// operations keep 4 (fewer registers).
+template <class OP, int LPN>
+constexpr int kBlock = (sizeof(typename OP::Coef) == 8) ? 4 : 256 / LPN;
+
+// First entry of lane sl's block in a row starting at e0.
+template <class OP, int LPN>
+__device__ __forceinline__ int64_t lane_start(int64_t e0, int sl)
+{
+    return e0 + (int64_t)(kBlock<OP, LPN> >= 8 ? 8 : kBlock<OP, LPN>) * sl;
 }
 
 // One round of a lane group: the family of tile node loc (blocks from `cur`, which the
 // caller loaded), with the next round's first block loaded into `nxt` beforehand.
 template <class OP, bool ENERGY, bool UE, int LPN>
 __device__ __forceinline__ void family_round(const TileView& tv, const Lds& l, int nt, int loc, int64_t e0,
-                                             int64_t e1, int sl, const Chunk<OP>& cur, double ivc4)
+                                             int64_t e1, int sl, const Chunk<OP, kBlock<OP, LPN>>& cur, double ivc4)
 {
-    constexpr int B = Chunk<OP>::B;
+    constexpr int B = kBlock<OP, LPN>;
     constexpr int CH = LPN * B;
     const int ss = (loc < nt) ? l.self[loc] : l.pad;
     const typename OP::SelfT me = OP::SelfT::make(l, tv, ss, UE);
     typename OP::Acc acc{};
 #pragma unroll
-    for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, cur.s[k], cur.c[k], me, acc);
+    for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, cur.slot(k), cur.c[k], me, acc);
     flush<OP>(acc);
     for (int64_t base = e0 + CH; base < e1; base += CH) {
-        Chunk<OP> more;
-        load_block<OP>(tv, base + B * sl, e1, l.pad, more);
+        Chunk<OP, B> more;
+        load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(base, sl), e1, l.pad, more);
 #pragma unroll
-        for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, more.s[k], more.c[k], me, acc);
+        for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, more.slot(k), more.c[k], me, acc);
         flush<OP>(acc);
     }
     OP::template reduce<LPN>(acc);
@@ -740,7 +783,7 @@ template <class OP, bool ENERGY, bool UE, int NW, int NPW>
 __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, int nt)
 {
     constexpr int LPN = 64 / NPW;
-    constexpr int B = Chunk<OP>::B;
+    constexpr int B = kBlock<OP, LPN>;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane / LPN, sl = lane % LPN;
     // FP32 records are scaled by S: the energy sum coef s (s r S) carries one factor S
@@ -749,8 +792,8 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, 
     if (first >= nt) return;
     int loc = first + g;
     int64_t e0 = (loc < nt) ? l.row[loc] : 0, e1 = (loc < nt) ? l.row[loc + 1] : 0;
-    Chunk<OP> c0, c1;
-    load_block<OP>(tv, e0 + B * sl, e1, l.pad, c0);
+    Chunk<OP, B> c0, c1;
+    load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(e0, sl), e1, l.pad, c0);
     if constexpr (!kF32Rec<OP>) {
         // FP64 operations: one block buffer and a copy per round (fewer registers)
         for (;;) {
@@ -762,7 +805,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, 
                     n0 = l.row[nloc];
                     n1 = l.row[nloc + 1];
                 }
-                load_block<OP>(tv, n0 + B * sl, n1, l.pad, c1);
+                load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(n0, sl), n1, l.pad, c1);
             }
             family_round<OP, ENERGY, UE, LPN>(tv, l, nt, loc, e0, e1, sl, c0, ivc4);
             if (nfirst >= nt) return;
@@ -783,7 +826,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, 
                 n0 = l.row[nloc];
                 n1 = l.row[nloc + 1];
             }
-            load_block<OP>(tv, n0 + B * sl, n1, l.pad, c1);
+            load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(n0, sl), n1, l.pad, c1);
         }
         family_round<OP, ENERGY, UE, LPN>(tv, l, nt, loc, e0, e1, sl, c0, ivc4);
         if (nfirst >= nt) break;
@@ -801,7 +844,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, 
                 n0 = l.row[nloc];
                 n1 = l.row[nloc + 1];
             }
-            load_block<OP>(tv, n0 + B * sl, n1, l.pad, c0);
+            load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(n0, sl), n1, l.pad, c0);
         }
         family_round<OP, ENERGY, UE, LPN>(tv, l, nt, loc, e0, e1, sl, c1, ivc4);
         if (nfirst >= nt) break;
@@ -873,8 +916,9 @@ struct EpiLoad {
 };
 
 // One RK4 stage (STAGE 0..3) for one tile.  Reads a.uin, writes a.uout.
+// At most 128 VGPRs (4 waves per SIMD): two 8-wave tiles per CU.
 template <int STAGE, int PREC, bool ENERGY, bool UE, int NW, int NPW>
-__global__ __launch_bounds__(64 * NW) void k_stage(StageArgs a, int step_idx)
+__global__ __launch_bounds__(64 * NW, 4) void k_stage(StageArgs a, int step_idx)
 {
     extern __shared__ double L[];
     __shared__ double sh_e[2][NW];
