@@ -78,7 +78,7 @@ def run_precision(args, spec, precision):
     ms_kernel, launches = prob.kernel_timing()
     U, _ = prob.state()
     res = dict(n=prob.n, E=prob.E, seconds=t1 - t0, init_seconds=t_init, kernel_ms=ms_kernel, launches=launches,
-               U=U, energy=prob.energy())
+               U=U, energy=prob.energy(), tuning=prob.tuning())
     prob.close()
     return res
 
@@ -213,6 +213,8 @@ def main():
             "precision": head_prec,
             "nodes": n,
             "family_entries": E,
+            "layout": head["tuning"],
+            "layout_fp64": res["fp64"]["tuning"] if "fp64" in res else None,
         },
         "fp64_value": (4.0 * E * args.steps / res["fp64"]["seconds"]) if "fp64" in res else None,
         "fp32c_over_fp64": (res["fp64"]["seconds"] / res["fp32c"]["seconds"]) if len(res) == 2 else None,

@@ -51,9 +51,11 @@ typedef struct {
     int use_graph;          /* capture each chunk of RK4 steps in a hipGraph (default 1) */
     int graph_chunk;        /* steps per captured graph (default 50) */
     int energies;           /* compute KE/EE every step (reference behaviour, default 1) */
-    int tile_edge;          /* tile edge in grid spacings, 2..8 (0 = autotuned at pdm_init) */
-    int waves_per_tile;     /* wavefronts per tile workgroup, 4 or 8 (0 = autotuned) */
-    int families_per_wave;  /* families a wavefront sums at once, 1, 2 or 4 (0 = autotuned) */
+    int tile_edge;          /* tile edge in grid spacings, 2..7 */
+    int waves_per_tile;     /* wavefronts per tile workgroup, 4 or 8 */
+    int families_per_wave;  /* families a wavefront sums at once, 1, 2 or 4; with all three 0 the
+                               layout is autotuned at pdm_init (specimens with >= 4M family
+                               entries; smaller ones take 4, 8, 2) */
     int verbose;
 } pdm_options;
 
@@ -168,6 +170,13 @@ int pdm_apply_tangent(pdm_handle* h, const double* u, double* y);
 /* Tecplot dump, same line format as pdSolve::WriteTecplot / pdNode::Print
  * (pdSolve.cpp:708-732, pdNode.cpp:418-433). */
 int pdm_write_tecplot(const pdm_handle* h, const char* path, int step);
+
+/* Tile layout of the gather kernels: tile edge (grid spacings), wavefronts per tile,
+ * families per wavefront, and, when the autotuner chose them (all three options 0 and
+ * at least 4M family entries), the measured time of one stage launch in ms (else 0).
+ * No counterpart in the reference; it reports the GPU layout choice. */
+int pdm_get_tuning(const pdm_handle* h, int* tile_edge, int* waves_per_tile, int* families_per_wave,
+                   double* stage_ms);
 
 /* Time of the last kernel launches, from HIP events on the handle's stream:
  * average duration in ms of the dominant (bond-force stage) kernel over the last

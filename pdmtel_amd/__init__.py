@@ -85,6 +85,8 @@ def lib() -> C.CDLL:
     L.pdm_solve_static.argtypes = [vp, C.c_double, C.c_int, C.POINTER(StaticStats)]
     L.pdm_write_tecplot.argtypes = [vp, C.c_char_p, C.c_int]
     L.pdm_last_kernel_timing.argtypes = [vp, dp, i64p]
+    ip = C.POINTER(C.c_int)
+    L.pdm_get_tuning.argtypes = [vp, ip, ip, ip, dp]
     _lib = L
     return L
 
@@ -214,6 +216,12 @@ class Problem:
         ms, n = C.c_double(), C.c_int64()
         self._chk(self.L.pdm_last_kernel_timing(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def tuning(self) -> dict:
+        """Tile layout in use (autotuned at init for large specimens when no layout option is set)."""
+        t, w, f, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self._chk(self.L.pdm_get_tuning(self.h, C.byref(t), C.byref(w), C.byref(f), C.byref(ms)))
+        return dict(tile_edge=t.value, waves_per_tile=w.value, families_per_wave=f.value, tuned_stage_ms=ms.value)
 
     def solve_dynamic(self, out_dir: str) -> None:
         self._chk(self.L.pdm_solve_dynamic(self.h, out_dir.encode()))

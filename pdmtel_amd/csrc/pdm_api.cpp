@@ -250,6 +250,103 @@ void plan_steps(pdm_handle* h, int want, std::vector<StepParam>& sp)
     }
 }
 
+constexpr int64_t kTuneMinEntries = 4000000;  // smaller specimens take the default layout
+
+// Release the tile layout (everything build_tiles allocates), so it can be rebuilt.
+void free_tiles(pdm_handle* h)
+{
+    dfree(h->permd); dfree(h->Xi); dfree(h->u[0]); dfree(h->u[1]); dfree(h->rpi); dfree(h->slot);
+    dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->mati); dfree(h->bci); dfree(h->nb);
+    dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr); dfree(h->halo);
+    dfree(h->part); dfree(h->part2);
+    h->ntl = 0;
+    h->ntiles = 0;
+}
+
+// Per-tile energy partials for the current layout.
+int alloc_parts(pdm_handle* h)
+{
+    dfree(h->part);
+    dfree(h->part2);
+    HCHK(hipMalloc(&h->part, sizeof(double) * 2 * std::max<int64_t>(h->ntl, 1)));
+    HCHK(hipMemsetAsync(h->part, 0, sizeof(double) * 2 * std::max<int64_t>(h->ntl, 1), h->stream));
+    HCHK(hipMalloc(&h->part2, sizeof(double) * 2 * energy_part2_size(h->ntl)));
+    return PDM_OK;
+}
+
+// Zero the dynamic state and the stage ping-pong buffers.
+int zero_state(pdm_handle* h)
+{
+    const int64_t n = std::max<int64_t>(h->n, 1);
+    for (d4* p : {h->U, h->V, h->Vb, h->k1, h->k23, h->u[0], h->u[1]})
+        HCHK(hipMemsetAsync(p, 0, sizeof(d4) * n, h->stream));
+    HCHK(hipMemsetAsync(h->elast, 0, sizeof(double) * n, h->stream));
+    HCHK(hipStreamSynchronize(h->stream));
+    return PDM_OK;
+}
+
+// Autotuner: time whole RK4 steps on this specimen for every candidate tile edge,
+// wavefronts per tile and families per wavefront, keep the fastest.  The stage
+// kernel's speed depends on the halo-to-tile ratio (tile edge), on the LDS and
+// register footprint that sets the tiles resident per CU, and on how many lanes share
+// a family; the best point differs between precisions and horizons.
+int autotune(pdm_handle* h)
+{
+    static const int kEdges[] = {4, 5, 6};
+    static const int kWaves[] = {8, 4};
+    static const int kFams[] = {2, 4};
+    double best = 1e300;
+    int bt = 0, bw = 8, bf = 2;
+    int rc = ensure_chunk(h, 4);
+    if (rc) return rc;
+    for (int t : kEdges) {
+        h->tile_edge[0] = h->tile_edge[1] = h->tile_edge[2] = t;
+        rc = build_tiles(h);
+        if (rc == PDM_E_LIMIT) {  // this edge does not fit the LDS or halo limits
+            free_tiles(h);
+            h->err.clear();
+            continue;
+        }
+        if (rc) return rc;
+        rc = alloc_parts(h);
+        if (rc) return rc;
+        const StepParam sp[4] = {{h->dt_stable, h->dt_stable}, {h->dt_stable, h->dt_stable},
+                                 {h->dt_stable, h->dt_stable}, {h->dt_stable, h->dt_stable}};
+        HCHK(hipMemcpyAsync(h->steps, sp, sizeof sp, hipMemcpyHostToDevice, h->stream));
+        for (int w : kWaves)
+            for (int f : kFams) {
+                h->waves = w;
+                h->fam_per_wave = f;
+                rc = enqueue_steps(h, 1);  // warm-up (first launch sets the LDS limit)
+                if (rc) return rc;
+                HCHK(hipEventRecord(h->ev0, h->stream));
+                rc = enqueue_steps(h, 3);
+                if (rc) return rc;
+                HCHK(hipEventRecord(h->ev1, h->stream));
+                HCHK(hipEventSynchronize(h->ev1));
+                float ms = 0.f;
+                HCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+                const double per = ms / 12.0;
+                if (h->opt.verbose)
+                    fprintf(stderr, "pdm_init: tune edge %d waves %d families %d: %.4f ms per stage\n", t, w, f, per);
+                if (per < best) {
+                    best = per;
+                    bt = t;
+                    bw = w;
+                    bf = f;
+                }
+            }
+        free_tiles(h);
+    }
+    if (!bt) return fail(h, PDM_E_LIMIT, "no tile edge fits this horizon");
+    h->tile_edge[0] = h->tile_edge[1] = h->tile_edge[2] = bt;
+    h->waves = bw;
+    h->fam_per_wave = bf;
+    h->tuned = 1;
+    h->tune_ms = best;
+    return build_tiles(h);
+}
+
 }  // namespace
 
 extern "C" {
@@ -403,14 +500,8 @@ int pdm_init(pdm_handle* h)
         if (f != 1 && f != 2 && f != 4) return fail(h, PDM_E_ARG, "families_per_wave must be 1, 2 or 4");
         h->fam_per_wave = f;
     }
-    rc = build_tiles(h);
-    if (rc) return rc;
-    if (h->opt.verbose)
-        fprintf(stderr, "pdm_init: nodes=%lld entries=%lld tiles=%lld (edge %dx%dx%d dx) tile_max=%d halo_max=%d "
-                        "halo_mean=%.1f lds=%zu B\n",
-                (long long)h->n, (long long)h->E, (long long)h->ntl, h->tile_edge[0], h->tile_edge[1],
-                h->tile_edge[2], h->tile_max, h->halo_max, h->ntl ? (double)h->halo_total / (double)h->ntl : 0.0,
-                stage_lds_bytes(tile_view(h), !h->uniform_ecrit, true, h->opt.precision != PDM_FP64));
+    // CalcStableTimeStep (pdDataManager2.cpp:6-10)
+    h->dt_stable = d.fixed_dt_on ? d.fixed_dt : d.safety * h->min_bond / h->max_c;
     int64_t n = std::max<int64_t>(h->n, 1);
     for (d4** p : {&h->U, &h->V, &h->Vb, &h->k1, &h->k23}) {
         HCHK(hipMalloc(p, sizeof(d4) * n));
@@ -418,12 +509,23 @@ int pdm_init(pdm_handle* h)
     }
     HCHK(hipMalloc(&h->elast, sizeof(double) * n));
     HCHK(hipMemsetAsync(h->elast, 0, sizeof(double) * n, h->stream));
-    HCHK(hipMalloc(&h->part, sizeof(double) * 2 * std::max<int64_t>(h->ntl, 1)));
-    HCHK(hipMemsetAsync(h->part, 0, sizeof(double) * 2 * std::max<int64_t>(h->ntl, 1), h->stream));
-    HCHK(hipMalloc(&h->part2, sizeof(double) * 2 * energy_part2_size(h->ntl)));
-    HCHK(hipStreamSynchronize(h->stream));
-    // CalcStableTimeStep (pdDataManager2.cpp:6-10) and the driver's clock
-    h->dt_stable = d.fixed_dt_on ? d.fixed_dt : d.safety * h->min_bond / h->max_c;
+    const bool tune = !h->opt.tile_edge && !h->opt.waves_per_tile && !h->opt.families_per_wave &&
+                      h->E >= kTuneMinEntries;
+    rc = tune ? autotune(h) : build_tiles(h);
+    if (rc) return rc;
+    rc = alloc_parts(h);
+    if (rc) return rc;
+    rc = zero_state(h);
+    if (rc) return rc;
+    h->last_kernel_ms = 0.0;
+    h->last_kernel_launches = 0;
+    if (h->opt.verbose)
+        fprintf(stderr, "pdm_init: nodes=%lld entries=%lld tiles=%lld (edge %dx%dx%d dx) tile_max=%d halo_max=%d "
+                        "halo_mean=%.1f lds=%zu B\n",
+                (long long)h->n, (long long)h->E, (long long)h->ntl, h->tile_edge[0], h->tile_edge[1],
+                h->tile_edge[2], h->tile_max, h->halo_max, h->ntl ? (double)h->halo_total / (double)h->ntl : 0.0,
+                stage_lds_bytes(tile_view(h), !h->uniform_ecrit, true, h->opt.precision != PDM_FP64));
+    // the driver's clock
     h->dt_cur = h->dt_stable;
     h->t = 0.0;
     h->step = 0;
@@ -592,6 +694,18 @@ int pdm_get_energy(const pdm_handle* h, double* ke, double* ee)
     if (rc) return rc;
     if (ke) *ke = h->ke;
     if (ee) *ee = h->ee;
+    return PDM_OK;
+}
+
+int pdm_get_tuning(const pdm_handle* h, int* tile_edge, int* waves_per_tile, int* families_per_wave,
+                   double* stage_ms)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (tile_edge) *tile_edge = h->tile_edge[0];
+    if (waves_per_tile) *waves_per_tile = h->waves;
+    if (families_per_wave) *families_per_wave = h->fam_per_wave;
+    if (stage_ms) *stage_ms = h->tuned ? h->tune_ms : 0.0;
     return PDM_OK;
 }
 

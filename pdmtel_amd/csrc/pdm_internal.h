@@ -185,6 +185,8 @@ struct pdm_handle {
     int tile_edge[3] = {4, 4, 4};   // tile edge in grid spacings
     int waves = 8;                  // wavefronts per tile workgroup
     int fam_per_wave = 2;           // families a wavefront sums at once
+    int tuned = 0;                  // the layout was chosen by timing candidates at pdm_init
+    double tune_ms = 0.0;           // stage launch time of the chosen layout (ms)
     int uniform_ecrit = 1;
 
     pdm::d4 *U = nullptr, *V = nullptr, *Vb = nullptr, *k1 = nullptr, *k23 = nullptr;

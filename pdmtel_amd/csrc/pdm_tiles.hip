@@ -416,6 +416,18 @@ int build_tiles(pdm_handle* h)
     uint32_t* packed = nullptr;
     int64_t* halo_len = nullptr;
     int* err = nullptr;
+    // temporaries are released on every return path (after the stream has drained)
+    struct Temps {
+        hipStream_t s;
+        std::vector<void**> p;
+        ~Temps()
+        {
+            (void)hipStreamSynchronize(s);
+            for (void** q : p)
+                if (*q) (void)hipFree(*q);
+        }
+    } temps{s, {(void**)&ipermd, (void**)&tileidd, (void**)&bcref, (void**)&ecm, (void**)&bei, (void**)&packed,
+                (void**)&halo_len, (void**)&err}};
     const int64_t n1 = std::max<int64_t>(n, 1), E1 = std::max<int64_t>(Ei, 1);
     HIPCHK(hipMalloc(&h->permd, sizeof(int32_t) * n1));
     HIPCHK(hipMalloc(&ipermd, sizeof(int32_t) * n1));
@@ -552,14 +564,6 @@ int build_tiles(pdm_handle* h)
         HIPCHK(hipMemsetAsync(h->u[b], 0, sizeof(d4) * n1, s));
     }
     HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(hipFree(ipermd));
-    HIPCHK(hipFree(tileidd));
-    HIPCHK(hipFree(bcref));
-    HIPCHK(hipFree(ecm));
-    HIPCHK(hipFree(bei));
-    HIPCHK(hipFree(packed));
-    HIPCHK(hipFree(halo_len));
-    HIPCHK(hipFree(err));
     return PDM_OK;
 }
 
