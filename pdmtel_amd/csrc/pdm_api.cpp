@@ -159,6 +159,8 @@ int upload_state(pdm_handle* h, const double* U, const double* V)
     return PDM_OK;
 }
 
+int alloc_parts(pdm_handle* h);
+
 int ensure_chunk(pdm_handle* h, int cap)
 {
     if (h->chunk_cap >= cap) return PDM_OK;
@@ -172,7 +174,7 @@ int ensure_chunk(pdm_handle* h, int cap)
         h->graph_exec = nullptr;
         h->graph_steps = 0;
     }
-    return PDM_OK;
+    return h->tiles ? alloc_parts(h) : PDM_OK;  // one row of tile partials per step
 }
 
 // Enqueue `count` steps whose parameters sit in h->steps[0..count).
@@ -185,8 +187,9 @@ int enqueue_steps(pdm_handle* h, int count)
             a.uout = h->u[(st + 1) & 1];
             HCHK(launch_stage(st, h->opt.precision, a, k, h->stream));
         }
-        if (a.energies) HCHK(launch_energy_final(h->part, h->ntl, h->part2, h->energy + 2 * k, h->stream));
     }
+    // per-step KE / EE totals, one launch for the chunk
+    if (a.energies) HCHK(launch_energy_steps(h->part, h->ntl, count, h->energy, h->stream));
     return PDM_OK;
 }
 
@@ -258,19 +261,21 @@ void free_tiles(pdm_handle* h)
     dfree(h->permd); dfree(h->Xi); dfree(h->u[0]); dfree(h->u[1]); dfree(h->rpi); dfree(h->slot);
     dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->mati); dfree(h->bci); dfree(h->nb);
     dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr); dfree(h->halo);
-    dfree(h->part); dfree(h->part2);
+    dfree(h->part);
+    h->part_cap = 0;
     h->ntl = 0;
     h->ntiles = 0;
 }
 
-// Per-tile energy partials for the current layout.
+// Per-step, per-tile energy partials for the current layout and chunk capacity.
 int alloc_parts(pdm_handle* h)
 {
     dfree(h->part);
-    dfree(h->part2);
-    HCHK(hipMalloc(&h->part, sizeof(double) * 2 * std::max<int64_t>(h->ntl, 1)));
-    HCHK(hipMemsetAsync(h->part, 0, sizeof(double) * 2 * std::max<int64_t>(h->ntl, 1), h->stream));
-    HCHK(hipMalloc(&h->part2, sizeof(double) * 2 * energy_part2_size(h->ntl)));
+    const int64_t steps = std::max<int64_t>(h->chunk_cap, 1);
+    const size_t bytes = sizeof(double) * 2 * (size_t)std::max<int64_t>(h->ntl, 1) * (size_t)steps;
+    HCHK(hipMalloc(&h->part, bytes));
+    HCHK(hipMemsetAsync(h->part, 0, bytes, h->stream));
+    h->part_cap = steps;
     return PDM_OK;
 }
 
@@ -401,7 +406,7 @@ int pdm_destroy(pdm_handle* h)
     dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->mati); dfree(h->bci); dfree(h->nb);
     dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr); dfree(h->halo);
     dfree(h->U); dfree(h->V); dfree(h->Vb); dfree(h->k1); dfree(h->k23);
-    dfree(h->elast); dfree(h->part); dfree(h->part2); dfree(h->energy); dfree(h->steps);
+    dfree(h->elast); dfree(h->part); dfree(h->energy); dfree(h->steps);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);

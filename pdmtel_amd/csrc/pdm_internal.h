@@ -112,7 +112,7 @@ struct StageArgs {
     d4* k1;
     d4* k23;
     double* elast;          // per-node elastic energy density, summed over the 4 stages
-    double* part;           // per-tile partial KE / EE (stage 4)
+    double* part;           // per-step, per-tile partial KE / EE of stage 4: [step][tile][2]
     const StepParam* steps; // per-step t, dt
     double damping;
     double vol;             // pow(dx, 3): node volume for energies
@@ -191,8 +191,8 @@ struct pdm_handle {
 
     pdm::d4 *U = nullptr, *V = nullptr, *Vb = nullptr, *k1 = nullptr, *k23 = nullptr;
     double* elast = nullptr;
-    double* part = nullptr;        // per-tile partial energies
-    double* part2 = nullptr;       // second-level partials
+    double* part = nullptr;        // per-step, per-tile partial energies ([chunk_cap][ntl][2])
+    int64_t part_cap = 0;          // steps the partial buffer holds
     double* energy = nullptr;      // per-step KE, EE (chunk)
     pdm::StepParam* steps = nullptr;
     int chunk_cap = 0;
@@ -230,7 +230,7 @@ TileView tile_view(const pdm_handle* h);
 
 // kernels (pdm_kernels.hip)
 hipError_t launch_stage(int stage, int precision, const StageArgs& a, int step_idx, hipStream_t s);
-hipError_t launch_energy_final(const double* part, int64_t nparts, double* part2, double* out2, hipStream_t s);
+hipError_t launch_energy_steps(const double* part, int64_t ntl, int nsteps, double* out, hipStream_t s);
 hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, double* elast, hipStream_t s);
 hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s);
 hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s);
@@ -238,6 +238,5 @@ hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream
 // static solve (pdm_cg.cpp / pdm_cg.hip)
 int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_stats* stats);
 size_t stage_lds_bytes(const TileView& tv, bool with_ecrit, bool with_epi, bool f32);
-int64_t energy_part2_size(int64_t nparts);
 
 }  // namespace pdm

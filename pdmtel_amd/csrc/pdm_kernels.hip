@@ -1054,21 +1054,22 @@ __global__ __launch_bounds__(64 * NW, 4) void k_stage(StageArgs a, int step_idx)
         if (threadIdx.x == 0) {
             double ks = 0.0, es = 0.0;
             for (int w = 0; w < NW; ++w) { ks += sh_e[0][w]; es += sh_e[1][w]; }
-            a.part[2 * k] = ks;
-            a.part[2 * k + 1] = es;
+            double* part = a.part + 2 * (tv.ntl * (int64_t)step_idx + k);  // [step][tile]
+            part[0] = ks;
+            part[1] = es;
         }
     }
 }
 
-// Sum pairs (ke, ee): block b reduces a contiguous chunk of `in` into out[2b], out[2b+1].
-__global__ __launch_bounds__(256) void k_reduce_pairs(const double* __restrict__ in, int64_t n, int64_t chunk,
+// Per-step energy totals: block s sums the ntl tile partials (ke, ee) of step s into
+// out[2s], out[2s+1], in a fixed order.
+__global__ __launch_bounds__(256) void k_reduce_steps(const double* __restrict__ part, int64_t ntl,
                                                       double* __restrict__ out)
 {
     __shared__ double sk[256], se[256];
-    const int64_t b0 = (int64_t)blockIdx.x * chunk;
-    const int64_t b1 = (b0 + chunk < n) ? b0 + chunk : n;
+    const double* in = part + 2 * ntl * (int64_t)blockIdx.x;
     double ks = 0.0, es = 0.0;
-    for (int64_t q = b0 + threadIdx.x; q < b1; q += 256) {
+    for (int64_t q = threadIdx.x; q < ntl; q += 256) {
         ks += in[2 * q];
         es += in[2 * q + 1];
     }
@@ -1198,7 +1199,6 @@ size_t stage_lds_bytes(const TileView& tv, bool with_ecrit, bool with_epi, bool 
     return lds_bytes(tv.halo_max, tv.tile_max, with_ecrit, with_epi, f32);
 }
 
-int64_t energy_part2_size(int64_t nparts) { return nparts / 1024 + 2; }
 
 hipError_t launch_stage(int stage, int precision, const StageArgs& a, int step_idx, hipStream_t s)
 {
@@ -1211,15 +1211,10 @@ hipError_t launch_stage(int stage, int precision, const StageArgs& a, int step_i
     }
 }
 
-hipError_t launch_energy_final(const double* part, int64_t nparts, double* part2, double* out2, hipStream_t s)
+hipError_t launch_energy_steps(const double* part, int64_t ntl, int nsteps, double* out, hipStream_t s)
 {
-    const int64_t chunk = 1024;
-    int64_t nb = (nparts + chunk - 1) / chunk;
-    if (nb < 1) nb = 1;
-    k_reduce_pairs<<<dim3((unsigned)nb), dim3(256), 0, s>>>(part, nparts, chunk, part2);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    k_reduce_pairs<<<dim3(1), dim3(256), 0, s>>>(part2, nb, nb, out2);
+    if (nsteps <= 0) return hipSuccess;
+    k_reduce_steps<<<dim3((unsigned)nsteps), dim3(256), 0, s>>>(part, ntl, out);
     return hipGetLastError();
 }
 
