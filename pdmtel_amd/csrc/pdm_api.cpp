@@ -115,9 +115,33 @@ StageArgs stage_args(pdm_handle* h)
     a.elast = h->elast;
     a.part = h->part;
     a.steps = h->steps;
+    a.mats = h->mats_d;
+    a.bcs = h->bcs_d;
+    a.nspec = h->nspec;
     a.energies = h->opt.energies ? 1 : 0;
-    fill_stage_tables(h, a);
     return a;
+}
+
+// Constants and tables of a single specimen (after each tile build: the FP32 record
+// scale depends on the tile edge).
+int upload_spec(pdm_handle* h)
+{
+    if (!h->specs_d) {
+        HCHK(hipMalloc(&h->specs_d, sizeof(SpecDev)));
+        HCHK(hipMalloc(&h->mats_d, sizeof(MatDev) * kMaxMat));
+        HCHK(hipMalloc(&h->bcs_d, sizeof(BCDev) * kMaxBC));
+        HCHK(hipMalloc(&h->spec_tile_ptr_d, sizeof(int64_t) * 2));
+    }
+    SpecDev sp{};
+    std::vector<MatDev> mats(kMaxMat);
+    std::vector<BCDev> bcs(kMaxBC);
+    fill_spec_tables(h, sp, mats.data(), bcs.data());
+    const int64_t tp[2] = {0, h->ntl};
+    HCHK(hipMemcpy(h->specs_d, &sp, sizeof sp, hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(h->mats_d, mats.data(), sizeof(MatDev) * kMaxMat, hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(h->bcs_d, bcs.data(), sizeof(BCDev) * kMaxBC, hipMemcpyHostToDevice));
+    HCHK(hipMemcpy(h->spec_tile_ptr_d, tp, sizeof tp, hipMemcpyHostToDevice));
+    return PDM_OK;
 }
 
 // Host AoS (reference order) <-> device d4 (internal order).
@@ -189,7 +213,8 @@ int enqueue_steps(pdm_handle* h, int count)
         }
     }
     // per-step KE / EE totals, one launch for the chunk
-    if (a.energies) HCHK(launch_energy_steps(h->part, h->ntl, count, h->energy, h->stream));
+    if (a.energies)
+        HCHK(launch_energy_steps(h->part, h->ntl, h->spec_tile_ptr_d, h->nspec, count, h->energy, h->stream));
     return PDM_OK;
 }
 
@@ -315,6 +340,8 @@ int autotune(pdm_handle* h)
         if (rc) return rc;
         rc = alloc_parts(h);
         if (rc) return rc;
+        rc = upload_spec(h);
+        if (rc) return rc;
         const StepParam sp[4] = {{h->dt_stable, h->dt_stable}, {h->dt_stable, h->dt_stable},
                                  {h->dt_stable, h->dt_stable}, {h->dt_stable, h->dt_stable}};
         HCHK(hipMemcpyAsync(h->steps, sp, sizeof sp, hipMemcpyHostToDevice, h->stream));
@@ -407,6 +434,7 @@ int pdm_destroy(pdm_handle* h)
     dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr); dfree(h->halo);
     dfree(h->U); dfree(h->V); dfree(h->Vb); dfree(h->k1); dfree(h->k23);
     dfree(h->elast); dfree(h->part); dfree(h->energy); dfree(h->steps);
+    dfree(h->specs_d); dfree(h->mats_d); dfree(h->bcs_d); dfree(h->tile_spec_d); dfree(h->spec_tile_ptr_d);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -517,6 +545,8 @@ int pdm_init(pdm_handle* h)
     const bool tune = !h->opt.tile_edge && !h->opt.waves_per_tile && !h->opt.families_per_wave &&
                       h->E >= kTuneMinEntries;
     rc = tune ? autotune(h) : build_tiles(h);
+    if (rc) return rc;
+    rc = upload_spec(h);
     if (rc) return rc;
     rc = alloc_parts(h);
     if (rc) return rc;

@@ -72,6 +72,17 @@ struct StepParam {
     double t, dt;
 };
 
+// Constants of one specimen.  A launch covers the tiles of one or many specimens;
+// each tile reads its specimen's entry (tile_spec), and its material and BC tables at
+// mats[spec * kMaxMat], bcs[spec * kMaxBC].
+struct SpecDev {
+    double ecrit0;        // common ecrit of the specimen's materials (when uniform_ecrit)
+    double inv_vol_cube;  // 1 / dx^3
+    double fscale;        // fixed-point scale of the FP32 halo records
+    double damping;       // damping_coeff
+    double vol;           // pow(dx, 3): node volume for the energies
+};
+
 // Tile-gather inputs shared by every tiled kernel.
 struct TileView {
     const int32_t* tiles;      // launch index -> tile id
@@ -92,10 +103,9 @@ struct TileView {
     int xcd_remap;
     int waves;                 // wavefronts per tile workgroup (4 or 8)
     int fam_per_wave;          // families a wavefront sums at once (1, 2 or 4)
-    int uniform_ecrit;         // all materials share ecrit: no per-neighbour ecrit in LDS
-    double ecrit0;             // that common ecrit
-    double inv_vol_cube;       // 1 / (dx*dx*dx)
-    double fscale;             // scale S of the FP32 halo records (fixed-point positions)
+    int uniform_ecrit;         // every specimen's materials share one ecrit: none per neighbour in LDS
+    const SpecDev* specs;      // per-specimen constants
+    const int32_t* tile_spec;  // launch index -> specimen (nullptr: one specimen)
 };
 
 // Arguments of the RK4 stage kernels.
@@ -113,13 +123,11 @@ struct StageArgs {
     d4* k23;
     double* elast;          // per-node elastic energy density, summed over the 4 stages
     double* part;           // per-step, per-tile partial KE / EE of stage 4: [step][tile][2]
-    const StepParam* steps; // per-step t, dt
-    double damping;
-    double vol;             // pow(dx, 3): node volume for energies
-    int nmat, nbc;
+    const StepParam* steps; // [step][specimen] t, dt
+    const MatDev* mats;     // [specimen][kMaxMat]
+    const BCDev* bcs;       // [specimen][kMaxBC]
+    int nspec;
     int energies;
-    MatDev mats[kMaxMat];
-    BCDev bcs[kMaxBC];
 };
 
 // Bond-build parameters.
@@ -177,6 +185,12 @@ struct pdm_handle {
     int32_t* tiles = nullptr;
     int64_t* tile_start = nullptr;
     pdm::d4* tref = nullptr;            // launch index -> tile centre
+    pdm::SpecDev* specs_d = nullptr;    // per-specimen constants (one entry for a single specimen)
+    pdm::MatDev* mats_d = nullptr;      // [specimen][kMaxMat]
+    pdm::BCDev* bcs_d = nullptr;        // [specimen][kMaxBC]
+    int32_t* tile_spec_d = nullptr;     // launch index -> specimen (batches only)
+    int64_t* spec_tile_ptr_d = nullptr; // specimen -> first launch index (nspec + 1)
+    int nspec = 1;
     double fscale = 1.0;                // fixed-point scale of the FP32 halo records
     int64_t* halo_ptr = nullptr;
     int32_t* halo = nullptr;
@@ -218,7 +232,7 @@ int count_max_family(double horizon, double dx, double tol);
 double bdryeff_type_one(const Deck& d, double spring, double E, double h);
 double time_factor(const BoundaryCondition& b, double t);
 bool is_bc_node(const Deck& d, int bcid);
-void fill_stage_tables(const pdm_handle* h, StageArgs& a);
+void fill_spec_tables(const pdm_handle* h, SpecDev& sp, MatDev* mats, BCDev* bcs);
 
 // reference-order family search and boundary-effect factors (pdm_bonds.hip)
 int build_bonds(pdm_handle* h);
@@ -230,7 +244,8 @@ TileView tile_view(const pdm_handle* h);
 
 // kernels (pdm_kernels.hip)
 hipError_t launch_stage(int stage, int precision, const StageArgs& a, int step_idx, hipStream_t s);
-hipError_t launch_energy_steps(const double* part, int64_t ntl, int nsteps, double* out, hipStream_t s);
+hipError_t launch_energy_steps(const double* part, int64_t ntl, const int64_t* spec_tile_ptr, int nspec, int nsteps,
+                               double* out, hipStream_t s);
 hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, double* elast, hipStream_t s);
 hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s);
 hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s);

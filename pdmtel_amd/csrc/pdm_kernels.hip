@@ -39,6 +39,9 @@ namespace {
 
 constexpr double kFar = 1.0e3;          // coordinate of the padding record (m)
 
+// Specimen of launch index k (one specimen when the map is absent).
+__device__ __forceinline__ int tile_specimen(const TileView& tv, int64_t k) { return tv.tile_spec ? tv.tile_spec[k] : 0; }
+
 __device__ __forceinline__ int64_t launch_index(int remap)
 {
     // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
@@ -293,11 +296,12 @@ __device__ __forceinline__ u32x4 frec_a(const d4& x, const d4& v, const d4& xr, 
 // record, the row starts and own slots.  Two records per thread per pass, both id
 // loads issued before the record loads.
 template <bool F32>
-__device__ __forceinline__ void stage_tile(const TileView& tv, const d4* __restrict__ u, int64_t k, int64_t h0,
-                                           int H, int64_t a0, int nt, const Lds& l, bool with_ecrit)
+__device__ __forceinline__ void stage_tile(const TileView& tv, const SpecDev& sp, const d4* __restrict__ u,
+                                           int64_t k, int64_t h0, int H, int64_t a0, int nt, const Lds& l,
+                                           bool with_ecrit)
 {
     d4 xr{}, ur{};
-    const double S = tv.fscale;
+    const double S = sp.fscale;
     if (F32) {
         xr = tv.tref[k];
         ur = u[a0 + nt / 2];
@@ -343,10 +347,10 @@ __device__ __forceinline__ void stage_tile(const TileView& tv, const d4* __restr
 // The family owner's own record.
 struct Self {
     double x, y, z, u, v, w, ecrit;
-    __device__ static Self make(const Lds& l, const TileView& tv, int ss, bool ue)
+    __device__ static Self make(const Lds& l, const SpecDev& sp, int ss, bool ue)
     {
         const Rec r = l.rec[ss];
-        return Self{r.x, r.y, r.z, r.u, r.v, r.w, ue ? tv.ecrit0 : l.ec[ss]};
+        return Self{r.x, r.y, r.z, r.u, r.v, r.w, ue ? sp.ecrit0 : l.ec[ss]};
     }
 };
 
@@ -354,12 +358,12 @@ struct SelfF {
     int x, y, z;
     float u, v, w;
     float ecrit;
-    __device__ static SelfF make(const Lds& l, const TileView& tv, int ss, bool ue)
+    __device__ static SelfF make(const Lds& l, const SpecDev& sp, int ss, bool ue)
     {
         const u32x4 a = l.fa[ss];
         const f32x2 b = l.fb[ss];
         return SelfF{(int)a.x, (int)a.y, (int)a.z, __uint_as_float(a.w), b.x, b.y,
-                     ue ? (float)tv.ecrit0 : l.fec[ss]};
+                     ue ? (float)sp.ecrit0 : l.fec[ss]};
     }
 };
 
@@ -753,13 +757,14 @@ __device__ __forceinline__ int64_t lane_start(int64_t e0, int sl)
 // One round of a lane group: the family of tile node loc (blocks from `cur`, which the
 // caller loaded), with the next round's first block loaded into `nxt` beforehand.
 template <class OP, bool ENERGY, bool UE, int LPN>
-__device__ __forceinline__ void family_round(const TileView& tv, const Lds& l, int nt, int loc, int64_t e0,
-                                             int64_t e1, int sl, const Chunk<OP, kBlock<OP, LPN>>& cur, double ivc4)
+__device__ __forceinline__ void family_round(const TileView& tv, const SpecDev& sp, const Lds& l, int nt, int loc,
+                                             int64_t e0, int64_t e1, int sl, const Chunk<OP, kBlock<OP, LPN>>& cur,
+                                             double ivc4)
 {
     constexpr int B = kBlock<OP, LPN>;
     constexpr int CH = LPN * B;
     const int ss = (loc < nt) ? l.self[loc] : l.pad;
-    const typename OP::SelfT me = OP::SelfT::make(l, tv, ss, UE);
+    const typename OP::SelfT me = OP::SelfT::make(l, sp, ss, UE);
     typename OP::Acc acc{};
 #pragma unroll
     for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, cur.slot(k), cur.c[k], me, acc);
@@ -780,14 +785,14 @@ __device__ __forceinline__ void family_round(const TileView& tv, const Lds& l, i
 // is loaded before round r is computed.  Rounds alternate between two block
 // buffers, so no registers are copied between rounds.
 template <class OP, bool ENERGY, bool UE, int NW, int NPW>
-__device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, int nt)
+__device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev& sp, const Lds& l, int nt)
 {
     constexpr int LPN = 64 / NPW;
     constexpr int B = kBlock<OP, LPN>;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane / LPN, sl = lane % LPN;
     // FP32 records are scaled by S: the energy sum coef s (s r S) carries one factor S
-    const double ivc4 = 0.25 * tv.inv_vol_cube * (kF32Rec<OP> ? 1.0 / tv.fscale : 1.0);
+    const double ivc4 = 0.25 * sp.inv_vol_cube * (kF32Rec<OP> ? 1.0 / sp.fscale : 1.0);
     int first = wave * NPW;
     if (first >= nt) return;
     int loc = first + g;
@@ -807,7 +812,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, 
                 }
                 load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(n0, sl), n1, l.pad, c1);
             }
-            family_round<OP, ENERGY, UE, LPN>(tv, l, nt, loc, e0, e1, sl, c0, ivc4);
+            family_round<OP, ENERGY, UE, LPN>(tv, sp, l, nt, loc, e0, e1, sl, c0, ivc4);
             if (nfirst >= nt) return;
             first = nfirst;
             loc = nloc;
@@ -828,7 +833,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, 
             }
             load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(n0, sl), n1, l.pad, c1);
         }
-        family_round<OP, ENERGY, UE, LPN>(tv, l, nt, loc, e0, e1, sl, c0, ivc4);
+        family_round<OP, ENERGY, UE, LPN>(tv, sp, l, nt, loc, e0, e1, sl, c0, ivc4);
         if (nfirst >= nt) break;
         first = nfirst;
         loc = nloc;
@@ -846,7 +851,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const Lds& l, 
             }
             load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(n0, sl), n1, l.pad, c0);
         }
-        family_round<OP, ENERGY, UE, LPN>(tv, l, nt, loc, e0, e1, sl, c1, ivc4);
+        family_round<OP, ENERGY, UE, LPN>(tv, sp, l, nt, loc, e0, e1, sl, c1, ivc4);
         if (nfirst >= nt) break;
         first = nfirst;
         loc = nloc;
@@ -930,21 +935,28 @@ __global__ __launch_bounds__(64 * NW, 4) void k_stage(StageArgs a, int step_idx)
     const int nt = (int)(tv.tile_start[tid + 1] - a0);
     const int64_t h0 = tv.halo_ptr[k];
     const int H = (int)(tv.halo_ptr[k + 1] - h0);
+    const int spec = tile_specimen(tv, k);
+    const SpecDev sp = tv.specs[spec];
     constexpr bool F32 = kF32Rec<typename OpFor<PREC>::Force>;
     const Lds l = lds_view(L, H, tv.tile_max, !UE, true, F32);
     // the stage-update inputs (first node of each thread) are loaded together with the halo
     EpiLoad<STAGE, ENERGY> pre;
     const bool has_pre = (int)threadIdx.x < nt;
     if (has_pre) pre.load(a, a0 + threadIdx.x);
-    stage_tile<F32>(tv, a.uin, k, h0, H, a0, nt, l, !UE);
+    stage_tile<F32>(tv, sp, a.uin, k, h0, H, a0, nt, l, !UE);
     {
-        const double* src = reinterpret_cast<const double*>(a.mats);
-        double* dst = reinterpret_cast<double*>(l.mats);
-        constexpr int nw = (int)((sizeof(MatDev) * kMaxMat + sizeof(BCDev) * kMaxBC) / sizeof(double));
+        // the specimen's material and BC tables
         static_assert(sizeof(MatDev) % 8 == 0 && sizeof(BCDev) % 8 == 0, "tables are double words");
-        static_assert(offsetof(StageArgs, bcs) == offsetof(StageArgs, mats) + sizeof(MatDev) * kMaxMat,
-                      "bcs follow mats");
-        for (int q = threadIdx.x; q < nw; q += blockDim.x) dst[q] = src[q];
+        constexpr int nm = (int)(sizeof(MatDev) * kMaxMat / sizeof(double));
+        constexpr int nb = (int)(sizeof(BCDev) * kMaxBC / sizeof(double));
+        const double* ms = reinterpret_cast<const double*>(a.mats + (int64_t)spec * kMaxMat);
+        const double* bs = reinterpret_cast<const double*>(a.bcs + (int64_t)spec * kMaxBC);
+        double* md = reinterpret_cast<double*>(l.mats);
+        double* bd = reinterpret_cast<double*>(l.bcs);
+        for (int q = threadIdx.x; q < nm + nb; q += blockDim.x) {
+            if (q < nm) md[q] = ms[q];
+            else bd[q - nm] = bs[q - nm];
+        }
     }
     if (has_pre) pre.store(l.epi[threadIdx.x]);
     for (int loc = threadIdx.x + 64 * NW; loc < nt; loc += 64 * NW) {
@@ -953,11 +965,11 @@ __global__ __launch_bounds__(64 * NW, 4) void k_stage(StageArgs a, int step_idx)
         more.store(l.epi[loc]);
     }
     __syncthreads();
-    tile_families<typename OpFor<PREC>::Force, ENERGY, UE, NW, NPW>(tv, l, nt);
+    tile_families<typename OpFor<PREC>::Force, ENERGY, UE, NW, NPW>(tv, sp, l, nt);
     __syncthreads();
 
-    const StepParam sp = a.steps[step_idx];
-    const double dt = sp.dt;
+    const StepParam stp = a.steps[(int64_t)step_idx * a.nspec + spec];
+    const double dt = stp.dt;
     const double dts = (STAGE < 2) ? 0.5 * dt : dt;
     double ke_i = 0.0, ee_i = 0.0;
     for (int loc = threadIdx.x; loc < nt; loc += 64 * NW) {
@@ -977,7 +989,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_stage(StageArgs a, int step_idx)
         if (b >= 0) {
             const BCDev& bd = l.bcs[b];
             bcnode = bd.bcnode != 0;
-            fac = time_factor_dev(bd, sp.t);
+            fac = time_factor_dev(bd, stp.t);
             if (bd.type == 4)
                 for (int c = 0; c < 3; ++c) Ft[c] += bd.trac[c] * fac;
         }
@@ -1027,16 +1039,16 @@ __global__ __launch_bounds__(64 * NW, 4) void k_stage(StageArgs a, int step_idx)
             const double* Vp = ep.v[4];
             for (int c = 0; c < 3; ++c) {
                 if (bcnode) Up[c] = Up[c] + dt6 * (k1p[c] + kk[c] + 2.0 * k23p[c]);
-                else Up[c] = Up[c] + dt6 * (k1p[c] + kk[c] + 2.0 * k23p[c]) - dt * a.damping * Vp[c] * dt / rho;
+                else Up[c] = Up[c] + dt6 * (k1p[c] + kk[c] + 2.0 * k23p[c]) - dt * sp.damping * Vp[c] * dt / rho;
             }
-            for (int c = 0; c < 3; ++c) vbp[c] += -a.damping * Vp[c] * dt / rho;
+            for (int c = 0; c < 3; ++c) vbp[c] += -sp.damping * Vp[c] * dt / rho;
             a.V[i] = d4{vbp[0], vbp[1], vbp[2], 0.0};
             for (int c = 0; c < 3; ++c) nup[c] = Up[c];
             if (ENERGY) {
                 const double elt = ep.el + eli;
                 const double velsq = vbp[0] * vbp[0] + vbp[1] * vbp[1] + vbp[2] * vbp[2];
-                ee_i += a.vol * elt;
-                ke_i += 0.5 * rho * velsq * a.vol;
+                ee_i += sp.vol * elt;
+                ke_i += 0.5 * rho * velsq * sp.vol;
             }
         }
         if (b >= 0 || STAGE == 3) a.U[i] = d4{Up[0], Up[1], Up[2], 0.0};
@@ -1061,15 +1073,19 @@ __global__ __launch_bounds__(64 * NW, 4) void k_stage(StageArgs a, int step_idx)
     }
 }
 
-// Per-step energy totals: block s sums the ntl tile partials (ke, ee) of step s into
-// out[2s], out[2s+1], in a fixed order.
+// Per-step, per-specimen energy totals: block b = step * nspec + spec sums the tile
+// partials (ke, ee) of that step over the specimen's tiles into out[2b], out[2b+1], in a
+// fixed order.
 __global__ __launch_bounds__(256) void k_reduce_steps(const double* __restrict__ part, int64_t ntl,
+                                                      const int64_t* __restrict__ spec_tile_ptr, int nspec,
                                                       double* __restrict__ out)
 {
     __shared__ double sk[256], se[256];
-    const double* in = part + 2 * ntl * (int64_t)blockIdx.x;
+    const int64_t step = blockIdx.x / nspec, spec = blockIdx.x % nspec;
+    const double* in = part + 2 * ntl * step;
+    const int64_t t0 = spec_tile_ptr[spec], t1 = spec_tile_ptr[spec + 1];
     double ks = 0.0, es = 0.0;
-    for (int64_t q = threadIdx.x; q < ntl; q += 256) {
+    for (int64_t q = t0 + threadIdx.x; q < t1; q += 256) {
         ks += in[2 * q];
         es += in[2 * q + 1];
     }
@@ -1104,9 +1120,10 @@ __global__ __launch_bounds__(64 * NW) void k_gather(TileView tv, const d4* __res
     const int H = (int)(tv.halo_ptr[k + 1] - h0);
     constexpr bool F32 = kF32Rec<OP>;
     const Lds l = lds_view(L, H, tv.tile_max, !UE, false, F32);
-    stage_tile<F32>(tv, u, k, h0, H, a0, nt, l, !UE);
+    const SpecDev sp = tv.specs[tile_specimen(tv, k)];
+    stage_tile<F32>(tv, sp, u, k, h0, H, a0, nt, l, !UE);
     __syncthreads();
-    tile_families<OP, true, UE, NW, NPW>(tv, l, nt);
+    tile_families<OP, true, UE, NW, NPW>(tv, sp, l, nt);
     __syncthreads();
     for (int loc = threadIdx.x; loc < nt; loc += 64 * NW) {
         const double* res = l.res + 4 * loc;
@@ -1211,10 +1228,11 @@ hipError_t launch_stage(int stage, int precision, const StageArgs& a, int step_i
     }
 }
 
-hipError_t launch_energy_steps(const double* part, int64_t ntl, int nsteps, double* out, hipStream_t s)
+hipError_t launch_energy_steps(const double* part, int64_t ntl, const int64_t* spec_tile_ptr, int nspec, int nsteps,
+                               double* out, hipStream_t s)
 {
-    if (nsteps <= 0) return hipSuccess;
-    k_reduce_steps<<<dim3((unsigned)nsteps), dim3(256), 0, s>>>(part, ntl, out);
+    if (nsteps <= 0 || nspec <= 0) return hipSuccess;
+    k_reduce_steps<<<dim3((unsigned)(nsteps * nspec)), dim3(256), 0, s>>>(part, ntl, spec_tile_ptr, nspec, out);
     return hipGetLastError();
 }
 

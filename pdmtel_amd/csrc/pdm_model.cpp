@@ -177,19 +177,20 @@ bool is_bc_node(const Deck& d, int bcid)
     return true;
 }
 
-void fill_stage_tables(const pdm_handle* h, StageArgs& a)
+void fill_spec_tables(const pdm_handle* h, SpecDev& sp, MatDev* mats, BCDev* bcs)
 {
     const Deck& d = h->deck;
-    a.nmat = (int)d.mats.size();
-    a.nbc = (int)d.bcs.size();
-    for (int m = 0; m < a.nmat; ++m) {
-        a.mats[m].rho = d.mats[m].rho;
-        for (int c = 0; c < 3; ++c) a.mats[m].g[c] = d.mats[m].g[c];
-        a.mats[m].ecrit = d.mats[m].ecrit;
-        a.mats[m].spring = h->spring[m];
+    std::memset(mats, 0, sizeof(MatDev) * kMaxMat);
+    std::memset(bcs, 0, sizeof(BCDev) * kMaxBC);
+    const int nmat = (int)d.mats.size(), nbc = (int)d.bcs.size();
+    for (int m = 0; m < nmat; ++m) {
+        mats[m].rho = d.mats[m].rho;
+        for (int c = 0; c < 3; ++c) mats[m].g[c] = d.mats[m].g[c];
+        mats[m].ecrit = d.mats[m].ecrit;
+        mats[m].spring = h->spring[m];
     }
-    for (int b = 0; b < a.nbc; ++b) {
-        BCDev& o = a.bcs[b];
+    for (int b = 0; b < nbc; ++b) {
+        BCDev& o = bcs[b];
         const BoundaryCondition& s = d.bcs[b];
         std::memset(&o, 0, sizeof(o));
         o.type = s.type;
@@ -200,8 +201,11 @@ void fill_stage_tables(const pdm_handle* h, StageArgs& a)
         o.load = s.load;
         for (int c = 0; c < 3; ++c) o.trac[c] = h->trac_density[b][c];
     }
-    a.damping = d.damping;
-    a.vol = std::pow(d.dx, 3.0);
+    sp.ecrit0 = d.mats.empty() ? 0.0 : d.mats[0].ecrit;
+    sp.inv_vol_cube = 1.0 / (d.dx * d.dx * d.dx);
+    sp.fscale = h->fscale;
+    sp.damping = d.damping;
+    sp.vol = std::pow(d.dx, 3.0);
 }
 
 }  // namespace pdm

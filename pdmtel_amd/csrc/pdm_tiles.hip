@@ -318,9 +318,8 @@ TileView tile_view(const pdm_handle* h)
     v.waves = h->waves;
     v.fam_per_wave = h->fam_per_wave;
     v.uniform_ecrit = h->uniform_ecrit;
-    v.ecrit0 = h->deck.mats.empty() ? 0.0 : h->deck.mats[0].ecrit;
-    v.inv_vol_cube = 1.0 / (h->deck.dx * h->deck.dx * h->deck.dx);
-    v.fscale = h->fscale;
+    v.specs = h->specs_d;
+    v.tile_spec = h->tile_spec_d;
     return v;
 }
 
