@@ -183,6 +183,43 @@ int pdm_get_tuning(const pdm_handle* h, int* tile_edge, int* waves_per_tile, int
  * pdm_step_rk4 call, and the number of launches it averaged. */
 int pdm_last_kernel_timing(const pdm_handle* h, double* avg_ms, int64_t* launches);
 
+/* ------------------------------------------------------------------ batches
+ * Many specimens (decks) stepped together: every RK4 stage is one launch over all
+ * specimens' tiles, scheduled by a CSR of specimens over the tile list; each specimen
+ * keeps its own deck, time step, clock, boundary conditions and energies.  A specimen
+ * whose run (total_step / total_time) has ended takes steps of dt = 0, which leave its
+ * state unchanged.  The reference has no batch driver; a batch replaces running
+ * pdSolve::InitialProblem + SolveDynamic (pdSolve.cpp:10-130) once per deck. */
+typedef struct pdm_batch pdm_batch;
+
+/* Parse nspec decks (files, or texts in memory); no GPU work.  Options are shared. */
+int pdm_batch_create(const char* const* input_paths, int nspec, const pdm_options* opt, pdm_batch** out);
+int pdm_batch_create_from_text(const char* const* deck_texts, int nspec, const pdm_options* opt, pdm_batch** out);
+int pdm_batch_destroy(pdm_batch* b);
+const char* pdm_batch_last_error(const pdm_batch* b);
+
+/* InitialProblem for every specimen, then one combined tile layout (autotuned when the
+ * layout options are 0 and the batch holds at least 4M family entries). */
+int pdm_batch_init(pdm_batch* b);
+
+int pdm_batch_num_specimens(const pdm_batch* b);
+int64_t pdm_batch_num_nodes(const pdm_batch* b, int spec);    /* spec < 0: all specimens */
+int64_t pdm_batch_num_entries(const pdm_batch* b, int spec);  /* spec < 0: all specimens */
+
+/* Advance every specimen by up to nsteps steps of its own SolveDynamic loop; *done =
+ * the largest count any specimen took. */
+int pdm_batch_step_rk4(pdm_batch* b, int nsteps, int* done);
+
+/* Per-specimen state (3N, reference node order), clock and energies, as the single-
+ * specimen calls. */
+int pdm_batch_get_state(const pdm_batch* b, int spec, double* U, double* V);
+int pdm_batch_set_state(pdm_batch* b, int spec, const double* U, const double* V);
+int pdm_batch_get_clock(const pdm_batch* b, int spec, double* t, int* step, double* dt_next);
+int pdm_batch_get_energy(const pdm_batch* b, int spec, double* ke, double* ee);
+int pdm_batch_get_tuning(const pdm_batch* b, int* tile_edge, int* waves_per_tile, int* families_per_wave,
+                         double* stage_ms);
+int pdm_batch_last_kernel_timing(const pdm_batch* b, double* avg_ms, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -261,10 +261,11 @@ class Oracle:
         self.L.orc_get_state(self.h, _dp(U), _dp(V))
         return U.reshape(-1, 3), V.reshape(-1, 3)
 
-    def set_state(self, U: np.ndarray, V: np.ndarray) -> None:
-        U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1)
-        V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
-        self.L.orc_set_state(self.h, _dp(U), _dp(V))
+    def set_state(self, U, V) -> None:
+        """U, V: 3N arrays (reference node order), or None to keep that field."""
+        U = None if U is None else np.ascontiguousarray(U, dtype=np.float64).reshape(-1)
+        V = None if V is None else np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
+        self.L.orc_set_state(self.h, _dp(U) if U is not None else None, _dp(V) if V is not None else None)
 
     def static_system(self):
         m, uc, b = np.zeros(3 * self.n), np.zeros(3 * self.n), np.zeros(3 * self.n)

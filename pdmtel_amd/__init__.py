@@ -87,6 +87,24 @@ def lib() -> C.CDLL:
     L.pdm_last_kernel_timing.argtypes = [vp, dp, i64p]
     ip = C.POINTER(C.c_int)
     L.pdm_get_tuning.argtypes = [vp, ip, ip, ip, dp]
+    cpp = C.POINTER(C.c_char_p)
+    L.pdm_batch_create.argtypes = [cpp, C.c_int, C.POINTER(Options), C.POINTER(vp)]
+    L.pdm_batch_create_from_text.argtypes = [cpp, C.c_int, C.POINTER(Options), C.POINTER(vp)]
+    L.pdm_batch_destroy.argtypes = [vp]
+    L.pdm_batch_last_error.restype = C.c_char_p
+    L.pdm_batch_last_error.argtypes = [vp]
+    L.pdm_batch_init.argtypes = [vp]
+    L.pdm_batch_num_specimens.argtypes = [vp]
+    for name in ("pdm_batch_num_nodes", "pdm_batch_num_entries"):
+        getattr(L, name).restype = C.c_int64
+        getattr(L, name).argtypes = [vp, C.c_int]
+    L.pdm_batch_step_rk4.argtypes = [vp, C.c_int, ip]
+    L.pdm_batch_get_state.argtypes = [vp, C.c_int, dp, dp]
+    L.pdm_batch_set_state.argtypes = [vp, C.c_int, dp, dp]
+    L.pdm_batch_get_clock.argtypes = [vp, C.c_int, dp, ip, dp]
+    L.pdm_batch_get_energy.argtypes = [vp, C.c_int, dp, dp]
+    L.pdm_batch_get_tuning.argtypes = [vp, ip, ip, ip, dp]
+    L.pdm_batch_last_kernel_timing.argtypes = [vp, dp, i64p]
     _lib = L
     return L
 
@@ -233,3 +251,76 @@ class Problem:
 
     def write_tecplot(self, path: str, step: int = 0) -> None:
         self._chk(self.L.pdm_write_tecplot(self.h, path.encode(), int(step)))
+
+
+class Batch:
+    """Many specimens stepped together (pdm_batch_*): one launch per RK4 stage over all tiles."""
+
+    def __init__(self, decks, precision: int = PDM_FP32C, init: bool = True, **opts):
+        self.L = lib()
+        self.h = C.c_void_p()
+        o = options(precision, **opts)
+        arr = (C.c_char_p * len(decks))(*[d.encode() for d in decks])
+        rc = self.L.pdm_batch_create_from_text(arr, len(decks), C.byref(o), C.byref(self.h))
+        if rc != 0:
+            raise PdmError(rc, self.L.pdm_batch_last_error(None).decode())
+        self.nspec = len(decks)
+        if init:
+            self.init()
+
+    def _chk(self, rc: int) -> None:
+        if rc != 0:
+            raise PdmError(rc, self.L.pdm_batch_last_error(self.h).decode())
+
+    def init(self) -> None:
+        self._chk(self.L.pdm_batch_init(self.h))
+        self.n = [int(self.L.pdm_batch_num_nodes(self.h, k)) for k in range(self.nspec)]
+        self.E = [int(self.L.pdm_batch_num_entries(self.h, k)) for k in range(self.nspec)]
+
+    def close(self) -> None:
+        if getattr(self, "h", None) and self.h.value:
+            self.L.pdm_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nsteps: int) -> int:
+        done = C.c_int()
+        self._chk(self.L.pdm_batch_step_rk4(self.h, int(nsteps), C.byref(done)))
+        return done.value
+
+    def state(self, spec: int) -> Tuple[np.ndarray, np.ndarray]:
+        U = np.zeros(3 * self.n[spec])
+        V = np.zeros(3 * self.n[spec])
+        self._chk(self.L.pdm_batch_get_state(self.h, spec, _dp(U), _dp(V)))
+        return U.reshape(-1, 3), V.reshape(-1, 3)
+
+    def set_state(self, spec: int, U: Optional[np.ndarray], V: Optional[np.ndarray]) -> None:
+        Ua = np.ascontiguousarray(U, dtype=np.float64).reshape(-1) if U is not None else None
+        Va = np.ascontiguousarray(V, dtype=np.float64).reshape(-1) if V is not None else None
+        self._chk(self.L.pdm_batch_set_state(self.h, spec, _dp(Ua) if Ua is not None else None,
+                                             _dp(Va) if Va is not None else None))
+
+    def clock(self, spec: int) -> Tuple[float, int, float]:
+        t, s, dt = C.c_double(), C.c_int(), C.c_double()
+        self._chk(self.L.pdm_batch_get_clock(self.h, spec, C.byref(t), C.byref(s), C.byref(dt)))
+        return t.value, s.value, dt.value
+
+    def energy(self, spec: int) -> Tuple[float, float]:
+        ke, ee = C.c_double(), C.c_double()
+        self._chk(self.L.pdm_batch_get_energy(self.h, spec, C.byref(ke), C.byref(ee)))
+        return ke.value, ee.value
+
+    def tuning(self) -> dict:
+        t, w, f, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self._chk(self.L.pdm_batch_get_tuning(self.h, C.byref(t), C.byref(w), C.byref(f), C.byref(ms)))
+        return dict(tile_edge=t.value, waves_per_tile=w.value, families_per_wave=f.value, tuned_stage_ms=ms.value)
+
+    def kernel_timing(self) -> Tuple[float, int]:
+        ms, n = C.c_double(), C.c_int64()
+        self._chk(self.L.pdm_batch_last_kernel_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value

@@ -190,8 +190,8 @@ int ensure_chunk(pdm_handle* h, int cap)
     if (h->chunk_cap >= cap) return PDM_OK;
     dfree(h->steps);
     dfree(h->energy);
-    HCHK(hipMalloc(&h->steps, sizeof(StepParam) * cap));
-    HCHK(hipMalloc(&h->energy, sizeof(double) * 2 * cap));
+    HCHK(hipMalloc(&h->steps, sizeof(StepParam) * cap * h->nspec));
+    HCHK(hipMalloc(&h->energy, sizeof(double) * 2 * cap * h->nspec));
     h->chunk_cap = cap;
     if (h->graph_exec) {
         (void)hipGraphExecDestroy(h->graph_exec);
@@ -218,13 +218,14 @@ int enqueue_steps(pdm_handle* h, int count)
     return PDM_OK;
 }
 
+// Run `count` steps; sp holds count rows of nspec StepParams ([step][specimen]).
 int run_chunk(pdm_handle* h, const std::vector<StepParam>& sp)
 {
-    int count = (int)sp.size();
+    const int count = (int)(sp.size() / (size_t)h->nspec);
     if (count == 0) return PDM_OK;
     int rc = ensure_chunk(h, std::max(count, h->opt.graph_chunk));
     if (rc) return rc;
-    HCHK(hipMemcpyAsync(h->steps, sp.data(), sizeof(StepParam) * count, hipMemcpyHostToDevice, h->stream));
+    HCHK(hipMemcpyAsync(h->steps, sp.data(), sizeof(StepParam) * sp.size(), hipMemcpyHostToDevice, h->stream));
     bool graph = h->opt.use_graph && count == h->opt.graph_chunk;
     if (graph && !h->graph_exec) {
         hipGraph_t g;
@@ -250,10 +251,11 @@ int run_chunk(pdm_handle* h, const std::vector<StepParam>& sp)
     h->last_kernel_ms += ms;
     h->last_kernel_launches += 4LL * count;
     if (h->opt.energies) {
-        double e[2];
-        HCHK(hipMemcpy(e, h->energy + 2 * (count - 1), sizeof e, hipMemcpyDeviceToHost));
-        h->ke = e[0];
-        h->ee = e[1];
+        h->last_energy.resize(2 * (size_t)h->nspec);
+        HCHK(hipMemcpy(h->last_energy.data(), h->energy + 2 * (int64_t)(count - 1) * h->nspec,
+                       sizeof(double) * 2 * h->nspec, hipMemcpyDeviceToHost));
+        h->ke = h->last_energy[0];
+        h->ee = h->last_energy[1];
     }
     return PDM_OK;
 }
@@ -313,6 +315,90 @@ int zero_state(pdm_handle* h)
     HCHK(hipMemsetAsync(h->elast, 0, sizeof(double) * n, h->stream));
     HCHK(hipStreamSynchronize(h->stream));
     return PDM_OK;
+}
+
+// Device, stream and events of a handle that does not share its batch's stream.
+int init_device(pdm_handle* h)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= h->opt.device || h->opt.device < 0)
+        return fail(h, PDM_E_HIP, "no HIP device " + std::to_string(h->opt.device));
+    h->dev = h->opt.device;
+    HCHK(hipSetDevice(h->dev));
+    if (!h->stream) {
+        HCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        HCHK(hipEventCreate(&h->ev0));
+        HCHK(hipEventCreate(&h->ev1));
+    }
+    return PDM_OK;
+}
+
+// InitNode, InitMaterial, traction densities, InitBond and bdryEff, the stable time step.
+int init_model(pdm_handle* h)
+{
+    int rc = build_nodes(h);
+    if (rc) return rc;
+    init_materials(h);
+    // traction density of each TRACTION bc (extension): total force / (nodes carrying it) / dx^3
+    const Deck& d = h->deck;
+    std::memset(h->trac_density, 0, sizeof(h->trac_density));
+    if (d.traction_body) {
+        std::vector<int64_t> cnt(d.bcs.size(), 0);
+        for (int64_t i = 0; i < h->n; ++i)
+            if (h->bc[i] >= 0) cnt[h->bc[i]]++;
+        double vol = std::pow(d.dx, 3.0);
+        for (size_t b = 0; b < d.bcs.size(); ++b)
+            if (d.bcs[b].type == 4 && cnt[b] > 0)
+                for (int c = 0; c < 3; ++c)
+                    if (d.bcs[b].f[c] == 1) h->trac_density[b][c] = d.bcs[b].v[c] / (double)cnt[b] / vol;
+    }
+    rc = build_bonds(h);
+    if (rc) return rc;
+    // CalcStableTimeStep (pdDataManager2.cpp:6-10)
+    h->dt_stable = d.fixed_dt_on ? d.fixed_dt : d.safety * h->min_bond / h->max_c;
+    return PDM_OK;
+}
+
+// Tile layout from the options (0 = default or autotuned).
+int layout_options(pdm_handle* h)
+{
+    if (h->opt.tile_edge) {
+        if (h->opt.tile_edge < 2 || h->opt.tile_edge > 7) return fail(h, PDM_E_ARG, "tile_edge must be 2..7");
+        h->tile_edge[0] = h->tile_edge[1] = h->tile_edge[2] = h->opt.tile_edge;
+    }
+    if (h->opt.waves_per_tile) {
+        if (h->opt.waves_per_tile != 4 && h->opt.waves_per_tile != 8)
+            return fail(h, PDM_E_ARG, "waves_per_tile must be 4 or 8");
+        h->waves = h->opt.waves_per_tile;
+    }
+    if (h->opt.families_per_wave) {
+        int f = h->opt.families_per_wave;
+        if (f != 1 && f != 2 && f != 4) return fail(h, PDM_E_ARG, "families_per_wave must be 1, 2 or 4");
+        h->fam_per_wave = f;
+    }
+    return PDM_OK;
+}
+
+// FP64 dynamic state arrays (N nodes), zeroed.
+int alloc_state(pdm_handle* h)
+{
+    int64_t n = std::max<int64_t>(h->n, 1);
+    for (d4** p : {&h->U, &h->V, &h->Vb, &h->k1, &h->k23}) {
+        HCHK(hipMalloc(p, sizeof(d4) * n));
+        HCHK(hipMemsetAsync(*p, 0, sizeof(d4) * n, h->stream));
+    }
+    HCHK(hipMalloc(&h->elast, sizeof(double) * n));
+    HCHK(hipMemsetAsync(h->elast, 0, sizeof(double) * n, h->stream));
+    return PDM_OK;
+}
+
+// The driver's clock at t = 0.
+void start_clock(pdm_handle* h)
+{
+    h->dt_cur = h->dt_stable;
+    h->t = 0.0;
+    h->step = 0;
+    h->last_step = h->deck.total_step;
 }
 
 // Autotuner: time whole RK4 steps on this specimen for every candidate tile edge,
@@ -379,6 +465,207 @@ int autotune(pdm_handle* h)
     return build_tiles(h);
 }
 
+// ------------------------------------------------------------ batches
+// A batch is one combined handle whose node, entry, tile and halo arrays are the
+// specimens' arrays laid end to end (ids and offsets shifted), with a launch-index ->
+// specimen map and per-specimen constants: one launch per RK4 stage covers every
+// specimen's tiles (a CSR of specimens over the tile schedule).  Each specimen keeps
+// its own deck, nodes, reference CSR and clock in a member handle.
+
+// Lay the members' tile layouts (built with the batch's layout) end to end in B.
+int batch_concat(pdm_handle* B)
+{
+    pdm_handle* const h = B;  // for HCHK
+    hipStream_t s = B->stream;
+    const int S = (int)B->members.size();
+    int64_t N = 0, Ei = 0, E = 0, ntl = 0, ntiles = 0, halo = 0;
+    int hmax = 0, tmax = 0, ue = 1;
+    for (pdm_handle* m : B->members) {
+        N += m->n;
+        Ei += m->Ei;
+        E += m->E;
+        ntl += m->ntl;
+        ntiles += m->ntiles;
+        halo += m->halo_total;
+        hmax = std::max(hmax, m->halo_max);
+        tmax = std::max(tmax, m->tile_max);
+        ue = ue && m->uniform_ecrit;
+    }
+    if (N > INT32_MAX) return fail(B, PDM_E_LIMIT, "a batch holds more than 2^31 - 1 nodes");
+    B->n = N;
+    B->E = E;
+    B->Ei = Ei;
+    B->ntl = ntl;
+    B->ntiles = ntiles;
+    B->halo_total = halo;
+    B->halo_max = hmax;
+    B->tile_max = tmax;
+    B->uniform_ecrit = ue;
+    const bool f64 = B->opt.precision == PDM_FP64;
+    const int64_t n1 = std::max<int64_t>(N, 1), e1 = std::max<int64_t>(Ei, 1), t1 = std::max<int64_t>(ntl, 1);
+    HCHK(hipMalloc(&B->Xi, sizeof(d4) * n1));
+    for (int b = 0; b < 2; ++b) {
+        HCHK(hipMalloc(&B->u[b], sizeof(d4) * n1));
+        HCHK(hipMemsetAsync(B->u[b], 0, sizeof(d4) * n1, s));
+    }
+    HCHK(hipMalloc(&B->mati, sizeof(int32_t) * n1));
+    HCHK(hipMalloc(&B->bci, sizeof(int32_t) * n1));
+    HCHK(hipMalloc(&B->nb, sizeof(int32_t) * n1));
+    HCHK(hipMalloc(&B->self_slot, sizeof(uint16_t) * n1));
+    HCHK(hipMalloc(&B->rpi, sizeof(int64_t) * (N + 1)));
+    HCHK(hipMalloc(&B->slot, sizeof(uint16_t) * e1));
+    if (f64) HCHK(hipMalloc(&B->coef64, sizeof(double) * e1));
+    else HCHK(hipMalloc(&B->coef32, sizeof(float) * e1));
+    HCHK(hipMalloc(&B->tiles, sizeof(int32_t) * t1));
+    HCHK(hipMalloc(&B->tref, sizeof(d4) * t1));
+    HCHK(hipMalloc(&B->tile_start, sizeof(int64_t) * (ntiles + 1)));
+    HCHK(hipMalloc(&B->halo_ptr, sizeof(int64_t) * (ntl + 1)));
+    HCHK(hipMalloc(&B->halo, sizeof(int32_t) * std::max<int64_t>(halo, 1)));
+    HCHK(hipMalloc(&B->tile_spec_d, sizeof(int32_t) * t1));
+    HCHK(hipMalloc(&B->spec_tile_ptr_d, sizeof(int64_t) * (S + 1)));
+    B->spec_node_ptr.assign(S + 1, 0);
+    std::vector<int64_t> stp(S + 1, 0);
+    int64_t no = 0, eo = 0, to = 0, tio = 0, ho = 0;
+    for (int k = 0; k < S; ++k) {
+        pdm_handle* m = B->members[k];
+        B->spec_node_ptr[k] = no;
+        stp[k] = to;
+        HCHK(hipMemcpyAsync(B->Xi + no, m->Xi, sizeof(d4) * m->n, hipMemcpyDeviceToDevice, s));
+        HCHK(hipMemcpyAsync(B->mati + no, m->mati, sizeof(int32_t) * m->n, hipMemcpyDeviceToDevice, s));
+        HCHK(hipMemcpyAsync(B->bci + no, m->bci, sizeof(int32_t) * m->n, hipMemcpyDeviceToDevice, s));
+        HCHK(hipMemcpyAsync(B->nb + no, m->nb, sizeof(int32_t) * m->n, hipMemcpyDeviceToDevice, s));
+        HCHK(hipMemcpyAsync(B->self_slot + no, m->self_slot, sizeof(uint16_t) * m->n, hipMemcpyDeviceToDevice, s));
+        HCHK(copy_add_i64(B->rpi + no, m->rpi, m->n, eo, s));
+        HCHK(hipMemcpyAsync(B->slot + eo, m->slot, sizeof(uint16_t) * m->Ei, hipMemcpyDeviceToDevice, s));
+        if (f64) HCHK(hipMemcpyAsync(B->coef64 + eo, m->coef64, sizeof(double) * m->Ei, hipMemcpyDeviceToDevice, s));
+        else HCHK(hipMemcpyAsync(B->coef32 + eo, m->coef32, sizeof(float) * m->Ei, hipMemcpyDeviceToDevice, s));
+        HCHK(copy_add_i32(B->tiles + to, m->tiles, m->ntl, (int32_t)tio, s));
+        HCHK(hipMemcpyAsync(B->tref + to, m->tref, sizeof(d4) * m->ntl, hipMemcpyDeviceToDevice, s));
+        HCHK(copy_add_i64(B->tile_start + tio, m->tile_start, m->ntiles, no, s));
+        HCHK(copy_add_i64(B->halo_ptr + to, m->halo_ptr, m->ntl, ho, s));
+        HCHK(copy_add_i32(B->halo + ho, m->halo, m->halo_total, (int32_t)no, s));
+        HCHK(fill_i32(B->tile_spec_d + to, m->ntl, k, s));
+        no += m->n;
+        eo += m->Ei;
+        to += m->ntl;
+        tio += m->ntiles;
+        ho += m->halo_total;
+    }
+    B->spec_node_ptr[S] = no;
+    stp[S] = to;
+    HCHK(hipMemcpyAsync(B->rpi + N, &Ei, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(B->tile_start + ntiles, &N, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(B->halo_ptr + ntl, &halo, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(B->spec_tile_ptr_d, stp.data(), sizeof(int64_t) * (S + 1), hipMemcpyHostToDevice, s));
+    // per-specimen constants and tables
+    std::vector<SpecDev> sp(S);
+    std::vector<MatDev> mats((size_t)S * kMaxMat);
+    std::vector<BCDev> bcs((size_t)S * kMaxBC);
+    for (int k = 0; k < S; ++k)
+        fill_spec_tables(B->members[k], sp[k], mats.data() + (size_t)k * kMaxMat, bcs.data() + (size_t)k * kMaxBC);
+    if (!B->specs_d) {
+        HCHK(hipMalloc(&B->specs_d, sizeof(SpecDev) * S));
+        HCHK(hipMalloc(&B->mats_d, sizeof(MatDev) * kMaxMat * S));
+        HCHK(hipMalloc(&B->bcs_d, sizeof(BCDev) * kMaxBC * S));
+    }
+    HCHK(hipMemcpyAsync(B->specs_d, sp.data(), sizeof(SpecDev) * S, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(B->mats_d, mats.data(), sizeof(MatDev) * kMaxMat * S, hipMemcpyHostToDevice, s));
+    HCHK(hipMemcpyAsync(B->bcs_d, bcs.data(), sizeof(BCDev) * kMaxBC * S, hipMemcpyHostToDevice, s));
+    HCHK(hipStreamSynchronize(s));
+    return PDM_OK;
+}
+
+// Build every member's tiles with edge t, lay them end to end in B, release the members' copies.
+int batch_layout(pdm_handle* B, int t)
+{
+    free_tiles(B);
+    dfree(B->tile_spec_d);
+    dfree(B->spec_tile_ptr_d);
+    for (size_t k = 0; k < B->members.size(); ++k) {
+        pdm_handle* m = B->members[k];
+        m->tile_edge[0] = m->tile_edge[1] = m->tile_edge[2] = t;
+        m->waves = B->waves;
+        m->fam_per_wave = B->fam_per_wave;
+        int rc = build_tiles(m);
+        if (rc) {
+            free_tiles(m);
+            return fail(B, rc, "specimen " + std::to_string(k) + ": " + m->err);
+        }
+    }
+    B->tile_edge[0] = B->tile_edge[1] = B->tile_edge[2] = t;
+    int rc = batch_concat(B);
+    for (pdm_handle* m : B->members) free_tiles(m);
+    if (rc) return rc;
+    // the stage kernels' LDS footprint (FP64 records are the larger)
+    size_t need = stage_lds_bytes(tile_view(B), !B->uniform_ecrit, true, false);
+    if (need > kLdsBytes) return fail(B, PDM_E_LIMIT, "tile LDS footprint exceeds the LDS of a CU");
+    return alloc_parts(B);
+}
+
+// One row of StepParams (dt per specimen) for timing.
+int batch_timing_steps(pdm_handle* B)
+{
+    pdm_handle* const h = B;  // for HCHK
+    std::vector<StepParam> sp(4 * (size_t)B->nspec);
+    for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < B->nspec; ++k) sp[(size_t)r * B->nspec + k] = {B->members[k]->dt_stable, B->members[k]->dt_stable};
+    HCHK(hipMemcpyAsync(B->steps, sp.data(), sizeof(StepParam) * sp.size(), hipMemcpyHostToDevice, B->stream));
+    return PDM_OK;
+}
+
+// Autotuner over the batch: tile edges 4 and 5, waves per tile 8 and 4, families per wave 2 and 4.
+int batch_autotune(pdm_handle* B)
+{
+    pdm_handle* const h = B;  // for HCHK
+    static const int kEdges[] = {4, 5};
+    static const int kWaves[] = {8, 4};
+    static const int kFams[] = {2, 4};
+    double best = 1e300;
+    int bt = 0, bw = 8, bf = 2;
+    int rc = ensure_chunk(B, 4);
+    if (rc) return rc;
+    for (int t : kEdges) {
+        rc = batch_layout(B, t);
+        if (rc == PDM_E_LIMIT) {
+            B->err.clear();
+            continue;
+        }
+        if (rc) return rc;
+        rc = batch_timing_steps(B);
+        if (rc) return rc;
+        for (int w : kWaves)
+            for (int f : kFams) {
+                B->waves = w;
+                B->fam_per_wave = f;
+                rc = enqueue_steps(B, 1);
+                if (rc) return rc;
+                HCHK(hipEventRecord(B->ev0, B->stream));
+                rc = enqueue_steps(B, 3);
+                if (rc) return rc;
+                HCHK(hipEventRecord(B->ev1, B->stream));
+                HCHK(hipEventSynchronize(B->ev1));
+                float ms = 0.f;
+                HCHK(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+                const double per = ms / 12.0;
+                if (B->opt.verbose)
+                    fprintf(stderr, "pdm_batch_init: tune edge %d waves %d families %d: %.4f ms per stage\n", t, w, f,
+                            per);
+                if (per < best) {
+                    best = per;
+                    bt = t;
+                    bw = w;
+                    bf = f;
+                }
+            }
+    }
+    if (!bt) return fail(B, PDM_E_LIMIT, "no tile edge fits these specimens");
+    B->waves = bw;
+    B->fam_per_wave = bf;
+    B->tuned = 1;
+    B->tune_ms = best;
+    return (B->tile_edge[0] == bt) ? PDM_OK : batch_layout(B, bt);
+}
+
 }  // namespace
 
 extern "C" {
@@ -437,7 +724,7 @@ int pdm_destroy(pdm_handle* h)
     dfree(h->specs_d); dfree(h->mats_d); dfree(h->bcs_d); dfree(h->tile_spec_d); dfree(h->spec_tile_ptr_d);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream && !h->shared_stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return PDM_OK;
 }
@@ -492,56 +779,14 @@ int pdm_init(pdm_handle* h)
 {
     if (!h) return PDM_E_ARG;
     if (h->inited) return fail(h, PDM_E_STATE, "pdm_init called twice");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= h->opt.device || h->opt.device < 0)
-        return fail(h, PDM_E_HIP, "no HIP device " + std::to_string(h->opt.device));
-    h->dev = h->opt.device;
-    HCHK(hipSetDevice(h->dev));
-    if (!h->stream) {
-        HCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-        HCHK(hipEventCreate(&h->ev0));
-        HCHK(hipEventCreate(&h->ev1));
-    }
-    int rc = build_nodes(h);
+    int rc = init_device(h);
     if (rc) return rc;
-    init_materials(h);
-    // traction density of each TRACTION bc (extension): total force / (nodes carrying it) / dx^3
-    const Deck& d = h->deck;
-    std::memset(h->trac_density, 0, sizeof(h->trac_density));
-    if (d.traction_body) {
-        std::vector<int64_t> cnt(d.bcs.size(), 0);
-        for (int64_t i = 0; i < h->n; ++i)
-            if (h->bc[i] >= 0) cnt[h->bc[i]]++;
-        double vol = std::pow(d.dx, 3.0);
-        for (size_t b = 0; b < d.bcs.size(); ++b)
-            if (d.bcs[b].type == 4 && cnt[b] > 0)
-                for (int c = 0; c < 3; ++c)
-                    if (d.bcs[b].f[c] == 1) h->trac_density[b][c] = d.bcs[b].v[c] / (double)cnt[b] / vol;
-    }
-    rc = build_bonds(h);
+    rc = init_model(h);
     if (rc) return rc;
-    if (h->opt.tile_edge) {
-        if (h->opt.tile_edge < 2 || h->opt.tile_edge > 7) return fail(h, PDM_E_ARG, "tile_edge must be 2..7");
-        h->tile_edge[0] = h->tile_edge[1] = h->tile_edge[2] = h->opt.tile_edge;
-    }
-    if (h->opt.waves_per_tile) {
-        if (h->opt.waves_per_tile != 4 && h->opt.waves_per_tile != 8) return fail(h, PDM_E_ARG, "waves_per_tile must be 4 or 8");
-        h->waves = h->opt.waves_per_tile;
-    }
-    if (h->opt.families_per_wave) {
-        int f = h->opt.families_per_wave;
-        if (f != 1 && f != 2 && f != 4) return fail(h, PDM_E_ARG, "families_per_wave must be 1, 2 or 4");
-        h->fam_per_wave = f;
-    }
-    // CalcStableTimeStep (pdDataManager2.cpp:6-10)
-    h->dt_stable = d.fixed_dt_on ? d.fixed_dt : d.safety * h->min_bond / h->max_c;
-    int64_t n = std::max<int64_t>(h->n, 1);
-    for (d4** p : {&h->U, &h->V, &h->Vb, &h->k1, &h->k23}) {
-        HCHK(hipMalloc(p, sizeof(d4) * n));
-        HCHK(hipMemsetAsync(*p, 0, sizeof(d4) * n, h->stream));
-    }
-    HCHK(hipMalloc(&h->elast, sizeof(double) * n));
-    HCHK(hipMemsetAsync(h->elast, 0, sizeof(double) * n, h->stream));
+    rc = layout_options(h);
+    if (rc) return rc;
+    rc = alloc_state(h);
+    if (rc) return rc;
     const bool tune = !h->opt.tile_edge && !h->opt.waves_per_tile && !h->opt.families_per_wave &&
                       h->E >= kTuneMinEntries;
     rc = tune ? autotune(h) : build_tiles(h);
@@ -560,11 +805,7 @@ int pdm_init(pdm_handle* h)
                 (long long)h->n, (long long)h->E, (long long)h->ntl, h->tile_edge[0], h->tile_edge[1],
                 h->tile_edge[2], h->tile_max, h->halo_max, h->ntl ? (double)h->halo_total / (double)h->ntl : 0.0,
                 stage_lds_bytes(tile_view(h), !h->uniform_ecrit, true, h->opt.precision != PDM_FP64));
-    // the driver's clock
-    h->dt_cur = h->dt_stable;
-    h->t = 0.0;
-    h->step = 0;
-    h->last_step = d.total_step;
+    start_clock(h);
     h->inited = true;
     return PDM_OK;
 }
@@ -841,6 +1082,293 @@ int pdm_solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_stat
     if (rc) return rc;
     HCHK(hipSetDevice(h->dev));
     return solve_static(h, rel_tol, max_iterations, stats);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batches (C ABI)
+
+struct pdm_batch {
+    pdm_handle* h = nullptr;  // combined arrays; the specimens are h->members
+};
+
+namespace {
+
+int batch_create(const std::vector<std::string>& texts, const pdm_options* opt, pdm_batch** out)
+{
+    if (!out) {
+        g_create_err = "null output handle pointer";
+        return PDM_E_ARG;
+    }
+    *out = nullptr;
+    if (texts.empty()) {
+        g_create_err = "a batch needs at least one specimen";
+        return PDM_E_ARG;
+    }
+    pdm_handle* B = nullptr;
+    // the combined handle carries the options; its own deck is the first specimen's
+    int rc = create_common(texts[0], opt, &B);
+    if (rc) {
+        g_create_err = "specimen 0: " + g_create_err;
+        return rc;
+    }
+    for (size_t k = 0; k < texts.size(); ++k) {
+        pdm_handle* m = nullptr;
+        rc = create_common(texts[k], opt, &m);
+        if (rc) {
+            g_create_err = "specimen " + std::to_string(k) + ": " + g_create_err;
+            for (pdm_handle* q : B->members) pdm_destroy(q);
+            B->members.clear();
+            pdm_destroy(B);
+            return rc;
+        }
+        m->shared_stream = true;
+        B->members.push_back(m);
+    }
+    B->nspec = (int)texts.size();
+    pdm_batch* b = new pdm_batch();
+    b->h = B;
+    *out = b;
+    return PDM_OK;
+}
+
+int batch_spec(const pdm_batch* b, int spec)
+{
+    if (!b || !b->h) return PDM_E_ARG;
+    if (!b->h->inited) return fail(b->h, PDM_E_STATE, "pdm_batch_init has not run");
+    if (spec < 0 || spec >= b->h->nspec) return fail(b->h, PDM_E_ARG, "specimen index out of range");
+    return PDM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdm_batch_create(const char* const* input_paths, int nspec, const pdm_options* opt, pdm_batch** out)
+{
+    if (!input_paths || nspec <= 0) {
+        g_create_err = "no input paths";
+        return PDM_E_ARG;
+    }
+    std::vector<std::string> texts(nspec);
+    for (int k = 0; k < nspec; ++k) {
+        if (!input_paths[k] || !read_file(input_paths[k], texts[k])) {
+            g_create_err = std::string("cannot read input file ") + (input_paths[k] ? input_paths[k] : "(null)");
+            return PDM_E_IO;
+        }
+    }
+    return batch_create(texts, opt, out);
+}
+
+int pdm_batch_create_from_text(const char* const* deck_texts, int nspec, const pdm_options* opt, pdm_batch** out)
+{
+    if (!deck_texts || nspec <= 0) {
+        g_create_err = "no deck texts";
+        return PDM_E_ARG;
+    }
+    std::vector<std::string> texts(nspec);
+    for (int k = 0; k < nspec; ++k) {
+        if (!deck_texts[k]) {
+            g_create_err = "null deck text";
+            return PDM_E_ARG;
+        }
+        texts[k] = deck_texts[k];
+    }
+    return batch_create(texts, opt, out);
+}
+
+int pdm_batch_destroy(pdm_batch* b)
+{
+    if (!b) return PDM_E_ARG;
+    if (b->h) {
+        if (b->h->stream) (void)hipStreamSynchronize(b->h->stream);
+        for (pdm_handle* m : b->h->members) pdm_destroy(m);
+        b->h->members.clear();
+        pdm_destroy(b->h);
+    }
+    delete b;
+    return PDM_OK;
+}
+
+const char* pdm_batch_last_error(const pdm_batch* b)
+{
+    if (!b || !b->h) return g_create_err.c_str();
+    return b->h->err.c_str();
+}
+
+int pdm_batch_init(pdm_batch* b)
+{
+    if (!b || !b->h) return PDM_E_ARG;
+    pdm_handle* B = b->h;
+    if (B->inited) return fail(B, PDM_E_STATE, "pdm_batch_init called twice");
+    int rc = init_device(B);
+    if (rc) return rc;
+    for (size_t k = 0; k < B->members.size(); ++k) {
+        pdm_handle* m = B->members[k];
+        m->dev = B->dev;
+        m->stream = B->stream;
+        rc = init_model(m);
+        if (rc) return fail(B, rc, "specimen " + std::to_string(k) + ": " + m->err);
+        start_clock(m);
+        m->inited = true;
+    }
+    rc = layout_options(B);
+    if (rc) return rc;
+    int64_t E = 0, N = 0;
+    for (pdm_handle* m : B->members) {
+        E += m->E;
+        N += m->n;
+    }
+    B->n = N;
+    rc = alloc_state(B);
+    if (rc) return rc;
+    const bool tune = !B->opt.tile_edge && !B->opt.waves_per_tile && !B->opt.families_per_wave &&
+                      E >= kTuneMinEntries;
+    rc = tune ? batch_autotune(B) : batch_layout(B, B->tile_edge[0]);
+    if (rc) return rc;
+    rc = zero_state(B);
+    if (rc) return rc;
+    B->last_kernel_ms = 0.0;
+    B->last_kernel_launches = 0;
+    if (B->opt.verbose)
+        fprintf(stderr, "pdm_batch_init: %d specimens, nodes=%lld entries=%lld tiles=%lld edge %d waves %d "
+                        "families %d halo_max=%d\n",
+                B->nspec, (long long)B->n, (long long)B->E, (long long)B->ntl, B->tile_edge[0], B->waves,
+                B->fam_per_wave, B->halo_max);
+    B->inited = true;
+    return PDM_OK;
+}
+
+int pdm_batch_num_specimens(const pdm_batch* b) { return (b && b->h) ? b->h->nspec : -1; }
+
+int64_t pdm_batch_num_nodes(const pdm_batch* b, int spec)
+{
+    if (!b || !b->h) return -1;
+    if (spec < 0) return b->h->n;
+    return (spec < b->h->nspec) ? b->h->members[spec]->n : -1;
+}
+
+int64_t pdm_batch_num_entries(const pdm_batch* b, int spec)
+{
+    if (!b || !b->h) return -1;
+    if (spec < 0) return b->h->E;
+    return (spec < b->h->nspec) ? b->h->members[spec]->E : -1;
+}
+
+int pdm_batch_step_rk4(pdm_batch* b, int nsteps, int* done)
+{
+    int rc = b ? (b->h && b->h->inited ? PDM_OK : PDM_E_STATE) : PDM_E_ARG;
+    if (rc) return rc;
+    pdm_handle* B = b->h;
+    pdm_handle* const h = B;  // for HCHK
+    if (nsteps < 0) return fail(B, PDM_E_ARG, "negative step count");
+    HCHK(hipSetDevice(B->dev));
+    const int S = B->nspec;
+    int total = 0;
+    std::vector<std::vector<StepParam>> per(S);
+    std::vector<StepParam> flat;
+    while (total < nsteps) {
+        const int want = std::min(B->opt.graph_chunk, nsteps - total);
+        int cnt = 0;
+        for (int k = 0; k < S; ++k) {
+            plan_steps(B->members[k], want, per[k]);
+            cnt = std::max(cnt, (int)per[k].size());
+        }
+        if (cnt == 0) break;
+        // a specimen whose run has ended takes dt = 0 steps, which leave its state unchanged
+        flat.assign((size_t)cnt * S, StepParam{0.0, 0.0});
+        for (int k = 0; k < S; ++k)
+            for (int r = 0; r < cnt; ++r)
+                flat[(size_t)r * S + k] = (r < (int)per[k].size()) ? per[k][r] : StepParam{B->members[k]->t, 0.0};
+        rc = run_chunk(B, flat);
+        if (rc) return rc;
+        if (B->opt.energies) {
+            // each specimen's energies of its last real step (a dt = 0 step recomputes
+            // the elastic energy from four identical trial fields)
+            std::vector<double> en(2 * (size_t)cnt * S);
+            HCHK(hipMemcpy(en.data(), B->energy, sizeof(double) * en.size(), hipMemcpyDeviceToHost));
+            for (int k = 0; k < S; ++k) {
+                if (per[k].empty()) continue;
+                const size_t r = per[k].size() - 1;
+                B->members[k]->ke = en[2 * (r * S + k)];
+                B->members[k]->ee = en[2 * (r * S + k) + 1];
+            }
+        }
+        total += cnt;
+        if (cnt < want) break;
+    }
+    if (done) *done = total;
+    return PDM_OK;
+}
+
+int pdm_batch_get_state(const pdm_batch* b, int spec, double* U, double* V)
+{
+    int rc = batch_spec(b, spec);
+    if (rc) return rc;
+    const pdm_handle* B = b->h;
+    const pdm_handle* const h = B;  // for HCHK
+    const pdm_handle* m = B->members[spec];
+    const int64_t off = B->spec_node_ptr[spec], n = m->n;
+    std::vector<d4> buf(n);
+    HCHK(hipStreamSynchronize(B->stream));
+    if (U) {
+        HCHK(hipMemcpy(buf.data(), B->U + off, sizeof(d4) * n, hipMemcpyDeviceToHost));
+        from_internal(m, buf, U);
+    }
+    if (V) {
+        HCHK(hipMemcpy(buf.data(), B->V + off, sizeof(d4) * n, hipMemcpyDeviceToHost));
+        from_internal(m, buf, V);
+    }
+    return PDM_OK;
+}
+
+int pdm_batch_set_state(pdm_batch* b, int spec, const double* U, const double* V)
+{
+    int rc = batch_spec(b, spec);
+    if (rc) return rc;
+    pdm_handle* B = b->h;
+    pdm_handle* const h = B;  // for HCHK
+    const pdm_handle* m = B->members[spec];
+    const int64_t off = B->spec_node_ptr[spec], n = m->n;
+    std::vector<d4> buf;
+    HCHK(hipStreamSynchronize(B->stream));
+    if (U) {
+        to_internal(m, U, buf);
+        HCHK(hipMemcpy(B->U + off, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
+        HCHK(hipMemcpy(B->u[0] + off, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
+    }
+    if (V) {
+        to_internal(m, V, buf);
+        HCHK(hipMemcpy(B->V + off, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
+    }
+    return PDM_OK;
+}
+
+int pdm_batch_get_clock(const pdm_batch* b, int spec, double* t, int* step, double* dt_next)
+{
+    int rc = batch_spec(b, spec);
+    if (rc) return rc;
+    return pdm_get_clock(b->h->members[spec], t, step, dt_next);
+}
+
+int pdm_batch_get_energy(const pdm_batch* b, int spec, double* ke, double* ee)
+{
+    int rc = batch_spec(b, spec);
+    if (rc) return rc;
+    return pdm_get_energy(b->h->members[spec], ke, ee);
+}
+
+int pdm_batch_get_tuning(const pdm_batch* b, int* tile_edge, int* waves_per_tile, int* families_per_wave,
+                         double* stage_ms)
+{
+    if (!b || !b->h) return PDM_E_ARG;
+    return pdm_get_tuning(b->h, tile_edge, waves_per_tile, families_per_wave, stage_ms);
+}
+
+int pdm_batch_last_kernel_timing(const pdm_batch* b, double* avg_ms, int64_t* launches)
+{
+    if (!b || !b->h) return PDM_E_ARG;
+    return pdm_last_kernel_timing(b->h, avg_ms, launches);
 }
 
 }  // extern "C"

@@ -191,6 +191,10 @@ struct pdm_handle {
     int32_t* tile_spec_d = nullptr;     // launch index -> specimen (batches only)
     int64_t* spec_tile_ptr_d = nullptr; // specimen -> first launch index (nspec + 1)
     int nspec = 1;
+    std::vector<pdm_handle*> members; // batch: the specimens (parsed decks, nodes, bonds, clocks)
+    std::vector<int64_t> spec_node_ptr; // batch: specimen -> first node of the combined arrays
+    bool shared_stream = false;     // a batch member uses its batch's stream
+    std::vector<double> last_energy; // [spec][2] KE, EE of each specimen's last step
     double fscale = 1.0;                // fixed-point scale of the FP32 halo records
     int64_t* halo_ptr = nullptr;
     int32_t* halo = nullptr;
@@ -244,6 +248,9 @@ TileView tile_view(const pdm_handle* h);
 
 // kernels (pdm_kernels.hip)
 hipError_t launch_stage(int stage, int precision, const StageArgs& a, int step_idx, hipStream_t s);
+hipError_t copy_add_i64(int64_t* dst, const int64_t* src, int64_t n, int64_t off, hipStream_t s);
+hipError_t copy_add_i32(int32_t* dst, const int32_t* src, int64_t n, int32_t off, hipStream_t s);
+hipError_t fill_i32(int32_t* dst, int64_t n, int32_t v, hipStream_t s);
 hipError_t launch_energy_steps(const double* part, int64_t ntl, const int64_t* spec_tile_ptr, int nspec, int nsteps,
                                double* out, hipStream_t s);
 hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, double* elast, hipStream_t s);

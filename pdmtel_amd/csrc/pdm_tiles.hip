@@ -294,7 +294,41 @@ __global__ void k_coef(CoefArgs a)
     if (lane == 0) a.nb[i] = cnt;
 }
 
+template <class T>
+__global__ void k_copy_add(T* __restrict__ dst, const T* __restrict__ src, int64_t n, T off)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] + off;
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ dst, int64_t n, int32_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
 }  // namespace
+
+hipError_t copy_add_i64(int64_t* dst, const int64_t* src, int64_t n, int64_t off, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_copy_add<int64_t><<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(dst, src, n, off);
+    return hipGetLastError();
+}
+
+hipError_t copy_add_i32(int32_t* dst, const int32_t* src, int64_t n, int32_t off, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_copy_add<int32_t><<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(dst, src, n, off);
+    return hipGetLastError();
+}
+
+hipError_t fill_i32(int32_t* dst, int64_t n, int32_t v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    k_fill_i32<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(dst, n, v);
+    return hipGetLastError();
+}
 
 TileView tile_view(const pdm_handle* h)
 {

@@ -183,3 +183,32 @@ def mid_slit(spec: Dict[str, Any], angle_deg: float, axis_t=(0.0, 0.0, 1.0), edg
             w = tuple(-x for x in w)
         c = [c[k] + big * w[k] for k in range(3)]
     return dict(type=2, c=tuple(c), n=n, t=tuple(axis_t), half_t=big, half_s=big)
+
+
+def specimen_specs(count: int, seed: int = 0, nodes=(1e3, 1e5), steps: int = 1000, jitter: float = 0.01):
+    """Random specimens for batched runs, each a bar spec with an edge crack.
+
+    Per specimen, from numpy's default_rng(seed): node count log-uniform in
+    ``nodes``; two box aspect ratios (length : width, width : depth) uniform in
+    [1, 5]; delta / dx from {3.015, 4.015}; crack normal at an angle uniform in
+    [0, 90] degrees to x; E uniform in [50, 200] GPa; node jitter ``jitter`` dx
+    with the specimen's own seed.  The x = 0 layer is fixed and the x = L layer
+    pulled with 1 MPa, as in ``bar_spec``.
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = math.exp(rng.uniform(math.log(nodes[0]), math.log(nodes[1])))
+        r1, r2 = rng.uniform(1.0, 5.0, size=2)
+        nz = max(2, int(round((n / (r1 * r2 * r2)) ** (1.0 / 3.0))))
+        ny = max(2, int(round(r2 * nz)))
+        nx = max(4, int(round(r1 * ny)))
+        delta = float(rng.choice([3.015, 4.015]))
+        angle = float(rng.uniform(0.0, 90.0))
+        E = float(rng.uniform(50e9, 200e9))
+        spec = bar_spec(nx, ny, nz, E=E, delta_over_dx=delta, steps=steps, jitter=jitter,
+                        seed=int(rng.integers(0, 2 ** 31)))
+        spec["title"] = f"specimen_{k}"
+        spec["slits"] = [mid_slit(spec, angle, edge=True)]
+        out.append(spec)
+    return out
