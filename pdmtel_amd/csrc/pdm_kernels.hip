@@ -199,12 +199,14 @@ struct alignas(16) Rec {
     double x, y, z, u, v, w;
 };
 
-// Halo record of the FP32 operations, relative to the tile's reference point (the
-// tile centre, and the displacement of the tile's middle node) and scaled by
-// S = tv.fscale (2^30 over the largest reach of a halo): the position as int32 fixed
+// Halo record of the FP32 operations, scaled by S = SpecDev::fscale (2^30 over the
+// largest reach of a halo): the position relative to the tile centre as int32 fixed
 // point, so xi = X_j - X_i is an exact integer difference rounded once to FP32, and the
-// displacement as FP32.  Two arrays: (x, y, z as int32 bits, ux) and (uy, uz).  The
-// bond force is scale-free in these units; the energy carries one factor 1/S.
+// displacement u S in FP64, so eta = u_j - u_i is formed exactly before its rounding
+// to FP32 (an FP32 displacement, even relative to a tile reference, rounds eta at the
+// scale of the displacement field across the halo instead of eta's own).  Three
+// arrays: (x, y, z as int32, ecrit as FP32 bits), (ux, uy), uz: 40 bytes.  The bond
+// force is scale-free in these units; the energy carries one factor 1/S.
 
 // Stage-update inputs of one tile node, loaded while the halo is staged.
 struct EpiRec {
@@ -218,10 +220,10 @@ struct EpiRec {
 // for the stage kernels also the stage-update inputs and the material / BC tables.
 struct Lds {
     Rec* rec;        // FP64 records
-    u32x4* fa;       // FP32 records
-    f32x2* fb;
+    u32x4* fa;       // FP32 records: fixed-point position, ecrit
+    f64x2* fu;       //   u S (x, y)
+    double* fz;      //   u S (z)
     double* ec;      // per-record ecrit (FP64 records)
-    float* fec;      // per-record ecrit (FP32 records)
     double* res;
     int64_t* row;
     int* self;
@@ -236,7 +238,7 @@ __host__ __device__ inline size_t round16(size_t b) { return (b + 15) / 16 * 16;
 __host__ __device__ inline size_t lds_rec_bytes(int H, bool with_ecrit, bool f32)
 {
     const size_t n = (size_t)H + 1;
-    if (f32) return round16(n * sizeof(u32x4)) + round16(n * sizeof(f32x2)) + (with_ecrit ? round16(n * 4) : 0);
+    if (f32) return round16(n * sizeof(u32x4)) + round16(n * sizeof(f64x2)) + round16(n * sizeof(double));
     return n * sizeof(Rec) + (with_ecrit ? round16(n * 8) : 0);
 }
 
@@ -257,12 +259,10 @@ __device__ __forceinline__ Lds lds_view(double* base, int H, int tile_max, bool 
     if (f32) {
         l.fa = reinterpret_cast<u32x4*>(p);
         p += round16(n * sizeof(u32x4));
-        l.fb = reinterpret_cast<f32x2*>(p);
-        p += round16(n * sizeof(f32x2));
-        if (with_ecrit) {
-            l.fec = reinterpret_cast<float*>(p);
-            p += round16(n * 4);
-        }
+        l.fu = reinterpret_cast<f64x2*>(p);
+        p += round16(n * sizeof(f64x2));
+        l.fz = reinterpret_cast<double*>(p);
+        p += round16(n * sizeof(double));
     } else {
         l.rec = reinterpret_cast<Rec*>(p);
         p += n * sizeof(Rec);
@@ -286,10 +286,12 @@ __device__ __forceinline__ Lds lds_view(double* base, int H, int tile_max, bool 
 
 constexpr unsigned kFarFix = 1u << 30;  // fixed-point coordinate of the padding record
 
-__device__ __forceinline__ u32x4 frec_a(const d4& x, const d4& v, const d4& xr, const d4& ur, double S)
+__device__ __forceinline__ void frec_store(const Lds& l, int q, const d4& x, const d4& v, const d4& xr, double S)
 {
-    return u32x4{(unsigned)__double2int_rn((x.x - xr.x) * S), (unsigned)__double2int_rn((x.y - xr.y) * S),
-                 (unsigned)__double2int_rn((x.z - xr.z) * S), __float_as_uint((float)((v.x - ur.x) * S))};
+    l.fa[q] = u32x4{(unsigned)__double2int_rn((x.x - xr.x) * S), (unsigned)__double2int_rn((x.y - xr.y) * S),
+                    (unsigned)__double2int_rn((x.z - xr.z) * S), __float_as_uint((float)x.w)};
+    l.fu[q] = f64x2{v.x * S, v.y * S};
+    l.fz[q] = v.z * S;
 }
 
 // Stage the tile's halo records (record q = internal node halo[h0 + q]), the padding
@@ -300,12 +302,9 @@ __device__ __forceinline__ void stage_tile(const TileView& tv, const SpecDev& sp
                                            int64_t k, int64_t h0, int H, int64_t a0, int nt, const Lds& l,
                                            bool with_ecrit)
 {
-    d4 xr{}, ur{};
+    d4 xr{};
     const double S = sp.fscale;
-    if (F32) {
-        xr = tv.tref[k];
-        ur = u[a0 + nt / 2];
-    }
+    if (F32) xr = tv.tref[k];
     for (int q0 = threadIdx.x; q0 < H; q0 += 2 * blockDim.x) {
         const int q1 = q0 + blockDim.x;
         const int32_t id0 = tv.halo[h0 + q0];
@@ -313,14 +312,8 @@ __device__ __forceinline__ void stage_tile(const TileView& tv, const SpecDev& sp
         const d4 x0 = tv.X[id0], v0 = u[id0];
         const d4 x1 = tv.X[id1], v1 = u[id1];
         if (F32) {
-            l.fa[q0] = frec_a(x0, v0, xr, ur, S);
-            l.fb[q0] = f32x2{(float)((v0.y - ur.y) * S), (float)((v0.z - ur.z) * S)};
-            if (with_ecrit) l.fec[q0] = (float)x0.w;
-            if (q1 < H) {
-                l.fa[q1] = frec_a(x1, v1, xr, ur, S);
-                l.fb[q1] = f32x2{(float)((v1.y - ur.y) * S), (float)((v1.z - ur.z) * S)};
-                if (with_ecrit) l.fec[q1] = (float)x1.w;
-            }
+            frec_store(l, q0, x0, v0, xr, S);
+            if (q1 < H) frec_store(l, q1, x1, v1, xr, S);
         } else {
             l.rec[q0] = Rec{x0.x, x0.y, x0.z, v0.x, v0.y, v0.z};
             if (with_ecrit) l.ec[q0] = x0.w;
@@ -332,9 +325,9 @@ __device__ __forceinline__ void stage_tile(const TileView& tv, const SpecDev& sp
     }
     if (threadIdx.x == 0) {
         if (F32) {
-            l.fa[H] = u32x4{kFarFix, kFarFix, kFarFix, 0u};
-            l.fb[H] = f32x2{0.0f, 0.0f};
-            if (with_ecrit) l.fec[H] = 1.0f;
+            l.fa[H] = u32x4{kFarFix, kFarFix, kFarFix, __float_as_uint(1.0f)};
+            l.fu[H] = f64x2{0.0, 0.0};
+            l.fz[H] = 0.0;
         } else {
             l.rec[H] = Rec{kFar, kFar, kFar, 0.0, 0.0, 0.0};
             if (with_ecrit) l.ec[H] = 1.0;
@@ -356,14 +349,13 @@ struct Self {
 
 struct SelfF {
     int x, y, z;
-    float u, v, w;
+    double u, v, w;
     float ecrit;
     __device__ static SelfF make(const Lds& l, const SpecDev& sp, int ss, bool ue)
     {
         const u32x4 a = l.fa[ss];
-        const f32x2 b = l.fb[ss];
-        return SelfF{(int)a.x, (int)a.y, (int)a.z, __uint_as_float(a.w), b.x, b.y,
-                     ue ? (float)sp.ecrit0 : l.fec[ss]};
+        const f64x2 b = l.fu[ss];
+        return SelfF{(int)a.x, (int)a.y, (int)a.z, b.x, b.y, l.fz[ss], ue ? (float)sp.ecrit0 : __uint_as_float(a.w)};
     }
 };
 
@@ -429,9 +421,10 @@ template <bool ENERGY, bool UE>
 __device__ __forceinline__ Bond32 bond32(const Lds& l, unsigned s, float c, const SelfF& me)
 {
     const u32x4 ra = l.fa[s];
-    const f32x2 rb = l.fb[s];
+    const f64x2 rb = l.fu[s];
+    const double rz = l.fz[s];
     const float x1 = (float)((int)ra.x - me.x), x2 = (float)((int)ra.y - me.y), x3 = (float)((int)ra.z - me.z);
-    const float e1 = __uint_as_float(ra.w) - me.u, e2 = rb.x - me.v, e3 = rb.y - me.w;
+    const float e1 = (float)(rb.x - me.u), e2 = (float)(rb.y - me.v), e3 = (float)(rz - me.w);
     const float a1 = x1 + e1, a2 = x2 + e2, a3 = x3 + e3;
     const float r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
     const float q = __builtin_fmaf(e3, x3 + a3, __builtin_fmaf(e2, x2 + a2, e1 * (x1 + a1)));
@@ -444,7 +437,7 @@ __device__ __forceinline__ Bond32 bond32(const Lds& l, unsigned s, float c, cons
     // u <= 0 || s <= ecrit, with u = s r and r > 0
     float ecrit = me.ecrit;
     if (!UE) {
-        const float ej = l.fec[s];
+        const float ej = __uint_as_float(ra.w);
         ecrit = (ej < ecrit) ? ej : ecrit;
     }
     const float cs = (st <= __builtin_fmaxf(ecrit, 0.0f)) ? c * st : 0.0f;
@@ -584,9 +577,9 @@ struct OpT32 {
     __device__ static void bond(const Lds& l, unsigned s, float c, const SelfF& me, Acc& a)
     {
         const u32x4 ra = l.fa[s];
-        const f32x2 rb = l.fb[s];
+        const f64x2 rb = l.fu[s];
         const float x1 = (float)((int)ra.x - me.x), x2 = (float)((int)ra.y - me.y), x3 = (float)((int)ra.z - me.z);
-        const float d0 = me.u - __uint_as_float(ra.w), d1 = me.v - rb.x, d2 = me.w - rb.y;
+        const float d0 = (float)(me.u - rb.x), d1 = (float)(me.v - rb.y), d2 = (float)(me.w - l.fz[s]);
         const float r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
         const float rinv = __builtin_amdgcn_rsqf(r2);
         const float proj = __builtin_fmaf(x3, d2, __builtin_fmaf(x2, d1, x1 * d0)) * c * (rinv * rinv * rinv);

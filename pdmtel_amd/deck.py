@@ -185,7 +185,8 @@ def mid_slit(spec: Dict[str, Any], angle_deg: float, axis_t=(0.0, 0.0, 1.0), edg
     return dict(type=2, c=tuple(c), n=n, t=tuple(axis_t), half_t=big, half_s=big)
 
 
-def specimen_specs(count: int, seed: int = 0, nodes=(1e3, 1e5), steps: int = 1000, jitter: float = 0.01):
+def specimen_specs(count: int, seed: int = 0, nodes=(1e3, 1e5), steps: int = 1000, jitter: float = 0.01,
+                   min_side: int = 5, safety: float = 0.8):
     """Random specimens for batched runs, each a bar spec with an edge crack.
 
     Per specimen, from numpy's default_rng(seed): node count log-uniform in
@@ -193,20 +194,24 @@ def specimen_specs(count: int, seed: int = 0, nodes=(1e3, 1e5), steps: int = 100
     [1, 5]; delta / dx from {3.015, 4.015}; crack normal at an angle uniform in
     [0, 90] degrees to x; E uniform in [50, 200] GPa; node jitter ``jitter`` dx
     with the specimen's own seed.  The x = 0 layer is fixed and the x = L layer
-    pulled with 1 MPa, as in ``bar_spec``.
+    pulled with 1 MPa, as in ``bar_spec``.  Every side holds at least
+    ``min_side`` nodes: thinner sections (2 or 3 layers, all of them surface
+    layers) take bdryEff factors that make the stable-dt estimate of the
+    reference unstable, so such boxes are thickened (the node count then exceeds
+    the drawn one).
     """
     rng = np.random.default_rng(seed)
     out = []
     for k in range(count):
         n = math.exp(rng.uniform(math.log(nodes[0]), math.log(nodes[1])))
         r1, r2 = rng.uniform(1.0, 5.0, size=2)
-        nz = max(2, int(round((n / (r1 * r2 * r2)) ** (1.0 / 3.0))))
-        ny = max(2, int(round(r2 * nz)))
-        nx = max(4, int(round(r1 * ny)))
+        nz = max(min_side, int(round((n / (r1 * r2 * r2)) ** (1.0 / 3.0))))
+        ny = max(min_side, int(round(r2 * nz)))
+        nx = max(min_side, int(round(r1 * ny)))
         delta = float(rng.choice([3.015, 4.015]))
         angle = float(rng.uniform(0.0, 90.0))
         E = float(rng.uniform(50e9, 200e9))
-        spec = bar_spec(nx, ny, nz, E=E, delta_over_dx=delta, steps=steps, jitter=jitter,
+        spec = bar_spec(nx, ny, nz, E=E, delta_over_dx=delta, steps=steps, jitter=jitter, safety=safety,
                         seed=int(rng.integers(0, 2 ** 31)))
         spec["title"] = f"specimen_{k}"
         spec["slits"] = [mid_slit(spec, angle, edge=True)]
