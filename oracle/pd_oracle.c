@@ -502,8 +502,9 @@ static double bdryeff_energy(const orc_state* s, int64_t i)
     double vol = pow(p->dx, 3.0);
     double sum = 0.0;
     const double* xi_ = &s->X[3 * i];
+    /* slit cuts do not enter: the factor is that of the intact family (a slit is a
+     * pre-crack introduced after the surface calibration) */
     for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
-        if (s->cut[e]) continue;
         const double* xj = &s->X[3 * (int64_t)s->col[e]];
         double xi1 = xj[0] - xi_[0], xi2 = xj[1] - xi_[1], xi3 = xj[2] - xi_[2];
         double eta1 = eps11 * xi1 + eps12 * xi2 + eps13 * xi3;

@@ -255,8 +255,9 @@ __global__ void k_bdryeff_energy(BdryArgs a)
     double eps12 = 0.0, eps13 = 0.0, eps21 = 0.0, eps23 = 0.0, eps31 = 0.0, eps32 = 0.0;
     d4 xi = a.X[i];
     double sum = 0.0;
+    // slit cuts do not enter: the factor is that of the intact family (a slit is a
+    // pre-crack introduced after the surface calibration)
     for (int64_t e = a.row_ptr[i]; e < a.row_ptr[i + 1]; ++e) {
-        if (a.cut[e]) continue;
         d4 xj = a.X[a.col[e]];
         double xi1 = xj.x - xi.x, xi2 = xj.y - xi.y, xi3 = xj.z - xi.z;
         double eta1 = eps11 * xi1 + eps12 * xi2 + eps13 * xi3;
