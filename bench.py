@@ -136,7 +136,9 @@ def run_batch(args):
               f"{t1 - t0:.3f} s for {args.steps} steps, stage {ms:.4f} ms", file=sys.stderr, flush=True)
         b.close()
     E, n = res["fp32c"]["E"], res["fp32c"]["n"]
-    err = max(float(np.abs(a - c).max() / max(np.abs(c).max(), 1e-300)) for a, c in zip(res["fp32c"]["U"], res["fp64"]["U"]))
+    errs = np.array([float(np.abs(a - c).max() / max(np.abs(c).max(), 1e-300))
+                     for a, c in zip(res["fp32c"]["U"], res["fp64"]["U"])])
+    err = float(errs.max())
     rate = 4.0 * E * args.steps / res["fp32c"]["seconds"]
     bytes_launch = E * ENTRY_BYTES["fp32c"] + n * NODE_BYTES
     ks = res["fp32c"]["kernel_ms"] * 1e-3
@@ -153,6 +155,8 @@ def run_batch(args):
         "fp64_value": 4.0 * E * args.steps / res["fp64"]["seconds"],
         "fp32c_over_fp64": res["fp64"]["seconds"] / res["fp32c"]["seconds"],
         "fp32c_max_rel_disp_err_vs_fp64": err,
+        "fp32c_rel_err_per_specimen": {"median": float(np.median(errs)), "p99": float(np.quantile(errs, 0.99)),
+                                       "over_1e-6": int((errs > 1e-6).sum())},
         "roofline": {"bound": "hbm", "achieved": bytes_launch / ks / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": bytes_launch / ks / 1e9 / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_stage (fused bond-force gather + RK4 stage), all specimens per launch",
