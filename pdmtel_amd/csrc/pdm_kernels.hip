@@ -204,9 +204,16 @@ struct alignas(16) Rec {
 // point, so xi = X_j - X_i is an exact integer difference rounded once to FP32, and the
 // displacement u S in FP64, so eta = u_j - u_i is formed exactly before its rounding
 // to FP32 (an FP32 displacement, even relative to a tile reference, rounds eta at the
-// scale of the displacement field across the halo instead of eta's own).  Three
-// arrays: (x, y, z as int32, ecrit as FP32 bits), (ux, uy), uz: 40 bytes.  The bond
-// force is scale-free in these units; the energy carries one factor 1/S.
+// scale of the displacement field across the halo instead of eta's own).  One 48-byte
+// record per node (x, y, z as int32, ecrit as FP32 bits | ux, uy | uz), so one address
+// serves its three LDS reads.  The bond force is scale-free in these units; the energy
+// carries one factor 1/S.
+struct alignas(16) RecF {
+    u32x4 a;     // fixed-point x, y, z; ecrit (FP32 bits)
+    f64x2 u;     // u S (x, y)
+    double w;    // u S (z)
+    double pad;
+};
 
 // Stage-update inputs of one tile node, loaded while the halo is staged.
 struct EpiRec {
@@ -220,9 +227,7 @@ struct EpiRec {
 // for the stage kernels also the stage-update inputs and the material / BC tables.
 struct Lds {
     Rec* rec;        // FP64 records
-    u32x4* fa;       // FP32 records: fixed-point position, ecrit
-    f64x2* fu;       //   u S (x, y)
-    double* fz;      //   u S (z)
+    RecF* recf;      // FP32 records
     double* ec;      // per-record ecrit (FP64 records)
     double* res;
     int64_t* row;
@@ -238,7 +243,7 @@ __host__ __device__ inline size_t round16(size_t b) { return (b + 15) / 16 * 16;
 __host__ __device__ inline size_t lds_rec_bytes(int H, bool with_ecrit, bool f32)
 {
     const size_t n = (size_t)H + 1;
-    if (f32) return round16(n * sizeof(u32x4)) + round16(n * sizeof(f64x2)) + round16(n * sizeof(double));
+    if (f32) return n * sizeof(RecF);
     return n * sizeof(Rec) + (with_ecrit ? round16(n * 8) : 0);
 }
 
@@ -257,12 +262,8 @@ __device__ __forceinline__ Lds lds_view(double* base, int H, int tile_max, bool 
     char* p = reinterpret_cast<char*>(base);
     const size_t n = (size_t)H + 1;
     if (f32) {
-        l.fa = reinterpret_cast<u32x4*>(p);
-        p += round16(n * sizeof(u32x4));
-        l.fu = reinterpret_cast<f64x2*>(p);
-        p += round16(n * sizeof(f64x2));
-        l.fz = reinterpret_cast<double*>(p);
-        p += round16(n * sizeof(double));
+        l.recf = reinterpret_cast<RecF*>(p);
+        p += n * sizeof(RecF);
     } else {
         l.rec = reinterpret_cast<Rec*>(p);
         p += n * sizeof(Rec);
@@ -288,10 +289,13 @@ constexpr unsigned kFarFix = 1u << 30;  // fixed-point coordinate of the padding
 
 __device__ __forceinline__ void frec_store(const Lds& l, int q, const d4& x, const d4& v, const d4& xr, double S)
 {
-    l.fa[q] = u32x4{(unsigned)__double2int_rn((x.x - xr.x) * S), (unsigned)__double2int_rn((x.y - xr.y) * S),
-                    (unsigned)__double2int_rn((x.z - xr.z) * S), __float_as_uint((float)x.w)};
-    l.fu[q] = f64x2{v.x * S, v.y * S};
-    l.fz[q] = v.z * S;
+    RecF r;
+    r.a = u32x4{(unsigned)__double2int_rn((x.x - xr.x) * S), (unsigned)__double2int_rn((x.y - xr.y) * S),
+                (unsigned)__double2int_rn((x.z - xr.z) * S), __float_as_uint((float)x.w)};
+    r.u = f64x2{v.x * S, v.y * S};
+    r.w = v.z * S;
+    r.pad = 0.0;
+    l.recf[q] = r;
 }
 
 // Stage the tile's halo records (record q = internal node halo[h0 + q]), the padding
@@ -325,9 +329,12 @@ __device__ __forceinline__ void stage_tile(const TileView& tv, const SpecDev& sp
     }
     if (threadIdx.x == 0) {
         if (F32) {
-            l.fa[H] = u32x4{kFarFix, kFarFix, kFarFix, __float_as_uint(1.0f)};
-            l.fu[H] = f64x2{0.0, 0.0};
-            l.fz[H] = 0.0;
+            RecF r;
+            r.a = u32x4{kFarFix, kFarFix, kFarFix, __float_as_uint(1.0f)};
+            r.u = f64x2{0.0, 0.0};
+            r.w = 0.0;
+            r.pad = 0.0;
+            l.recf[H] = r;
         } else {
             l.rec[H] = Rec{kFar, kFar, kFar, 0.0, 0.0, 0.0};
             if (with_ecrit) l.ec[H] = 1.0;
@@ -353,9 +360,10 @@ struct SelfF {
     float ecrit;
     __device__ static SelfF make(const Lds& l, const SpecDev& sp, int ss, bool ue)
     {
-        const u32x4 a = l.fa[ss];
-        const f64x2 b = l.fu[ss];
-        return SelfF{(int)a.x, (int)a.y, (int)a.z, b.x, b.y, l.fz[ss], ue ? (float)sp.ecrit0 : __uint_as_float(a.w)};
+        const RecF& r = l.recf[ss];
+        const u32x4 a = r.a;
+        const f64x2 b = r.u;
+        return SelfF{(int)a.x, (int)a.y, (int)a.z, b.x, b.y, r.w, ue ? (float)sp.ecrit0 : __uint_as_float(a.w)};
     }
 };
 
@@ -420,9 +428,10 @@ struct Bond32 {
 template <bool ENERGY, bool UE>
 __device__ __forceinline__ Bond32 bond32(const Lds& l, unsigned s, float c, const SelfF& me)
 {
-    const u32x4 ra = l.fa[s];
-    const f64x2 rb = l.fu[s];
-    const double rz = l.fz[s];
+    const RecF& rec = l.recf[s];
+    const u32x4 ra = rec.a;
+    const f64x2 rb = rec.u;
+    const double rz = rec.w;
     const float x1 = (float)((int)ra.x - me.x), x2 = (float)((int)ra.y - me.y), x3 = (float)((int)ra.z - me.z);
     const float e1 = (float)(rb.x - me.u), e2 = (float)(rb.y - me.v), e3 = (float)(rz - me.w);
     const float a1 = x1 + e1, a2 = x2 + e2, a3 = x3 + e3;
@@ -576,10 +585,11 @@ struct OpT32 {
     template <bool ENERGY, bool UE>
     __device__ static void bond(const Lds& l, unsigned s, float c, const SelfF& me, Acc& a)
     {
-        const u32x4 ra = l.fa[s];
-        const f64x2 rb = l.fu[s];
+        const RecF& rec = l.recf[s];
+        const u32x4 ra = rec.a;
+        const f64x2 rb = rec.u;
         const float x1 = (float)((int)ra.x - me.x), x2 = (float)((int)ra.y - me.y), x3 = (float)((int)ra.z - me.z);
-        const float d0 = (float)(me.u - rb.x), d1 = (float)(me.v - rb.y), d2 = (float)(me.w - l.fz[s]);
+        const float d0 = (float)(me.u - rb.x), d1 = (float)(me.v - rb.y), d2 = (float)(me.w - rec.w);
         const float r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
         const float rinv = __builtin_amdgcn_rsqf(r2);
         const float proj = __builtin_fmaf(x3, d2, __builtin_fmaf(x2, d1, x1 * d0)) * c * (rinv * rinv * rinv);
