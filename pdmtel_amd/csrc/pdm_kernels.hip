@@ -420,7 +420,7 @@ struct OpF64 {  // PDM_FP64, reference arithmetic
 };
 
 struct Bond32 {
-    float g, a1, a2, a3, e;  // force factor, xi + eta, energy term coef * s * u
+    float g, a1, a2, a3, e, cs;  // force factor, xi + eta, u, coef * s
 };
 
 // FP32 bond force from the scaled FP32 records, cancellation-free stretch:
@@ -455,7 +455,8 @@ __device__ __forceinline__ Bond32 bond32(const Lds& l, unsigned s, float c, cons
     b.a1 = a1;
     b.a2 = a2;
     b.a3 = a3;
-    b.e = ENERGY ? cs * (st * rr) : 0.0f;
+    b.e = ENERGY ? st * rr : 0.0f;  // u; the energy term is cs u
+    b.cs = cs;
     return b;
 }
 
@@ -474,7 +475,7 @@ struct OpF32 {  // PDM_FP32C: FP32 bond force, FP32 block sums, FP64 family accu
         a.p0 = __builtin_fmaf(b.g, b.a1, a.p0);
         a.p1 = __builtin_fmaf(b.g, b.a2, a.p1);
         a.p2 = __builtin_fmaf(b.g, b.a3, a.p2);
-        if (ENERGY) a.el += b.e;
+        if (ENERGY) a.el = __builtin_fmaf(b.cs, b.e, a.el);
     }
     __device__ static void flush(Acc& a)
     {
@@ -514,7 +515,7 @@ struct OpF32TF {  // PDM_FP32C2: FP32 bond force, two-float family sums
         two_sum(a.h0, b.g * b.a1, t, err); a.h0 = t; a.l0 += err;
         two_sum(a.h1, b.g * b.a2, t, err); a.h1 = t; a.l1 += err;
         two_sum(a.h2, b.g * b.a3, t, err); a.h2 = t; a.l2 += err;
-        if (ENERGY) a.el += b.e;
+        if (ENERGY) a.el = __builtin_fmaf(b.cs, b.e, a.el);
     }
     template <int LPN>
     __device__ static void reduce(Acc& a)
