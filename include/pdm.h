@@ -60,8 +60,8 @@ typedef struct {
 } pdm_options;
 
 typedef struct {
-    int iterations;         /* inner CG iterations, all refinement sweeps */
-    int refinements;        /* outer FP64 residual / refinement sweeps */
+    int iterations;         /* PCG iterations */
+    int refinements;        /* FP64 residual evaluations (each replaces the CG residual) */
     double rel_residual;    /* final ||b - K u|| / ||b|| in FP64 */
     double seconds;         /* wall time of the solve */
 } pdm_static_stats;
@@ -159,7 +159,8 @@ int pdm_get_energy(const pdm_handle* h, double* ke, double* ee);
 int pdm_solve_dynamic(pdm_handle* h, const char* out_dir);
 
 /* Static equilibrium K u = b on the tangent of the same bond operator, matrix-free
- * Jacobi-preconditioned CG with FP64 residual refinement to rel_tol.  Replaces the
+ * Jacobi-preconditioned CG (matvecs in the handle's precision) whose residual is
+ * replaced by the FP64 residual every 25 iterations, to rel_tol in FP64.  Replaces the
  * dense SolveStatic/AssembleEquation/Gauss path (pdSolve.cpp:132-673), which is
  * broken in the reference.  On success U holds the solution and V = 0. */
 int pdm_solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_stats* stats);

@@ -217,3 +217,23 @@ def specimen_specs(count: int, seed: int = 0, nodes=(1e3, 1e5), steps: int = 100
         spec["slits"] = [mid_slit(spec, angle, edge=True)]
         out.append(spec)
     return out
+
+
+def plate_spec(nx: int = 400, ny: int = 400, nz: int = 50, jitter: float = 0.01, seed: int = 0, steps: int = 500,
+               crack_fraction: float = 0.5) -> Dict[str, Any]:
+    """Plate with two crossing slits through the thickness at its centre.
+
+    Normals (1, 0, 0) and (0, 1, 0); each slit spans ``crack_fraction`` of the plate's
+    width across its line (centred), so the plate stays in one piece.  Boundary
+    conditions as in ``bar_spec``: x = 0 layer fixed, 1 MPa on the x = L layer.
+    """
+    spec = bar_spec(nx, ny, nz, jitter=jitter, seed=seed, steps=steps)
+    spec["title"] = f"plate_{nx}x{ny}x{nz}"
+    dx = spec["grid_spacing"]
+    c = tuple(spec["grid_center"])
+    big = 10.0 * max(nx, ny, nz) * dx
+    spec["slits"] = [
+        dict(type=2, c=c, n=(1.0, 0.0, 0.0), t=(0.0, 0.0, 1.0), half_t=big, half_s=0.5 * crack_fraction * ny * dx),
+        dict(type=2, c=c, n=(0.0, 1.0, 0.0), t=(0.0, 0.0, 1.0), half_t=big, half_s=0.5 * crack_fraction * nx * dx),
+    ]
+    return spec
