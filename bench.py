@@ -58,7 +58,10 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_bar200x40x40_traffic.txt"
 
 def workload_spec(args, steps):
     import pdmtel_amd as pm
+    from pdmtel_amd.deck import plate_spec
 
+    if args.workload == "plate":
+        return plate_spec(args.nx, args.ny, args.nz, jitter=0.01, seed=0, steps=steps)
     return pm.bar_spec(args.nx, args.ny, args.nz, jitter=0.01, seed=0, steps=steps)
 
 
@@ -89,11 +92,12 @@ def run_static(args):
     import pdmtel_amd as pm
 
     spec = workload_spec(args, 1)
-    spec["slits"] = [pm.mid_slit(spec, 30.0, edge=True)]
+    if args.workload == "bar":
+        spec["slits"] = [pm.mid_slit(spec, 30.0, edge=True)]
     text = pm.deck_text(spec)
     out = {}
     for name, prec in (("fp32c", pm.PDM_FP32C), ("fp64", pm.PDM_FP64)):
-        prob = pm.Problem(text, precision=prec)
+        prob = pm.Problem(text, precision=prec, verbose=1)  # progress lines on stderr
         t0 = time.perf_counter()
         st = prob.solve_static(rel_tol=1e-10, max_iterations=200000)
         wall = time.perf_counter() - t0
@@ -180,7 +184,7 @@ def cpu_baseline(spec, steps):
 
 def recorded_traffic(args, prec):
     """FETCH_SIZE + WRITE_SIZE bytes per stage launch recorded for the default workload, else None."""
-    if (args.nx, args.ny, args.nz) != DEFAULT_DIMS or not os.path.exists(TRAFFIC_FILE):
+    if args.workload != "bar" or (args.nx, args.ny, args.nz) != DEFAULT_DIMS or not os.path.exists(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):
         m = re.match(r"^traffic_bytes_per_launch\s+(\w+)\s+([0-9.eE+]+)", line)
@@ -217,7 +221,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-static", action="store_true")
     ap.add_argument("--calib-copy", type=int, default=0, help="run only a copy kernel of this many bytes")
-    ap.add_argument("--workload", choices=["bar", "batch"], default="bar")
+    ap.add_argument("--workload", choices=["bar", "batch", "plate"], default="bar",
+                    help="bar: 200x40x40 bar (default); batch: random specimens; plate: --nx --ny --nz plate "
+                         "with two crossing slits (400 400 50 for the 8M-node plate)")
     ap.add_argument("--specimens", type=int, default=512)
     args = ap.parse_args()
 
@@ -273,8 +279,10 @@ def main():
         "dtype": "f32+f64" if head_prec == "fp32c" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"bar {args.nx}x{args.ny}x{args.nz}, {n} nodes, {E} family entries, "
-                        f"delta=3.015dx, jitter 0.01dx seed 0, RK4 {args.steps} steps",
+            "workload": f"{args.workload} {args.nx}x{args.ny}x{args.nz}, {n} nodes, {E} family entries, "
+                        f"delta=3.015dx, jitter 0.01dx seed 0"
+                        + (", two crossing slits" if args.workload == "plate" else "")
+                        + f", RK4 {args.steps} steps",
             "precision": head_prec,
             "nodes": n,
             "family_entries": E,
@@ -308,8 +316,10 @@ def main():
     if not args.skip_static:
         st, serr = run_static(args)
         line["static_solve"] = {
-            "workload": f"bar {args.nx}x{args.ny}x{args.nz} jitter 0.01dx seed 0, edge crack to the mid-plane, normal 30 deg to x, "
-                        "Jacobi-PCG to 1e-10 relative residual",
+            "workload": (f"bar {args.nx}x{args.ny}x{args.nz} jitter 0.01dx seed 0, edge crack to the mid-plane, "
+                         "normal 30 deg to x, Jacobi-PCG to 1e-10 relative residual") if args.workload == "bar" else
+                        (f"plate {args.nx}x{args.ny}x{args.nz} jitter 0.01dx seed 0, two crossing slits, "
+                         "Jacobi-PCG to 1e-10 relative residual"),
             "wall_s": st["fp32c"]["wall"], "iterations": st["fp32c"]["iterations"],
             "refinements": st["fp32c"]["refinements"], "rel_residual": st["fp32c"]["rel_residual"],
             "fp64_wall_s": st["fp64"]["wall"], "fp64_iterations": st["fp64"]["iterations"],
