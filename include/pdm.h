@@ -88,7 +88,9 @@ int pdm_options_default(pdm_options* opt);
 /* Parse a pds_input.txt deck (CRLF or LF) and create a handle; no GPU work.
  * Replaces pdDataManager::ReadInputFile (pdDataManager.cpp:4-758) with
  * ReadKeyword/FindKeyword (pdDataManager.h:270-421) and SetWkDir's input
- * check (pdDataManager3.cpp:1503-1541). */
+ * check (pdDataManager3.cpp:1503-1541).  Integration: CCI (cci_on 1) in any precision;
+ * fixed Gaussian integration (cci_on 0, fgi_on 1, intg_point 1..6 or 8) in PDM_FP64;
+ * adaptive integration (ai_on) is refused with PDM_E_MODEL. */
 int pdm_create(const char* input_path, const pdm_options* opt, pdm_handle** out);
 
 /* Same, from deck text in memory. */
@@ -134,8 +136,18 @@ int pdm_get_timestep(const pdm_handle* h, double* dt, double* min_bond_length);
 /* Family force sum F_i = sum_j f_ij for a displacement field u (3N host), every
  * node's displacement taken from u: the CCI family loop of CalcBondBasedMotionRK4
  * (pdDataManager.cpp:1795-1864) with CalcBondForceCCI (pdDataManager2.cpp:1293-1351).
- * Runs the handle's precision.  elast (N, may be NULL) receives the per-node energy density. */
+ * Runs the handle's precision.  elast (N, may be NULL) receives the per-node energy density.
+ * Fixed Gaussian integration: the sum over each neighbour's Gauss points
+ * (pdDataManager.cpp:1918-1957), every Gauss point interpolated from u
+ * (CalcIntgPointDisplRK4's non-BC branch, pdDataManager.cpp:2426-2479). */
 int pdm_compute_forces(pdm_handle* h, const double* u, double* F, double* elast);
+
+/* Fixed Gaussian integration: Gauss points per node (n^3; 0 for CCI), and their positions
+ * and current displacements, [node][point][xyz] in reference node order with points in
+ * SetGaussianPoint's order (pdDataManager2.cpp:72-104; pdNode::GetGaussianPoint and the
+ * pdGauss getters, pdNode.h:100-102, pdGauss.h).  Xg or Ug may be NULL. */
+int pdm_num_gauss_points(const pdm_handle* h);
+int pdm_get_gauss(const pdm_handle* h, double* Xg, double* Ug);
 
 /* Set / get the dynamic state U, V (3N each, host; NULL skips). */
 int pdm_set_state(pdm_handle* h, const double* U, const double* V);
@@ -158,14 +170,14 @@ int pdm_get_energy(const pdm_handle* h, double* ke, double* ee);
  * step (pdSolve.cpp:24-130, 708-732). */
 int pdm_solve_dynamic(pdm_handle* h, const char* out_dir);
 
-/* Static equilibrium K u = b on the tangent of the same bond operator, matrix-free
+/* Static equilibrium K u = b on the tangent of the same bond operator (CCI decks), matrix-free
  * Jacobi-preconditioned CG (matvecs in the handle's precision) whose residual is
  * replaced by the FP64 residual every 25 iterations, to rel_tol in FP64.  Replaces the
  * dense SolveStatic/AssembleEquation/Gauss path (pdSolve.cpp:132-673), which is
  * broken in the reference.  On success U holds the solution and V = 0. */
 int pdm_solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_stats* stats);
 
-/* Apply the tangent operator y = K u (3N host each), in the handle's precision. */
+/* Apply the tangent operator y = K u (3N host each), in the handle's precision (CCI decks). */
 int pdm_apply_tangent(pdm_handle* h, const double* u, double* y);
 
 /* Tecplot dump, same line format as pdSolve::WriteTecplot / pdNode::Print
@@ -193,7 +205,7 @@ int pdm_last_kernel_timing(const pdm_handle* h, double* avg_ms, int64_t* launche
  * pdSolve::InitialProblem + SolveDynamic (pdSolve.cpp:10-130) once per deck. */
 typedef struct pdm_batch pdm_batch;
 
-/* Parse nspec decks (files, or texts in memory); no GPU work.  Options are shared. */
+/* Parse nspec decks (files, or texts in memory); no GPU work.  Options are shared.  CCI decks only. */
 int pdm_batch_create(const char* const* input_paths, int nspec, const pdm_options* opt, pdm_batch** out);
 int pdm_batch_create_from_text(const char* const* deck_texts, int nspec, const pdm_options* opt, pdm_batch** out);
 int pdm_batch_destroy(pdm_batch* b);

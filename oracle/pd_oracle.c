@@ -39,6 +39,19 @@
  *                pdDataManager.cpp:1246-1290)        brute-force segment/rectangle test in numpy
  *   tangent K u  pdDataManager2.cpp:1763-1852 (operator form)
  *                                                     finite difference of the force (eta -> 0)
+ *   Gauss rules  pdDataManager3.cpp:1706-1799         numpy.polynomial.legendre.leggauss; the
+ *                                                     rule integrates x^(2n-1) exactly
+ *   Gauss points pdDataManager2.cpp:72-104            closed-form positions x_j + (dx/2) a
+ *   MLS          pdDataManager2.cpp:106-256,1547-1614 reproduces quadratic fields (the reference's
+ *                                                     ConsistencyCheck criteria); equals an
+ *                                                     independent numpy weighted least squares
+ *   FGI force    pdDataManager.cpp:1918-1957          n = 1 equals the modified-CCI force;
+ *                                                     eta = 0 -> 0; rigid translation -> 0; the
+ *                                                     force converges in n to a numpy cubature
+ *                                                     of the bond force over node j's cell
+ *   Gauss-pt u   pdDataManager.cpp:2402-2495,2604-2703
+ *                                                     affine fields interpolate exactly; BC
+ *                                                     points take the prescribed value
  */
 #include "pd_oracle.h"
 
@@ -78,6 +91,14 @@ struct orc_state {
     double max_c;
     int max_family;
     double* trac;  /* 3N traction body-force density (extension) */
+    /* fixed Gaussian integration (fgi_on): ng = n^3 points per node */
+    int ng;
+    double* gofs;  /* 3 ng: point offsets 0.5 dx a from the node */
+    double* gw;    /* 3 ng: the three 1-D weights of each point */
+    int64_t* cb_rp;/* N+1: MLS contributing nodes of each node (n > 1) */
+    int32_t* cb;
+    double* phi;   /* ng * cb_rp[N]: node i's block [g][k] at ng * cb_rp[i] */
+    double* G;     /* 3 N ng: Gauss-point displacements */
     /* dynamic state */
     double *U, *V, *Vb, *elast;
     double t, dt_cur, dt_stable, ke, ee;
@@ -308,6 +329,7 @@ void orc_destroy(orc_state* s)
     free(s->X); free(s->mat); free(s->bc); free(s->rp); free(s->col); free(s->cut);
     free(s->be); free(s->spring); free(s->trac);
     free(s->U); free(s->V); free(s->Vb); free(s->elast);
+    free(s->gofs); free(s->gw); free(s->cb_rp); free(s->cb); free(s->phi); free(s->G);
     free(s);
 }
 
@@ -601,6 +623,282 @@ static void init_traction(orc_state* s)
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* fixed Gaussian integration                                           */
+
+/* Gauss-Legendre rule of order n on [-1, 1] (the rule SetGaussianPointsValue tabulates,
+ * pdDataManager3.cpp:1706-1799), found by Newton's method on the three-term recurrence
+ * (n+1) P_{n+1} = (2n+1) x P_n - n P_{n-1}, with w = 2 / ((1 - x^2) P_n'(x)^2).  Points are
+ * listed in the reference's order: the zero (odd n), then each positive root in increasing
+ * order followed by its negative; for n = 8 the four positive roots in increasing order,
+ * then their negatives.  n = 7 and n > 8 have no rule in the reference. */
+static void legendre(int n, double x, double* pn, double* dpn)
+{
+    double p0 = 1.0, p1 = x;
+    for (int q = 1; q < n; ++q) {
+        double p2 = ((2.0 * q + 1.0) * x * p1 - q * p0) / (q + 1.0);
+        p0 = p1;
+        p1 = p2;
+    }
+    *pn = p1;
+    *dpn = n * (x * p1 - p0) / (x * x - 1.0);
+}
+
+static int gauss_rule(int n, double* a, double* w)
+{
+    if (n < 1 || n == 7 || n > 8) return -1;
+    if (n == 1) { a[0] = 0.0; w[0] = 2.0; return 0; }
+    const double pi = acos(-1.0);
+    int m = n / 2;
+    double r[4], rw[4], pn, dp;
+    for (int k = 0; k < m; ++k) {
+        double x = cos(pi * (k + 0.75) / (n + 0.5));  /* near the k-th largest root */
+        for (int it = 0; it < 100; ++it) {
+            legendre(n, x, &pn, &dp);
+            double step = pn / dp;
+            x -= step;
+            if (fabs(step) < 1e-17) break;
+        }
+        legendre(n, x, &pn, &dp);
+        r[m - 1 - k] = x;  /* increasing */
+        rw[m - 1 - k] = 2.0 / ((1.0 - x * x) * dp * dp);
+    }
+    int q = 0;
+    if (n & 1) {
+        legendre(n, 0.0, &pn, &dp);
+        a[q] = 0.0;
+        w[q] = 2.0 / (dp * dp);
+        ++q;
+    }
+    for (int k = 0; k < m; ++k) {
+        if (n == 8) {
+            a[q] = r[k]; w[q] = rw[k];
+            a[q + m] = -r[k]; w[q + m] = rw[k];
+            ++q;
+        } else {
+            a[q] = r[k]; w[q] = rw[k]; ++q;
+            a[q] = -r[k]; w[q] = rw[k]; ++q;
+        }
+    }
+    return 0;
+}
+
+int orc_gauss_rule(int n, double* a, double* w) { return gauss_rule(n, a, w); }
+
+/* Quadratic MLS basis (GetBasis, pdDataManager2.cpp:1547-1562) at a point given relative
+ * to the node and in units of dx.  The reference evaluates it at global coordinates; the
+ * quadratic basis spans the same space after any shift and scaling, so the shape functions
+ * are the same, and the moment matrix is far better conditioned (about 20 against 1e12). */
+static void mls_basis(double x, double y, double z, double* b)
+{
+    b[0] = 1.0; b[1] = x; b[2] = y; b[3] = z;
+    b[4] = x * y; b[5] = x * z; b[6] = y * z;
+    b[7] = x * x; b[8] = y * y; b[9] = z * z;
+}
+
+/* Quartic spline weight (GetWeight, pdDataManager2.cpp:1564-1577) */
+static double mls_weight(const double* xk, const double* xg, double rw)
+{
+    double a = xg[0] - xk[0], b = xg[1] - xk[1], c = xg[2] - xk[2];
+    double r = sqrt(a * a + b * b + c * c);
+    double d = r / rw;
+    if (d > 1.0) return 0.0;
+    return (1 - 6.0 * d * d + 8.0 * pow(d, 3.0) - 3.0 * pow(d, 4.0));
+}
+
+/* Gauss-Jordan elimination with full pivoting on the 10 x 10 system A x = b (the solver
+ * of pdMatrix::Gauss, pdMatrix.cpp:251-340).  A and b are overwritten.  -1 if singular. */
+static int gj_solve(double A[10][10], double* b, double* x)
+{
+    int colp[10];
+    for (int k = 0; k < 10; ++k) colp[k] = k;
+    for (int k = 0; k < 10; ++k) {
+        int pr = k, pc = k;
+        double big = -1.0;
+        for (int r = k; r < 10; ++r)
+            for (int c = k; c < 10; ++c)
+                if (fabs(A[r][c]) > big) { big = fabs(A[r][c]); pr = r; pc = c; }
+        if (!(big > 0.0)) return -1;
+        if (pr != k) {
+            for (int c = 0; c < 10; ++c) { double t = A[k][c]; A[k][c] = A[pr][c]; A[pr][c] = t; }
+            double t = b[k]; b[k] = b[pr]; b[pr] = t;
+        }
+        if (pc != k) {
+            for (int r = 0; r < 10; ++r) { double t = A[r][k]; A[r][k] = A[r][pc]; A[r][pc] = t; }
+            int t = colp[k]; colp[k] = colp[pc]; colp[pc] = t;
+        }
+        double piv = A[k][k];
+        for (int c = 0; c < 10; ++c) A[k][c] /= piv;
+        b[k] /= piv;
+        for (int r = 0; r < 10; ++r) {
+            if (r == k || A[r][k] == 0.0) continue;
+            double f = A[r][k];
+            for (int c = 0; c < 10; ++c) A[r][c] -= f * A[k][c];
+            b[r] -= f * b[k];
+        }
+    }
+    for (int k = 0; k < 10; ++k) x[colp[k]] = b[k];
+    return 0;
+}
+
+/* SetGaussianPoint and InitIntegrationPoint's FGI branch (pdDataManager2.cpp:72-104,
+ * 106-256): n^3 points per node (x index outermost, z innermost) at x_i + (dx/2) a, zero
+ * displacement; for n > 1 the MLS contributing nodes (the node itself, then its family
+ * within rw, or within 1.5 rw when that leaves fewer than 10) and their coefficients at
+ * every point.  Every weight uses rw, as the reference's per-node support size does. */
+static int init_fgi(orc_state* s)
+{
+    const orc_problem* p = &s->p;
+    int n1 = p->intg_point;
+    double a[8], w[8];
+    if (gauss_rule(n1, a, w)) return -6;
+    int ng = n1 * n1 * n1;
+    int64_t n = s->n;
+    s->ng = ng;
+    s->gofs = (double*)malloc(sizeof(double) * 3 * ng);
+    s->gw = (double*)malloc(sizeof(double) * 3 * ng);
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n1; ++j)
+            for (int k = 0; k < n1; ++k) {
+                int g = (i * n1 + j) * n1 + k;
+                s->gofs[3 * g] = 0.5 * p->dx * a[i];
+                s->gofs[3 * g + 1] = 0.5 * p->dx * a[j];
+                s->gofs[3 * g + 2] = 0.5 * p->dx * a[k];
+                s->gw[3 * g] = w[i]; s->gw[3 * g + 1] = w[j]; s->gw[3 * g + 2] = w[k];
+            }
+    s->G = (double*)calloc(3 * (size_t)(n ? n : 1) * ng, sizeof(double));
+    if (n1 == 1) return 0;
+    s->cb_rp = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
+    s->cb_rp[0] = 0;
+    ivec lst = {0, 0, NULL};
+    for (int64_t i = 0; i < n; ++i) {
+        const double* xi = &s->X[3 * i];
+        for (int pass = 0; pass < 2; ++pass) {
+            double rad = pass ? p->rw * 1.5 : p->rw;
+            int64_t start = lst.cnt;
+            ivec_push(&lst, (int32_t)i);
+            for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
+                const double* xj = &s->X[3 * s->col[e]];
+                double d = sqrt(sq(xj[0] - xi[0]) + sq(xj[1] - xi[1]) + sq(xj[2] - xi[2]));
+                if (d <= rad) ivec_push(&lst, s->col[e]);
+            }
+            if (lst.cnt - start >= 10 || pass == 1) break;
+            lst.cnt = start;
+        }
+        s->cb_rp[i + 1] = lst.cnt;
+    }
+    s->cb = lst.v;
+    s->phi = (double*)malloc(sizeof(double) * (size_t)ng * (size_t)(lst.cnt ? lst.cnt : 1));
+    int bad = 0;
+    for (int64_t i = 0; i < n && !bad; ++i) {
+        const double* xi = &s->X[3 * i];
+        int64_t c0 = s->cb_rp[i], m = s->cb_rp[i + 1] - c0;
+        double* ph = s->phi + (size_t)ng * c0;
+        for (int g = 0; g < ng; ++g) {
+            double xg[3] = {s->gofs[3 * g] + xi[0], s->gofs[3 * g + 1] + xi[1], s->gofs[3 * g + 2] + xi[2]};
+            double A[10][10], bb[10], sol[10], pg[10];
+            memset(A, 0, sizeof A);
+            for (int64_t k = 0; k < m; ++k) {
+                const double* xk = &s->X[3 * s->cb[c0 + k]];
+                double pk[10];
+                mls_basis((xk[0] - xi[0]) / p->dx, (xk[1] - xi[1]) / p->dx, (xk[2] - xi[2]) / p->dx, pk);
+                double wk = mls_weight(xk, xg, p->rw);
+                for (int r = 0; r < 10; ++r)
+                    for (int c = 0; c < 10; ++c) A[r][c] += wk * pk[r] * pk[c];
+            }
+            mls_basis((xg[0] - xi[0]) / p->dx, (xg[1] - xi[1]) / p->dx, (xg[2] - xi[2]) / p->dx, pg);
+            memcpy(bb, pg, sizeof bb);
+            if (gj_solve(A, bb, sol)) { bad = 1; break; }
+            for (int64_t k = 0; k < m; ++k) {
+                const double* xk = &s->X[3 * s->cb[c0 + k]];
+                double pk[10];
+                mls_basis((xk[0] - xi[0]) / p->dx, (xk[1] - xi[1]) / p->dx, (xk[2] - xi[2]) / p->dx, pk);
+                double v = 0.0;
+                for (int r = 0; r < 10; ++r) v += pk[r] * sol[r];
+                ph[(size_t)g * m + k] = mls_weight(xk, xg, p->rw) * v;
+            }
+        }
+    }
+    return bad ? -7 : 0;
+}
+
+/* CalcIntgPointDisplRK4 (pdDataManager.cpp:2402-2495) with UpdateIntgPointStatusWithBC
+ * (:2604-2703).  mode 0: as the reference (BC nodes: DISPL / DISPLGRAD components take the
+ * prescribed value at time t, other components keep theirs; other nodes: MLS interpolation of
+ * u, or a copy for n = 1); mode 1: only the non-BC nodes; mode 2: every node interpolated. */
+static int bc_node(const orc_state* s, int64_t i);
+static void gauss_update(orc_state* s, double t, const double* u, int mode)
+{
+    const orc_problem* p = &s->p;
+    int ng = s->ng;
+    for (int64_t i = 0; i < s->n; ++i) {
+        double* G = s->G + (size_t)3 * ng * i;
+        if (mode != 2 && bc_node(s, i)) {
+            if (mode == 1) continue;
+            const orc_bc* bc = &p->bc[s->bc[i]];
+            double fac = orc_time_factor(bc, t);
+            for (int g = 0; g < ng; ++g) {
+                double xg[3];
+                for (int c = 0; c < 3; ++c) xg[c] = s->gofs[3 * g + c] + s->X[3 * i + c];
+                for (int c = 0; c < 3; ++c) {
+                    if (bc->f[c] != 1) continue;
+                    if (bc->type == 1) G[3 * g + c] = bc->v[c] * fac;
+                    else if (bc->type == 3)
+                        G[3 * g + c] = (bc->v[3 * c] * xg[0] + bc->v[3 * c + 1] * xg[1] + bc->v[3 * c + 2] * xg[2]) * fac;
+                }
+            }
+            continue;
+        }
+        if (!s->phi) {
+            for (int c = 0; c < 3; ++c) G[c] = u[3 * i + c];
+            continue;
+        }
+        int64_t c0 = s->cb_rp[i], m = s->cb_rp[i + 1] - c0;
+        const double* ph = s->phi + (size_t)ng * c0;
+        for (int g = 0; g < ng; ++g) {
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+            for (int64_t k = 0; k < m; ++k) {
+                int64_t j = s->cb[c0 + k];
+                double f = ph[(size_t)g * m + k];
+                a0 += u[3 * j] * f;
+                a1 += u[3 * j + 1] * f;
+                a2 += u[3 * j + 2] * f;
+            }
+            G[3 * g] = a0; G[3 * g + 1] = a1; G[3 * g + 2] = a2;
+        }
+    }
+}
+
+/* The fixed-Gaussian branch of CalcBondBasedMotionRK4's family loop (pdDataManager.cpp:
+ * 1918-1957) for the entry (i, j): the CCI bond force at each of node j's Gauss points with
+ * that point's effective weight (w1 w2 w3)^(1/3), summed with the weights dx^3/8 w1 w2 w3. */
+static void fgi_entry(const orc_state* s, int64_t i, int64_t j, const double* ui, double* F, double* el)
+{
+    const orc_problem* p = &s->p;
+    const orc_mat* mi = &p->mat[s->mat[i]];
+    const orc_mat* mj = &p->mat[s->mat[j]];
+    double ecrit = (mj->ecrit < mi->ecrit) ? mj->ecrit : mi->ecrit;
+    double ci = s->spring[s->mat[i]], cj = s->spring[s->mat[j]];
+    double spring = (cj < ci) ? cj : ci;
+    double bij = 0.5 * (s->be[i] + s->be[j]);
+    const double* G = s->G + (size_t)3 * s->ng * j;
+    F[0] = F[1] = F[2] = 0.0;
+    for (int g = 0; g < s->ng; ++g) {
+        double xg[3], xi[3], eta[3];
+        for (int c = 0; c < 3; ++c) {
+            xg[c] = s->gofs[3 * g + c] + s->X[3 * j + c];
+            xi[c] = xg[c] - s->X[3 * i + c];
+            eta[c] = G[3 * g + c] - ui[c];
+        }
+        double w1 = s->gw[3 * g], w2 = s->gw[3 * g + 1], w3 = s->gw[3 * g + 2];
+        double effWeight = pow(w1 * w2 * w3, 1.0 / 3.0);
+        double f[3], e1;
+        orc_bond_force_cci(p, xi, eta, ecrit, spring, bij, effWeight, f, &e1);
+        *el += e1;
+        for (int c = 0; c < 3; ++c) F[c] += (pow(p->dx, 3) / 8.0) * w1 * w2 * w3 * f[c];
+    }
+}
+
 int orc_init(orc_state* s, int method, int nthreads)
 {
     (void)nthreads;
@@ -611,6 +909,10 @@ int orc_init(orc_state* s, int method, int nthreads)
     init_material(s);
     init_bdryeff(s);
     init_traction(s);
+    if (s->p.fgi_on && !s->p.cci_on) {
+        rc = init_fgi(s);
+        if (rc) return rc;
+    }
     int64_t n = s->n;
     s->U = (double*)calloc(3 * (n ? n : 1), sizeof(double));
     s->V = (double*)calloc(3 * (n ? n : 1), sizeof(double));
@@ -672,6 +974,28 @@ static void family_force(const orc_state* s, int64_t i, const double* u, double*
 
 void orc_forces(const orc_state* s, const double* u, double* F, double* elast, int nthreads)
 {
+    if (s->ng) {
+        /* FGI: every node's Gauss points interpolated from u (no BC values), in a scratch
+         * copy so that the stepping state keeps its own */
+        orc_state* w = (orc_state*)s;
+        double* keep = w->G;
+        w->G = (double*)malloc(sizeof(double) * 3 * (size_t)(s->n ? s->n : 1) * s->ng);
+        gauss_update(w, 0.0, u, 2);
+        for (int64_t i = 0; i < s->n; ++i) {
+            double acc[3] = {0.0, 0.0, 0.0}, ea = 0.0;
+            for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
+                if (s->cut[e]) continue;
+                double f[3];
+                fgi_entry(s, i, s->col[e], &u[3 * i], f, &ea);
+                acc[0] += f[0]; acc[1] += f[1]; acc[2] += f[2];
+            }
+            for (int c = 0; c < 3; ++c) F[3 * i + c] = acc[c];
+            if (elast) elast[i] = ea;
+        }
+        free(w->G);
+        w->G = keep;
+        return;
+    }
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 256) if (nthreads != 1)
@@ -760,6 +1084,14 @@ static void rk4_stage(orc_state* s, double dts, double t, const double* trial, d
         for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
             if (s->cut[e]) continue;
             int64_t j = s->col[e];
+            if (s->ng) {
+                /* fixed Gaussian integration: node j's Gauss points */
+                double f[3], el = 0.0;
+                fgi_entry(s, i, j, ui, f, &el);
+                s->elast[i] += el;
+                for (int c = 0; c < 3; ++c) s->Vb[3 * i + c] += (f[c] + bf[c]) * dts / denst;
+                continue;
+            }
             const orc_mat* mj = &p->mat[s->mat[j]];
             double xi[3] = {s->X[3 * j] - s->X[3 * i], s->X[3 * j + 1] - s->X[3 * i + 1], s->X[3 * j + 2] - s->X[3 * i + 2]};
             const double* uj = bc_node(s, j) ? &s->U[3 * j] : &trial[3 * j];
@@ -793,7 +1125,8 @@ static void rk4_stage(orc_state* s, double dts, double t, const double* trial, d
     }
 }
 
-/* RK4, pdDataManager.cpp:1994-2134 (CCI: CalcIntgPointDisplRK4 is a no-op) */
+/* RK4, pdDataManager.cpp:1994-2134.  CalcIntgPointDisplRK4 runs before stages 2-4 and after
+ * the update (a no-op for CCI): stage 1 uses the Gauss-point displacements of the previous step. */
 static void rk4(orc_state* s, double dt, double t, int nthreads)
 {
     const orc_problem* p = &s->p;
@@ -807,10 +1140,13 @@ static void rk4(orc_state* s, double dt, double t, int nthreads)
     memcpy(u1, s->U, sizeof(double) * n3);
     rk4_stage(s, dtHalf, t, u1, k1, nthreads);
     for (int64_t i = 0; i < n3; ++i) u234[i] = u1[i] + 0.5 * k1[i];
+    if (s->ng) gauss_update(s, t, u234, 0);
     rk4_stage(s, dtHalf, t, u234, k24, nthreads);
     for (int64_t i = 0; i < n3; ++i) u234[i] = u1[i] + 0.5 * k24[i];
+    if (s->ng) gauss_update(s, t, u234, 0);
     rk4_stage(s, dt, t, u234, k3, nthreads);
     for (int64_t i = 0; i < n3; ++i) { u234[i] = u1[i] + k3[i]; k3[i] += k24[i]; }
+    if (s->ng) gauss_update(s, t, u234, 0);
     rk4_stage(s, dt, t, u234, k24, nthreads);
     for (int64_t i = 0; i < n; ++i) {
         double denst = p->mat[s->mat[i]].rho;
@@ -832,6 +1168,7 @@ static void rk4(orc_state* s, double dt, double t, int nthreads)
         s->ee += vol * s->elast[i];
         s->ke += 0.5 * denst * velsq * vol;
     }
+    if (s->ng) gauss_update(s, t, s->U, 0);
     free(u1); free(u234); free(k1); free(k24); free(k3);
 }
 
@@ -899,9 +1236,11 @@ void orc_get_state(const orc_state* s, double* U, double* V)
     if (U) memcpy(U, s->U, sizeof(double) * 3 * s->n);
     if (V) memcpy(V, s->V, sizeof(double) * 3 * s->n);
 }
+/* U also re-interpolates the Gauss points of the non-BC nodes, as the end of a step does. */
 void orc_set_state(orc_state* s, const double* U, const double* V)
 {
     if (U) memcpy(s->U, U, sizeof(double) * 3 * s->n);
+    if (U && s->ng) gauss_update(s, 0.0, s->U, 1);
     if (V) memcpy(s->V, V, sizeof(double) * 3 * s->n);
 }
 double orc_time(const orc_state* s) { return s->t; }
@@ -909,3 +1248,23 @@ double orc_dt_current(const orc_state* s) { return s->dt_cur; }
 double orc_ke(const orc_state* s) { return s->ke; }
 double orc_ee(const orc_state* s) { return s->ee; }
 int orc_steps_done(const orc_state* s) { return s->steps_done; }
+int orc_num_gauss(const orc_state* s) { return s->ng; }
+int64_t orc_num_contrib(const orc_state* s) { return s->cb_rp ? s->cb_rp[s->n] : 0; }
+void orc_get_gauss(const orc_state* s, double* Xg, double* Ug)
+{
+    for (int64_t i = 0; i < s->n; ++i)
+        for (int g = 0; g < s->ng; ++g)
+            for (int c = 0; c < 3; ++c) {
+                size_t q = ((size_t)i * s->ng + g) * 3 + c;
+                if (Xg) Xg[q] = s->gofs[3 * g + c] + s->X[3 * i + c];
+                if (Ug) Ug[q] = s->G[q];
+            }
+}
+void orc_get_mls(const orc_state* s, int64_t* cb_rp, int32_t* cb, double* phi)
+{
+    if (!s->cb_rp) return;
+    int64_t m = s->cb_rp[s->n];
+    if (cb_rp) memcpy(cb_rp, s->cb_rp, sizeof(int64_t) * (s->n + 1));
+    if (cb) memcpy(cb, s->cb, sizeof(int32_t) * m);
+    if (phi) memcpy(phi, s->phi, sizeof(double) * (size_t)s->ng * m);
+}

@@ -109,6 +109,17 @@ double orc_ke(const orc_state* s);
 double orc_ee(const orc_state* s);
 int orc_steps_done(const orc_state* s);
 
+/* Fixed Gaussian integration (fgi_on, cci_on 0): n^3 Gauss points per node.
+ * orc_gauss_rule: the order-n Gauss-Legendre rule in the reference's point order (-1: no rule).
+ * orc_get_gauss: point positions and displacements, [node][point][xyz].
+ * orc_get_mls: contributing nodes (row pointer N+1, ids) and coefficients, node i's block
+ * [point][contributing node] at num_gauss * cb_rp[i]. */
+int orc_gauss_rule(int n, double* a, double* w);
+int orc_num_gauss(const orc_state* s);
+int64_t orc_num_contrib(const orc_state* s);
+void orc_get_gauss(const orc_state* s, double* Xg, double* Ug);
+void orc_get_mls(const orc_state* s, int64_t* cb_rp, int32_t* cb, double* phi);
+
 /* 0: reference pow(x, 2.0) via glibc (default); 1: correctly rounded square x*x */
 void orc_set_square_mode(int exact);
 

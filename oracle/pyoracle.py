@@ -108,6 +108,12 @@ def lib() -> C.CDLL:
         L.orc_static_system.argtypes = [vp, dp, dp, dp]
         L.orc_time_factor.restype = C.c_double
         L.orc_time_factor.argtypes = [C.POINTER(_BC), C.c_double]
+        L.orc_gauss_rule.argtypes = [C.c_int, dp, dp]
+        L.orc_num_gauss.argtypes = [vp]
+        L.orc_num_contrib.restype = C.c_int64
+        L.orc_num_contrib.argtypes = [vp]
+        L.orc_get_gauss.argtypes = [vp, dp, dp]
+        L.orc_get_mls.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), dp]
         _lib = L
     return _lib
 
@@ -196,6 +202,14 @@ def _dp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def gauss_rule(n: int):
+    """The oracle's order-n Gauss-Legendre rule (abscissae, weights), reference point order."""
+    a, w = np.zeros(8), np.zeros(8)
+    if lib().orc_gauss_rule(int(n), _dp(a), _dp(w)) != 0:
+        raise ValueError(f"no Gauss rule of order {n}")
+    return a[:n].copy(), w[:n].copy()
+
+
 class Oracle:
     """One oracle problem instance (init = InitNode/InitBond/InitMaterial/InitBdryEffectFactor)."""
 
@@ -266,6 +280,29 @@ class Oracle:
         U = None if U is None else np.ascontiguousarray(U, dtype=np.float64).reshape(-1)
         V = None if V is None else np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
         self.L.orc_set_state(self.h, _dp(U) if U is not None else None, _dp(V) if V is not None else None)
+
+    def gauss(self):
+        """Gauss-point positions and displacements, each (N, n^3, 3); None for CCI."""
+        ng = int(self.L.orc_num_gauss(self.h))
+        if ng == 0:
+            return None
+        Xg, Ug = np.zeros(3 * ng * self.n), np.zeros(3 * ng * self.n)
+        self.L.orc_get_gauss(self.h, _dp(Xg), _dp(Ug))
+        return Xg.reshape(self.n, ng, 3), Ug.reshape(self.n, ng, 3)
+
+    def mls(self):
+        """MLS contributing nodes and coefficients: (cb_rp, cb, phi), node i's coefficients
+        phi[ng*cb_rp[i] : ng*cb_rp[i+1]] as [point][contributing node]; None for n = 1."""
+        m = int(self.L.orc_num_contrib(self.h))
+        if m == 0:
+            return None
+        ng = int(self.L.orc_num_gauss(self.h))
+        rp = np.zeros(self.n + 1, dtype=np.int64)
+        cb = np.zeros(m, dtype=np.int32)
+        phi = np.zeros(ng * m)
+        self.L.orc_get_mls(self.h, rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                           cb.ctypes.data_as(C.POINTER(C.c_int32)), _dp(phi))
+        return rp, cb, phi
 
     def static_system(self):
         m, uc, b = np.zeros(3 * self.n), np.zeros(3 * self.n), np.zeros(3 * self.n)

@@ -28,6 +28,7 @@ ERRORS = {
     -1: "PDM_E_ARG", -2: "PDM_E_IO", -3: "PDM_E_PARSE", -4: "PDM_E_MODEL", -5: "PDM_E_HIP",
     -6: "PDM_E_STATE", -7: "PDM_E_LIMIT", -8: "PDM_E_CONVERGE",
 }
+globals().update({name: code for code, name in ERRORS.items()})
 
 
 class PdmError(RuntimeError):
@@ -87,6 +88,8 @@ def lib() -> C.CDLL:
     L.pdm_last_kernel_timing.argtypes = [vp, dp, i64p]
     ip = C.POINTER(C.c_int)
     L.pdm_get_tuning.argtypes = [vp, ip, ip, ip, dp]
+    L.pdm_num_gauss_points.argtypes = [vp]
+    L.pdm_get_gauss.argtypes = [vp, dp, dp]
     cpp = C.POINTER(C.c_char_p)
     L.pdm_batch_create.argtypes = [cpp, C.c_int, C.POINTER(Options), C.POINTER(vp)]
     L.pdm_batch_create_from_text.argtypes = [cpp, C.c_int, C.POINTER(Options), C.POINTER(vp)]
@@ -196,6 +199,16 @@ class Problem:
         el = np.zeros(self.n)
         self._chk(self.L.pdm_compute_forces(self.h, _dp(u), _dp(F), _dp(el)))
         return (F.reshape(-1, 3), el) if with_elast else F.reshape(-1, 3)
+
+    def gauss(self):
+        """Gauss-point positions and displacements, each (N, n^3, 3); None for a CCI deck."""
+        ng = int(self.L.pdm_num_gauss_points(self.h))
+        if ng <= 0:
+            return None
+        Xg = np.zeros(3 * ng * self.n)
+        Ug = np.zeros(3 * ng * self.n)
+        self._chk(self.L.pdm_get_gauss(self.h, _dp(Xg), _dp(Ug)))
+        return Xg.reshape(self.n, ng, 3), Ug.reshape(self.n, ng, 3)
 
     def tangent(self, u: np.ndarray) -> np.ndarray:
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)

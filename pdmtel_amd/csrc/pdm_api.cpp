@@ -65,11 +65,28 @@ int create_common(const std::string& text, const pdm_options* opt, pdm_handle** 
         return rc;
     }
     const Deck& d = h->deck;
-    if (!d.cci_on) {
-        g_create_err = d.fgi_on ? "fixed Gaussian integration (fgi_on) is not supported by this build"
-                                : "adaptive integration (ai_on) is not supported by this build";
+    if (!d.cci_on && !d.fgi_on) {
+        g_create_err = "adaptive integration (ai_on) is not supported by this build";
         delete h;
         return PDM_E_MODEL;
+    }
+    if (!d.cci_on) {
+        const int n = d.intg_point;
+        if (n < 1 || n == 7 || n > 8) {
+            g_create_err = "intg_point must be 1..6 or 8 (the reference's Gauss rules)";
+            delete h;
+            return PDM_E_PARSE;
+        }
+        if (n > 1 && !(d.rw > 0.0)) {
+            g_create_err = "support_domain_size must be positive";
+            delete h;
+            return PDM_E_PARSE;
+        }
+        if (h->opt.precision != PDM_FP64) {
+            g_create_err = "fixed Gaussian integration runs in PDM_FP64 only";
+            delete h;
+            return PDM_E_MODEL;
+        }
     }
     if (d.mats.size() > (size_t)kMaxMat || d.bcs.size() > (size_t)kMaxBC || d.slits.size() > (size_t)kMaxSlit) {
         g_create_err = "too many materials / boundary conditions / slits";
@@ -175,6 +192,12 @@ int upload_state(pdm_handle* h, const double* U, const double* V)
         to_internal(h, U, buf);
         HCHK(hipMemcpy(h->U, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
         HCHK(hipMemcpy(h->u[0], buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
+        if (h->fgi_ng) {  // the non-BC nodes' Gauss points follow U, as at the end of a step
+            FgiArgs f = fgi_args(h);
+            f.mode = 1;
+            HCHK(launch_fgi_gauss(f, h->u[0], 0, h->stream));
+            HCHK(hipStreamSynchronize(h->stream));
+        }
     }
     if (V) {
         to_internal(h, V, buf);
@@ -205,11 +228,20 @@ int ensure_chunk(pdm_handle* h, int cap)
 int enqueue_steps(pdm_handle* h, int count)
 {
     StageArgs a = stage_args(h);
+    const bool fgi = h->fgi_ng > 0;
+    const FgiArgs f = fgi ? fgi_args(h) : FgiArgs{};
     for (int k = 0; k < count; ++k) {
         for (int st = 0; st < 4; ++st) {
             a.uin = h->u[st & 1];
             a.uout = h->u[(st + 1) & 1];
-            HCHK(launch_stage(st, h->opt.precision, a, k, h->stream));
+            if (!fgi) {
+                HCHK(launch_stage(st, h->opt.precision, a, k, h->stream));
+                continue;
+            }
+            // Gauss points follow each stage's new displacements: stages 2-4 read them, and
+            // the next step's stage 1 reads those of the updated U (pdDataManager.cpp:2043-2125)
+            HCHK(launch_fgi_stage(st, a, f, k, h->stream));
+            HCHK(launch_fgi_gauss(f, st == 3 ? a.uout : f.trial, k, h->stream));
         }
     }
     // per-step KE / EE totals, one launch for the chunk
@@ -313,6 +345,7 @@ int zero_state(pdm_handle* h)
     for (d4* p : {h->U, h->V, h->Vb, h->k1, h->k23, h->u[0], h->u[1]})
         HCHK(hipMemsetAsync(p, 0, sizeof(d4) * n, h->stream));
     HCHK(hipMemsetAsync(h->elast, 0, sizeof(double) * n, h->stream));
+    if (h->gauss) HCHK(hipMemsetAsync(h->gauss, 0, sizeof(d4) * n * h->fgi_ng, h->stream));
     HCHK(hipStreamSynchronize(h->stream));
     return PDM_OK;
 }
@@ -722,6 +755,7 @@ int pdm_destroy(pdm_handle* h)
     dfree(h->U); dfree(h->V); dfree(h->Vb); dfree(h->k1); dfree(h->k23);
     dfree(h->elast); dfree(h->part); dfree(h->energy); dfree(h->steps);
     dfree(h->specs_d); dfree(h->mats_d); dfree(h->bcs_d); dfree(h->tile_spec_d); dfree(h->spec_tile_ptr_d);
+    free_fgi(h);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream && !h->shared_stream) (void)hipStreamDestroy(h->stream);
@@ -787,12 +821,17 @@ int pdm_init(pdm_handle* h)
     if (rc) return rc;
     rc = alloc_state(h);
     if (rc) return rc;
+    const bool fgi = !h->deck.cci_on;
     const bool tune = !h->opt.tile_edge && !h->opt.waves_per_tile && !h->opt.families_per_wave &&
-                      h->E >= kTuneMinEntries;
+                      h->E >= kTuneMinEntries && !fgi;
     rc = tune ? autotune(h) : build_tiles(h);
     if (rc) return rc;
     rc = upload_spec(h);
     if (rc) return rc;
+    if (fgi) {
+        rc = build_fgi(h);
+        if (rc) return rc;
+    }
     rc = alloc_parts(h);
     if (rc) return rc;
     rc = zero_state(h);
@@ -868,7 +907,21 @@ int pdm_compute_forces(pdm_handle* h, const double* u, double* F, double* elast)
     HCHK(hipMalloc(&Fd, sizeof(d4) * n));
     HCHK(hipMalloc(&ed, sizeof(double) * n));
     HCHK(hipMemcpy(ud, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
-    HCHK(launch_forces(h->opt.precision, tile_view(h), ud, Fd, ed, h->stream));
+    if (h->fgi_ng) {
+        // every node's Gauss points interpolated from u, into scratch (the stepping state keeps its own)
+        d4* gd = nullptr;
+        HCHK(hipMalloc(&gd, sizeof(d4) * n * h->fgi_ng));
+        FgiArgs f = fgi_args(h);
+        f.mode = 2;
+        f.gout = gd;
+        f.gin = gd;
+        HCHK(launch_fgi_gauss(f, ud, 0, h->stream));
+        HCHK(launch_fgi_forces(stage_args(h), f, ud, Fd, ed, h->stream));
+        HCHK(hipStreamSynchronize(h->stream));
+        HCHK(hipFree(gd));
+    } else {
+        HCHK(launch_forces(h->opt.precision, tile_view(h), ud, Fd, ed, h->stream));
+    }
     HCHK(hipStreamSynchronize(h->stream));
     HCHK(hipMemcpy(buf.data(), Fd, sizeof(d4) * n, hipMemcpyDeviceToHost));
     from_internal(h, buf, F);
@@ -883,11 +936,46 @@ int pdm_compute_forces(pdm_handle* h, const double* u, double* F, double* elast)
     return PDM_OK;
 }
 
+int pdm_num_gauss_points(const pdm_handle* h) { return h ? h->fgi_ng : -1; }
+
+int pdm_get_gauss(const pdm_handle* h, double* Xg, double* Ug)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    const int ng = h->fgi_ng;
+    if (ng == 0) return fail(h, PDM_E_MODEL, "the deck has no Gauss points (CCI)");
+    const int64_t n = h->n;
+    if (Xg) {
+        std::vector<double> tab((size_t)kFgiTab * ng);
+        HCHK(hipMemcpy(tab.data(), h->fgi_tab, sizeof(double) * tab.size(), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i)
+            for (int g = 0; g < ng; ++g)
+                for (int c = 0; c < 3; ++c) Xg[(i * ng + g) * 3 + c] = tab[(size_t)kFgiTab * g + c] + h->X[3 * i + c];
+    }
+    if (Ug) {
+        std::vector<d4> G((size_t)n * ng);
+        HCHK(hipStreamSynchronize(h->stream));
+        HCHK(hipMemcpy(G.data(), h->gauss, sizeof(d4) * G.size(), hipMemcpyDeviceToHost));
+        for (int64_t a = 0; a < n; ++a) {
+            const int64_t i = h->perm[a];
+            for (int g = 0; g < ng; ++g) {
+                const d4& v = G[(size_t)a * ng + g];
+                double* o = &Ug[(i * ng + g) * 3];
+                o[0] = v.x;
+                o[1] = v.y;
+                o[2] = v.z;
+            }
+        }
+    }
+    return PDM_OK;
+}
+
 int pdm_apply_tangent(pdm_handle* h, const double* u, double* y)
 {
     int rc = require_init(h);
     if (rc) return rc;
     if (!u || !y) return fail(h, PDM_E_ARG, "null u or y");
+    if (h->fgi_ng) return fail(h, PDM_E_MODEL, "the tangent operator is built for CCI decks only");
     int64_t n = h->n;
     if (n == 0) return PDM_OK;
     std::vector<d4> buf;
@@ -1080,6 +1168,7 @@ int pdm_solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_stat
 {
     int rc = require_init(h);
     if (rc) return rc;
+    if (h->fgi_ng) return fail(h, PDM_E_MODEL, "the static solve is built for CCI decks only");
     HCHK(hipSetDevice(h->dev));
     return solve_static(h, rel_tol, max_iterations, stats);
 }
@@ -1124,6 +1213,13 @@ int batch_create(const std::vector<std::string>& texts, const pdm_options* opt, 
         }
         m->shared_stream = true;
         B->members.push_back(m);
+        if (!m->deck.cci_on) {
+            g_create_err = "specimen " + std::to_string(k) + ": batches hold CCI decks only";
+            for (pdm_handle* q : B->members) pdm_destroy(q);
+            B->members.clear();
+            pdm_destroy(B);
+            return PDM_E_MODEL;
+        }
     }
     B->nspec = (int)texts.size();
     pdm_batch* b = new pdm_batch();

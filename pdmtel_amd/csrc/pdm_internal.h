@@ -130,6 +130,31 @@ struct StageArgs {
     int energies;
 };
 
+// Fixed Gaussian integration (fgi_on): ng = n^3 Gauss points per node, internal order.
+// tab[8 g ..]: the point's offset (0.5 dx a) from its node in x, y, z; its force weight
+// dx^3/8 w1 w2 w3; its beta cell size (w1 w2 w3)^(1/3) dx / 2; its energy factor
+// ((w1 w2 w3)^(1/3) / 2)^3.
+constexpr int kFgiTab = 8;
+
+struct FgiArgs {
+    const double* tab;      // [ng][kFgiTab]
+    int ng;
+    int mode;               // k_fgi_gauss: 0 as the reference, 1 non-BC nodes only, 2 every node
+    double horizon;
+    const d4* gin;          // Gauss-point displacements [N][ng] read by the family sums
+    d4* gout;               // written by k_fgi_gauss
+    const int64_t* cb_ptr;  // MLS contributing nodes (n > 1): [N + 1]
+    const int32_t* cb;
+    const double* phi;      // coefficient k of point g of node i at (cb_ptr[i] + k) * ng + g
+    const d4* X;
+    const int32_t* bc;
+    const BCDev* bcs;
+    const StepParam* steps; // the step's time (mode 0)
+    int64_t n;
+    d4* trial;              // RK4 trial displacement of every node (stages 1-3 outputs)
+    d4* u0;                 // U at the start of the step (BC nodes)
+};
+
 // Bond-build parameters.
 struct BondArgs {
     const d4* X;
@@ -222,6 +247,16 @@ struct pdm_handle {
     double last_kernel_ms = 0;
     int64_t last_kernel_launches = 0;
 
+    // fixed Gaussian integration (fgi_on)
+    int fgi_n = 0, fgi_ng = 0;
+    double* fgi_tab = nullptr;
+    pdm::d4* gauss = nullptr;          // [N][ng] Gauss-point displacements
+    pdm::d4* fgi_trial = nullptr;      // [N] RK4 trial displacements the Gauss points interpolate
+    pdm::d4* fgi_u0 = nullptr;         // [N] U at the start of the step
+    int64_t* cb_ptr = nullptr;         // MLS contributing nodes (n > 1)
+    int32_t* cb = nullptr;
+    double* phi = nullptr;
+
     // graph cache
     hipGraphExec_t graph_exec = nullptr;
     int graph_steps = 0;
@@ -256,6 +291,14 @@ hipError_t launch_energy_steps(const double* part, int64_t ntl, const int64_t* s
 hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, double* elast, hipStream_t s);
 hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s);
 hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s);
+
+// fixed Gaussian integration (pdm_fgi.hip, kernels in pdm_kernels.hip)
+int build_fgi(pdm_handle* h);
+void free_fgi(pdm_handle* h);
+FgiArgs fgi_args(const pdm_handle* h);
+hipError_t launch_fgi_gauss(const FgiArgs& f, const d4* u, int step_idx, hipStream_t s);
+hipError_t launch_fgi_stage(int stage, const StageArgs& a, const FgiArgs& f, int step_idx, hipStream_t s);
+hipError_t launch_fgi_forces(const StageArgs& a, const FgiArgs& f, const d4* u, d4* F, double* elast, hipStream_t s);
 
 // static solve (pdm_cg.cpp / pdm_cg.hip)
 int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_stats* stats);

@@ -32,6 +32,7 @@
 #include <type_traits>
 #include <utility>
 
+#include "pdm_geom.h"
 #include "pdm_internal.h"
 
 namespace pdm {
@@ -924,54 +925,31 @@ struct EpiLoad {
     }
 };
 
-// One RK4 stage (STAGE 0..3) for one tile.  Reads a.uin, writes a.uout.
-// At most 128 VGPRs (4 waves per SIMD): two 8-wave tiles per CU.
-template <int STAGE, int PREC, bool ENERGY, bool UE, int NW, int NPW>
-__global__ __launch_bounds__(64 * NW, 4) void k_stage(StageArgs a, int step_idx)
+// The specimen's material and BC tables into LDS.
+__device__ __forceinline__ void load_tables(const StageArgs& a, int spec, const Lds& l)
 {
-    extern __shared__ double L[];
-    __shared__ double sh_e[2][NW];
-    const TileView& tv = a.tv;
-    const int64_t k = launch_index(tv.xcd_remap);
-    if (k >= tv.ntl) return;
-    const int32_t tid = tv.tiles[k];
-    const int64_t a0 = tv.tile_start[tid];
-    const int nt = (int)(tv.tile_start[tid + 1] - a0);
-    const int64_t h0 = tv.halo_ptr[k];
-    const int H = (int)(tv.halo_ptr[k + 1] - h0);
-    const int spec = tile_specimen(tv, k);
-    const SpecDev sp = tv.specs[spec];
-    constexpr bool F32 = kF32Rec<typename OpFor<PREC>::Force>;
-    const Lds l = lds_view(L, H, tv.tile_max, !UE, true, F32);
-    // the stage-update inputs (first node of each thread) are loaded together with the halo
-    EpiLoad<STAGE, ENERGY> pre;
-    const bool has_pre = (int)threadIdx.x < nt;
-    if (has_pre) pre.load(a, a0 + threadIdx.x);
-    stage_tile<F32>(tv, sp, a.uin, k, h0, H, a0, nt, l, !UE);
-    {
-        // the specimen's material and BC tables
-        static_assert(sizeof(MatDev) % 8 == 0 && sizeof(BCDev) % 8 == 0, "tables are double words");
-        constexpr int nm = (int)(sizeof(MatDev) * kMaxMat / sizeof(double));
-        constexpr int nb = (int)(sizeof(BCDev) * kMaxBC / sizeof(double));
-        const double* ms = reinterpret_cast<const double*>(a.mats + (int64_t)spec * kMaxMat);
-        const double* bs = reinterpret_cast<const double*>(a.bcs + (int64_t)spec * kMaxBC);
-        double* md = reinterpret_cast<double*>(l.mats);
-        double* bd = reinterpret_cast<double*>(l.bcs);
-        for (int q = threadIdx.x; q < nm + nb; q += blockDim.x) {
-            if (q < nm) md[q] = ms[q];
-            else bd[q - nm] = bs[q - nm];
-        }
+    static_assert(sizeof(MatDev) % 8 == 0 && sizeof(BCDev) % 8 == 0, "tables are double words");
+    constexpr int nm = (int)(sizeof(MatDev) * kMaxMat / sizeof(double));
+    constexpr int nb = (int)(sizeof(BCDev) * kMaxBC / sizeof(double));
+    const double* ms = reinterpret_cast<const double*>(a.mats + (int64_t)spec * kMaxMat);
+    const double* bs = reinterpret_cast<const double*>(a.bcs + (int64_t)spec * kMaxBC);
+    double* md = reinterpret_cast<double*>(l.mats);
+    double* bd = reinterpret_cast<double*>(l.bcs);
+    for (int q = threadIdx.x; q < nm + nb; q += blockDim.x) {
+        if (q < nm) md[q] = ms[q];
+        else bd[q - nm] = bs[q - nm];
     }
-    if (has_pre) pre.store(l.epi[threadIdx.x]);
-    for (int loc = threadIdx.x + 64 * NW; loc < nt; loc += 64 * NW) {
-        EpiLoad<STAGE, ENERGY> more;
-        more.load(a, a0 + loc);
-        more.store(l.epi[loc]);
-    }
-    __syncthreads();
-    tile_families<typename OpFor<PREC>::Force, ENERGY, UE, NW, NPW>(tv, sp, l, nt);
-    __syncthreads();
+}
 
+// The stage update of a tile's nodes from their family sums in l.res (fx, fy, fz, energy
+// density): velocity buffer, BC overwrite, k, the next stage's displacements and, at stage 4,
+// the RK4 update and the tile's KE / EE partials (CalcBondBasedMotionRK4's tail and RK4,
+// pdDataManager.cpp:1958-1990, 2058-2110).
+template <int STAGE, bool ENERGY, int NW>
+__device__ __forceinline__ void tile_update(const StageArgs& a, const SpecDev& sp, const Lds& l, int64_t k, int64_t a0,
+                                            int nt, int step_idx, int spec, double (*sh_e)[NW])
+{
+    const TileView& tv = a.tv;
     const StepParam stp = a.steps[(int64_t)step_idx * a.nspec + spec];
     const double dt = stp.dt;
     const double dts = (STAGE < 2) ? 0.5 * dt : dt;
@@ -1077,6 +1055,43 @@ __global__ __launch_bounds__(64 * NW, 4) void k_stage(StageArgs a, int step_idx)
     }
 }
 
+// One RK4 stage (STAGE 0..3) for one tile.  Reads a.uin, writes a.uout.
+// At most 128 VGPRs (4 waves per SIMD): two 8-wave tiles per CU.
+template <int STAGE, int PREC, bool ENERGY, bool UE, int NW, int NPW>
+__global__ __launch_bounds__(64 * NW, 4) void k_stage(StageArgs a, int step_idx)
+{
+    extern __shared__ double L[];
+    __shared__ double sh_e[2][NW];
+    const TileView& tv = a.tv;
+    const int64_t k = launch_index(tv.xcd_remap);
+    if (k >= tv.ntl) return;
+    const int32_t tid = tv.tiles[k];
+    const int64_t a0 = tv.tile_start[tid];
+    const int nt = (int)(tv.tile_start[tid + 1] - a0);
+    const int64_t h0 = tv.halo_ptr[k];
+    const int H = (int)(tv.halo_ptr[k + 1] - h0);
+    const int spec = tile_specimen(tv, k);
+    const SpecDev sp = tv.specs[spec];
+    constexpr bool F32 = kF32Rec<typename OpFor<PREC>::Force>;
+    const Lds l = lds_view(L, H, tv.tile_max, !UE, true, F32);
+    // the stage-update inputs (first node of each thread) are loaded together with the halo
+    EpiLoad<STAGE, ENERGY> pre;
+    const bool has_pre = (int)threadIdx.x < nt;
+    if (has_pre) pre.load(a, a0 + threadIdx.x);
+    stage_tile<F32>(tv, sp, a.uin, k, h0, H, a0, nt, l, !UE);
+    load_tables(a, spec, l);
+    if (has_pre) pre.store(l.epi[threadIdx.x]);
+    for (int loc = threadIdx.x + 64 * NW; loc < nt; loc += 64 * NW) {
+        EpiLoad<STAGE, ENERGY> more;
+        more.load(a, a0 + loc);
+        more.store(l.epi[loc]);
+    }
+    __syncthreads();
+    tile_families<typename OpFor<PREC>::Force, ENERGY, UE, NW, NPW>(tv, sp, l, nt);
+    __syncthreads();
+    tile_update<STAGE, ENERGY, NW>(a, sp, l, k, a0, nt, step_idx, spec, sh_e);
+}
+
 // Per-step, per-specimen energy totals: block b = step * nspec + spec sums the tile
 // partials (ke, ee) of that step over the specimen's tiles into out[2b], out[2b+1], in a
 // fixed order.
@@ -1133,6 +1148,196 @@ __global__ __launch_bounds__(64 * NW) void k_gather(TileView tv, const d4* __res
         const double* res = l.res + 4 * loc;
         y[a0 + loc] = d4{res[0], res[1], res[2], 0.0};
         if (elast) elast[a0 + loc] = res[3];
+    }
+}
+
+// ------------------------------------------------------------ fixed Gaussian integration
+// The fixed-Gaussian branch of CalcBondBasedMotionRK4's family loop (pdDataManager.cpp:
+// 1918-1957): for each unbroken entry (i, j), the CCI bond force (pdDataManager2.cpp:
+// 1293-1351, volume outside) at each Gauss point g of node j, with xi = X_g - X_i, eta =
+// u_g - u_i, beta over the cell size (w1 w2 w3)^(1/3) dx / 2, summed with the weights
+// dx^3/8 w1 w2 w3.  coef64 holds bdryEff_ij (0: cut entry or row padding).  One wavefront per
+// node, lanes over the row's entries; returns (fx, fy, fz, energy density) in every lane.
+__device__ __forceinline__ void fgi_family(const TileView& tv, const FgiArgs& f, const double* __restrict__ tab,
+                                           const MatDev* mats, const int32_t* __restrict__ mat, int64_t h0, int64_t i,
+                                           const d4& ui, double acc[4])
+{
+    const int lane = threadIdx.x & 63;
+    const d4 xi = tv.X[i];
+    const MatDev& mi = mats[mat[i]];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0, el = 0.0;
+    for (int64_t e = tv.rpi[i] + lane; e < tv.rpi[i + 1]; e += 64) {
+        const double bij = tv.coef64[e];
+        if (bij == 0.0) continue;
+        const int32_t j = tv.halo[h0 + tv.slot[e]];
+        const d4 xj = tv.X[j];
+        const MatDev& mj = mats[mat[j]];
+        const double ecrit = (mj.ecrit < mi.ecrit) ? mj.ecrit : mi.ecrit;
+        const double spring = (mj.spring < mi.spring) ? mj.spring : mi.spring;
+        const d4* __restrict__ gj = f.gin + (int64_t)j * f.ng;
+        for (int g = 0; g < f.ng; ++g) {
+            const double* t = tab + kFgiTab * g;
+            const d4 ug = gj[g];
+            const double xi1 = (t[0] + xj.x) - xi.x, xi2 = (t[1] + xj.y) - xi.y, xi3 = (t[2] + xj.z) - xi.z;
+            const double a1 = xi1 + (ug.x - ui.x), a2 = xi2 + (ug.y - ui.y), a3 = xi3 + (ug.z - ui.z);
+            const double r = sqrt(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
+            const double p = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+            const double beta = g_beta(r, f.horizon, t[4], false);
+            const double u = p - r;
+            const double st = u / r;
+            if (u <= 0.0 || (u > 0.0 && st <= ecrit)) {
+                const double c = bij * beta * spring * st;
+                f0 += t[3] * ((c * a1) / p);
+                f1 += t[3] * ((c * a2) / p);
+                f2 += t[3] * ((c * a3) / p);
+                el += bij * 0.5 * (0.5 * beta * (spring / r) * u * u) * t[5];
+            }
+        }
+    }
+    acc[0] = wave_sum(f0);
+    acc[1] = wave_sum(f1);
+    acc[2] = wave_sum(f2);
+    acc[3] = wave_sum(el);
+}
+
+constexpr int kFgiWaves = 4;
+
+// Gauss-point displacements, one thread per (node, point): CalcIntgPointDisplRK4
+// (pdDataManager.cpp:2402-2495).  A BC node's points take the prescribed DISPL / DISPLGRAD
+// components at the step's time (UpdateIntgPointStatusWithBC, :2604-2703) and keep the others;
+// the other nodes' points interpolate u with the MLS coefficients (n > 1) or copy it (n = 1).
+// f.mode 1 updates the non-BC nodes only, mode 2 interpolates every node.
+__global__ __launch_bounds__(256) void k_fgi_gauss(FgiArgs f, const d4* __restrict__ u, int step_idx)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= f.n * f.ng) return;
+    const int64_t a = q / f.ng;
+    const int g = (int)(q - a * f.ng);
+    const int b = f.bc[a];
+    if (f.mode != 2 && b >= 0 && f.bcs[b].bcnode) {
+        if (f.mode == 1) return;
+        const BCDev& bd = f.bcs[b];
+        const double fac = time_factor_dev(bd, f.steps[step_idx].t);
+        const double* t = f.tab + kFgiTab * g;
+        const d4 x = f.X[a];
+        const double xg[3] = {t[0] + x.x, t[1] + x.y, t[2] + x.z};
+        d4 G = f.gout[q];
+        double v[3] = {G.x, G.y, G.z};
+        for (int c = 0; c < 3; ++c) {
+            if (bd.f[c] != 1) continue;
+            if (bd.type == 1) v[c] = bd.v[c] * fac;
+            else if (bd.type == 3) v[c] = (bd.v[3 * c] * xg[0] + bd.v[3 * c + 1] * xg[1] + bd.v[3 * c + 2] * xg[2]) * fac;
+        }
+        f.gout[q] = d4{v[0], v[1], v[2], 0.0};
+        return;
+    }
+    if (!f.phi) {
+        const d4 x = u[a];
+        f.gout[q] = d4{x.x, x.y, x.z, 0.0};
+        return;
+    }
+    const int64_t k0 = f.cb_ptr[a], k1 = f.cb_ptr[a + 1];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int64_t k = k0; k < k1; ++k) {
+        const double c = f.phi[k * f.ng + g];
+        const d4 x = u[f.cb[k]];
+        s0 += x.x * c;
+        s1 += x.y * c;
+        s2 += x.z * c;
+    }
+    f.gout[q] = d4{s0, s1, s2, 0.0};
+}
+
+// LDS of the FGI kernels: the stage layout without halo records, then the point table.
+__host__ __device__ inline size_t fgi_lds_bytes(int tile_max, int ng)
+{
+    return lds_bytes(0, tile_max, false, true, false) + sizeof(double) * kFgiTab * (size_t)ng;
+}
+
+// One RK4 stage of a fixed-Gaussian specimen for one tile: family sums over the neighbours'
+// Gauss points (a.uin: this stage's displacement of the family owner, f.gin: the Gauss
+// points'), then the same stage update as k_stage.
+template <int STAGE, bool ENERGY>
+__global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_stage(StageArgs a, FgiArgs f, int step_idx)
+{
+    extern __shared__ double L[];
+    __shared__ double sh_e[2][kFgiWaves];
+    const TileView& tv = a.tv;
+    const int64_t k = blockIdx.x;
+    const int32_t tid = tv.tiles[k];
+    const int64_t a0 = tv.tile_start[tid];
+    const int nt = (int)(tv.tile_start[tid + 1] - a0);
+    const int64_t h0 = tv.halo_ptr[k];
+    const SpecDev sp = tv.specs[0];
+    const Lds l = lds_view(L, 0, tv.tile_max, false, true, false);
+    double* tab = reinterpret_cast<double*>(reinterpret_cast<char*>(L) + lds_bytes(0, tv.tile_max, false, true, false));
+    for (int q = threadIdx.x; q < kFgiTab * f.ng; q += blockDim.x) tab[q] = f.tab[q];
+    load_tables(a, 0, l);
+    for (int loc = threadIdx.x; loc < nt; loc += blockDim.x) {
+        EpiLoad<STAGE, ENERGY> in;
+        in.load(a, a0 + loc);
+        in.store(l.epi[loc]);
+    }
+    __syncthreads();
+    for (int loc = threadIdx.x >> 6; loc < nt; loc += kFgiWaves) {
+        const int64_t i = a0 + loc;
+        double acc[4];
+        fgi_family(tv, f, tab, l.mats, a.mat, h0, i, a.uin[i], acc);
+        if ((threadIdx.x & 63) == 0)
+            for (int c = 0; c < 4; ++c) l.res[4 * loc + c] = acc[c];
+    }
+    __syncthreads();
+    tile_update<STAGE, ENERGY, kFgiWaves>(a, sp, l, k, a0, nt, step_idx, 0, sh_e);
+    if (STAGE == 3) return;  // the Gauss points then follow the new U (a.uout)
+    // The Gauss points interpolate the RK4 trial displacement displStep = u1 + c k of every
+    // node (pdDataManager.cpp:2040-2056), where the force loop reads a BC node's own U
+    // (a.uout): a BC node's trial is its step-start U plus c k.
+    __syncthreads();
+    const double dts = (STAGE < 2) ? 0.5 * a.steps[step_idx].dt : a.steps[step_idx].dt;
+    const double c = (STAGE < 2) ? 0.5 : 1.0;
+    for (int loc = threadIdx.x; loc < nt; loc += blockDim.x) {
+        const int64_t i = a0 + loc;
+        const EpiRec& ep = l.epi[loc];
+        if (ep.b < 0 || !l.bcs[ep.b].bcnode) {
+            f.trial[i] = a.uout[i];
+            continue;
+        }
+        d4 u1;
+        if (STAGE == 0) {
+            u1 = d4{ep.v[1][0], ep.v[1][1], ep.v[1][2], 0.0};
+            f.u0[i] = u1;
+        } else {
+            u1 = f.u0[i];
+        }
+        const d4 vb = a.Vb[i];
+        f.trial[i] = d4{u1.x + c * (vb.x * dts), u1.y + c * (vb.y * dts), u1.z + c * (vb.z * dts), 0.0};
+    }
+}
+
+// Fixed-Gaussian family sums of every node for the displacement u (f.gin interpolated from u).
+__global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_forces(StageArgs a, FgiArgs f, const d4* __restrict__ u,
+                                                               d4* __restrict__ F, double* __restrict__ elast)
+{
+    extern __shared__ double L[];
+    const TileView& tv = a.tv;
+    const int64_t k = blockIdx.x;
+    const int32_t tid = tv.tiles[k];
+    const int64_t a0 = tv.tile_start[tid];
+    const int nt = (int)(tv.tile_start[tid + 1] - a0);
+    const int64_t h0 = tv.halo_ptr[k];
+    const Lds l = lds_view(L, 0, tv.tile_max, false, true, false);
+    double* tab = reinterpret_cast<double*>(reinterpret_cast<char*>(L) + lds_bytes(0, tv.tile_max, false, true, false));
+    for (int q = threadIdx.x; q < kFgiTab * f.ng; q += blockDim.x) tab[q] = f.tab[q];
+    load_tables(a, 0, l);
+    __syncthreads();
+    for (int loc = threadIdx.x >> 6; loc < nt; loc += kFgiWaves) {
+        const int64_t i = a0 + loc;
+        double acc[4];
+        fgi_family(tv, f, tab, l.mats, a.mat, h0, i, u[i], acc);
+        if ((threadIdx.x & 63) == 0) {
+            F[i] = d4{acc[0], acc[1], acc[2], 0.0};
+            if (elast) elast[i] = acc[3];
+        }
     }
 }
 
@@ -1249,6 +1454,51 @@ hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, 
     case PDM_FP32C: return launch_gather_u<OpF32>(tv, u, F, elast, lds, s);
     default: return launch_gather_u<OpF32TF>(tv, u, F, elast, lds, s);
     }
+}
+
+template <int STAGE>
+hipError_t launch_fgi_stage_t(const StageArgs& a, const FgiArgs& f, int step_idx, size_t lds, hipStream_t s)
+{
+    if (a.energies) {
+        hipError_t e = set_lds<k_fgi_stage<STAGE, true>>(lds);
+        if (e != hipSuccess) return e;
+        k_fgi_stage<STAGE, true><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, step_idx);
+    } else {
+        hipError_t e = set_lds<k_fgi_stage<STAGE, false>>(lds);
+        if (e != hipSuccess) return e;
+        k_fgi_stage<STAGE, false><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, step_idx);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fgi_stage(int stage, const StageArgs& a, const FgiArgs& f, int step_idx, hipStream_t s)
+{
+    if (a.tv.ntl == 0) return hipSuccess;
+    const size_t lds = fgi_lds_bytes(a.tv.tile_max, f.ng);
+    switch (stage) {
+    case 0: return launch_fgi_stage_t<0>(a, f, step_idx, lds, s);
+    case 1: return launch_fgi_stage_t<1>(a, f, step_idx, lds, s);
+    case 2: return launch_fgi_stage_t<2>(a, f, step_idx, lds, s);
+    default: return launch_fgi_stage_t<3>(a, f, step_idx, lds, s);
+    }
+}
+
+hipError_t launch_fgi_gauss(const FgiArgs& f, const d4* u, int step_idx, hipStream_t s)
+{
+    const int64_t m = f.n * f.ng;
+    if (m == 0) return hipSuccess;
+    k_fgi_gauss<<<dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s>>>(f, u, step_idx);
+    return hipGetLastError();
+}
+
+hipError_t launch_fgi_forces(const StageArgs& a, const FgiArgs& f, const d4* u, d4* F, double* elast, hipStream_t s)
+{
+    if (a.tv.ntl == 0) return hipSuccess;
+    const size_t lds = fgi_lds_bytes(a.tv.tile_max, f.ng);
+    hipError_t e = set_lds<k_fgi_forces>(lds);
+    if (e != hipSuccess) return e;
+    k_fgi_forces<<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, u, F, elast);
+    return hipGetLastError();
 }
 
 hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s)

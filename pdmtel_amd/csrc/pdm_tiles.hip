@@ -255,11 +255,14 @@ struct CoefArgs {
     int64_t n;
     double horizon, dx, vcube;
     bool original_cci;
+    bool fgi;                // fixed Gaussian integration: coef64 = bdryEff_ij only
     double spring[kMaxMat];
 };
 
 // coef = bdryEff_ij * dx^3 * beta * c_ij: the leading factors of CalcBondForceCCI's force
-// expression (pdDataManager2.cpp:1329-1331) in the same association order.
+// expression (pdDataManager2.cpp:1329-1331) in the same association order.  Under fixed
+// Gaussian integration beta and the weights belong to each Gauss point, so coef64 holds
+// bdryEff_ij alone (0 for a cut bond).
 __global__ void k_coef(CoefArgs a)
 {
     int64_t i = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
@@ -286,7 +289,7 @@ __global__ void k_coef(CoefArgs a)
         double bij = 0.5 * (bi + a.bei[j]);
         double beta = g_beta(r, a.horizon, a.dx, a.original_cci);
         double c = cut ? 0.0 : bij * a.vcube * beta * spring;
-        a.coef64[e] = c;
+        a.coef64[e] = cut ? 0.0 : (a.fgi ? bij : c);
         a.coef32[e] = (float)c;
         cnt += cut ? 0 : 1;
     }
@@ -581,6 +584,7 @@ int build_tiles(pdm_handle* h)
     ca.dx = d.dx;
     ca.vcube = d.dx * d.dx * d.dx;
     ca.original_cci = d.cci_on && !d.cci_mdf;
+    ca.fgi = !d.cci_on && d.fgi_on;
     for (size_t m = 0; m < d.mats.size() && m < (size_t)kMaxMat; ++m) ca.spring[m] = h->spring[m];
     if (n) {
         k_coef<<<dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s>>>(ca);

@@ -91,6 +91,18 @@ def test_defaults_and_extensions():
     assert inf.jitter == 0.01 and inf.jitter_seed == 7 and inf.num_slits == 1 and inf.traction_body == 1
 
 
+def test_fgi_deck_values():
+    spec = pm.bar_spec(4, 3, 3)
+    spec["cci"] = None
+    spec["fgi"] = (1, 2)
+    inf = info(pm.deck_text(spec))
+    assert inf.cci_on == 0 and inf.fgi_on == 1 and inf.intg_point == 2 and inf.mls_on == 1
+    assert math_close(inf.support_domain_size, 3.0 * 1e-3)
+    spec["fgi"] = (1, 1)
+    inf = info(pm.deck_text(spec))
+    assert inf.intg_point == 1 and inf.mls_on == 0
+
+
 def math_close(a, b):
     return abs(a - b) <= 1e-15 * abs(b)
 
@@ -107,6 +119,15 @@ def test_error_codes():
     assert rc == -3 and b"no integration method" in L.pdm_last_error(None)
     rc, _ = create(text.replace("material_1\n 1", "material_1\n 7"))
     assert rc == -3
+    fgi = text.replace("cci_on\n 1 1", "cci_on\n 0 0\nfgi_on\n 1\nintg_point\n 7")
+    rc, _ = create(fgi)
+    assert rc == -3 and b"intg_point" in L.pdm_last_error(None)
+    rc, _ = create(text.replace("cci_on\n 1 1", "cci_on\n 0 0\nfgi_on\n 0\nai_on\n 1 1"))
+    assert rc == -4 and b"adaptive" in L.pdm_last_error(None)
+    o32 = pm.options(pm.PDM_FP32C)
+    h = C.c_void_p()
+    rc = L.pdm_create_from_text(fgi.replace("intg_point\n 7", "intg_point\n 2").encode(), C.byref(o32), C.byref(h))
+    assert rc == -4 and b"FP64" in L.pdm_last_error(None)
     h = C.c_void_p()
     o = pm.options()
     assert L.pdm_create(b"/nonexistent/pds_input.txt", C.byref(o), C.byref(h)) == -2

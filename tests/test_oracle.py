@@ -409,3 +409,177 @@ def test_axis_slit_matches_reference_plane_test():
             crosses = (p1 - box[0]) * (p1 - box[1]) <= 0 and (p3 - box[4]) * (p3 - box[5]) <= 0
         got = L.orc_pair_cut(C.byref(prob), pyoracle._dp(xi), pyoracle._dp(np.ascontiguousarray(xj)))
         assert bool(got) == crosses
+
+
+# ---------------------------------------------------------------- fixed Gaussian integration
+def fgi_spec(spec, n):
+    s = dict(spec)
+    s["cci"] = None
+    s["fgi"] = (1, n)
+    return s
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 8])
+def test_gauss_rules_match_legendre(n):
+    a, w = pyoracle.gauss_rule(n)
+    x, v = np.polynomial.legendre.leggauss(n)
+    order = np.argsort(a)
+    assert np.allclose(a[order], x, rtol=0, atol=2e-15)
+    assert np.allclose(w[order], v, rtol=0, atol=2e-15)
+    # the reference's point order: zero first for odd n, +/- pairs from the centre out
+    # (n = 8: the four positive points, then the four negative ones)
+    if n == 8:
+        assert np.all(np.diff(a[:4]) > 0) and np.array_equal(a[4:], -a[:4])
+    elif n > 1:
+        pos = a[n % 2::2]
+        assert np.all(pos > 0) and np.all(np.diff(pos) > 0) and np.array_equal(a[n % 2 + 1::2], -pos)
+    # exact for x^(2n-1) and x^(2n-2) on [-1, 1]
+    assert abs(np.dot(w, a ** (2 * n - 1))) < 1e-14
+    assert abs(np.dot(w, a ** (2 * n - 2)) - 2.0 / (2 * n - 1)) < 1e-14
+
+
+def test_gauss_rule_orders_without_reference_table():
+    for n in (0, 7, 9):
+        with pytest.raises(ValueError):
+            pyoracle.gauss_rule(n)
+
+
+def test_gauss_point_positions():
+    spec = fgi_spec(small_bar(nx=5, ny=4, nz=4, jitter=0.01, seed=1), 3)
+    o = pyoracle.Oracle(spec)
+    X, _, _ = o.nodes()
+    Xg, Ug = o.gauss()
+    x, _ = np.polynomial.legendre.leggauss(3)
+    want = X[:, None, :] + 0.5 * DX * np.stack(np.meshgrid(x, x, x, indexing="ij"), -1).reshape(-1, 3)[None]
+    for i in range(o.n):
+        got = np.array(sorted(map(tuple, Xg[i])))
+        ref = np.array(sorted(map(tuple, want[i])))
+        assert np.allclose(got, ref, rtol=0, atol=1e-18)
+    assert np.all(Ug == 0.0)
+
+
+def test_fgi_one_point_is_modified_cci():
+    """One Gauss point (weight 2 per axis) is the modified CCI rule: same forces, same steps."""
+    pyoracle.set_square_mode(True)
+    try:
+        base = small_bar(nx=8, ny=6, nz=5, jitter=0.01, seed=2, steps=20)
+        oc, o1 = pyoracle.Oracle(base), pyoracle.Oracle(fgi_spec(base, 1))
+        u = np.random.default_rng(0).uniform(-1e-6, 1e-6, size=(oc.n, 3))
+        Fc, ec = oc.forces(u, with_elast=True)
+        F1, e1 = o1.forces(u, with_elast=True)
+        assert np.abs(F1 - Fc).max() <= 1e-14 * np.abs(Fc).max()
+        assert np.abs(e1 - ec).max() <= 1e-14 * np.abs(ec).max()
+        oc.run(5)
+        o1.run(5)
+        Uc, Vc = oc.state()
+        U1, V1 = o1.state()
+        # the per-bond association (dx^3 / 8 * 8 outside the force) differs in the last bits; this
+        # bar's step load amplifies a 1e-16 perturbation of the state to ~1e-13 within five steps
+        assert np.abs(U1 - Uc).max() <= 1e-11 * np.abs(Uc).max()
+        assert np.abs(V1 - Vc).max() <= 1e-11 * np.abs(Vc).max()
+    finally:
+        pyoracle.set_square_mode(False)
+
+
+def quartic(d):
+    return np.where(d > 1.0, 0.0, 1 - 6 * d ** 2 + 8 * d ** 3 - 3 * d ** 4)
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_mls_reproduces_quadratics_and_matches_lstsq(n):
+    spec = fgi_spec(small_bar(nx=6, ny=5, nz=5, jitter=0.02, seed=3), n)
+    rw = 3.0 * DX
+    o = pyoracle.Oracle(spec)
+    X, _, _ = o.nodes()
+    Xg, _ = o.gauss()
+    rp, cb, phi = o.mls()
+    ng = n ** 3
+    rng = np.random.default_rng(4)
+    worst = 0.0
+    for i in range(o.n):
+        k = cb[rp[i]:rp[i + 1]]
+        assert k[0] == i and len(k) >= 10
+        rows = phi[ng * rp[i]:ng * rp[i + 1]].reshape(ng, len(k))
+        d = (X[k][None, :, :] - Xg[i][:, None, :]) / DX            # (ng, m, 3)
+        worst = max(worst, np.abs(rows.sum(1) - 1).max())
+        for a in range(3):
+            worst = max(worst, np.abs((rows * d[..., a]).sum(1)).max())
+            for b in range(a, 3):
+                worst = max(worst, np.abs((rows * d[..., a] * d[..., b]).sum(1)).max())
+        if i in rng.choice(o.n, 6, replace=False):
+            # weighted least squares through numpy: phi = W P (P^T W P)^-1 p(x_g)
+            for g in range(ng):
+                w = quartic(np.linalg.norm(X[k] - Xg[i, g], axis=1) / rw)
+                q = (X[k] - Xg[i, g]) / DX
+                P = np.column_stack([np.ones(len(k)), q, q[:, 0] * q[:, 1], q[:, 0] * q[:, 2], q[:, 1] * q[:, 2],
+                                     q ** 2])
+                sw = np.sqrt(w)
+                # min || sqrt(W) (P c - e_k) || per unit vector: phi_k = (P (P^T W P)^-1 P^T W)_0k at q = 0
+                coef = np.linalg.lstsq(sw[:, None] * P, np.diag(sw), rcond=None)[0]
+                assert np.allclose(coef[0], rows[g], rtol=0, atol=1e-12)
+    assert worst < 1e-10
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_fgi_patch_test_and_translation(n):
+    """Regular lattice: an affine field leaves a full-family node unloaded (lattice symmetry,
+    exact MLS for affine fields); a rigid translation loads nobody."""
+    spec = fgi_spec(small_bar(nx=9, ny=9, nz=9), n)
+    spec["bdryeff_on"] = 0
+    o = pyoracle.Oracle(spec)
+    X, _, _ = o.nodes()
+    rng = np.random.default_rng(5)
+    u = (X - X.mean(0)) @ (rng.normal(size=(3, 3)) * 1e-4).T
+    F = o.forces(u)
+    scale = np.abs(F).max()
+    rp, _, _ = o.csr()
+    full = np.flatnonzero(np.diff(rp) == o.max_family)
+    assert len(full) > 0 and np.abs(F[full]).max() <= 1e-11 * scale
+    Ft = o.forces(np.tile(rng.normal(size=3) * 1e-6, (o.n, 1)))
+    assert np.abs(Ft).max() <= 1e-11 * scale
+
+
+def test_fgi_converges_in_number_of_points():
+    """Uniform expansion: the family sums of boundary nodes converge as the rule grows, and keep
+    the lattice's mirror symmetry."""
+    base = small_bar(nx=10, ny=9, nz=9)
+    base["bdryeff_on"] = 0
+    X = None
+    res = {}
+    for n in (2, 3, 4, 5, 6, 8):
+        o = pyoracle.Oracle(fgi_spec(base, n))
+        if X is None:
+            X, _, _ = o.nodes()
+        res[n] = o.forces(1e-4 * (X - X.mean(0)))
+    corner = 0
+    face = int(np.argmin(np.abs(X[:, 0] - X[:, 0].min()) + np.linalg.norm(X[:, 1:] - X[:, 1:].mean(0), axis=1)))
+    ref = res[8]
+    for n, F in res.items():
+        tol = 1.5e-2 if n == 2 else 6e-3
+        for i in (corner, face):
+            assert np.linalg.norm(F[i] - ref[i]) <= tol * np.linalg.norm(ref[i]), (n, i)
+        assert np.abs(F[face, 1:]).max() <= 1e-12 * abs(F[face, 0])
+        assert np.ptp(F[corner]) <= 1e-12 * np.abs(F[corner]).max()
+
+
+def test_fgi_gauss_points_follow_state_and_bcs():
+    """After a step: BC nodes' points hold the prescribed components (the rest keep 0); the other
+    points interpolate U with the MLS coefficients."""
+    spec = fgi_spec(small_bar(nx=8, ny=5, nz=5, jitter=0.01, seed=6, steps=10), 2)
+    spec["bcs"][0] = dict(type=1, flags=(1, 0, 1), value=(2e-7, 0.0, -1e-7), t_end=1.0, load=1)
+    o = pyoracle.Oracle(spec)
+    o.run(2)
+    _, _, bc = o.nodes()
+    U, _ = o.state()
+    _, Ug = o.gauss()
+    rp, cb, phi = o.mls()
+    fac = o.t / 1.0  # ramp load
+    bcn = bc == 0
+    assert bcn.sum() > 0
+    assert np.allclose(Ug[bcn][..., 0], 2e-7 * fac, rtol=1e-15, atol=0)
+    assert np.all(Ug[bcn][..., 1] == 0.0)
+    assert np.allclose(Ug[bcn][..., 2], -1e-7 * fac, rtol=1e-15, atol=0)
+    for i in np.flatnonzero(bc < 0)[:20]:
+        k = cb[rp[i]:rp[i + 1]]
+        rows = phi[8 * rp[i]:8 * rp[i + 1]].reshape(8, len(k))
+        assert np.allclose(Ug[i], rows @ U[k], rtol=1e-14, atol=1e-24)
