@@ -111,6 +111,59 @@ def run_static(args):
     return out, err
 
 
+FGI_FLOPS_PER_POINT = 55  # FP64 operations of one Gauss-point bond force (sqrt and division as one)
+FP64_PEAK_TFLOPS = 78.6
+
+
+def run_fgi(args):
+    """Fixed Gaussian integration (fgi_on 1, intg_point n): FP64 RK4 on the bar, one JSON line."""
+    import pdmtel_amd as pm
+
+    n1 = args.intg_point
+    spec = pm.bar_spec(args.nx, args.ny, args.nz, jitter=0.01, seed=0, steps=args.warmup + args.steps + 10)
+    spec["cci"] = None
+    spec["fgi"] = (1, n1)
+    t0 = time.perf_counter()
+    prob = pm.Problem(pm.deck_text(spec), precision=pm.PDM_FP64, graph_chunk=args.chunk, energies=1)
+    t_init = time.perf_counter() - t0
+    print(f"# fgi n={n1}: init {t_init:.2f} s, {prob.n} nodes, {prob.E} entries", file=sys.stderr, flush=True)
+    if args.warmup:
+        prob.step(args.warmup)
+    t0 = time.perf_counter()
+    done = prob.step(args.steps)
+    sec = time.perf_counter() - t0
+    assert done == args.steps, f"ran {done} of {args.steps} steps"
+    ms, launches = prob.kernel_timing()
+    n, E, ng = prob.n, prob.E, n1 ** 3
+    flops = E * ng * FGI_FLOPS_PER_POINT
+    achieved = flops / (ms * 1e-3) / 1e12
+    line = {
+        "metric": f"bond-updates/s (RK4, fixed Gaussian integration {n1}x{n1}x{n1}, FP64)",
+        "value": 4.0 * E * args.steps / sec,
+        "unit": "bond-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"bar {args.nx}x{args.ny}x{args.nz}, {n} nodes, {E} family entries, delta=3.015dx, "
+                               f"jitter 0.01dx seed 0, fgi_on 1 intg_point {n1}, RK4 {args.steps} steps",
+                   "gauss_points_per_node": ng, "init_seconds": t_init},
+        "gauss_point_evaluations_per_s": 4.0 * E * ng * args.steps / sec,
+        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_fgi_stage + k_fgi_gauss (one RK4 stage)", "kernel_avg_ms": ms, "launches": launches,
+                     "flops_per_launch": flops},
+        "cpu_baseline": None,
+    }
+    prob.close()
+    print(json.dumps(line))
+
+
 def run_batch(args):
     """Batched specimens: FP32C and FP64 runs of the same random specimens."""
     import numpy as np
@@ -221,9 +274,11 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-static", action="store_true")
     ap.add_argument("--calib-copy", type=int, default=0, help="run only a copy kernel of this many bytes")
-    ap.add_argument("--workload", choices=["bar", "batch", "plate"], default="bar",
+    ap.add_argument("--workload", choices=["bar", "batch", "plate", "fgi"], default="bar",
                     help="bar: 200x40x40 bar (default); batch: random specimens; plate: --nx --ny --nz plate "
-                         "with two crossing slits (400 400 50 for the 8M-node plate)")
+                         "with two crossing slits (400 400 50 for the 8M-node plate); fgi: the bar with fixed "
+                         "Gaussian integration (--intg-point), FP64")
+    ap.add_argument("--intg-point", type=int, default=2)
     ap.add_argument("--specimens", type=int, default=512)
     args = ap.parse_args()
 
@@ -236,6 +291,9 @@ def main():
         return
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "fgi":
+        run_fgi(args)
         return
 
     import numpy as np
