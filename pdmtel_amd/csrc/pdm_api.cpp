@@ -139,6 +139,24 @@ StageArgs stage_args(pdm_handle* h)
     return a;
 }
 
+// Per-entry beta corrections of the FP32 force kernels for the current layout and
+// specimen constants (the FP32 records depend on both).
+int build_corr(pdm_handle* h)
+{
+    if (h->opt.precision == PDM_FP64 || h->ntl == 0) return PDM_OK;
+    int* cl = nullptr;
+    HCHK(hipMalloc(&cl, sizeof(int)));
+    HCHK(hipMemsetAsync(cl, 0, sizeof(int), h->stream));
+    HCHK(launch_fly_corr(tile_view(h), h->u[0], h->corr, cl, h->stream));
+    int clamped = 0;
+    HCHK(hipStreamSynchronize(h->stream));
+    HCHK(hipMemcpy(&clamped, cl, sizeof(int), hipMemcpyDeviceToHost));
+    HCHK(hipFree(cl));
+    if (h->opt.verbose)
+        fprintf(stderr, "pdm_init: %d family entries outside the 8-bit coefficient correction range\n", clamped);
+    return PDM_OK;
+}
+
 // Constants and tables of a single specimen (after each tile build: the FP32 record
 // scale depends on the tile edge).
 int upload_spec(pdm_handle* h)
@@ -158,7 +176,7 @@ int upload_spec(pdm_handle* h)
     HCHK(hipMemcpy(h->mats_d, mats.data(), sizeof(MatDev) * kMaxMat, hipMemcpyHostToDevice));
     HCHK(hipMemcpy(h->bcs_d, bcs.data(), sizeof(BCDev) * kMaxBC, hipMemcpyHostToDevice));
     HCHK(hipMemcpy(h->spec_tile_ptr_d, tp, sizeof tp, hipMemcpyHostToDevice));
-    return PDM_OK;
+    return build_corr(h);
 }
 
 // Host AoS (reference order) <-> device d4 (internal order).
@@ -318,9 +336,9 @@ constexpr int64_t kTuneMinEntries = 4000000;  // smaller specimens take the defa
 void free_tiles(pdm_handle* h)
 {
     dfree(h->permd); dfree(h->Xi); dfree(h->u[0]); dfree(h->u[1]); dfree(h->rpi); dfree(h->slot);
-    dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->mati); dfree(h->bci); dfree(h->nb);
-    dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr); dfree(h->halo);
-    dfree(h->part);
+    dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->bk); dfree(h->corr); dfree(h->mati);
+    dfree(h->bci); dfree(h->nb); dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr);
+    dfree(h->halo); dfree(h->part);
     h->part_cap = 0;
     h->ntl = 0;
     h->ntiles = 0;
@@ -548,7 +566,11 @@ int batch_concat(pdm_handle* B)
     HCHK(hipMalloc(&B->rpi, sizeof(int64_t) * (N + 1)));
     HCHK(hipMalloc(&B->slot, sizeof(uint16_t) * e1));
     if (f64) HCHK(hipMalloc(&B->coef64, sizeof(double) * e1));
-    else HCHK(hipMalloc(&B->coef32, sizeof(float) * e1));
+    else {  // the FP32 stage kernels read no coef32; coef64 only while the corrections are built
+        HCHK(hipMalloc(&B->bk, sizeof(BK) * n1));
+        HCHK(hipMalloc(&B->corr, e1));
+        HCHK(hipMalloc(&B->coef64, sizeof(double) * e1));
+    }
     HCHK(hipMalloc(&B->tiles, sizeof(int32_t) * t1));
     HCHK(hipMalloc(&B->tref, sizeof(d4) * t1));
     HCHK(hipMalloc(&B->tile_start, sizeof(int64_t) * (ntiles + 1)));
@@ -570,8 +592,8 @@ int batch_concat(pdm_handle* B)
         HCHK(hipMemcpyAsync(B->self_slot + no, m->self_slot, sizeof(uint16_t) * m->n, hipMemcpyDeviceToDevice, s));
         HCHK(copy_add_i64(B->rpi + no, m->rpi, m->n, eo, s));
         HCHK(hipMemcpyAsync(B->slot + eo, m->slot, sizeof(uint16_t) * m->Ei, hipMemcpyDeviceToDevice, s));
-        if (f64) HCHK(hipMemcpyAsync(B->coef64 + eo, m->coef64, sizeof(double) * m->Ei, hipMemcpyDeviceToDevice, s));
-        else HCHK(hipMemcpyAsync(B->coef32 + eo, m->coef32, sizeof(float) * m->Ei, hipMemcpyDeviceToDevice, s));
+        HCHK(hipMemcpyAsync(B->coef64 + eo, m->coef64, sizeof(double) * m->Ei, hipMemcpyDeviceToDevice, s));
+        if (!f64) HCHK(hipMemcpyAsync(B->bk + no, m->bk, sizeof(BK) * m->n, hipMemcpyDeviceToDevice, s));
         HCHK(copy_add_i32(B->tiles + to, m->tiles, m->ntl, (int32_t)tio, s));
         HCHK(hipMemcpyAsync(B->tref + to, m->tref, sizeof(d4) * m->ntl, hipMemcpyDeviceToDevice, s));
         HCHK(copy_add_i64(B->tile_start + tio, m->tile_start, m->ntiles, no, s));
@@ -605,7 +627,9 @@ int batch_concat(pdm_handle* B)
     HCHK(hipMemcpyAsync(B->mats_d, mats.data(), sizeof(MatDev) * kMaxMat * S, hipMemcpyHostToDevice, s));
     HCHK(hipMemcpyAsync(B->bcs_d, bcs.data(), sizeof(BCDev) * kMaxBC * S, hipMemcpyHostToDevice, s));
     HCHK(hipStreamSynchronize(s));
-    return PDM_OK;
+    const int rc = build_corr(B);
+    if (!f64) dfree(B->coef64);
+    return rc;
 }
 
 // Build every member's tiles with edge t, lay them end to end in B, release the members' copies.
@@ -750,9 +774,9 @@ int pdm_destroy(pdm_handle* h)
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     dfree(h->Xd); dfree(h->rp); dfree(h->col); dfree(h->cut); dfree(h->be); dfree(h->matd);
     dfree(h->permd); dfree(h->Xi); dfree(h->u[0]); dfree(h->u[1]); dfree(h->rpi); dfree(h->slot);
-    dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->mati); dfree(h->bci); dfree(h->nb);
-    dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr); dfree(h->halo);
-    dfree(h->U); dfree(h->V); dfree(h->Vb); dfree(h->k1); dfree(h->k23);
+    dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->bk); dfree(h->corr); dfree(h->mati);
+    dfree(h->bci); dfree(h->nb); dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr);
+    dfree(h->halo); dfree(h->U); dfree(h->V); dfree(h->Vb); dfree(h->k1); dfree(h->k23);
     dfree(h->elast); dfree(h->part); dfree(h->energy); dfree(h->steps);
     dfree(h->specs_d); dfree(h->mats_d); dfree(h->bcs_d); dfree(h->tile_spec_d); dfree(h->spec_tile_ptr_d);
     free_fgi(h);

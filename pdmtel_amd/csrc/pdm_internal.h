@@ -16,9 +16,12 @@
 //    Xi d4[N]      position, .w = ecrit of the node's material
 //    u[2] d4[N]    displacement gathered by the next stage (ping-pong)
 //    rpi int64[N+1], slot uint16[Ei], coef64 FP64[Ei] / coef32 FP32[Ei]
-//      rows padded to a multiple of kRowAlign entries (Ei >= E); padding entries
-//      point at the tile's padding record with coef 0
+//      rows padded to a multiple of kRowAlign entries (Ei >= E); padding entries and
+//      cut bonds point at the tile's padding record, with coef 0
 //      coef = bdryEff_ij * dx^3 * beta(r_ij) * c_ij, 0 for a cut bond
+//    bk BK[N]      bdryEff and dx^3 c / 2 in FP32, and corr uint8[Ei]: the FP32 force
+//                  kernels form each entry's coefficient from these and beta(r), corrected
+//                  per entry by (corr - 128) 2^-26, instead of reading coef32
 //    U, V, Vb, k1, k23 d4[N]  FP64 RK4 state; mat, bc, nb int32[N]
 #pragma once
 
@@ -45,6 +48,12 @@ constexpr int kRowAlign = 8;          // internal rows padded to whole blocks of
 
 struct alignas(32) d4 {
     double x, y, z, w;
+};
+
+// Per node, FP32: the boundary-effect factor and half the node volume times the spring
+// constant, from which the FP32 force kernels form coef = (be_i + be_j) min(kc_i, kc_j) beta.
+struct alignas(8) BK {
+    float be, kc;
 };
 
 struct MatDev {
@@ -81,6 +90,9 @@ struct SpecDev {
     double fscale;        // fixed-point scale of the FP32 halo records
     double damping;       // damping_coeff
     double vol;           // pow(dx, 3): node volume for the energies
+    // beta(r) of the FP32 records (r in record units, S m), offset by the correction bias:
+    // med3(beta0 - r beta1, -128 2^-26, 1 - 128 2^-26)
+    float beta0, beta1;
 };
 
 // Tile-gather inputs shared by every tiled kernel.
@@ -94,6 +106,8 @@ struct TileView {
     const uint16_t* slot;
     const double* coef64;
     const float* coef32;
+    const BK* bk;
+    const uint8_t* corr;
     const d4* X;
     const d4* tref;            // launch index -> tile centre (reference point of FP32 records)
     int64_t n;
@@ -206,6 +220,8 @@ struct pdm_handle {
     uint16_t* self_slot = nullptr;
     double* coef64 = nullptr;
     float* coef32 = nullptr;
+    pdm::BK* bk = nullptr;
+    uint8_t* corr = nullptr;
     int32_t *mati = nullptr, *bci = nullptr, *nb = nullptr;
     int32_t* tiles = nullptr;
     int64_t* tile_start = nullptr;
@@ -291,6 +307,9 @@ hipError_t launch_energy_steps(const double* part, int64_t ntl, const int64_t* s
 hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, double* elast, hipStream_t s);
 hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s);
 hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s);
+// Per-entry beta corrections of the FP32 force kernels (from coef64); *clamped counts the
+// entries whose correction left the 8-bit range.
+hipError_t launch_fly_corr(const TileView& tv, const d4* u, uint8_t* corr, int* clamped, hipStream_t s);
 
 // fixed Gaussian integration (pdm_fgi.hip, kernels in pdm_kernels.hip)
 int build_fgi(pdm_handle* h);

@@ -206,15 +206,17 @@ struct alignas(16) Rec {
 // displacement u S in FP64, so eta = u_j - u_i is formed exactly before its rounding
 // to FP32 (an FP32 displacement, even relative to a tile reference, rounds eta at the
 // scale of the displacement field across the halo instead of eta's own).  One 48-byte
-// record per node (x, y, z as int32, ecrit as FP32 bits | ux, uy | uz), so one address
-// serves its three LDS reads.  The bond force is scale-free in these units; the energy
-// carries one factor 1/S.
+// record per node (x, y, z as int32, ecrit as FP32 bits | ux, uy | uz, bdryEff and
+// dx^3 c / 2 as FP32), read as three 16-byte LDS loads from one address.  The bond force
+// is scale-free in these units; the energy carries one factor 1/S.
 struct alignas(16) RecF {
     u32x4 a;     // fixed-point x, y, z; ecrit (FP32 bits)
     f64x2 u;     // u S (x, y)
-    double w;    // u S (z)
-    double pad;
+    f64x2 w;     // u S (z); bdryEff and dx^3 c / 2 as two FP32 in the second double
 };
+
+__device__ __forceinline__ float lo_f(double d) { return __int_as_float((int)__double2loint(d)); }
+__device__ __forceinline__ float hi_f(double d) { return __int_as_float((int)__double2hiint(d)); }
 
 // Stage-update inputs of one tile node, loaded while the halo is staged.
 struct EpiRec {
@@ -288,14 +290,14 @@ __device__ __forceinline__ Lds lds_view(double* base, int H, int tile_max, bool 
 
 constexpr unsigned kFarFix = 1u << 30;  // fixed-point coordinate of the padding record
 
-__device__ __forceinline__ void frec_store(const Lds& l, int q, const d4& x, const d4& v, const d4& xr, double S)
+__device__ __forceinline__ void frec_store(const Lds& l, int q, const d4& x, const d4& v, const BK& b, const d4& xr,
+                                           double S)
 {
     RecF r;
     r.a = u32x4{(unsigned)__double2int_rn((x.x - xr.x) * S), (unsigned)__double2int_rn((x.y - xr.y) * S),
                 (unsigned)__double2int_rn((x.z - xr.z) * S), __float_as_uint((float)x.w)};
     r.u = f64x2{v.x * S, v.y * S};
-    r.w = v.z * S;
-    r.pad = 0.0;
+    r.w = f64x2{v.z * S, __hiloint2double(__float_as_int(b.kc), __float_as_int(b.be))};
     l.recf[q] = r;
 }
 
@@ -317,8 +319,9 @@ __device__ __forceinline__ void stage_tile(const TileView& tv, const SpecDev& sp
         const d4 x0 = tv.X[id0], v0 = u[id0];
         const d4 x1 = tv.X[id1], v1 = u[id1];
         if (F32) {
-            frec_store(l, q0, x0, v0, xr, S);
-            if (q1 < H) frec_store(l, q1, x1, v1, xr, S);
+            const BK b0 = tv.bk[id0], b1 = tv.bk[id1];
+            frec_store(l, q0, x0, v0, b0, xr, S);
+            if (q1 < H) frec_store(l, q1, x1, v1, b1, xr, S);
         } else {
             l.rec[q0] = Rec{x0.x, x0.y, x0.z, v0.x, v0.y, v0.z};
             if (with_ecrit) l.ec[q0] = x0.w;
@@ -333,8 +336,7 @@ __device__ __forceinline__ void stage_tile(const TileView& tv, const SpecDev& sp
             RecF r;
             r.a = u32x4{kFarFix, kFarFix, kFarFix, __float_as_uint(1.0f)};
             r.u = f64x2{0.0, 0.0};
-            r.w = 0.0;
-            r.pad = 0.0;
+            r.w = f64x2{0.0, 0.0};  // bdryEff 0, c 0: coefficient 0
             l.recf[H] = r;
         } else {
             l.rec[H] = Rec{kFar, kFar, kFar, 0.0, 0.0, 0.0};
@@ -359,12 +361,16 @@ struct SelfF {
     int x, y, z;
     double u, v, w;
     float ecrit;
+    float be, kc;         // bdryEff, dx^3 c / 2
+    float beta0, beta1;   // beta(r) constants of the specimen
     __device__ static SelfF make(const Lds& l, const SpecDev& sp, int ss, bool ue)
     {
         const RecF& r = l.recf[ss];
         const u32x4 a = r.a;
         const f64x2 b = r.u;
-        return SelfF{(int)a.x, (int)a.y, (int)a.z, b.x, b.y, r.w, ue ? (float)sp.ecrit0 : __uint_as_float(a.w)};
+        const f64x2 w = r.w;
+        return SelfF{(int)a.x, (int)a.y, (int)a.z, b.x, b.y, w.x, ue ? (float)sp.ecrit0 : __uint_as_float(a.w),
+                     lo_f(w.y), hi_f(w.y), sp.beta0, sp.beta1};
     }
 };
 
@@ -424,26 +430,62 @@ struct Bond32 {
     float g, a1, a2, a3, e, cs;  // force factor, xi + eta, u, coef * s
 };
 
+// ---- coefficients of the FP32 force operations
+// coef = bdryEff_ij dx^3 beta(r) c_ij is formed from the two records as
+// (be_i + be_j) min(kc_i, kc_j) (beta_32(r) + (corr - 128) 2^-26), kc = dx^3 c / 2, where
+// beta_32 is beta evaluated in FP32 from the record geometry and corr (uint8 per entry)
+// takes it to the FP64 coefficient within 2^-27 (k_fly_corr): one byte per entry instead
+// of an FP32 coefficient, and the correction also absorbs the FP32 rounding of the
+// constants and of be, kc.  The -128 2^-26 offset is folded into beta_32's constant and
+// clamp bounds.  Entries with coefficient 0 (cut bonds, beta 0, row padding) read the
+// padding record (be = kc = 0).
+constexpr float kCorrStep = 0x1p-26f;
+
+// r^2, 1/r and r of a record pair, in record units (S m).
+__device__ __forceinline__ void rec_r(const u32x4& ra, const SelfF& me, float& r2, float& rinv, float& rr)
+{
+    const float x1 = (float)((int)ra.x - me.x), x2 = (float)((int)ra.y - me.y), x3 = (float)((int)ra.z - me.z);
+    r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
+    rinv = __builtin_amdgcn_rsqf(r2);
+    rr = r2 * rinv;
+}
+
+// beta (pdDataManager3.cpp:1801-1831) in FP32, (delta + dx/2 - r) / dx clamped to [0, 1], minus
+// 128 2^-26.
+__device__ __forceinline__ float fly_beta(float rr, const SelfF& me)
+{
+    return __builtin_amdgcn_fmed3f(__builtin_fmaf(-rr, me.beta1, me.beta0), -128.0f * kCorrStep,
+                                   1.0f - 128.0f * kCorrStep);
+}
+
+// (be_i + be_j) min(kc_i, kc_j) from the neighbour record's second double.
+__device__ __forceinline__ float fly_base(const SelfF& me, double wy)
+{
+    return (me.be + lo_f(wy)) * __builtin_fminf(me.kc, hi_f(wy));
+}
+
 // FP32 bond force from the scaled FP32 records, cancellation-free stretch:
 // q = |xi+eta|^2 - |xi|^2 = eta . (2 xi + eta), s = (p - r) / r = q / ((p + r) r).
+// cu: the entry's coefficient correction byte, as a float.
 template <bool ENERGY, bool UE>
-__device__ __forceinline__ Bond32 bond32(const Lds& l, unsigned s, float c, const SelfF& me)
+__device__ __forceinline__ Bond32 bond32(const Lds& l, unsigned s, float cu, const SelfF& me)
 {
     const RecF& rec = l.recf[s];
     const u32x4 ra = rec.a;
     const f64x2 rb = rec.u;
-    const double rz = rec.w;
+    const f64x2 rw = rec.w;
+    const double rz = rw.x;
     const float x1 = (float)((int)ra.x - me.x), x2 = (float)((int)ra.y - me.y), x3 = (float)((int)ra.z - me.z);
     const float e1 = (float)(rb.x - me.u), e2 = (float)(rb.y - me.v), e3 = (float)(rz - me.w);
     const float a1 = x1 + e1, a2 = x2 + e2, a3 = x3 + e3;
-    const float r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
+    float r2, rinv, rr;
+    rec_r(ra, me, r2, rinv, rr);
     const float q = __builtin_fmaf(e3, x3 + a3, __builtin_fmaf(e2, x2 + a2, e1 * (x1 + a1)));
     const float p2 = r2 + q;
-    const float rinv = __builtin_amdgcn_rsqf(r2);
     const float pinv = __builtin_amdgcn_rsqf(p2);
-    const float rr = r2 * rinv;
     const float p = p2 * pinv;
     const float st = q * __builtin_amdgcn_rcpf((p + rr) * rr);
+    const float c = fly_base(me, rw.y) * __builtin_fmaf(cu, kCorrStep, fly_beta(rr, me));
     // u <= 0 || s <= ecrit, with u = s r and r > 0
     float ecrit = me.ecrit;
     if (!UE) {
@@ -463,6 +505,7 @@ __device__ __forceinline__ Bond32 bond32(const Lds& l, unsigned s, float c, cons
 
 struct OpF32 {  // PDM_FP32C: FP32 bond force, FP32 block sums, FP64 family accumulator
     using Coef = float;
+    static constexpr bool kCoef = false;  // coefficients formed from the records and correction bytes
     using SelfT = SelfF;
     struct Acc {
         double f0, f1, f2;
@@ -504,6 +547,7 @@ struct OpF32 {  // PDM_FP32C: FP32 bond force, FP32 block sums, FP64 family accu
 
 struct OpF32TF {  // PDM_FP32C2: FP32 bond force, two-float family sums
     using Coef = float;
+    static constexpr bool kCoef = false;
     using SelfT = SelfF;
     struct Acc {
         float h0, h1, h2, l0, l1, l2, el;
@@ -591,7 +635,7 @@ struct OpT32 {
         const u32x4 ra = rec.a;
         const f64x2 rb = rec.u;
         const float x1 = (float)((int)ra.x - me.x), x2 = (float)((int)ra.y - me.y), x3 = (float)((int)ra.z - me.z);
-        const float d0 = (float)(me.u - rb.x), d1 = (float)(me.v - rb.y), d2 = (float)(me.w - rec.w);
+        const float d0 = (float)(me.u - rb.x), d1 = (float)(me.v - rb.y), d2 = (float)(me.w - rec.w.x);
         const float r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
         const float rinv = __builtin_amdgcn_rsqf(r2);
         const float proj = __builtin_fmaf(x3, d2, __builtin_fmaf(x2, d1, x1 * d0)) * c * (rinv * rinv * rinv);
@@ -651,6 +695,19 @@ struct OpD64 {
 template <class OP>
 constexpr bool kF32Rec = std::is_same<typename OP::SelfT, SelfF>::value;
 
+// Whether an operation streams a per-entry coefficient (the FP32 force operations form
+// theirs from the records and stream one correction byte per entry instead).
+template <class OP, class = void>
+struct CoefStream {
+    static constexpr bool value = true;
+};
+template <class OP>
+struct CoefStream<OP, std::enable_if_t<!OP::kCoef>> {
+    static constexpr bool value = false;
+};
+template <class OP>
+constexpr bool kCoefStream = CoefStream<OP>::value;
+
 // Operations without FP32 partial sums have nothing to flush.
 template <class OP, class = void>
 struct HasFlush {
@@ -666,12 +723,21 @@ __device__ __forceinline__ void flush(typename OP::Acc& a)
     if constexpr (HasFlush<OP>::value) OP::flush(a);
 }
 
-// One lane's block: B consecutive family entries, slots packed two per word.
-template <class OP, int B>
+// One lane's block: B consecutive family entries, slots packed two per word, and the
+// coefficients or, for the FP32 force operations, the correction bytes packed four per word.
+template <class OP, int B, bool STREAM = kCoefStream<OP>>
 struct Chunk {
     unsigned sp[B / 2];
     typename OP::Coef c[B];
     __device__ __forceinline__ unsigned slot(int k) const { return (sp[k >> 1] >> ((k & 1) * 16)) & 0xFFFFu; }
+    __device__ __forceinline__ typename OP::Coef coef(int k) const { return c[k]; }
+};
+template <class OP, int B>
+struct Chunk<OP, B, false> {
+    unsigned sp[B / 2];
+    unsigned cb[B / 4];
+    __device__ __forceinline__ unsigned slot(int k) const { return (sp[k >> 1] >> ((k & 1) * 16)) & 0xFFFFu; }
+    __device__ __forceinline__ float coef(int k) const { return (float)((cb[k >> 2] >> ((k & 3) * 8)) & 0xFFu); }
 };
 
 // Entries [e, e + 8) of a row ending at e1 (rows are whole blocks of kRowAlign = 8
@@ -685,7 +751,11 @@ __device__ __forceinline__ void load8(const TileView& tv, int64_t e, int64_t e1,
         const u32x4 sv = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(tv.slot + e));
 #pragma unroll
         for (int k = 0; k < 4; ++k) ch.sp[o / 2 + k] = sv[k];
-        if constexpr (sizeof(typename OP::Coef) == 4) {
+        if constexpr (!kCoefStream<OP>) {
+            const u32x2 cv = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(tv.corr + e));
+            ch.cb[o / 4] = cv[0];
+            ch.cb[o / 4 + 1] = cv[1];
+        } else if constexpr (sizeof(typename OP::Coef) == 4) {
             const f32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e));
             const f32x4 c1 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e + 4));
 #pragma unroll
@@ -704,8 +774,12 @@ __device__ __forceinline__ void load8(const TileView& tv, int64_t e, int64_t e1,
     } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k) ch.sp[o / 2 + k] = pad2;
+        if constexpr (kCoefStream<OP>) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) ch.c[o + k] = 0;
+            for (int k = 0; k < 8; ++k) ch.c[o + k] = 0;
+        } else {
+            ch.cb[o / 4] = ch.cb[o / 4 + 1] = 0x80808080u;  // zero corrections
+        }
     }
 }
 
@@ -726,7 +800,9 @@ __device__ __forceinline__ void load_block(const TileView& tv, int64_t e, int64_
             const u32x2 sv = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(tv.slot + e));
             ch.sp[0] = sv[0];
             ch.sp[1] = sv[1];
-            if constexpr (sizeof(typename OP::Coef) == 4) {
+            if constexpr (!kCoefStream<OP>) {
+                ch.cb[0] = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(tv.corr + e));
+            } else if constexpr (sizeof(typename OP::Coef) == 4) {
                 const f32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e));
 #pragma unroll
                 for (int k = 0; k < 4; ++k) ch.c[k] = c0[k];
@@ -740,8 +816,12 @@ __device__ __forceinline__ void load_block(const TileView& tv, int64_t e, int64_
             }
         } else {
             ch.sp[0] = ch.sp[1] = pad2;
+            if constexpr (kCoefStream<OP>) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) ch.c[k] = 0;
+                for (int k = 0; k < 4; ++k) ch.c[k] = 0;
+            } else {
+                ch.cb[0] = 0x80808080u;
+            }
         }
     }
 }
@@ -772,13 +852,13 @@ __device__ __forceinline__ void family_round(const TileView& tv, const SpecDev& 
     const typename OP::SelfT me = OP::SelfT::make(l, sp, ss, UE);
     typename OP::Acc acc{};
 #pragma unroll
-    for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, cur.slot(k), cur.c[k], me, acc);
+    for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, cur.slot(k), cur.coef(k), me, acc);
     flush<OP>(acc);
     for (int64_t base = e0 + CH; base < e1; base += CH) {
         Chunk<OP, B> more;
         load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(base, sl), e1, l.pad, more);
 #pragma unroll
-        for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, more.slot(k), more.c[k], me, acc);
+        for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, more.slot(k), more.coef(k), me, acc);
         flush<OP>(acc);
     }
     OP::template reduce<LPN>(acc);
@@ -1151,6 +1231,45 @@ __global__ __launch_bounds__(64 * NW) void k_gather(TileView tv, const d4* __res
     }
 }
 
+// Correction bytes of the FP32 force operations: for each entry, q = rint((coef64 / base -
+// fly_beta) / 2^-26) clamped to 0..255, with base and fly_beta evaluated by the same
+// functions, on the same FP32 records, as the force kernels.  One workgroup per
+// tile; *clamped counts entries whose correction fell outside the byte.
+__global__ __launch_bounds__(256) void k_fly_corr(TileView tv, const d4* __restrict__ u, uint8_t* __restrict__ corr,
+                                                  int* __restrict__ clamped)
+{
+    extern __shared__ double L[];
+    const int64_t k = blockIdx.x;
+    const int32_t tid = tv.tiles[k];
+    const int64_t a0 = tv.tile_start[tid];
+    const int nt = (int)(tv.tile_start[tid + 1] - a0);
+    const int64_t h0 = tv.halo_ptr[k];
+    const int H = (int)(tv.halo_ptr[k + 1] - h0);
+    const SpecDev sp = tv.specs[tile_specimen(tv, k)];
+    const Lds l = lds_view(L, H, tv.tile_max, false, false, true);
+    stage_tile<true>(tv, sp, u, k, h0, H, a0, nt, l, false);
+    __syncthreads();
+    int bad = 0;
+    for (int loc = 0; loc < nt; ++loc) {
+        const SelfF me = SelfF::make(l, sp, l.self[loc], true);
+        for (int64_t e = l.row[loc] + threadIdx.x; e < l.row[loc + 1]; e += blockDim.x) {
+            const RecF& rec = l.recf[tv.slot[e]];
+            float r2, rinv, rr;
+            rec_r(rec.a, me, r2, rinv, rr);
+            const float base = fly_base(me, rec.w.y);
+            const float b32 = fly_beta(rr, me);
+            int q = 128;
+            if (base > 0.0f) {
+                const double d = rint((tv.coef64[e] / (double)base - (double)b32) / (double)kCorrStep);
+                bad += (d < 0.0 || d > 255.0) ? 1 : 0;
+                q = (int)fmin(fmax(d, 0.0), 255.0);
+            }
+            corr[e] = (uint8_t)q;
+        }
+    }
+    if (bad) atomicAdd(clamped, bad);
+}
+
 // ------------------------------------------------------------ fixed Gaussian integration
 // The fixed-Gaussian branch of CalcBondBasedMotionRK4's family loop (pdDataManager.cpp:
 // 1918-1957): for each unbroken entry (i, j), the CCI bond force (pdDataManager2.cpp:
@@ -1498,6 +1617,16 @@ hipError_t launch_fgi_forces(const StageArgs& a, const FgiArgs& f, const d4* u, 
     hipError_t e = set_lds<k_fgi_forces>(lds);
     if (e != hipSuccess) return e;
     k_fgi_forces<<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, u, F, elast);
+    return hipGetLastError();
+}
+
+hipError_t launch_fly_corr(const TileView& tv, const d4* u, uint8_t* corr, int* clamped, hipStream_t s)
+{
+    if (tv.ntl == 0) return hipSuccess;
+    const size_t lds = lds_bytes(tv.halo_max, tv.tile_max, false, false, true);
+    hipError_t e = set_lds<k_fly_corr>(lds);
+    if (e != hipSuccess) return e;
+    k_fly_corr<<<dim3((unsigned)tv.ntl), dim3(256), lds, s>>>(tv, u, corr, clamped);
     return hipGetLastError();
 }
 

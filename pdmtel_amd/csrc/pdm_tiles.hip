@@ -120,7 +120,20 @@ struct HaloArgs {
     uint16_t* slot;
     uint16_t* self_slot;
     int* error;
+    // entries whose CCI beta is 0 read the padding record too (beta_mode 1 modified, 2 original
+    // CCI; 0: keep them, as fixed Gaussian integration weights each Gauss point itself)
+    const d4* Xi;
+    double horizon, dx;
+    int beta_mode;
 };
+
+// beta of the entry (i, j) from the internal positions, as k_coef evaluates it.
+__device__ __forceinline__ double entry_beta(const d4& xi, const d4& xj, double horizon, double dx, bool original)
+{
+    double x1 = xj.x - xi.x, x2 = xj.y - xi.y, x3 = xj.z - xi.z;
+    double r = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+    return g_beta(r, horizon, dx, original);
+}
 
 // One workgroup per tile.  WRITE=false: halo length; WRITE=true: halo list and slots.
 template <bool WRITE>
@@ -234,6 +247,21 @@ __global__ __launch_bounds__(256) void k_halo(HaloArgs a)
             continue;
         }
         int32_t c = (int32_t)(pk >> 1);
+        bool zero = (pk & 1u) != 0;  // a cut bond reads the padding record (coefficient 0)
+        if (!zero && a.beta_mode) {
+            // the row owner: the last node of the tile whose row starts at or before e
+            int64_t lo = a0, hi = a1 - 1;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi + 1) / 2;
+                if (a.rpi[mid] <= e) lo = mid;
+                else hi = mid - 1;
+            }
+            zero = entry_beta(a.Xi[lo], a.Xi[c], a.horizon, a.dx, a.beta_mode == 2) == 0.0;
+        }
+        if (zero) {
+            a.slot[e] = (uint16_t)H;
+            continue;
+        }
         int32_t tj = a.tileid[c];
         int64_t jx = tj % ntx, jy = (tj / ntx) % nty, jz = tj / (ntx * nty);
         int g = (int)((jx - tx + Rx) + gx * ((jy - ty + Ry) + gy * (jz - tz + Rz)));
@@ -251,6 +279,7 @@ struct CoefArgs {
     const double* bei;
     double* coef64;
     float* coef32;
+    BK* bk;
     int32_t* nb;
     int64_t n;
     double horizon, dx, vcube;
@@ -271,6 +300,7 @@ __global__ void k_coef(CoefArgs a)
     d4 xi = a.Xi[i];
     double bi = a.bei[i];
     double ci = a.spring[a.mati[i]];
+    if (lane == 0) a.bk[i] = BK{(float)bi, (float)(0.5 * a.vcube * ci)};
     int cnt = 0;
     for (int64_t e = a.rpi[i] + lane; e < a.rpi[i + 1]; e += 64) {
         uint32_t pk = a.packed[e];
@@ -282,12 +312,10 @@ __global__ void k_coef(CoefArgs a)
         int32_t j = (int32_t)(pk >> 1);
         bool cut = (pk & 1u) != 0;
         d4 xj = a.Xi[j];
-        double x1 = xj.x - xi.x, x2 = xj.y - xi.y, x3 = xj.z - xi.z;
-        double r = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
         double cj = a.spring[a.mati[j]];
         double spring = (cj < ci) ? cj : ci;
         double bij = 0.5 * (bi + a.bei[j]);
-        double beta = g_beta(r, a.horizon, a.dx, a.original_cci);
+        double beta = entry_beta(xi, xj, a.horizon, a.dx, a.original_cci);
         double c = cut ? 0.0 : bij * a.vcube * beta * spring;
         a.coef64[e] = cut ? 0.0 : (a.fgi ? bij : c);
         a.coef32[e] = (float)c;
@@ -345,6 +373,8 @@ TileView tile_view(const pdm_handle* h)
     v.slot = h->slot;
     v.coef64 = h->coef64;
     v.coef32 = h->coef32;
+    v.bk = h->bk;
+    v.corr = h->corr;
     v.X = h->Xi;
     v.tref = h->tref;
     v.n = h->n;
@@ -486,6 +516,8 @@ int build_tiles(pdm_handle* h)
     HIPCHK(hipMalloc(&h->slot, sizeof(uint16_t) * E1));
     HIPCHK(hipMalloc(&h->coef64, sizeof(double) * E1));
     HIPCHK(hipMalloc(&h->coef32, sizeof(float) * E1));
+    HIPCHK(hipMalloc(&h->bk, sizeof(BK) * n1));
+    HIPCHK(hipMalloc(&h->corr, E1));
     HIPCHK(hipMalloc(&err, sizeof(int)));
     double ech[kMaxMat] = {0};
     for (size_t m = 0; m < d.mats.size(); ++m) ech[m] = d.mats[m].ecrit;
@@ -528,6 +560,10 @@ int build_tiles(pdm_handle* h)
     ha.g = g;
     ha.halo_len = halo_len;
     ha.error = err;
+    ha.Xi = h->Xi;
+    ha.horizon = d.horizon;
+    ha.dx = d.dx;
+    ha.beta_mode = !d.cci_on ? 0 : (d.cci_mdf ? 1 : 2);
     if (h->ntl) {
         k_halo<false><<<dim3((unsigned)h->ntl), dim3(256), 0, s>>>(ha);
         HIPCHK(hipGetLastError());
@@ -578,6 +614,7 @@ int build_tiles(pdm_handle* h)
     ca.bei = bei;
     ca.coef64 = h->coef64;
     ca.coef32 = h->coef32;
+    ca.bk = h->bk;
     ca.nb = h->nb;
     ca.n = n;
     ca.horizon = d.horizon;
