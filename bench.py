@@ -29,8 +29,9 @@ launch use the per-unit figures of the bond-update unit: 8 B per family
 entry for FP32 coefficients (int32 neighbour + FP32 coefficient; 12 B with
 FP64 coefficients) plus 264 B per node (trial u, X, Vbuf: 72 B, and the
 fused stage update, 8 vectors of 24 B).  The library's own format is smaller
-(uint16 LDS slot + coefficient: 6 B / 10 B per entry); `format_bytes` gives
-that figure.  `traffic` is FETCH_SIZE + WRITE_SIZE per launch from rocprofv3
+(uint16 LDS slot + one correction byte from which the FP32 kernels form the
+coefficient: 3 B per entry; uint16 slot + FP64 coefficient: 10 B); `format_bytes`
+gives that figure.  `traffic` is FETCH_SIZE + WRITE_SIZE per launch from rocprofv3
 counter runs on this workload, scaled by a plain copy kernel's readings, as
 recorded under profiles/ (null for any other workload).
 
@@ -50,7 +51,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 ENTRY_BYTES = {"fp32c": 8, "fp64": 12}        # int32 neighbour + coefficient
-FORMAT_ENTRY_BYTES = {"fp32c": 6, "fp64": 10}  # uint16 slot + coefficient (this library's format)
+FORMAT_ENTRY_BYTES = {"fp32c": 3, "fp64": 10}  # uint16 slot + correction byte / FP64 coefficient (this library's format)
 NODE_BYTES = 72 + 8 * 24                       # trial u, X, Vbuf + fused stage update (8 x 24 B)
 DEFAULT_DIMS = (200, 40, 40)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_bar200x40x40_traffic.txt")
