@@ -36,7 +36,9 @@ counter runs on this workload, scaled by a plain copy kernel's readings, as
 recorded under profiles/ (null for any other workload).
 
 cpu_baseline: the oracle (a serial restatement of the reference's RK4 loop)
-on the same bar, a few steps on 1 thread of this host.
+on the same bar, a few steps on 1 thread of this host (value, cores 1), then
+the next few steps with its node loop on all the cores given to this process
+(all_cores; OMP_NUM_THREADS, else the CPU count).
 """
 
 import argparse
@@ -233,7 +235,14 @@ def cpu_baseline(spec, steps):
     t0 = time.perf_counter()
     o.run(steps)
     dt = time.perf_counter() - t0
-    return dict(value=4.0 * o.E * steps / dt, seconds=dt, E=o.E, n=o.n, steps=steps)
+    # the same restatement with its node loop on all host cores given to this process (OpenMP)
+    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    o.nthreads = cores
+    t0 = time.perf_counter()
+    o.run(steps)
+    dt_all = time.perf_counter() - t0
+    return dict(value=4.0 * o.E * steps / dt, seconds=dt, E=o.E, n=o.n, steps=steps,
+                value_all=4.0 * o.E * steps / dt_all, seconds_all=dt_all, cores_all=cores)
 
 
 def recorded_traffic(args, prec):
@@ -393,6 +402,9 @@ def main():
             "kind": "port",
             "sample": f"oracle serial restatement of the reference RK4 loop, same {cb['n']}-node bar, "
                       f"{cb['steps']} RK4 steps ({4 * cb['E'] * cb['steps']} bond-updates) in {cb['seconds']:.1f} s",
+            "all_cores": {"value": cb["value_all"], "cores": cb["cores_all"],
+                          "sample": f"the next {cb['steps']} steps with the node loop on {cb['cores_all']} OpenMP "
+                                    f"threads, {cb['seconds_all']:.1f} s"},
         }
     print(json.dumps(line))
 
