@@ -125,6 +125,29 @@ def test_rk4_graph_matches_eager():
         assert np.array_equal(Ua, Ub) and np.array_equal(Va, Vb)
 
 
+@pytest.mark.parametrize("case", ["bar_jitter", "bar_delta4", "bar_cci_original", "slits"])
+@pytest.mark.parametrize("precision", [pm.PDM_FP32C, pm.PDM_FP32C2])
+def test_forces_fp32(case, precision):
+    """FP32 bond forces (coefficients formed from the records and the per-entry correction
+    bytes; cut and zero-beta entries on the padding record) against the FP64 oracle."""
+    if case == "slits":
+        spec = make("bar_jitter")
+        spec["slits"] = [pm.mid_slit(spec, 30.0), dict(type=1, box=(0.0, 1.0, 0.004, 0.004, 0.0, 1.0))]
+    else:
+        spec = make(case)
+    prob, orc = pair(spec, precision=precision)
+    rng = np.random.default_rng(9)
+    X, _, _ = prob.nodes()
+    for name, u in (("random", rng.uniform(-1e-6, 1e-6, size=(prob.n, 3))),
+                    ("affine", X @ np.array([[1.4e-5, 0, 0], [0, -3.5e-6, 0], [0, 2e-6, -3.5e-6]]).T)):
+        F, el = prob.forces(u, with_elast=True)
+        Fo, elo = orc.forces(u, with_elast=True)
+        err = np.abs(F - Fo).max() / np.abs(Fo).max()
+        eerr = np.abs(el - elo).max() / np.abs(elo).max()
+        print(f"[forces fp32] {case} precision={precision} {name} err={err:.3e} energy={eerr:.3e}", flush=True)
+        assert err <= 1e-6 and eerr <= 1e-6, (name, err, eerr)
+
+
 def test_rk4_fp32c_error():
     spec = make("bar_jitter", steps=40)
     p64 = pm.Problem(pm.deck_text(spec), precision=pm.PDM_FP64)
