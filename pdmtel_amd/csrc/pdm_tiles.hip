@@ -325,6 +325,45 @@ __global__ void k_coef(CoefArgs a)
     if (lane == 0) a.nb[i] = cnt;
 }
 
+// LDS-bank-aware order of each row (one thread per row).  The FP32 gathers read a row in
+// rounds of 256 entries; at each step, 8 consecutive lanes read the positions
+// 64 B + 8 j + a (j = 0..7) of one group (B, a), and a 48-byte record s starts in bank
+// group 3 s mod 16.  Entries sorted by slot mod 16 are dealt to the positions ordered by
+// (j, group), so equal residues land in different groups and the 8 records one group
+// reads fall in distinct banks.  The order within a row is otherwise free: every entry,
+// padding included, carries its own weight.
+__global__ void k_bank_order(const int64_t* __restrict__ rpi, uint16_t* __restrict__ slot,
+                             uint32_t* __restrict__ packed, uint16_t* __restrict__ tslot,
+                             uint32_t* __restrict__ tpacked, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t e0 = rpi[i];
+    const int L = (int)(rpi[i + 1] - e0);
+    int start[16] = {0};
+    for (int p = 0; p < L; ++p) start[slot[e0 + p] & 15]++;
+    for (int r = 0, run = 0; r < 16; ++r) {
+        const int c = start[r];
+        start[r] = run;
+        run += c;
+    }
+    for (int p = 0; p < L; ++p) {
+        const int q = start[slot[e0 + p] & 15]++;
+        tslot[e0 + q] = slot[e0 + p];
+        tpacked[e0 + q] = packed[e0 + p];
+    }
+    int q = 0;
+    for (int j = 0; j < 8; ++j)
+        for (int b = 0; 64 * b < L; ++b)
+            for (int a = 0; a < 8; ++a) {
+                const int p = 64 * b + 8 * j + a;
+                if (p >= L) continue;
+                slot[e0 + p] = tslot[e0 + q];
+                packed[e0 + p] = tpacked[e0 + q];
+                ++q;
+            }
+}
+
 template <class T>
 __global__ void k_copy_add(T* __restrict__ dst, const T* __restrict__ src, int64_t n, T off)
 {
@@ -605,6 +644,17 @@ int build_tiles(pdm_handle* h)
     if (h->ntl) {
         k_halo<true><<<dim3((unsigned)h->ntl), dim3(256), 0, s>>>(ha);
         HIPCHK(hipGetLastError());
+    }
+    if (n) {
+        uint16_t* tslot = nullptr;
+        uint32_t* tpacked = nullptr;
+        HIPCHK(hipMalloc(&tslot, sizeof(uint16_t) * E1));
+        HIPCHK(hipMalloc(&tpacked, sizeof(uint32_t) * E1));
+        k_bank_order<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(h->rpi, h->slot, packed, tslot, tpacked, n);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipFree(tslot));
+        HIPCHK(hipFree(tpacked));
     }
     CoefArgs ca{};
     ca.Xi = h->Xi;
