@@ -1,14 +1,15 @@
 // pdm_kernels.hip -- LDS-staged bond-force gathers fused with the RK4 stage update.
 //
 // One workgroup per tile.  Phase 1 stages the tile's halo as 48-byte records
-// (position and this stage's displacement, FP64) plus the families' row
-// starts and own slots into LDS with coalesced loads; one extra record far
-// outside the body backs the row padding.  Phase 2: each 64-lane wavefront works
-// on NPW families at a time, 64/NPW lanes per family; every lane loads one
-// aligned block of B consecutive entries (uint16 LDS slots and coefficients,
-// streamed from HBM as 16-byte vector loads: B = 8 with FP32 coefficients, 4
-// with FP64), and the first block of the next round is loaded while the current
-// one is computed.  The neighbour's record is three ds_read_b128.  Family sums
+// (position and this stage's displacement; FP64, or fixed point and FP64 u S with the
+// FP32 coefficient inputs) plus the families' row starts and own slots into LDS with
+// coalesced loads; one extra record far outside the body, with zero weight, backs the
+// row padding and every zero-coefficient entry.  Phase 2: each 64-lane wavefront works
+// on NPW families at a time, 64/NPW lanes per family; every lane loads one aligned
+// block of B consecutive entries (uint16 LDS slots, and FP64 coefficients or FP32
+// correction bytes, streamed from HBM as vector loads: B = 256/(64/NPW) for the FP32
+// operations, 4 for FP64), and the first block of the next round is loaded while the
+// current one is computed.  The neighbour's record is three ds_read_b128.  Family sums
 // are reduced inside the lane group with DPP moves (quad_perm, row_half_mirror,
 // row_mirror) and the gfx950 permlane16/32 swaps, never through LDS.  Phase 3:
 // one thread per tile node applies the stage update (velocity buffer, boundary
@@ -21,10 +22,11 @@
 //               coef = bdryEff_ij * dx^3 * beta * c_ij precomputed in the same association
 //               order, so each bond force equals the reference's when pow(x, 2.0) is the
 //               correctly rounded square.
-//   PDM_FP32C:  xi and eta formed in FP64 and rounded to FP32; the stretch from the
-//               cancellation-free form s = (2 xi.eta + eta.eta) / ((p + r) r); FP32
-//               coefficients; the bond forces of a lane's block (8 entries) summed in
-//               FP32, each block sum added to an FP64 family accumulator.
+//   PDM_FP32C:  xi and eta formed exactly and rounded to FP32; the stretch from the
+//               cancellation-free form s = (2 xi.eta + eta.eta) / ((p + r) r); the
+//               coefficient formed from the two records and the entry's correction byte;
+//               the bond forces of a lane's block summed in FP32, each block sum added to
+//               an FP64 family accumulator.
 //   PDM_FP32C2: the same bond force with two-float (hi, lo) FP32 family sums, also across
 //               lanes.  The stage update is FP64 in every mode.
 #include <cmath>
