@@ -148,6 +148,28 @@ def test_forces_fp32(case, precision):
         assert err <= 1e-6 and eerr <= 1e-6, (name, err, eerr)
 
 
+@pytest.mark.parametrize("layout", [(2, 4, 1), (3, 8, 4), (5, 4, 2), (7, 8, 1)])
+@pytest.mark.parametrize("precision", [pm.PDM_FP64, pm.PDM_FP32C])
+def test_forces_every_layout(layout, precision):
+    """Tile edge, wavefronts per tile and families per wavefront change the blocks, the
+    lane mapping and the LDS footprint, not the forces."""
+    edge, waves, fams = layout
+    spec = make("bar_delta4")
+    spec["slits"] = [pm.mid_slit(spec, 30.0)]
+    prob, orc = pair(spec, precision=precision, tile_edge=edge, waves_per_tile=waves, families_per_wave=fams)
+    assert prob.tuning()["tile_edge"] == edge
+    pyoracle.set_square_mode(True)
+    try:
+        u = np.random.default_rng(4).uniform(-1e-6, 1e-6, size=(prob.n, 3))
+        F = prob.forces(u)
+        Fo = orc.forces(u)
+        err = np.abs(F - Fo).max() / np.abs(Fo).max()
+        print(f"[forces layout] {layout} precision={precision} err={err:.3e}", flush=True)
+        assert err <= (1e-12 if precision == pm.PDM_FP64 else 1e-6)
+    finally:
+        pyoracle.set_square_mode(False)
+
+
 def test_rk4_fp32c_error():
     spec = make("bar_jitter", steps=40)
     p64 = pm.Problem(pm.deck_text(spec), precision=pm.PDM_FP64)
