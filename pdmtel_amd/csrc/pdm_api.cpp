@@ -540,7 +540,7 @@ int batch_concat(pdm_handle* B)
         halo += m->halo_total;
         hmax = std::max(hmax, m->halo_max);
         tmax = std::max(tmax, m->tile_max);
-        ue = ue && m->uniform_ecrit;
+        ue = ue && m->uniform_mat;
     }
     if (N > INT32_MAX) return fail(B, PDM_E_LIMIT, "a batch holds more than 2^31 - 1 nodes");
     B->n = N;
@@ -551,7 +551,7 @@ int batch_concat(pdm_handle* B)
     B->halo_total = halo;
     B->halo_max = hmax;
     B->tile_max = tmax;
-    B->uniform_ecrit = ue;
+    B->uniform_mat = ue;
     const bool f64 = B->opt.precision == PDM_FP64;
     const int64_t n1 = std::max<int64_t>(N, 1), e1 = std::max<int64_t>(Ei, 1), t1 = std::max<int64_t>(ntl, 1);
     HCHK(hipMalloc(&B->Xi, sizeof(d4) * n1));
@@ -654,7 +654,7 @@ int batch_layout(pdm_handle* B, int t)
     for (pdm_handle* m : B->members) free_tiles(m);
     if (rc) return rc;
     // the stage kernels' LDS footprint (FP64 records are the larger)
-    size_t need = stage_lds_bytes(tile_view(B), !B->uniform_ecrit, true, false);
+    size_t need = stage_lds_bytes(tile_view(B), !B->uniform_mat, true, false);
     if (need > kLdsBytes) return fail(B, PDM_E_LIMIT, "tile LDS footprint exceeds the LDS of a CU");
     return alloc_parts(B);
 }
@@ -867,7 +867,7 @@ int pdm_init(pdm_handle* h)
                         "halo_mean=%.1f lds=%zu B\n",
                 (long long)h->n, (long long)h->E, (long long)h->ntl, h->tile_edge[0], h->tile_edge[1],
                 h->tile_edge[2], h->tile_max, h->halo_max, h->ntl ? (double)h->halo_total / (double)h->ntl : 0.0,
-                stage_lds_bytes(tile_view(h), !h->uniform_ecrit, true, h->opt.precision != PDM_FP64));
+                stage_lds_bytes(tile_view(h), !h->uniform_mat, true, h->opt.precision != PDM_FP64));
     start_clock(h);
     h->inited = true;
     return PDM_OK;

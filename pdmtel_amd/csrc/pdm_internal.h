@@ -85,11 +85,12 @@ struct StepParam {
 // each tile reads its specimen's entry (tile_spec), and its material and BC tables at
 // mats[spec * kMaxMat], bcs[spec * kMaxBC].
 struct SpecDev {
-    double ecrit0;        // common ecrit of the specimen's materials (when uniform_ecrit)
+    double ecrit0;        // common ecrit of the specimen's materials (when uniform_mat)
     double inv_vol_cube;  // 1 / dx^3
     double fscale;        // fixed-point scale of the FP32 halo records
     double damping;       // damping_coeff
     double vol;           // pow(dx, 3): node volume for the energies
+    double kc0;           // dx^3 c / 2 of the specimen's materials (when uniform_mat)
     // beta(r) of the FP32 records (r in record units, S m), offset by the correction bias:
     // med3(beta0 - r beta1, -128 2^-26, 1 - 128 2^-26)
     float beta0, beta1;
@@ -117,7 +118,8 @@ struct TileView {
     int xcd_remap;
     int waves;                 // wavefronts per tile workgroup (4 or 8)
     int fam_per_wave;          // families a wavefront sums at once (1, 2 or 4)
-    int uniform_ecrit;         // every specimen's materials share one ecrit: none per neighbour in LDS
+    int uniform_mat;           // each specimen's materials share one ecrit and one spring constant:
+                               // no per-neighbour ecrit in LDS, and kc0 scales the FP32 family sums
     const SpecDev* specs;      // per-specimen constants
     const int32_t* tile_spec;  // launch index -> specimen (nullptr: one specimen)
 };
@@ -246,7 +248,7 @@ struct pdm_handle {
     int fam_per_wave = 2;           // families a wavefront sums at once
     int tuned = 0;                  // the layout was chosen by timing candidates at pdm_init
     double tune_ms = 0.0;           // stage launch time of the chosen layout (ms)
-    int uniform_ecrit = 1;
+    int uniform_mat = 1;
 
     pdm::d4 *U = nullptr, *V = nullptr, *Vb = nullptr, *k1 = nullptr, *k23 = nullptr;
     double* elast = nullptr;

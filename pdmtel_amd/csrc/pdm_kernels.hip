@@ -460,9 +460,13 @@ __device__ __forceinline__ float fly_beta(float rr, const SelfF& me)
                                    1.0f - 128.0f * kCorrStep);
 }
 
-// (be_i + be_j) min(kc_i, kc_j) from the neighbour record's second double.
+// (be_i + be_j) min(kc_i, kc_j) from the neighbour record's second double; with one
+// material per specimen (UE) only be_i + be_j, and the family sums are scaled by kc0 in
+// FP64 when they are stored.
+template <bool UE>
 __device__ __forceinline__ float fly_base(const SelfF& me, double wy)
 {
+    if (UE) return me.be + lo_f(wy);
     return (me.be + lo_f(wy)) * __builtin_fminf(me.kc, hi_f(wy));
 }
 
@@ -487,7 +491,7 @@ __device__ __forceinline__ Bond32 bond32(const Lds& l, unsigned s, float cu, con
     const float pinv = __builtin_amdgcn_rsqf(p2);
     const float p = p2 * pinv;
     const float st = q * __builtin_amdgcn_rcpf((p + rr) * rr);
-    const float c = fly_base(me, rw.y) * __builtin_fmaf(cu, kCorrStep, fly_beta(rr, me));
+    const float c = fly_base<UE>(me, rw.y) * __builtin_fmaf(cu, kCorrStep, fly_beta(rr, me));
     // u <= 0 || s <= ecrit, with u = s r and r > 0
     float ecrit = me.ecrit;
     if (!UE) {
@@ -864,7 +868,12 @@ __device__ __forceinline__ void family_round(const TileView& tv, const SpecDev& 
         flush<OP>(acc);
     }
     OP::template reduce<LPN>(acc);
-    if (sl == 0 && loc < nt) OP::store(acc, ivc4, l.res + 4 * loc);
+    if (sl == 0 && loc < nt) {
+        double* out = l.res + 4 * loc;
+        OP::store(acc, ivc4, out);
+        if constexpr (!kCoefStream<OP> && UE)
+            for (int c = 0; c < 4; ++c) out[c] *= sp.kc0;  // the coefficient's common factor
+    }
 }
 
 // Family sums of the tile's nodes into l.res.  Round r of wavefront w covers nodes
@@ -1252,17 +1261,19 @@ __global__ __launch_bounds__(256) void k_fly_corr(TileView tv, const d4* __restr
     stage_tile<true>(tv, sp, u, k, h0, H, a0, nt, l, false);
     __syncthreads();
     int bad = 0;
+    const bool ue = tv.uniform_mat != 0;
     for (int loc = 0; loc < nt; ++loc) {
         const SelfF me = SelfF::make(l, sp, l.self[loc], true);
         for (int64_t e = l.row[loc] + threadIdx.x; e < l.row[loc + 1]; e += blockDim.x) {
             const RecF& rec = l.recf[tv.slot[e]];
             float r2, rinv, rr;
             rec_r(rec.a, me, r2, rinv, rr);
-            const float base = fly_base(me, rec.w.y);
+            const float base = ue ? fly_base<true>(me, rec.w.y) : fly_base<false>(me, rec.w.y);
+            const double scale = ue ? sp.kc0 : 1.0;
             const float b32 = fly_beta(rr, me);
             int q = 128;
             if (base > 0.0f) {
-                const double d = rint((tv.coef64[e] / (double)base - (double)b32) / (double)kCorrStep);
+                const double d = rint((tv.coef64[e] / ((double)base * scale) - (double)b32) / (double)kCorrStep);
                 bad += (d < 0.0 || d > 255.0) ? 1 : 0;
                 q = (int)fmin(fmax(d, 0.0), 255.0);
             }
@@ -1490,10 +1501,10 @@ template <int STAGE, int PREC, int NW, int NPW>
 hipError_t launch_stage_e(const StageArgs& a, int step_idx, size_t lds, hipStream_t s)
 {
     if (a.energies) {
-        if (a.tv.uniform_ecrit) return launch_stage_t<STAGE, PREC, true, true, NW, NPW>(a, step_idx, lds, s);
+        if (a.tv.uniform_mat) return launch_stage_t<STAGE, PREC, true, true, NW, NPW>(a, step_idx, lds, s);
         return launch_stage_t<STAGE, PREC, true, false, NW, NPW>(a, step_idx, lds, s);
     }
-    if (a.tv.uniform_ecrit) return launch_stage_t<STAGE, PREC, false, true, NW, NPW>(a, step_idx, lds, s);
+    if (a.tv.uniform_mat) return launch_stage_t<STAGE, PREC, false, true, NW, NPW>(a, step_idx, lds, s);
     return launch_stage_t<STAGE, PREC, false, false, NW, NPW>(a, step_idx, lds, s);
 }
 
@@ -1535,7 +1546,7 @@ hipError_t launch_gather_t(const TileView& tv, const d4* u, d4* y, double* elast
 template <class OP>
 hipError_t launch_gather_u(const TileView& tv, const d4* u, d4* y, double* elast, size_t lds, hipStream_t s)
 {
-    if (tv.uniform_ecrit) return launch_gather_t<OP, true>(tv, u, y, elast, lds, s);
+    if (tv.uniform_mat) return launch_gather_t<OP, true>(tv, u, y, elast, lds, s);
     return launch_gather_t<OP, false>(tv, u, y, elast, lds, s);
 }
 
@@ -1550,7 +1561,7 @@ size_t stage_lds_bytes(const TileView& tv, bool with_ecrit, bool with_epi, bool 
 hipError_t launch_stage(int stage, int precision, const StageArgs& a, int step_idx, hipStream_t s)
 {
     if (a.tv.ntl == 0) return hipSuccess;
-    size_t lds = stage_lds_bytes(a.tv, !a.tv.uniform_ecrit, true, precision != PDM_FP64);
+    size_t lds = stage_lds_bytes(a.tv, !a.tv.uniform_mat, true, precision != PDM_FP64);
     switch (precision) {
     case PDM_FP64: return launch_stage_p<PDM_FP64>(stage, a, step_idx, lds, s);
     case PDM_FP32C: return launch_stage_p<PDM_FP32C>(stage, a, step_idx, lds, s);
@@ -1569,7 +1580,7 @@ hipError_t launch_energy_steps(const double* part, int64_t ntl, const int64_t* s
 hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, double* elast, hipStream_t s)
 {
     if (tv.ntl == 0) return hipSuccess;
-    size_t lds = stage_lds_bytes(tv, !tv.uniform_ecrit, false, precision != PDM_FP64);
+    size_t lds = stage_lds_bytes(tv, !tv.uniform_mat, false, precision != PDM_FP64);
     switch (precision) {
     case PDM_FP64: return launch_gather_u<OpF64>(tv, u, F, elast, lds, s);
     case PDM_FP32C: return launch_gather_u<OpF32>(tv, u, F, elast, lds, s);
@@ -1635,13 +1646,13 @@ hipError_t launch_fly_corr(const TileView& tv, const d4* u, uint8_t* corr, int* 
 hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s)
 {
     if (tv.ntl == 0) return hipSuccess;
-    return launch_gather_u<OpD64>(tv, u, y, nullptr, stage_lds_bytes(tv, !tv.uniform_ecrit, false, false), s);
+    return launch_gather_u<OpD64>(tv, u, y, nullptr, stage_lds_bytes(tv, !tv.uniform_mat, false, false), s);
 }
 
 hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s)
 {
     if (tv.ntl == 0) return hipSuccess;
-    size_t lds = stage_lds_bytes(tv, !tv.uniform_ecrit, false, precision != PDM_FP64);
+    size_t lds = stage_lds_bytes(tv, !tv.uniform_mat, false, precision != PDM_FP64);
     if (precision == PDM_FP64) return launch_gather_u<OpT64>(tv, u, y, nullptr, lds, s);
     return launch_gather_u<OpT32>(tv, u, y, nullptr, lds, s);
 }

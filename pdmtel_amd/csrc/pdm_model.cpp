@@ -206,6 +206,7 @@ void fill_spec_tables(const pdm_handle* h, SpecDev& sp, MatDev* mats, BCDev* bcs
     sp.fscale = h->fscale;
     sp.damping = d.damping;
     sp.vol = std::pow(d.dx, 3.0);
+    sp.kc0 = h->spring.empty() ? 0.0 : 0.5 * (d.dx * d.dx * d.dx) * h->spring[0];
     // beta (pdDataManager3.cpp:1801-1831) in the FP32 record units (m x fscale)
     // (entries with beta 0, including original CCI's beyond delta, read the padding record)
     sp.beta0 = (float)((d.horizon + 0.5 * d.dx) / d.dx - 128.0 * 0x1p-26);

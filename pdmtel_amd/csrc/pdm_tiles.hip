@@ -423,7 +423,7 @@ TileView tile_view(const pdm_handle* h)
     v.xcd_remap = 1;
     v.waves = h->waves;
     v.fam_per_wave = h->fam_per_wave;
-    v.uniform_ecrit = h->uniform_ecrit;
+    v.uniform_mat = h->uniform_mat;
     v.specs = h->specs_d;
     v.tile_spec = h->tile_spec_d;
     return v;
@@ -511,10 +511,10 @@ int build_tiles(pdm_handle* h)
     }
     const int64_t Ei = rpih[n];
     h->Ei = Ei;
-    // uniform ecrit across materials?
-    h->uniform_ecrit = 1;
+    // one ecrit and one spring constant across the materials?
+    h->uniform_mat = 1;
     for (size_t m = 1; m < d.mats.size(); ++m)
-        if (d.mats[m].ecrit != d.mats[0].ecrit) h->uniform_ecrit = 0;
+        if (d.mats[m].ecrit != d.mats[0].ecrit || h->spring[m] != h->spring[0]) h->uniform_mat = 0;
 
     int32_t *ipermd = nullptr, *tileidd = nullptr, *bcref = nullptr;
     double *ecm = nullptr, *bei = nullptr;
@@ -629,7 +629,7 @@ int build_tiles(pdm_handle* h)
     h->halo_max = (int)hmax;
     {
         TileView tv = tile_view(h);
-        size_t need = stage_lds_bytes(tv, !h->uniform_ecrit, true, false);  // FP64 records: the larger
+        size_t need = stage_lds_bytes(tv, !h->uniform_mat, true, false);  // FP64 records: the larger
         if (need > kLdsBytes) {
             h->err = "tile LDS footprint " + std::to_string(need) + " B exceeds " + std::to_string(kLdsBytes) +
                      " B; use a smaller tile_edge";
