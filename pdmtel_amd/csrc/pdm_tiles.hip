@@ -645,7 +645,7 @@ int build_tiles(pdm_handle* h)
         k_halo<true><<<dim3((unsigned)h->ntl), dim3(256), 0, s>>>(ha);
         HIPCHK(hipGetLastError());
     }
-    if (n) {
+    if (n && d.cci_on) {  // the fixed-Gaussian kernels read neighbours from global memory, in id order
         uint16_t* tslot = nullptr;
         uint32_t* tpacked = nullptr;
         HIPCHK(hipMalloc(&tslot, sizeof(uint16_t) * E1));
