@@ -1289,7 +1289,8 @@ __global__ __launch_bounds__(256) void k_fly_corr(TileView tv, const d4* __restr
 // 1293-1351, volume outside) at each Gauss point g of node j, with xi = X_g - X_i, eta =
 // u_g - u_i, beta over the cell size (w1 w2 w3)^(1/3) dx / 2, summed with the weights
 // dx^3/8 w1 w2 w3.  coef64 holds bdryEff_ij (0: cut entry or row padding).  One wavefront per
-// node, lanes over the row's entries; returns (fx, fy, fz, energy density) in every lane.
+// node; its lanes step through the row's (entry, point) pairs, so that consecutive lanes
+// read consecutive Gauss points; returns (fx, fy, fz, energy density) in every lane.
 __device__ __forceinline__ void fgi_family(const TileView& tv, const FgiArgs& f, const double* __restrict__ tab,
                                            const MatDev* mats, const int32_t* __restrict__ mat, int64_t h0, int64_t i,
                                            const d4& ui, double acc[4])
@@ -1298,7 +1299,17 @@ __device__ __forceinline__ void fgi_family(const TileView& tv, const FgiArgs& f,
     const d4 xi = tv.X[i];
     const MatDev& mi = mats[mat[i]];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0, el = 0.0;
-    for (int64_t e = tv.rpi[i] + lane; e < tv.rpi[i + 1]; e += 64) {
+    const int ng = f.ng;
+    const int64_t eb = tv.rpi[i];
+    const int total = (int)(tv.rpi[i + 1] - eb) * ng;
+    const int qe = 64 / ng, rg = 64 % ng;
+    int eo = lane / ng, gg = lane - (lane / ng) * ng;
+    for (int idx = lane; idx < total; idx += 64) {
+        const int64_t e = eb + eo;
+        const int g = gg;
+        eo += qe;
+        gg += rg;
+        if (gg >= ng) { gg -= ng; ++eo; }
         const double bij = tv.coef64[e];
         if (bij == 0.0) continue;
         const int32_t j = tv.halo[h0 + tv.slot[e]];
@@ -1306,24 +1317,21 @@ __device__ __forceinline__ void fgi_family(const TileView& tv, const FgiArgs& f,
         const MatDev& mj = mats[mat[j]];
         const double ecrit = (mj.ecrit < mi.ecrit) ? mj.ecrit : mi.ecrit;
         const double spring = (mj.spring < mi.spring) ? mj.spring : mi.spring;
-        const d4* __restrict__ gj = f.gin + (int64_t)j * f.ng;
-        for (int g = 0; g < f.ng; ++g) {
-            const double* t = tab + kFgiTab * g;
-            const d4 ug = gj[g];
-            const double xi1 = (t[0] + xj.x) - xi.x, xi2 = (t[1] + xj.y) - xi.y, xi3 = (t[2] + xj.z) - xi.z;
-            const double a1 = xi1 + (ug.x - ui.x), a2 = xi2 + (ug.y - ui.y), a3 = xi3 + (ug.z - ui.z);
-            const double r = sqrt(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
-            const double p = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
-            const double beta = g_beta(r, f.horizon, t[4], false);
-            const double u = p - r;
-            const double st = u / r;
-            if (u <= 0.0 || (u > 0.0 && st <= ecrit)) {
-                const double c = bij * beta * spring * st;
-                f0 += t[3] * ((c * a1) / p);
-                f1 += t[3] * ((c * a2) / p);
-                f2 += t[3] * ((c * a3) / p);
-                el += bij * 0.5 * (0.5 * beta * (spring / r) * u * u) * t[5];
-            }
+        const double* t = tab + kFgiTab * g;
+        const d4 ug = f.gin[(int64_t)j * ng + g];
+        const double xi1 = (t[0] + xj.x) - xi.x, xi2 = (t[1] + xj.y) - xi.y, xi3 = (t[2] + xj.z) - xi.z;
+        const double a1 = xi1 + (ug.x - ui.x), a2 = xi2 + (ug.y - ui.y), a3 = xi3 + (ug.z - ui.z);
+        const double r = sqrt(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
+        const double p = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+        const double beta = g_beta(r, f.horizon, t[4], false);
+        const double u = p - r;
+        const double st = u / r;
+        if (u <= 0.0 || (u > 0.0 && st <= ecrit)) {
+            const double c = bij * beta * spring * st;
+            f0 += t[3] * ((c * a1) / p);
+            f1 += t[3] * ((c * a2) / p);
+            f2 += t[3] * ((c * a3) / p);
+            el += bij * 0.5 * (0.5 * beta * (spring / r) * u * u) * t[5];
         }
     }
     acc[0] = wave_sum(f0);
