@@ -25,8 +25,8 @@
 //   PDM_FP32C:  xi and eta formed exactly and rounded to FP32; the stretch from the
 //               cancellation-free form s = (2 xi.eta + eta.eta) / ((p + r) r); the
 //               coefficient formed from the two records and the entry's correction byte;
-//               the bond forces of a lane's block summed in FP32, each block sum added to
-//               an FP64 family accumulator.
+//               each lane's bond forces summed in FP32 and the lane sums of a family
+//               reduced in FP32, then widened to FP64 for the stage update.
 //   PDM_FP32C2: the same bond force with two-float (hi, lo) FP32 family sums, also across
 //               lanes.  The stage update is FP64 in every mode.
 #include <cmath>
@@ -509,12 +509,11 @@ __device__ __forceinline__ Bond32 bond32(const Lds& l, unsigned s, float cu, con
     return b;
 }
 
-struct OpF32 {  // PDM_FP32C: FP32 bond force, FP32 block sums, FP64 family accumulator
+struct OpF32 {  // PDM_FP32C: FP32 bond force, FP32 sums per lane and across the lane group
     using Coef = float;
     static constexpr bool kCoef = false;  // coefficients formed from the records and correction bytes
     using SelfT = SelfF;
     struct Acc {
-        double f0, f1, f2;
         float p0, p1, p2;
         float el;
     };
@@ -527,26 +526,19 @@ struct OpF32 {  // PDM_FP32C: FP32 bond force, FP32 block sums, FP64 family accu
         a.p2 = __builtin_fmaf(b.g, b.a3, a.p2);
         if (ENERGY) a.el = __builtin_fmaf(b.cs, b.e, a.el);
     }
-    __device__ static void flush(Acc& a)
-    {
-        a.f0 += (double)a.p0;
-        a.f1 += (double)a.p1;
-        a.f2 += (double)a.p2;
-        a.p0 = a.p1 = a.p2 = 0.0f;
-    }
     template <int LPN>
     __device__ static void reduce(Acc& a)
     {
-        a.f0 = group_sum<LPN>(a.f0);
-        a.f1 = group_sum<LPN>(a.f1);
-        a.f2 = group_sum<LPN>(a.f2);
+        a.p0 = group_sumf<LPN>(a.p0);
+        a.p1 = group_sumf<LPN>(a.p1);
+        a.p2 = group_sumf<LPN>(a.p2);
         a.el = group_sumf<LPN>(a.el);
     }
     __device__ static void store(const Acc& a, double ivc4, double* out)
     {
-        out[0] = a.f0;
-        out[1] = a.f1;
-        out[2] = a.f2;
+        out[0] = (double)a.p0;
+        out[1] = (double)a.p1;
+        out[2] = (double)a.p2;
         out[3] = (double)a.el * ivc4;
     }
 };
