@@ -22,13 +22,18 @@ section would leave the far half unsupported); matrix-free Jacobi-PCG to 1e-10
 relative residual, FP32 matvecs with FP64 refinement (and the FP64 solve for
 the error figure).  Wall seconds of pdm_solve_static.
 
+value counts bond-updates as the reference performs them: 4 per family entry per
+RK4 step (pdm_num_entries).  The gathers evaluate only the evaluated_entries
+(pdm_num_live_entries): a cut entry or one whose CCI beta is 0 contributes
+exactly 0 to every force and energy, and the body force still counts it.
+
 roofline: dominant kernel = the fused bond-force / RK4-stage kernel.  Its
 average launch time comes from HIP events recorded on the library's stream
 around the timed region (pdm_last_kernel_timing).  Algorithmic bytes per
-launch use the per-unit figures of the bond-update unit: 8 B per family
-entry for FP32 coefficients (int32 neighbour + FP32 coefficient; 12 B with
-FP64 coefficients) plus 264 B per node (trial u, X, Vbuf: 72 B, and the
-fused stage update, 8 vectors of 24 B).  The library's own format is smaller
+launch use the per-unit figures of the bond-update unit over the entries the
+kernel evaluates: 8 B per evaluated entry for FP32 coefficients (int32
+neighbour + FP32 coefficient; 12 B with FP64 coefficients) plus 264 B per
+node (trial u, X, Vbuf: 72 B, and the fused stage update, 8 vectors of 24 B).  The library's own format is smaller
 (uint16 LDS slot + one correction byte from which the FP32 kernels form the
 coefficient: 3 B per entry; uint16 slot + FP64 coefficient: 10 B); `format_bytes`
 gives that figure.  `traffic` is FETCH_SIZE + WRITE_SIZE per launch from rocprofv3
@@ -83,7 +88,7 @@ def run_precision(args, spec, precision):
     assert done == args.steps, f"ran {done} of {args.steps} steps"
     ms_kernel, launches = prob.kernel_timing()
     U, _ = prob.state()
-    res = dict(n=prob.n, E=prob.E, seconds=t1 - t0, init_seconds=t_init, kernel_ms=ms_kernel, launches=launches,
+    res = dict(n=prob.n, E=prob.E, E_live=prob.E_live, seconds=t1 - t0, init_seconds=t_init, kernel_ms=ms_kernel, launches=launches,
                U=U, energy=prob.energy(), tuning=prob.tuning())
     prob.close()
     return res
@@ -191,16 +196,16 @@ def run_batch(args):
         ms, launches = b.kernel_timing()
         U = [b.state(k)[0] for k in range(b.nspec)]
         res[name] = dict(seconds=t1 - t0, init=t_init, kernel_ms=ms, launches=launches, U=U, E=sum(b.E),
-                         n=sum(b.n), tuning=b.tuning())
+                         E_live=sum(b.E_live), n=sum(b.n), tuning=b.tuning())
         print(f"# batch {name}: {args.specimens} specimens, {sum(b.n)} nodes, {sum(b.E)} entries, init {t_init:.1f} s, "
               f"{t1 - t0:.3f} s for {args.steps} steps, stage {ms:.4f} ms", file=sys.stderr, flush=True)
         b.close()
-    E, n = res["fp32c"]["E"], res["fp32c"]["n"]
+    E, n, E_live = res["fp32c"]["E"], res["fp32c"]["n"], res["fp32c"]["E_live"]
     errs = np.array([float(np.abs(a - c).max() / max(np.abs(c).max(), 1e-300))
                      for a, c in zip(res["fp32c"]["U"], res["fp64"]["U"])])
     err = float(errs.max())
     rate = 4.0 * E * args.steps / res["fp32c"]["seconds"]
-    bytes_launch = E * ENTRY_BYTES["fp32c"] + n * NODE_BYTES
+    bytes_launch = E_live * ENTRY_BYTES["fp32c"] + n * NODE_BYTES
     ks = res["fp32c"]["kernel_ms"] * 1e-3
     line = {
         "metric": "bond-updates/s (RK4), mixed vs FP64 at <=1e-6 rel err; static-solve wall s",
@@ -210,7 +215,8 @@ def run_batch(args):
         "config": {"workload": f"batch of {args.specimens} random specimens (seed 0): nodes log-uniform in "
                                f"[1e3, 1e5], aspect ratios U[1,5], delta/dx in {{3.015, 4.015}}, edge crack at "
                                f"U[0,90] deg, E U[50,200] GPa; {n} nodes, {E} family entries; RK4 {args.steps} steps",
-                   "precision": "fp32c", "nodes": n, "family_entries": E, "specimens": args.specimens,
+                   "precision": "fp32c", "nodes": n, "family_entries": E, "evaluated_entries": E_live,
+                   "specimens": args.specimens,
                    "layout": res["fp32c"]["tuning"], "layout_fp64": res["fp64"]["tuning"]},
         "fp64_value": 4.0 * E * args.steps / res["fp64"]["seconds"],
         "fp32c_over_fp64": res["fp64"]["seconds"] / res["fp32c"]["seconds"],
@@ -323,10 +329,10 @@ def main():
               f"kernel {res['fp32c']['kernel_ms']:.4f} ms/launch", file=sys.stderr, flush=True)
     head_prec = "fp32c" if "fp32c" in res else "fp64"
     head = res[head_prec]
-    E, n = head["E"], head["n"]
+    E, n, E_live = head["E"], head["n"], head["E_live"]
     rate = 4.0 * E * args.steps / head["seconds"]
-    bytes_launch = E * ENTRY_BYTES[head_prec] + n * NODE_BYTES
-    format_bytes = E * FORMAT_ENTRY_BYTES[head_prec] + n * NODE_BYTES
+    bytes_launch = E_live * ENTRY_BYTES[head_prec] + n * NODE_BYTES
+    format_bytes = E_live * FORMAT_ENTRY_BYTES[head_prec] + n * NODE_BYTES
     kernel_s = head["kernel_ms"] * 1e-3
     achieved = bytes_launch / kernel_s / 1e9 if kernel_s > 0 else 0.0
     err = None
@@ -354,6 +360,7 @@ def main():
             "precision": head_prec,
             "nodes": n,
             "family_entries": E,
+            "evaluated_entries": E_live,
             "layout": head["tuning"],
             "layout_fp64": res["fp64"]["tuning"] if "fp64" in res else None,
         },
@@ -377,7 +384,7 @@ def main():
     }
     if "fp64" in res and head_prec != "fp64":
         k64 = res["fp64"]["kernel_ms"] * 1e-3
-        b64 = E * ENTRY_BYTES["fp64"] + n * NODE_BYTES
+        b64 = res["fp64"]["E_live"] * ENTRY_BYTES["fp64"] + n * NODE_BYTES
         line["roofline_fp64"] = {"achieved": b64 / k64 / 1e9, "frac": b64 / k64 / 1e9 / HBM_PEAK_GBS,
                                  "kernel_avg_ms": res["fp64"]["kernel_ms"], "bytes_per_launch": b64,
                                  "traffic": recorded_traffic(args, "fp64")}

@@ -117,6 +117,13 @@ int64_t pdm_num_nodes(const pdm_handle* h);
 int64_t pdm_num_bonds(const pdm_handle* h);     /* unique bonds */
 int64_t pdm_num_entries(const pdm_handle* h);   /* directed family entries = 2 x bonds */
 int pdm_max_family(const pdm_handle* h);        /* _numMaxFamily, pdDataManager.cpp:1345-1387 */
+/* Family entries the force gathers evaluate.  Under CCI a cut entry and an entry with
+ * beta = 0 (r >= delta + dx/2) contribute exactly 0 to the force and the elastic energy
+ * (CalcBondForceCCI, pdDataManager2.cpp:1293-1351, scales both by beta), so the gathers
+ * skip them; the body force still counts every unbroken entry.  Fixed Gaussian
+ * integration weights each Gauss point's beta and evaluates every entry.  No reference
+ * counterpart (the reference evaluates all pdm_num_entries). */
+int64_t pdm_num_live_entries(const pdm_handle* h);
 
 /* Node data: X (3N, may be NULL), material index (N, may be NULL),
  * boundary-condition index (N, -1 = none; may be NULL).  pdNode getters, pdNode.h:27-142. */
@@ -218,6 +225,7 @@ int pdm_batch_init(pdm_batch* b);
 int pdm_batch_num_specimens(const pdm_batch* b);
 int64_t pdm_batch_num_nodes(const pdm_batch* b, int spec);    /* spec < 0: all specimens */
 int64_t pdm_batch_num_entries(const pdm_batch* b, int spec);  /* spec < 0: all specimens */
+int64_t pdm_batch_num_live_entries(const pdm_batch* b, int spec);  /* as pdm_num_live_entries */
 
 /* Advance every specimen by up to nsteps steps of its own SolveDynamic loop; *done =
  * the largest count any specimen took. */

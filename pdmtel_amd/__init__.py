@@ -67,7 +67,7 @@ def lib() -> C.CDLL:
     L.pdm_last_error.restype = C.c_char_p
     L.pdm_last_error.argtypes = [vp]
     L.pdm_init.argtypes = [vp]
-    for name in ("pdm_num_nodes", "pdm_num_bonds", "pdm_num_entries"):
+    for name in ("pdm_num_nodes", "pdm_num_bonds", "pdm_num_entries", "pdm_num_live_entries"):
         getattr(L, name).restype = C.c_int64
         getattr(L, name).argtypes = [vp]
     L.pdm_max_family.argtypes = [vp]
@@ -98,7 +98,7 @@ def lib() -> C.CDLL:
     L.pdm_batch_last_error.argtypes = [vp]
     L.pdm_batch_init.argtypes = [vp]
     L.pdm_batch_num_specimens.argtypes = [vp]
-    for name in ("pdm_batch_num_nodes", "pdm_batch_num_entries"):
+    for name in ("pdm_batch_num_nodes", "pdm_batch_num_entries", "pdm_batch_num_live_entries"):
         getattr(L, name).restype = C.c_int64
         getattr(L, name).argtypes = [vp, C.c_int]
     L.pdm_batch_step_rk4.argtypes = [vp, C.c_int, ip]
@@ -150,6 +150,7 @@ class Problem:
         self._chk(self.L.pdm_init(self.h))
         self.n = int(self.L.pdm_num_nodes(self.h))
         self.E = int(self.L.pdm_num_entries(self.h))
+        self.E_live = int(self.L.pdm_num_live_entries(self.h))
 
     def close(self) -> None:
         if getattr(self, "h", None) and self.h.value:
@@ -289,6 +290,7 @@ class Batch:
         self._chk(self.L.pdm_batch_init(self.h))
         self.n = [int(self.L.pdm_batch_num_nodes(self.h, k)) for k in range(self.nspec)]
         self.E = [int(self.L.pdm_batch_num_entries(self.h, k)) for k in range(self.nspec)]
+        self.E_live = [int(self.L.pdm_batch_num_live_entries(self.h, k)) for k in range(self.nspec)]
 
     def close(self) -> None:
         if getattr(self, "h", None) and self.h.value:

@@ -529,11 +529,12 @@ int batch_concat(pdm_handle* B)
     pdm_handle* const h = B;  // for HCHK
     hipStream_t s = B->stream;
     const int S = (int)B->members.size();
-    int64_t N = 0, Ei = 0, E = 0, ntl = 0, ntiles = 0, halo = 0;
+    int64_t N = 0, Ei = 0, E = 0, Elive = 0, ntl = 0, ntiles = 0, halo = 0;
     int hmax = 0, tmax = 0, ue = 1;
     for (pdm_handle* m : B->members) {
         N += m->n;
         Ei += m->Ei;
+        Elive += m->Elive;
         E += m->E;
         ntl += m->ntl;
         ntiles += m->ntiles;
@@ -546,6 +547,7 @@ int batch_concat(pdm_handle* B)
     B->n = N;
     B->E = E;
     B->Ei = Ei;
+    B->Elive = Elive;
     B->ntl = ntl;
     B->ntiles = ntiles;
     B->halo_total = halo;
@@ -876,6 +878,7 @@ int pdm_init(pdm_handle* h)
 int64_t pdm_num_nodes(const pdm_handle* h) { return h ? h->n : -1; }
 int64_t pdm_num_entries(const pdm_handle* h) { return h ? h->E : -1; }
 int64_t pdm_num_bonds(const pdm_handle* h) { return h ? h->E / 2 : -1; }
+int64_t pdm_num_live_entries(const pdm_handle* h) { return h ? h->Elive : -1; }
 int pdm_max_family(const pdm_handle* h) { return h ? h->max_family : -1; }
 
 int pdm_get_nodes(const pdm_handle* h, double* X, int32_t* mat, int32_t* bc)
@@ -1373,6 +1376,13 @@ int64_t pdm_batch_num_entries(const pdm_batch* b, int spec)
     if (!b || !b->h) return -1;
     if (spec < 0) return b->h->E;
     return (spec < b->h->nspec) ? b->h->members[spec]->E : -1;
+}
+
+int64_t pdm_batch_num_live_entries(const pdm_batch* b, int spec)
+{
+    if (!b || !b->h) return -1;
+    if (spec < 0) return b->h->Elive;
+    return (spec < b->h->nspec) ? b->h->members[spec]->Elive : -1;
 }
 
 int pdm_batch_step_rk4(pdm_batch* b, int nsteps, int* done)

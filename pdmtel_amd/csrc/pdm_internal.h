@@ -197,6 +197,7 @@ struct pdm_handle {
     // host copies (reference order)
     int64_t n = 0, E = 0;
     int64_t Ei = 0;                 // internal (padded) family entries
+    int64_t Elive = 0;              // entries the gathers evaluate (CCI: uncut, beta > 0)
     std::vector<double> X;          // 3N AoS
     std::vector<int32_t> mat, bc;
     std::vector<int32_t> perm;      // internal -> reference id

@@ -781,19 +781,19 @@ __device__ __forceinline__ void load8(const TileView& tv, int64_t e, int64_t e1,
     }
 }
 
-// The lane's block of B entries: for B >= 8, sub-blocks of 8 entries at e, e + 8 LPN,
-// e + 16 LPN, ... (e a multiple of 8), so that each load instruction of the lane group
-// covers contiguous memory; for B = 4, the 4 entries at e.  One 16-byte slot load and
-// two 16-byte coefficient loads per 8 entries.
+// The lane's block of B entries at e (a multiple of B): for B = 8 one 16-byte slot
+// load, and one 8-byte correction load or two 16-byte coefficient loads; for B = 4 an
+// 8-byte slot load and a 4-byte correction load or one or two 16-byte coefficient loads.
+// Lane sl of a group reads the block at e0 + B sl, so each load instruction of the group
+// covers contiguous memory.
 template <class OP, int B, int LPN>
 __device__ __forceinline__ void load_block(const TileView& tv, int64_t e, int64_t e1, int pad, Chunk<OP, B>& ch)
 {
     const unsigned pad2 = (unsigned)pad * 0x10001u;
-    if constexpr (B >= 8) {
-#pragma unroll
-        for (int o = 0; o < B; o += 8) load8<OP, B>(tv, e + (int64_t)o * LPN, e1, pad2, ch, o);
+    if constexpr (B == 8) {
+        load8<OP, B>(tv, e, e1, pad2, ch, 0);
     } else {
-        static_assert(B == 4, "blocks of 4, 8 or 16 entries");
+        static_assert(B == 4, "blocks of 4 or 8 entries");
         if (e < e1) {
             const u32x2 sv = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(tv.slot + e));
             ch.sp[0] = sv[0];
@@ -824,39 +824,39 @@ __device__ __forceinline__ void load_block(const TileView& tv, int64_t e, int64_
     }
 }
 
-// Entries per lane block: one block per lane covers 256 entries of a family (an
-// interior family at delta = 3.015 dx holds 250) for the FP32 operations; the FP64
-// operations keep 4 (fewer registers).
-template <class OP, int LPN>
-constexpr int kBlock = (sizeof(typename OP::Coef) == 8) ? 4 : 256 / LPN;
+// A family is read as a first block of kBlock0 entries per lane (8 for the FP32 operations,
+// 4 for FP64, which keep fewer registers), loaded during the previous round, then blocks
+// of 4 entries per lane until the row ends, each loaded while the one before is computed.
+// With 16 lanes per family an FP32 family of up to 128 + 64 m entries takes 1 + m blocks:
+// 192 entry slots for an interior CCI family at delta = 3.015 dx (178 entries with beta > 0).
+template <class OP>
+constexpr int kBlock0 = (sizeof(typename OP::Coef) == 8) ? 4 : 8;
+constexpr int kBlockN = 4;
 
-// First entry of lane sl's block in a row starting at e0.
-template <class OP, int LPN>
-__device__ __forceinline__ int64_t lane_start(int64_t e0, int sl)
-{
-    return e0 + (int64_t)(kBlock<OP, LPN> >= 8 ? 8 : kBlock<OP, LPN>) * sl;
-}
-
-// One round of a lane group: the family of tile node loc (blocks from `cur`, which the
-// caller loaded), with the next round's first block loaded into `nxt` beforehand.
+// One round of a lane group: the family of tile node loc (first block from `cur`, which the
+// caller loaded), with the next round's first block loaded into the caller's other buffer
+// beforehand.
 template <class OP, bool ENERGY, bool UE, int LPN>
 __device__ __forceinline__ void family_round(const TileView& tv, const SpecDev& sp, const Lds& l, int nt, int loc,
-                                             int64_t e0, int64_t e1, int sl, const Chunk<OP, kBlock<OP, LPN>>& cur,
+                                             int64_t e0, int64_t e1, int sl, const Chunk<OP, kBlock0<OP>>& cur,
                                              double ivc4)
 {
-    constexpr int B = kBlock<OP, LPN>;
-    constexpr int CH = LPN * B;
+    constexpr int B0 = kBlock0<OP>;
     const int ss = (loc < nt) ? l.self[loc] : l.pad;
     const typename OP::SelfT me = OP::SelfT::make(l, sp, ss, UE);
     typename OP::Acc acc{};
+    int64_t base = e0 + LPN * B0;
+    Chunk<OP, kBlockN> nx;
+    if (base < e1) load_block<OP, kBlockN, LPN>(tv, base + kBlockN * sl, e1, l.pad, nx);
 #pragma unroll
-    for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, cur.slot(k), cur.coef(k), me, acc);
+    for (int k = 0; k < B0; ++k) OP::template bond<ENERGY, UE>(l, cur.slot(k), cur.coef(k), me, acc);
     flush<OP>(acc);
-    for (int64_t base = e0 + CH; base < e1; base += CH) {
-        Chunk<OP, B> more;
-        load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(base, sl), e1, l.pad, more);
+    while (base < e1) {
+        const Chunk<OP, kBlockN> c = nx;
+        base += LPN * kBlockN;
+        if (base < e1) load_block<OP, kBlockN, LPN>(tv, base + kBlockN * sl, e1, l.pad, nx);
 #pragma unroll
-        for (int k = 0; k < B; ++k) OP::template bond<ENERGY, UE>(l, more.slot(k), more.coef(k), me, acc);
+        for (int k = 0; k < kBlockN; ++k) OP::template bond<ENERGY, UE>(l, c.slot(k), c.coef(k), me, acc);
         flush<OP>(acc);
     }
     OP::template reduce<LPN>(acc);
@@ -876,7 +876,7 @@ template <class OP, bool ENERGY, bool UE, int NW, int NPW>
 __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev& sp, const Lds& l, int nt)
 {
     constexpr int LPN = 64 / NPW;
-    constexpr int B = kBlock<OP, LPN>;
+    constexpr int B = kBlock0<OP>;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane / LPN, sl = lane % LPN;
     // FP32 records are scaled by S: the energy sum coef s (s r S) carries one factor S
@@ -886,7 +886,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev&
     int loc = first + g;
     int64_t e0 = (loc < nt) ? l.row[loc] : 0, e1 = (loc < nt) ? l.row[loc + 1] : 0;
     Chunk<OP, B> c0, c1;
-    load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(e0, sl), e1, l.pad, c0);
+    load_block<OP, B, LPN>(tv, e0 + B * sl, e1, l.pad, c0);
     if constexpr (!kF32Rec<OP>) {
         // FP64 operations: one block buffer and a copy per round (fewer registers)
         for (;;) {
@@ -898,7 +898,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev&
                     n0 = l.row[nloc];
                     n1 = l.row[nloc + 1];
                 }
-                load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(n0, sl), n1, l.pad, c1);
+                load_block<OP, B, LPN>(tv, n0 + B * sl, n1, l.pad, c1);
             }
             family_round<OP, ENERGY, UE, LPN>(tv, sp, l, nt, loc, e0, e1, sl, c0, ivc4);
             if (nfirst >= nt) return;
@@ -919,7 +919,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev&
                 n0 = l.row[nloc];
                 n1 = l.row[nloc + 1];
             }
-            load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(n0, sl), n1, l.pad, c1);
+            load_block<OP, B, LPN>(tv, n0 + B * sl, n1, l.pad, c1);
         }
         family_round<OP, ENERGY, UE, LPN>(tv, sp, l, nt, loc, e0, e1, sl, c0, ivc4);
         if (nfirst >= nt) break;
@@ -937,7 +937,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev&
                 n0 = l.row[nloc];
                 n1 = l.row[nloc + 1];
             }
-            load_block<OP, B, LPN>(tv, lane_start<OP, LPN>(n0, sl), n1, l.pad, c0);
+            load_block<OP, B, LPN>(tv, n0 + B * sl, n1, l.pad, c0);
         }
         family_round<OP, ENERGY, UE, LPN>(tv, sp, l, nt, loc, e0, e1, sl, c1, ivc4);
         if (nfirst >= nt) break;

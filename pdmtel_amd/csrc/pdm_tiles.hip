@@ -104,6 +104,61 @@ __global__ void k_sort_rows_u32(const int64_t* __restrict__ rpi, uint32_t* __res
     for (int t = threadIdx.x; t < len; t += blockDim.x) v[b + t] = sh[t];
 }
 
+// beta of the entry (i, j) from the internal positions, as k_coef evaluates it.
+__device__ __forceinline__ double entry_beta(const d4& xi, const d4& xj, double horizon, double dx, bool original)
+{
+    double x1 = xj.x - xi.x, x2 = xj.y - xi.y, x3 = xj.z - xi.z;
+    double r = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
+    return g_beta(r, horizon, dx, original);
+}
+
+// CCI rows keep only the entries that carry a force.  A cut bond and an entry whose beta
+// is 0 (r >= delta + dx/2) contribute exactly 0 to the force, the elastic energy and the
+// tangent in every state (CalcBondForceCCI, pdDataManager2.cpp:1293-1351, multiplies its
+// force and energy by beta), so the gathers skip them; the body force still counts every
+// unbroken entry, so nb[a] holds that count from the full row.  One wavefront per row;
+// the kept entries move to the front of the row in order (every lane has read its entry
+// before the wavefront writes, and a write never passes the position being read).
+__global__ void k_live_rows(const int64_t* __restrict__ rpi, uint32_t* __restrict__ packed,
+                            const d4* __restrict__ Xi, double horizon, double dx, bool original, int64_t n,
+                            int32_t* __restrict__ nb, int64_t* __restrict__ live)
+{
+    const int64_t a = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (a >= n) return;
+    const int64_t e0 = rpi[a], e1 = rpi[a + 1];
+    const d4 xi = Xi[a];
+    int cnt = 0;
+    int64_t w = e0;
+    for (int64_t b = e0; b < e1; b += 64) {
+        const int64_t e = b + lane;
+        const uint32_t pk = (e < e1) ? packed[e] : kPadEntry;
+        const bool uncut = pk != kPadEntry && (pk & 1u) == 0;
+        const bool keep = uncut && entry_beta(xi, Xi[pk >> 1], horizon, dx, original) != 0.0;
+        cnt += uncut ? 1 : 0;
+        const uint64_t m = __ballot(keep);
+        if (keep) packed[w + __popcll(m & ((1ull << lane) - 1ull))] = pk;
+        w += __popcll(m);
+    }
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    if (lane == 0) {
+        nb[a] = cnt;
+        live[a] = w - e0;
+    }
+}
+
+// dst rows (row pointers rpo, padded) from the first live[a] entries of the src rows.
+__global__ void k_copy_rows(const int64_t* __restrict__ rps, const uint32_t* __restrict__ src,
+                            const int64_t* __restrict__ rpo, const int64_t* __restrict__ live, int64_t n,
+                            uint32_t* __restrict__ dst)
+{
+    const int64_t a = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (a >= n) return;
+    const int64_t s0 = rps[a], o0 = rpo[a], len = rpo[a + 1] - o0, lv = live[a];
+    for (int64_t k = lane; k < len; k += 64) dst[o0 + k] = (k < lv) ? src[s0 + k] : kPadEntry;
+}
+
 struct HaloArgs {
     const int32_t* tiles;
     const int64_t* tile_start;
@@ -120,20 +175,7 @@ struct HaloArgs {
     uint16_t* slot;
     uint16_t* self_slot;
     int* error;
-    // entries whose CCI beta is 0 read the padding record too (beta_mode 1 modified, 2 original
-    // CCI; 0: keep them, as fixed Gaussian integration weights each Gauss point itself)
-    const d4* Xi;
-    double horizon, dx;
-    int beta_mode;
 };
-
-// beta of the entry (i, j) from the internal positions, as k_coef evaluates it.
-__device__ __forceinline__ double entry_beta(const d4& xi, const d4& xj, double horizon, double dx, bool original)
-{
-    double x1 = xj.x - xi.x, x2 = xj.y - xi.y, x3 = xj.z - xi.z;
-    double r = sqrt(x1 * x1 + x2 * x2 + x3 * x3);
-    return g_beta(r, horizon, dx, original);
-}
 
 // One workgroup per tile.  WRITE=false: halo length; WRITE=true: halo list and slots.
 template <bool WRITE>
@@ -247,18 +289,7 @@ __global__ __launch_bounds__(256) void k_halo(HaloArgs a)
             continue;
         }
         int32_t c = (int32_t)(pk >> 1);
-        bool zero = (pk & 1u) != 0;  // a cut bond reads the padding record (coefficient 0)
-        if (!zero && a.beta_mode) {
-            // the row owner: the last node of the tile whose row starts at or before e
-            int64_t lo = a0, hi = a1 - 1;
-            while (lo < hi) {
-                const int64_t mid = (lo + hi + 1) / 2;
-                if (a.rpi[mid] <= e) lo = mid;
-                else hi = mid - 1;
-            }
-            zero = entry_beta(a.Xi[lo], a.Xi[c], a.horizon, a.dx, a.beta_mode == 2) == 0.0;
-        }
-        if (zero) {
+        if (pk & 1u) {  // a cut bond (fixed Gaussian rows keep them) reads the padding record
             a.slot[e] = (uint16_t)H;
             continue;
         }
@@ -280,7 +311,7 @@ struct CoefArgs {
     double* coef64;
     float* coef32;
     BK* bk;
-    int32_t* nb;
+    int32_t* nb;             // unbroken entries per row (null: counted elsewhere)
     int64_t n;
     double horizon, dx, vcube;
     bool original_cci;
@@ -322,16 +353,17 @@ __global__ void k_coef(CoefArgs a)
         cnt += cut ? 0 : 1;
     }
     for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    if (lane == 0) a.nb[i] = cnt;
+    if (lane == 0 && a.nb) a.nb[i] = cnt;
 }
 
-// LDS-bank-aware order of each row (one thread per row).  The FP32 gathers read a row in
-// rounds of 256 entries; at each step, 8 consecutive lanes read the positions
-// 64 B + 8 j + a (j = 0..7) of one group (B, a), and a 48-byte record s starts in bank
-// group 3 s mod 16.  Entries sorted by slot mod 16 are dealt to the positions ordered by
-// (j, group), so equal residues land in different groups and the 8 records one group
-// reads fall in distinct banks.  The order within a row is otherwise free: every entry,
-// padding included, carries its own weight.
+// LDS-bank-aware order of each row (one thread per row).  With 16 lanes per family the
+// FP32 gathers read a row's entries 0..127 as one block of 8 per lane and the rest in
+// blocks of 4 per lane (64 entries per block round).  In one LDS read, 8 consecutive lanes
+// read the positions of one group: 64 b + 8 j + a (j = 0..7) below 128, 128 + 32 b + 4 j + a
+// (j = 0..7) above; a 48-byte record s starts in bank group 3 s mod 16.  Entries sorted by
+// slot mod 16 are dealt to the positions ordered by (j, group), so equal residues land in
+// different groups and the 8 records one group reads fall in distinct banks.  The order
+// within a row is otherwise free: every entry, padding included, carries its own weight.
 __global__ void k_bank_order(const int64_t* __restrict__ rpi, uint16_t* __restrict__ slot,
                              uint32_t* __restrict__ packed, uint16_t* __restrict__ tslot,
                              uint32_t* __restrict__ tpacked, int64_t n)
@@ -353,8 +385,8 @@ __global__ void k_bank_order(const int64_t* __restrict__ rpi, uint16_t* __restri
         tpacked[e0 + q] = packed[e0 + p];
     }
     int q = 0;
-    for (int j = 0; j < 8; ++j)
-        for (int b = 0; 64 * b < L; ++b)
+    for (int j = 0; j < 8; ++j) {
+        for (int b = 0; b < 2; ++b)
             for (int a = 0; a < 8; ++a) {
                 const int p = 64 * b + 8 * j + a;
                 if (p >= L) continue;
@@ -362,6 +394,15 @@ __global__ void k_bank_order(const int64_t* __restrict__ rpi, uint16_t* __restri
                 packed[e0 + p] = tpacked[e0 + q];
                 ++q;
             }
+        for (int b = 0; 128 + 32 * b < L; ++b)
+            for (int a = 0; a < 4; ++a) {
+                const int p = 128 + 32 * b + 4 * j + a;
+                if (p >= L) continue;
+                slot[e0 + p] = tslot[e0 + q];
+                packed[e0 + p] = tpacked[e0 + q];
+                ++q;
+            }
+    }
 }
 
 template <class T>
@@ -552,11 +593,7 @@ int build_tiles(pdm_handle* h)
     HIPCHK(hipMalloc(&halo_len, sizeof(int64_t) * std::max<int64_t>(h->ntl, 1)));
     HIPCHK(hipMalloc(&h->self_slot, sizeof(uint16_t) * n1));
     HIPCHK(hipMalloc(&packed, sizeof(uint32_t) * E1));
-    HIPCHK(hipMalloc(&h->slot, sizeof(uint16_t) * E1));
-    HIPCHK(hipMalloc(&h->coef64, sizeof(double) * E1));
-    HIPCHK(hipMalloc(&h->coef32, sizeof(float) * E1));
     HIPCHK(hipMalloc(&h->bk, sizeof(BK) * n1));
-    HIPCHK(hipMalloc(&h->corr, E1));
     HIPCHK(hipMalloc(&err, sizeof(int)));
     double ech[kMaxMat] = {0};
     for (size_t m = 0; m < d.mats.size(); ++m) ech[m] = d.mats[m].ecrit;
@@ -589,6 +626,54 @@ int build_tiles(pdm_handle* h)
         k_sort_rows_u32<<<dim3((unsigned)n), dim3(256), 0, s>>>(h->rpi, packed, n);
         HIPCHK(hipGetLastError());
     }
+    // CCI: rows of the entries that carry a force (fixed Gaussian integration weights each
+    // Gauss point's beta itself and keeps every entry)
+    const bool compact = d.cci_on && n > 0;
+    h->Elive = h->E;
+    if (compact) {
+        int64_t *live = nullptr, *rpo = nullptr;
+        uint32_t* packed2 = nullptr;
+        struct Local {
+            hipStream_t s;
+            int64_t** a;
+            int64_t** b;
+            uint32_t** c;
+            ~Local()
+            {
+                (void)hipStreamSynchronize(s);
+                if (*a) (void)hipFree(*a);
+                if (*b) (void)hipFree(*b);
+                if (*c) (void)hipFree(*c);
+            }
+        } local{s, &live, &rpo, &packed2};
+        HIPCHK(hipMalloc(&live, sizeof(int64_t) * n));
+        k_live_rows<<<dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s>>>(h->rpi, packed, h->Xi, d.horizon, d.dx,
+                                                                         !d.cci_mdf, n, h->nb, live);
+        HIPCHK(hipGetLastError());
+        std::vector<int64_t> lv(n);
+        HIPCHK(hipMemcpyAsync(lv.data(), live, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<int64_t> rpo_h(n + 1, 0);
+        for (int64_t a = 0; a < n; ++a)
+            rpo_h[a + 1] = rpo_h[a] + (lv[a] + kRowAlign - 1) / kRowAlign * kRowAlign;
+        const int64_t E2 = rpo_h[n];
+        HIPCHK(hipMalloc(&rpo, sizeof(int64_t) * (n + 1)));
+        HIPCHK(hipMalloc(&packed2, sizeof(uint32_t) * std::max<int64_t>(E2, 1)));
+        HIPCHK(hipMemcpyAsync(rpo, rpo_h.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
+        k_copy_rows<<<dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s>>>(h->rpi, packed, rpo, live, n, packed2);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(h->rpi, rpo_h.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::swap(packed, packed2);  // the full rows are released with the other temporaries
+        h->Ei = E2;
+        h->Elive = 0;
+        for (int64_t a = 0; a < n; ++a) h->Elive += lv[a];
+    }
+    const int64_t Ec = std::max<int64_t>(h->Ei, 1);
+    HIPCHK(hipMalloc(&h->slot, sizeof(uint16_t) * Ec));
+    HIPCHK(hipMalloc(&h->coef64, sizeof(double) * Ec));
+    HIPCHK(hipMalloc(&h->coef32, sizeof(float) * Ec));
+    HIPCHK(hipMalloc(&h->corr, Ec));
     HaloArgs ha{};
     ha.tiles = h->tiles;
     ha.tile_start = h->tile_start;
@@ -599,10 +684,6 @@ int build_tiles(pdm_handle* h)
     ha.g = g;
     ha.halo_len = halo_len;
     ha.error = err;
-    ha.Xi = h->Xi;
-    ha.horizon = d.horizon;
-    ha.dx = d.dx;
-    ha.beta_mode = !d.cci_on ? 0 : (d.cci_mdf ? 1 : 2);
     if (h->ntl) {
         k_halo<false><<<dim3((unsigned)h->ntl), dim3(256), 0, s>>>(ha);
         HIPCHK(hipGetLastError());
@@ -648,8 +729,8 @@ int build_tiles(pdm_handle* h)
     if (n && d.cci_on) {  // the fixed-Gaussian kernels read neighbours from global memory, in id order
         uint16_t* tslot = nullptr;
         uint32_t* tpacked = nullptr;
-        HIPCHK(hipMalloc(&tslot, sizeof(uint16_t) * E1));
-        HIPCHK(hipMalloc(&tpacked, sizeof(uint32_t) * E1));
+        HIPCHK(hipMalloc(&tslot, sizeof(uint16_t) * Ec));
+        HIPCHK(hipMalloc(&tpacked, sizeof(uint32_t) * Ec));
         k_bank_order<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s>>>(h->rpi, h->slot, packed, tslot, tpacked, n);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s));
@@ -665,7 +746,7 @@ int build_tiles(pdm_handle* h)
     ca.coef64 = h->coef64;
     ca.coef32 = h->coef32;
     ca.bk = h->bk;
-    ca.nb = h->nb;
+    ca.nb = compact ? nullptr : h->nb;  // compacted rows: nb was counted from the full rows
     ca.n = n;
     ca.horizon = d.horizon;
     ca.dx = d.dx;

@@ -66,6 +66,50 @@ def test_bond_set_with_slits():
     assert np.array_equal(prob.bdryeff(), orc.bdryeff())
 
 
+@pytest.mark.parametrize("case", ["bar_jitter", "bar_delta4", "bar_cci_original"])
+def test_live_entries(case):
+    """The gathers evaluate the uncut entries whose CCI beta is nonzero: r < delta + dx/2
+    (modified beta), r <= delta (original), with r formed as the library forms it."""
+    spec = make(case)
+    spec["slits"] = [pm.mid_slit(spec, 30.0)]
+    prob = pm.Problem(pm.deck_text(spec), precision=pm.PDM_FP64)
+    rp, col, cut = prob.csr()
+    X, _, _ = prob.nodes()
+    X = X.reshape(-1, 3)
+    row = np.repeat(np.arange(prob.n), np.diff(rp))
+    d = X[col] - X[row]
+    r = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+    dx, delta = spec["grid_spacing"], spec["horizon"]
+    outer = delta if spec["cci"][1] == 0 else delta + 0.5 * dx
+    beta_pos = (r <= outer) & (delta + 0.5 * dx - r > 0.0)
+    want = int(((cut == 0) & beta_pos).sum())
+    print(f"[live entries] {case} E={prob.E} live={prob.E_live} want={want}", flush=True)
+    assert cut.sum() > 0 and want < prob.E
+    assert prob.E_live == want
+
+
+def test_rk4_gravity_with_slits():
+    """Gravity acts once per unbroken family entry (pdDataManager.cpp:1767-1992), entries with
+    beta = 0 included: the count survives the gathers skipping those entries."""
+    spec = make("bar_jitter", steps=30)
+    spec["slits"] = [pm.mid_slit(spec, 30.0)]
+    spec["materials"][0]["gravity"] = (3.0e4, -2.0e4, 1.0e4)
+    prob, orc = pair(spec, use_graph=0)
+    pyoracle.set_square_mode(True)
+    try:
+        for chunk in (2, 5):
+            prob.step(chunk)
+            orc.run(chunk)
+            U, V = prob.state()
+            Uo, Vo = orc.state()
+            eu = np.abs(U - Uo).max() / np.abs(Uo).max()
+            ev = np.abs(V - Vo).max() / np.abs(Vo).max()
+            print(f"[rk4 gravity] step={prob.clock()[1]} eU={eu:.3e} eV={ev:.3e}", flush=True)
+            assert eu <= 1e-11 and ev <= 1e-11
+    finally:
+        pyoracle.set_square_mode(False)
+
+
 @pytest.mark.parametrize("case", ["bar_12x6x6", "bar_jitter", "bar_delta4", "bar_cci_original"])
 def test_forces_fp64(case):
     spec = make(case)
