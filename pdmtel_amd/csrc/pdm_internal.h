@@ -16,9 +16,11 @@
 //    Xi d4[N]      position, .w = ecrit of the node's material
 //    u[2] d4[N]    displacement gathered by the next stage (ping-pong)
 //    rpi int64[N+1], slot uint16[Ei], coef64 FP64[Ei] / coef32 FP32[Ei]
-//      rows padded to a multiple of kRowAlign entries (Ei >= E); padding entries and
-//      cut bonds point at the tile's padding record, with coef 0
-//      coef = bdryEff_ij * dx^3 * beta(r_ij) * c_ij, 0 for a cut bond
+//      CCI rows hold the entries with beta > 0 and no cut (Elive of them; the others
+//      carry no force), fixed-Gaussian rows every entry; rows padded to a multiple of
+//      kRowAlign entries; padding entries and fixed-Gaussian cut bonds point at the
+//      tile's padding record, with coef 0
+//      coef = bdryEff_ij * dx^3 * beta(r_ij) * c_ij
 //    bk BK[N]      bdryEff and dx^3 c / 2 in FP32, and corr uint8[Ei]: the FP32 force
 //                  kernels form each entry's coefficient from these and beta(r), corrected
 //                  per entry by (corr - 128) 2^-26, instead of reading coef32

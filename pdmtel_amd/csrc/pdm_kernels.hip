@@ -439,8 +439,8 @@ struct Bond32 {
 // takes it to the FP64 coefficient within 2^-27 (k_fly_corr): one byte per entry instead
 // of an FP32 coefficient, and the correction also absorbs the FP32 rounding of the
 // constants and of be, kc.  The -128 2^-26 offset is folded into beta_32's constant and
-// clamp bounds.  Entries with coefficient 0 (cut bonds, beta 0, row padding) read the
-// padding record (be = kc = 0).
+// clamp bounds.  CCI rows hold no entry with coefficient 0 (cut bonds and beta 0 are
+// left out when the rows are built); row padding reads the padding record (be = kc = 0).
 constexpr float kCorrStep = 0x1p-26f;
 
 // r^2, 1/r and r of a record pair, in record units (S m).

@@ -5,9 +5,10 @@
 // CSR is the reference family mapped to internal ids and sorted.  For every
 // tile a halo list holds the internal ids of the tile's nodes and all their
 // neighbours, sorted; a family entry stores the uint16 position of its
-// neighbour in that list.  Rows are padded to a multiple of kRowAlign entries
-// (padding entries point at the tile's padding record with a zero coefficient) so
-// that a lane can load a whole aligned block of slots and coefficients at once.
+// neighbour in that list.  CCI rows keep only the entries that carry a force (no
+// cut, beta > 0).  Rows are padded to a multiple of kRowAlign entries (padding
+// entries point at the tile's padding record with a zero coefficient) so that a
+// lane can load a whole aligned block of slots and coefficients at once.
 // The halo of a tile lies in the (2R+1)^3 block of
 // tiles around it, R = ceil(reach / edge) per axis, reach = delta + dx/2 plus
 // a margin: an accepted pair has dmin < delta + 1e-4 dx, and dmin is at least
