@@ -121,8 +121,8 @@ int pdm_max_family(const pdm_handle* h);        /* _numMaxFamily, pdDataManager.
  * beta = 0 (r >= delta + dx/2) contribute exactly 0 to the force and the elastic energy
  * (CalcBondForceCCI, pdDataManager2.cpp:1293-1351, scales both by beta), so the gathers
  * skip them; the body force still counts every unbroken entry.  Fixed Gaussian
- * integration weights each Gauss point's beta and evaluates every entry.  No reference
- * counterpart (the reference evaluates all pdm_num_entries). */
+ * integration weights each Gauss point's beta and evaluates every uncut entry.  No
+ * reference counterpart (the reference evaluates all pdm_num_entries). */
 int64_t pdm_num_live_entries(const pdm_handle* h);
 
 /* Node data: X (3N, may be NULL), material index (N, may be NULL),

@@ -760,6 +760,12 @@ int build_tiles(pdm_handle* h)
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s));
+    if (!compact && n) {  // fixed Gaussian integration evaluates the uncut entries
+        std::vector<int32_t> nbh(n);
+        HIPCHK(hipMemcpy(nbh.data(), h->nb, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        h->Elive = 0;
+        for (int64_t a = 0; a < n; ++a) h->Elive += nbh[a];
+    }
     HIPCHK(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
     if (herr) {
         h->err = "a family entry lies outside its tile neighbourhood";

@@ -117,6 +117,16 @@ def test_graph_matches_eager_and_set_state():
         pyoracle.set_square_mode(False)
 
 
+def test_live_entries_are_the_uncut_ones():
+    """Each Gauss point carries its own beta, so only cut entries are left out."""
+    spec = fgi_spec(2)
+    spec["slits"] = [pm.mid_slit(spec, 30.0)]
+    prob = pm.Problem(pm.deck_text(spec))
+    _, _, cut = prob.csr()
+    assert cut.sum() > 0
+    assert prob.E_live == prob.E - int(cut.sum())
+
+
 def test_unsupported_combinations_fail_loudly():
     spec = fgi_spec(2)
     text = pm.deck_text(spec)
