@@ -39,8 +39,8 @@ extern "C" {
 
 /* precision of the bond-force gather and of CG matvecs */
 #define PDM_FP64 0          /* reference arithmetic in FP64 */
-#define PDM_FP32C 1         /* FP32 bond force (cancellation-free stretch), family sums in an FP64
-                               accumulator, FP64 state */
+#define PDM_FP32C 1         /* FP32 bond force (cancellation-free stretch), FP32 family sums,
+                               FP64 state */
 #define PDM_FP32C2 2        /* as PDM_FP32C with two-float (hi, lo) FP32 family sums */
 
 typedef struct pdm_handle pdm_handle;
@@ -120,9 +120,10 @@ int pdm_max_family(const pdm_handle* h);        /* _numMaxFamily, pdDataManager.
 /* Family entries the force gathers evaluate.  Under CCI a cut entry and an entry with
  * beta = 0 (r >= delta + dx/2) contribute exactly 0 to the force and the elastic energy
  * (CalcBondForceCCI, pdDataManager2.cpp:1293-1351, scales both by beta), so the gathers
- * skip them; the body force still counts every unbroken entry.  Fixed Gaussian
- * integration weights each Gauss point's beta and evaluates every uncut entry.  No
- * reference counterpart (the reference evaluates all pdm_num_entries). */
+ * skip them; the body force still counts every unbroken entry.  Under fixed Gaussian
+ * integration each Gauss point has its own beta: the uncut entries with at least one
+ * point of nonzero beta (see pdm_num_live_gauss_pairs).  No reference counterpart (the
+ * reference evaluates all pdm_num_entries). */
 int64_t pdm_num_live_entries(const pdm_handle* h);
 
 /* Node data: X (3N, may be NULL), material index (N, may be NULL),
@@ -155,6 +156,11 @@ int pdm_compute_forces(pdm_handle* h, const double* u, double* F, double* elast)
  * pdGauss getters, pdNode.h:100-102, pdGauss.h).  Xg or Ug may be NULL. */
 int pdm_num_gauss_points(const pdm_handle* h);
 int pdm_get_gauss(const pdm_handle* h, double* Xg, double* Ug);
+/* Fixed Gaussian integration: (entry, Gauss point) pairs the gathers evaluate, those of
+ * uncut entries whose point has beta > 0 (0 for CCI).  A pair with beta = 0 adds exactly 0
+ * to the force and the energy (pdDataManager.cpp:1918-1957 scales both by beta).  No
+ * reference counterpart (the reference evaluates every pair); for throughput accounting. */
+int64_t pdm_num_live_gauss_pairs(const pdm_handle* h);
 
 /* Set / get the dynamic state U, V (3N each, host; NULL skips). */
 int pdm_set_state(pdm_handle* h, const double* U, const double* V);

@@ -89,6 +89,8 @@ def lib() -> C.CDLL:
     ip = C.POINTER(C.c_int)
     L.pdm_get_tuning.argtypes = [vp, ip, ip, ip, dp]
     L.pdm_num_gauss_points.argtypes = [vp]
+    L.pdm_num_live_gauss_pairs.restype = C.c_int64
+    L.pdm_num_live_gauss_pairs.argtypes = [vp]
     L.pdm_get_gauss.argtypes = [vp, dp, dp]
     cpp = C.POINTER(C.c_char_p)
     L.pdm_batch_create.argtypes = [cpp, C.c_int, C.POINTER(Options), C.POINTER(vp)]
@@ -200,6 +202,11 @@ class Problem:
         el = np.zeros(self.n)
         self._chk(self.L.pdm_compute_forces(self.h, _dp(u), _dp(F), _dp(el)))
         return (F.reshape(-1, 3), el) if with_elast else F.reshape(-1, 3)
+
+    @property
+    def live_gauss_pairs(self) -> int:
+        """(entry, Gauss point) pairs the fixed-Gaussian gathers evaluate (0 for CCI)."""
+        return int(self.L.pdm_num_live_gauss_pairs(self.h))
 
     def gauss(self):
         """Gauss-point positions and displacements, each (N, n^3, 3); None for a CCI deck."""

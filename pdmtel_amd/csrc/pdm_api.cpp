@@ -964,6 +964,7 @@ int pdm_compute_forces(pdm_handle* h, const double* u, double* F, double* elast)
 }
 
 int pdm_num_gauss_points(const pdm_handle* h) { return h ? h->fgi_ng : -1; }
+int64_t pdm_num_live_gauss_pairs(const pdm_handle* h) { return h ? h->fgi_npairs : -1; }
 
 int pdm_get_gauss(const pdm_handle* h, double* Xg, double* Ug)
 {

@@ -246,13 +246,14 @@ __global__ __launch_bounds__(256) void k_mls_coef(MlsArgs m)
 
 // Bytes above which the MLS coefficient table is refused.
 constexpr double kMaxPhiBytes = 96.0 * 1024 * 1024 * 1024;
+constexpr double kMaxPairBytes = 64.0 * 1024 * 1024 * 1024;
 
 }  // namespace
 
 void free_fgi(pdm_handle* h)
 {
     for (void* p : {(void*)h->fgi_tab, (void*)h->gauss, (void*)h->fgi_trial, (void*)h->fgi_u0, (void*)h->cb_ptr,
-                    (void*)h->cb, (void*)h->phi})
+                    (void*)h->cb, (void*)h->phi, (void*)h->fgi_pair_ptr, (void*)h->fgi_pairs})
         if (p) (void)hipFree(p);
     h->fgi_tab = nullptr;
     h->gauss = nullptr;
@@ -261,6 +262,9 @@ void free_fgi(pdm_handle* h)
     h->cb_ptr = nullptr;
     h->cb = nullptr;
     h->phi = nullptr;
+    h->fgi_pair_ptr = nullptr;
+    h->fgi_pairs = nullptr;
+    h->fgi_npairs = 0;
 }
 
 // Gauss points, their table and (n > 1) the MLS lists and coefficients, after the tile build.
@@ -301,6 +305,38 @@ int build_fgi(pdm_handle* h)
     for (d4** p : {&h->fgi_trial, &h->fgi_u0}) {
         HIPCHK(hipMalloc(p, sizeof(d4) * std::max<int64_t>(n, 1)));
         HIPCHK(hipMemsetAsync(*p, 0, sizeof(d4) * std::max<int64_t>(n, 1), s));
+    }
+    {
+        // live (entry, Gauss point) pairs: the gathers skip the pairs whose beta is 0
+        HIPCHK(hipMalloc(&h->fgi_pair_ptr, sizeof(int64_t) * (n + 1)));
+        int64_t *cnt = nullptr, *ent = nullptr;
+        HIPCHK(hipMalloc(&cnt, sizeof(int64_t) * std::max<int64_t>(n, 1)));
+        HIPCHK(hipMalloc(&ent, sizeof(int64_t) * std::max<int64_t>(n, 1)));
+        const TileView tv = tile_view(h);
+        FgiArgs f = fgi_args(h);
+        hipError_t e = launch_fgi_pairs(tv, f, false, cnt, ent, nullptr, s);
+        int rc = (e == hipSuccess) ? scan_exclusive(h, cnt, h->fgi_pair_ptr, n) : PDM_E_HIP;
+        std::vector<int64_t> enth(n);
+        if (rc == PDM_OK && n) {
+            e = hipMemcpy(enth.data(), ent, sizeof(int64_t) * n, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(&h->fgi_npairs, h->fgi_pair_ptr + n, sizeof(int64_t), hipMemcpyDeviceToHost);
+        }
+        (void)hipFree(cnt);
+        (void)hipFree(ent);
+        if (e != hipSuccess) {
+            h->err = std::string("HIP error ") + hipGetErrorString(e) + " in the Gauss-point pair lists";
+            return PDM_E_HIP;
+        }
+        if (rc) return rc;
+        if ((double)h->fgi_npairs * sizeof(uint32_t) > kMaxPairBytes) {
+            h->err = "Gauss-point pair lists exceed 64 GiB (lower intg_point)";
+            return PDM_E_LIMIT;
+        }
+        h->Elive = 0;
+        for (int64_t a2 = 0; a2 < n; ++a2) h->Elive += enth[a2];
+        HIPCHK(hipMalloc(&h->fgi_pairs, sizeof(uint32_t) * std::max<int64_t>(h->fgi_npairs, 1)));
+        f = fgi_args(h);
+        HIPCHK(launch_fgi_pairs(tv, f, true, nullptr, nullptr, h->fgi_pairs, s));
     }
     if (n1 == 1 || n == 0) {
         HIPCHK(hipStreamSynchronize(s));
@@ -387,6 +423,8 @@ FgiArgs fgi_args(const pdm_handle* h)
     f.n = h->n;
     f.trial = h->fgi_trial;
     f.u0 = h->fgi_u0;
+    f.pair_ptr = h->fgi_pair_ptr;
+    f.pairs = h->fgi_pairs;
     return f;
 }
 

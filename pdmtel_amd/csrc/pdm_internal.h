@@ -171,6 +171,8 @@ struct FgiArgs {
     int64_t n;
     d4* trial;              // RK4 trial displacement of every node (stages 1-3 outputs)
     d4* u0;                 // U at the start of the step (BC nodes)
+    const int64_t* pair_ptr;  // [N + 1] live (entry, Gauss point) pairs of each internal row
+    const uint32_t* pairs;    // (entry offset in the row) << 16 | point
 };
 
 // Bond-build parameters.
@@ -277,6 +279,9 @@ struct pdm_handle {
     int64_t* cb_ptr = nullptr;         // MLS contributing nodes (n > 1)
     int32_t* cb = nullptr;
     double* phi = nullptr;
+    int64_t* fgi_pair_ptr = nullptr;   // [N + 1] live (entry, Gauss point) pairs per internal row
+    uint32_t* fgi_pairs = nullptr;
+    int64_t fgi_npairs = 0;
 
     // graph cache
     hipGraphExec_t graph_exec = nullptr;
@@ -323,6 +328,10 @@ FgiArgs fgi_args(const pdm_handle* h);
 hipError_t launch_fgi_gauss(const FgiArgs& f, const d4* u, int step_idx, hipStream_t s);
 hipError_t launch_fgi_stage(int stage, const StageArgs& a, const FgiArgs& f, int step_idx, hipStream_t s);
 hipError_t launch_fgi_forces(const StageArgs& a, const FgiArgs& f, const d4* u, d4* F, double* elast, hipStream_t s);
+// Live-pair lists: fill = false counts each row's live pairs (cnt) and the entries that hold
+// one (ent); fill = true writes them at f.pair_ptr.
+hipError_t launch_fgi_pairs(const TileView& tv, const FgiArgs& f, bool fill, int64_t* cnt, int64_t* ent,
+                            uint32_t* pairs, hipStream_t s);
 
 // static solve (pdm_cg.cpp / pdm_cg.hip)
 int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_stats* stats);

@@ -1281,8 +1281,9 @@ __global__ __launch_bounds__(256) void k_fly_corr(TileView tv, const d4* __restr
 // 1293-1351, volume outside) at each Gauss point g of node j, with xi = X_g - X_i, eta =
 // u_g - u_i, beta over the cell size (w1 w2 w3)^(1/3) dx / 2, summed with the weights
 // dx^3/8 w1 w2 w3.  coef64 holds bdryEff_ij (0: cut entry or row padding).  One wavefront per
-// node; its lanes step through the row's (entry, point) pairs, so that consecutive lanes
-// read consecutive Gauss points; returns (fx, fy, fz, energy density) in every lane.
+// node; its lanes step through the row's live (entry, point) pairs (f.pairs: uncut entries,
+// beta > 0), so that consecutive lanes read consecutive Gauss points; a pair left out adds
+// exactly 0 to every sum.  Returns (fx, fy, fz, energy density) in every lane.
 __device__ __forceinline__ void fgi_family(const TileView& tv, const FgiArgs& f, const double* __restrict__ tab,
                                            const MatDev* mats, const int32_t* __restrict__ mat, int64_t h0, int64_t i,
                                            const d4& ui, double acc[4])
@@ -1293,17 +1294,12 @@ __device__ __forceinline__ void fgi_family(const TileView& tv, const FgiArgs& f,
     double f0 = 0.0, f1 = 0.0, f2 = 0.0, el = 0.0;
     const int ng = f.ng;
     const int64_t eb = tv.rpi[i];
-    const int total = (int)(tv.rpi[i + 1] - eb) * ng;
-    const int qe = 64 / ng, rg = 64 % ng;
-    int eo = lane / ng, gg = lane - (lane / ng) * ng;
-    for (int idx = lane; idx < total; idx += 64) {
-        const int64_t e = eb + eo;
-        const int g = gg;
-        eo += qe;
-        gg += rg;
-        if (gg >= ng) { gg -= ng; ++eo; }
+    const int64_t p1 = f.pair_ptr[i + 1];
+    for (int64_t idx = f.pair_ptr[i] + lane; idx < p1; idx += 64) {
+        const uint32_t pk = f.pairs[idx];
+        const int64_t e = eb + (int64_t)(pk >> 16);
+        const int g = (int)(pk & 0xFFFFu);
         const double bij = tv.coef64[e];
-        if (bij == 0.0) continue;
         const int32_t j = tv.halo[h0 + tv.slot[e]];
         const d4 xj = tv.X[j];
         const MatDev& mj = mats[mat[j]];
@@ -1473,6 +1469,68 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_forces(StageArgs a, FgiA
     }
 }
 
+// Live (entry, Gauss point) pairs of each node's row, for fgi_family: an uncut entry
+// (coefficient != 0) and a point whose beta is nonzero, with r formed exactly as fgi_family
+// forms it.  One workgroup per tile, one wavefront per node, one lane per entry; the pairs
+// are written in (entry, point) order.
+__device__ __forceinline__ bool fgi_point_live(const FgiArgs& f, int g, const d4& xi, const d4& xj)
+{
+    const double* t = f.tab + kFgiTab * g;
+    const double xi1 = (t[0] + xj.x) - xi.x, xi2 = (t[1] + xj.y) - xi.y, xi3 = (t[2] + xj.z) - xi.z;
+    const double r = sqrt(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
+    return g_beta(r, f.horizon, t[4], false) != 0.0;
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_pairs(TileView tv, FgiArgs f, int64_t* __restrict__ cnt,
+                                                              int64_t* __restrict__ ent, uint32_t* __restrict__ pairs)
+{
+    const int64_t k = blockIdx.x;
+    const int32_t tid = tv.tiles[k];
+    const int64_t a0 = tv.tile_start[tid];
+    const int nt = (int)(tv.tile_start[tid + 1] - a0);
+    const int64_t h0 = tv.halo_ptr[k];
+    const int lane = threadIdx.x & 63;
+    const int ng = f.ng;
+    for (int loc = threadIdx.x >> 6; loc < nt; loc += kFgiWaves) {
+        const int64_t i = a0 + loc;
+        const d4 xi = tv.X[i];
+        const int64_t eb = tv.rpi[i], ee = tv.rpi[i + 1];
+        int64_t base = FILL ? f.pair_ptr[i] : 0;
+        int64_t rc = 0, re = 0;
+        for (int64_t e0 = eb; e0 < ee; e0 += 64) {
+            const int64_t e = e0 + lane;
+            const bool uncut = e < ee && tv.coef64[e] != 0.0;
+            d4 xj = xi;
+            if (uncut) xj = tv.X[tv.halo[h0 + tv.slot[e]]];  // cut and padding entries hold the padding slot
+            int c = 0;
+            if (uncut)
+                for (int g = 0; g < ng; ++g) c += fgi_point_live(f, g, xi, xj) ? 1 : 0;
+            // exclusive prefix of the lanes' counts
+            int incl = c;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int v = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += v;
+            }
+            const int total = __shfl(incl, 63, 64);
+            if (FILL && c > 0) {
+                int64_t o = base + (incl - c);
+                const uint32_t eo = (uint32_t)(e - eb) << 16;
+                for (int g = 0; g < ng; ++g)
+                    if (fgi_point_live(f, g, xi, xj)) pairs[o++] = eo | (uint32_t)g;
+            }
+            base += total;
+            rc += total;
+            re += __popcll(__ballot(c > 0));
+        }
+        if (!FILL && lane == 0) {
+            cnt[i] = rc;
+            ent[i] = re;
+        }
+    }
+}
+
+
 int64_t padded_grid(int64_t ntl) { return (ntl + 7) / 8 * 8; }
 
 // Raise a kernel's dynamic-LDS limit once per instantiation (gfx950 has 160 KB per CU).
@@ -1630,6 +1688,17 @@ hipError_t launch_fgi_forces(const StageArgs& a, const FgiArgs& f, const d4* u, 
     hipError_t e = set_lds<k_fgi_forces>(lds);
     if (e != hipSuccess) return e;
     k_fgi_forces<<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, u, F, elast);
+    return hipGetLastError();
+}
+
+hipError_t launch_fgi_pairs(const TileView& tv, const FgiArgs& f, bool fill, int64_t* cnt, int64_t* ent,
+                            uint32_t* pairs, hipStream_t s)
+{
+    if (tv.ntl == 0) return hipSuccess;
+    if (fill)
+        k_fgi_pairs<true><<<dim3((unsigned)tv.ntl), dim3(64 * kFgiWaves), 0, s>>>(tv, f, cnt, ent, pairs);
+    else
+        k_fgi_pairs<false><<<dim3((unsigned)tv.ntl), dim3(64 * kFgiWaves), 0, s>>>(tv, f, cnt, ent, pairs);
     return hipGetLastError();
 }
 

@@ -117,14 +117,31 @@ def test_graph_matches_eager_and_set_state():
         pyoracle.set_square_mode(False)
 
 
-def test_live_entries_are_the_uncut_ones():
-    """Each Gauss point carries its own beta, so only cut entries are left out."""
-    spec = fgi_spec(2)
+def test_live_entries_hold_a_point_with_nonzero_beta():
+    """The gathers step through the (entry, Gauss point) pairs of uncut entries whose point
+    has beta > 0 (cell size dx/2 at n = 2, so beta = 0 from r = delta + dx/4 on);
+    pdm_num_live_entries counts the entries holding at least one such pair.  r is formed as
+    the library forms it: (offset + X_j) - X_i."""
+    spec = fgi_spec(2, jitter=0.01, seed=4)
     spec["slits"] = [pm.mid_slit(spec, 30.0)]
     prob = pm.Problem(pm.deck_text(spec))
-    _, _, cut = prob.csr()
-    assert cut.sum() > 0
-    assert prob.E_live == prob.E - int(cut.sum())
+    rp, col, cut = prob.csr()
+    X, _, _ = prob.nodes()
+    X = X.reshape(-1, 3)
+    Xg, _ = prob.gauss()
+    row = np.repeat(np.arange(prob.n), np.diff(rp))
+    d = Xg[col] - X[row][:, None, :]
+    r = np.sqrt(d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1] + d[..., 2] * d[..., 2])
+    delta, cell = spec["horizon"], spec["grid_spacing"] / 2.0
+    inner, outer = delta - 0.5 * cell, delta + 0.5 * cell
+    live = (r <= inner) | ((r <= outer) & ((delta + 0.5 * cell - r) / cell != 0.0))
+    live &= (cut == 0)[:, None]
+    want = int(live.any(axis=1).sum())
+    print(f"[fgi live] E={prob.E} cut={int(cut.sum())} live entries={prob.E_live} want={want} "
+          f"live pairs={int(live.sum())} of {live.size}", flush=True)
+    assert cut.sum() > 0 and want < prob.E - int(cut.sum())
+    assert prob.E_live == want
+    assert prob.live_gauss_pairs == int(live.sum())
 
 
 def test_unsupported_combinations_fail_loudly():
