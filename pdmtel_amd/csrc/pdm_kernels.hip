@@ -781,57 +781,67 @@ __device__ __forceinline__ void load8(const TileView& tv, int64_t e, int64_t e1,
     }
 }
 
-// The lane's block of B entries at e (a multiple of B): for B = 8 one 16-byte slot
-// load, and one 8-byte correction load or two 16-byte coefficient loads; for B = 4 an
-// 8-byte slot load and a 4-byte correction load or one or two 16-byte coefficient loads.
-// Lane sl of a group reads the block at e0 + B sl, so each load instruction of the group
-// covers contiguous memory.
-template <class OP, int B, int LPN>
-__device__ __forceinline__ void load_block(const TileView& tv, int64_t e, int64_t e1, int pad, Chunk<OP, B>& ch)
+// Entries [e, e + 4) of a row ending at e1 into a chunk at entry offset o (e a multiple of
+// 4, rows whole blocks of 8), or, at or past e1, 4 references to the padding record with
+// coefficient 0: an 8-byte slot load and a 4-byte correction load or one or two 16-byte
+// coefficient loads.
+template <class OP, int B>
+__device__ __forceinline__ void load4(const TileView& tv, int64_t e, int64_t e1, unsigned pad2, Chunk<OP, B>& ch,
+                                      int o)
 {
-    const unsigned pad2 = (unsigned)pad * 0x10001u;
-    if constexpr (B == 8) {
-        load8<OP, B>(tv, e, e1, pad2, ch, 0);
-    } else {
-        static_assert(B == 4, "blocks of 4 or 8 entries");
-        if (e < e1) {
-            const u32x2 sv = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(tv.slot + e));
-            ch.sp[0] = sv[0];
-            ch.sp[1] = sv[1];
-            if constexpr (!kCoefStream<OP>) {
-                ch.cb[0] = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(tv.corr + e));
-            } else if constexpr (sizeof(typename OP::Coef) == 4) {
-                const f32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e));
+    if (e < e1) {
+        const u32x2 sv = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(tv.slot + e));
+        ch.sp[o / 2] = sv[0];
+        ch.sp[o / 2 + 1] = sv[1];
+        if constexpr (!kCoefStream<OP>) {
+            ch.cb[o / 4] = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(tv.corr + e));
+        } else if constexpr (sizeof(typename OP::Coef) == 4) {
+            const f32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e));
 #pragma unroll
-                for (int k = 0; k < 4; ++k) ch.c[k] = c0[k];
-            } else {
-                const f64x2 c0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(tv.coef64 + e));
-                const f64x2 c1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(tv.coef64 + e + 2));
-                ch.c[0] = c0[0];
-                ch.c[1] = c0[1];
-                ch.c[2] = c1[0];
-                ch.c[3] = c1[1];
-            }
+            for (int k = 0; k < 4; ++k) ch.c[o + k] = c0[k];
         } else {
-            ch.sp[0] = ch.sp[1] = pad2;
-            if constexpr (kCoefStream<OP>) {
+            const f64x2 c0 = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(tv.coef64 + e));
+            const f64x2 c1 = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(tv.coef64 + e + 2));
+            ch.c[o] = c0[0];
+            ch.c[o + 1] = c0[1];
+            ch.c[o + 2] = c1[0];
+            ch.c[o + 3] = c1[1];
+        }
+    } else {
+        ch.sp[o / 2] = ch.sp[o / 2 + 1] = pad2;
+        if constexpr (kCoefStream<OP>) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) ch.c[k] = 0;
-            } else {
-                ch.cb[0] = 0x80808080u;
-            }
+            for (int k = 0; k < 4; ++k) ch.c[o + k] = 0;
+        } else {
+            ch.cb[o / 4] = 0x80808080u;
         }
     }
 }
 
-// A family is read as a first block of kBlock0 entries per lane (8 for the FP32 operations,
-// 4 for FP64, which keep fewer registers), loaded during the previous round, then blocks
-// of 4 entries per lane until the row ends, each loaded while the one before is computed.
-// With 16 lanes per family an FP32 family of up to 128 + 64 m entries takes 1 + m blocks:
-// 192 entry slots for an interior CCI family at delta = 3.015 dx (178 entries with beta > 0).
+// A family is read as a first block of kBlock0 entries per lane, loaded during the
+// previous round, then blocks of 4 entries per lane until the row ends, each loaded while
+// the one before is computed.  FP32 operations: the first block is 8 entries at e0 + 8 l
+// plus 4 at e0 + 8 L + 4 l (lane l of L), so with 16 lanes it covers 192 entries, an
+// interior CCI family at delta = 3.015 dx (178 entries with beta > 0) whole; the FP64
+// operations keep 4 (fewer registers).  The later blocks of 4 lie at e0 + kBlock0 L + 4 l,
+// then 4 L further each time.  Each load instruction of a lane group covers contiguous
+// memory.
 template <class OP>
-constexpr int kBlock0 = (sizeof(typename OP::Coef) == 8) ? 4 : 8;
+constexpr int kBlock0 = (sizeof(typename OP::Coef) == 8) ? 4 : 12;
 constexpr int kBlockN = 4;
+
+template <class OP, int LPN>
+__device__ __forceinline__ void load_first(const TileView& tv, int64_t e0, int sl, int64_t e1, int pad,
+                                           Chunk<OP, kBlock0<OP>>& ch)
+{
+    const unsigned pad2 = (unsigned)pad * 0x10001u;
+    if constexpr (kBlock0<OP> == 12) {
+        load8<OP, 12>(tv, e0 + 8 * sl, e1, pad2, ch, 0);
+        load4<OP, 12>(tv, e0 + 8 * LPN + 4 * sl, e1, pad2, ch, 8);
+    } else {
+        load4<OP, 4>(tv, e0 + 4 * sl, e1, pad2, ch, 0);
+    }
+}
 
 // One round of a lane group: the family of tile node loc (first block from `cur`, which the
 // caller loaded), with the next round's first block loaded into the caller's other buffer
@@ -846,15 +856,16 @@ __device__ __forceinline__ void family_round(const TileView& tv, const SpecDev& 
     const typename OP::SelfT me = OP::SelfT::make(l, sp, ss, UE);
     typename OP::Acc acc{};
     int64_t base = e0 + LPN * B0;
+    const unsigned pad2 = (unsigned)l.pad * 0x10001u;
     Chunk<OP, kBlockN> nx;
-    if (base < e1) load_block<OP, kBlockN, LPN>(tv, base + kBlockN * sl, e1, l.pad, nx);
+    if (base < e1) load4<OP, kBlockN>(tv, base + kBlockN * sl, e1, pad2, nx, 0);
 #pragma unroll
     for (int k = 0; k < B0; ++k) OP::template bond<ENERGY, UE>(l, cur.slot(k), cur.coef(k), me, acc);
     flush<OP>(acc);
     while (base < e1) {
         const Chunk<OP, kBlockN> c = nx;
         base += LPN * kBlockN;
-        if (base < e1) load_block<OP, kBlockN, LPN>(tv, base + kBlockN * sl, e1, l.pad, nx);
+        if (base < e1) load4<OP, kBlockN>(tv, base + kBlockN * sl, e1, pad2, nx, 0);
 #pragma unroll
         for (int k = 0; k < kBlockN; ++k) OP::template bond<ENERGY, UE>(l, c.slot(k), c.coef(k), me, acc);
         flush<OP>(acc);
@@ -886,7 +897,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev&
     int loc = first + g;
     int64_t e0 = (loc < nt) ? l.row[loc] : 0, e1 = (loc < nt) ? l.row[loc + 1] : 0;
     Chunk<OP, B> c0, c1;
-    load_block<OP, B, LPN>(tv, e0 + B * sl, e1, l.pad, c0);
+    load_first<OP, LPN>(tv, e0, sl, e1, l.pad, c0);
     if constexpr (!kF32Rec<OP>) {
         // FP64 operations: one block buffer and a copy per round (fewer registers)
         for (;;) {
@@ -898,7 +909,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev&
                     n0 = l.row[nloc];
                     n1 = l.row[nloc + 1];
                 }
-                load_block<OP, B, LPN>(tv, n0 + B * sl, n1, l.pad, c1);
+                load_first<OP, LPN>(tv, n0, sl, n1, l.pad, c1);
             }
             family_round<OP, ENERGY, UE, LPN>(tv, sp, l, nt, loc, e0, e1, sl, c0, ivc4);
             if (nfirst >= nt) return;
@@ -919,7 +930,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev&
                 n0 = l.row[nloc];
                 n1 = l.row[nloc + 1];
             }
-            load_block<OP, B, LPN>(tv, n0 + B * sl, n1, l.pad, c1);
+            load_first<OP, LPN>(tv, n0, sl, n1, l.pad, c1);
         }
         family_round<OP, ENERGY, UE, LPN>(tv, sp, l, nt, loc, e0, e1, sl, c0, ivc4);
         if (nfirst >= nt) break;
@@ -937,7 +948,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev&
                 n0 = l.row[nloc];
                 n1 = l.row[nloc + 1];
             }
-            load_block<OP, B, LPN>(tv, n0 + B * sl, n1, l.pad, c0);
+            load_first<OP, LPN>(tv, n0, sl, n1, l.pad, c0);
         }
         family_round<OP, ENERGY, UE, LPN>(tv, sp, l, nt, loc, e0, e1, sl, c1, ivc4);
         if (nfirst >= nt) break;
