@@ -39,6 +39,7 @@ def test_batch_equals_single(precision):
         assert np.array_equal(U, Ub) and np.array_equal(V, Vb), k
         assert p.clock() == b.clock(k), k
         assert p.energy() == b.energy(k), k
+        assert p.E_live == b.E_live[k] and 0 < p.E_live < p.E, k
         p.close()
     b.close()
 
