@@ -26,6 +26,7 @@ value counts bond-updates as the reference performs them: 4 per family entry per
 RK4 step (pdm_num_entries).  The gathers evaluate only the evaluated_entries
 (pdm_num_live_entries): a cut entry or one whose CCI beta is 0 contributes
 exactly 0 to every force and energy, and the body force still counts it.
+evaluated_bond_updates_per_s gives the rate of the evaluated ones.
 
 roofline: dominant kernel = the fused bond-force / RK4-stage kernel.  Its
 average launch time comes from HIP events recorded on the library's stream
@@ -218,6 +219,7 @@ def run_batch(args):
                    "precision": "fp32c", "nodes": n, "family_entries": E, "evaluated_entries": E_live,
                    "specimens": args.specimens,
                    "layout": res["fp32c"]["tuning"], "layout_fp64": res["fp64"]["tuning"]},
+        "evaluated_bond_updates_per_s": 4.0 * E_live * args.steps / res["fp32c"]["seconds"],
         "fp64_value": 4.0 * E * args.steps / res["fp64"]["seconds"],
         "fp32c_over_fp64": res["fp64"]["seconds"] / res["fp32c"]["seconds"],
         "fp32c_max_rel_disp_err_vs_fp64": err,
@@ -364,6 +366,7 @@ def main():
             "layout": head["tuning"],
             "layout_fp64": res["fp64"]["tuning"] if "fp64" in res else None,
         },
+        "evaluated_bond_updates_per_s": 4.0 * E_live * args.steps / head["seconds"],
         "fp64_value": (4.0 * E * args.steps / res["fp64"]["seconds"]) if "fp64" in res else None,
         "fp32c_over_fp64": (res["fp64"]["seconds"] / res["fp32c"]["seconds"]) if len(res) == 2 else None,
         "fp32c_max_rel_disp_err_vs_fp64": err,
