@@ -4,12 +4,13 @@
 // (position and this stage's displacement; FP64, or fixed point and FP64 u S with the
 // FP32 coefficient inputs) plus the families' row starts and own slots into LDS with
 // coalesced loads; one extra record far outside the body, with zero weight, backs the
-// row padding and every zero-coefficient entry.  Phase 2: each 64-lane wavefront works
-// on NPW families at a time, 64/NPW lanes per family; every lane loads one aligned
-// block of B consecutive entries (uint16 LDS slots, and FP64 coefficients or FP32
-// correction bytes, streamed from HBM as vector loads: B = 256/(64/NPW) for the FP32
-// operations, 4 for FP64), and the first block of the next round is loaded while the
-// current one is computed.  The neighbour's record is three ds_read_b128.  Family sums
+// row padding (CCI rows hold only entries with a nonzero coefficient).  Phase 2: each
+// 64-lane wavefront works on NPW families at a time, 64/NPW lanes per family; every lane
+// loads aligned blocks of entries (uint16 LDS slots, and FP64 coefficients or FP32
+// correction bytes, streamed from HBM as vector loads): a first block of 12 (FP32
+// operations: 8 + 4) or 4 (FP64) loaded while the previous round is computed, then
+// blocks of 4 each loaded while the one before is computed.  The neighbour's record is
+// three ds_read_b128.  Family sums
 // are reduced inside the lane group with DPP moves (quad_perm, row_half_mirror,
 // row_mirror) and the gfx950 permlane16/32 swaps, never through LDS.  Phase 3:
 // one thread per tile node applies the stage update (velocity buffer, boundary
