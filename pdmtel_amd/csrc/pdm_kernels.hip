@@ -1378,6 +1378,8 @@ __global__ __launch_bounds__(256) void k_fgi_gauss(FgiArgs f, const d4* __restri
     }
     const int64_t k0 = f.cb_ptr[a], k1 = f.cb_ptr[a + 1];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    // unrolled so that several iterations' loads are in flight; the sums keep their order
+#pragma unroll 4
     for (int64_t k = k0; k < k1; ++k) {
         const double c = f.phi[k * f.ng + g];
         const d4 x = u[f.cb[k]];
