@@ -173,7 +173,7 @@ def test_rk4_graph_matches_eager():
 @pytest.mark.parametrize("precision", [pm.PDM_FP32C, pm.PDM_FP32C2])
 def test_forces_fp32(case, precision):
     """FP32 bond forces (coefficients formed from the records and the per-entry correction
-    bytes; cut and zero-beta entries on the padding record) against the FP64 oracle."""
+    bytes; cut and zero-beta entries left out of the rows) against the FP64 oracle."""
     if case == "slits":
         spec = make("bar_jitter")
         spec["slits"] = [pm.mid_slit(spec, 30.0), dict(type=1, box=(0.0, 1.0, 0.004, 0.004, 0.0, 1.0))]
