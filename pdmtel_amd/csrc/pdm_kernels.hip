@@ -1151,9 +1151,14 @@ __device__ __forceinline__ void tile_update(const StageArgs& a, const SpecDev& s
 }
 
 // One RK4 stage (STAGE 0..3) for one tile.  Reads a.uin, writes a.uout.
-// At most 128 VGPRs (4 waves per SIMD): two 8-wave tiles per CU.
+// FP64: at most 128 VGPRs (4 waves per SIMD, two 8-wave tiles per CU).  FP32 operations:
+// at most 80 (6 waves per SIMD), so that three 8-wave tiles share a CU when their LDS
+// allows it (about 51 KB per tile of edge 4 dx at delta = 3.015 dx): the staging and
+// epilogue of one tile then overlap the bond loops of the others.
+template <int PREC>
+constexpr int kStageWavesPerSimd = (PREC == PDM_FP64) ? 4 : 6;
 template <int STAGE, int PREC, bool ENERGY, bool UE, int NW, int NPW>
-__global__ __launch_bounds__(64 * NW, 4) void k_stage(StageArgs a, int step_idx)
+__global__ __launch_bounds__(64 * NW, kStageWavesPerSimd<PREC>) void k_stage(StageArgs a, int step_idx)
 {
     extern __shared__ double L[];
     __shared__ double sh_e[2][NW];
