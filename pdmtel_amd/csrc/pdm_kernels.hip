@@ -1151,14 +1151,14 @@ __device__ __forceinline__ void tile_update(const StageArgs& a, const SpecDev& s
 }
 
 // One RK4 stage (STAGE 0..3) for one tile.  Reads a.uin, writes a.uout.
-// FP64: at most 128 VGPRs (4 waves per SIMD, two 8-wave tiles per CU).  FP32 operations:
-// at most 80 (6 waves per SIMD), so that three 8-wave tiles share a CU when their LDS
-// allows it (about 51 KB per tile of edge 4 dx at delta = 3.015 dx): the staging and
-// epilogue of one tile then overlap the bond loops of the others.
-template <int PREC>
-constexpr int kStageWavesPerSimd = (PREC == PDM_FP64) ? 4 : 6;
-template <int STAGE, int PREC, bool ENERGY, bool UE, int NW, int NPW>
-__global__ __launch_bounds__(64 * NW, kStageWavesPerSimd<PREC>) void k_stage(StageArgs a, int step_idx)
+// At most 128 VGPRs (4 waves per SIMD, two 8-wave tiles per CU), or with OCC6 at most 80
+// (6 waves per SIMD): the launch picks OCC6 for the FP32 operations on uniform-material
+// tiles when the LDS holds more tiles per CU than the 128-VGPR kernel can (three 8-wave
+// tiles of edge 4 dx at delta = 3.015 dx, about 51 KB each), so that the staging and
+// epilogue of one tile overlap the bond loops of the others.  With room for fewer tiles
+// the tighter register budget would only spill.
+template <int STAGE, int PREC, bool ENERGY, bool UE, int NW, int NPW, bool OCC6 = false>
+__global__ __launch_bounds__(64 * NW, (OCC6 ? 6 : 4)) void k_stage(StageArgs a, int step_idx)
 {
     extern __shared__ double L[];
     __shared__ double sh_e[2][NW];
@@ -1564,12 +1564,12 @@ hipError_t set_lds(size_t bytes)
     return e;
 }
 
-template <int STAGE, int PREC, bool ENERGY, bool UE, int NW, int NPW>
+template <int STAGE, int PREC, bool ENERGY, bool UE, int NW, int NPW, bool OCC6 = false>
 hipError_t launch_stage_t(const StageArgs& a, int step_idx, size_t lds, hipStream_t s)
 {
-    hipError_t e = set_lds<k_stage<STAGE, PREC, ENERGY, UE, NW, NPW>>(lds);
+    hipError_t e = set_lds<k_stage<STAGE, PREC, ENERGY, UE, NW, NPW, OCC6>>(lds);
     if (e != hipSuccess) return e;
-    k_stage<STAGE, PREC, ENERGY, UE, NW, NPW>
+    k_stage<STAGE, PREC, ENERGY, UE, NW, NPW, OCC6>
         <<<dim3((unsigned)padded_grid(a.tv.ntl)), dim3(64 * NW), lds, s>>>(a, step_idx);
     return hipGetLastError();
 }
@@ -1577,11 +1577,23 @@ hipError_t launch_stage_t(const StageArgs& a, int step_idx, size_t lds, hipStrea
 template <int STAGE, int PREC, int NW, int NPW>
 hipError_t launch_stage_e(const StageArgs& a, int step_idx, size_t lds, hipStream_t s)
 {
+    // tiles per CU: 16 / NW waves at 4 per SIMD, 24 / NW at 6, LDS (dynamic + the static
+    // energy partials) permitting
+    const size_t per_tile = lds + sizeof(double) * 2 * NW;
+    const bool occ6 = PREC != PDM_FP64 && a.tv.uniform_mat && kLdsBytes / per_tile > (size_t)(16 / NW);
     if (a.energies) {
-        if (a.tv.uniform_mat) return launch_stage_t<STAGE, PREC, true, true, NW, NPW>(a, step_idx, lds, s);
+        if (a.tv.uniform_mat) {
+            if constexpr (PREC != PDM_FP64)
+                if (occ6) return launch_stage_t<STAGE, PREC, true, true, NW, NPW, true>(a, step_idx, lds, s);
+            return launch_stage_t<STAGE, PREC, true, true, NW, NPW>(a, step_idx, lds, s);
+        }
         return launch_stage_t<STAGE, PREC, true, false, NW, NPW>(a, step_idx, lds, s);
     }
-    if (a.tv.uniform_mat) return launch_stage_t<STAGE, PREC, false, true, NW, NPW>(a, step_idx, lds, s);
+    if (a.tv.uniform_mat) {
+        if constexpr (PREC != PDM_FP64)
+            if (occ6) return launch_stage_t<STAGE, PREC, false, true, NW, NPW, true>(a, step_idx, lds, s);
+        return launch_stage_t<STAGE, PREC, false, true, NW, NPW>(a, step_idx, lds, s);
+    }
     return launch_stage_t<STAGE, PREC, false, false, NW, NPW>(a, step_idx, lds, s);
 }
 
