@@ -457,6 +457,13 @@ void start_clock(pdm_handle* h)
 // kernel's speed depends on the halo-to-tile ratio (tile edge), on the LDS and
 // register footprint that sets the tiles resident per CU, and on how many lanes share
 // a family; the best point differs between precisions and horizons.
+// Steps the autotuners run per candidate layout: a few to warm up (the first launch of an
+// instantiation also sets its LDS limit), then enough timed steps that the ranking does not
+// hinge on the run-to-run spread of a few launches.  The step-parameter and energy-partial
+// buffers hold kTuneSteps steps meanwhile.
+constexpr int kTuneWarmSteps = 2;
+constexpr int kTuneSteps = 10;
+
 int autotune(pdm_handle* h)
 {
     static const int kEdges[] = {4, 5, 6};
@@ -464,7 +471,7 @@ int autotune(pdm_handle* h)
     static const int kFams[] = {2, 4};
     double best = 1e300;
     int bt = 0, bw = 8, bf = 2;
-    int rc = ensure_chunk(h, 4);
+    int rc = ensure_chunk(h, kTuneSteps);
     if (rc) return rc;
     for (int t : kEdges) {
         h->tile_edge[0] = h->tile_edge[1] = h->tile_edge[2] = t;
@@ -479,23 +486,23 @@ int autotune(pdm_handle* h)
         if (rc) return rc;
         rc = upload_spec(h);
         if (rc) return rc;
-        const StepParam sp[4] = {{h->dt_stable, h->dt_stable}, {h->dt_stable, h->dt_stable},
-                                 {h->dt_stable, h->dt_stable}, {h->dt_stable, h->dt_stable}};
+        StepParam sp[kTuneSteps];
+        for (StepParam& q : sp) q = {h->dt_stable, h->dt_stable};
         HCHK(hipMemcpyAsync(h->steps, sp, sizeof sp, hipMemcpyHostToDevice, h->stream));
         for (int w : kWaves)
             for (int f : kFams) {
                 h->waves = w;
                 h->fam_per_wave = f;
-                rc = enqueue_steps(h, 1);  // warm-up (first launch sets the LDS limit)
+                rc = enqueue_steps(h, kTuneWarmSteps);
                 if (rc) return rc;
                 HCHK(hipEventRecord(h->ev0, h->stream));
-                rc = enqueue_steps(h, 3);
+                rc = enqueue_steps(h, kTuneSteps);
                 if (rc) return rc;
                 HCHK(hipEventRecord(h->ev1, h->stream));
                 HCHK(hipEventSynchronize(h->ev1));
                 float ms = 0.f;
                 HCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-                const double per = ms / 12.0;
+                const double per = ms / (4.0 * kTuneSteps);
                 if (h->opt.verbose)
                     fprintf(stderr, "pdm_init: tune edge %d waves %d families %d: %.4f ms per stage\n", t, w, f, per);
                 if (per < best) {
@@ -665,8 +672,8 @@ int batch_layout(pdm_handle* B, int t)
 int batch_timing_steps(pdm_handle* B)
 {
     pdm_handle* const h = B;  // for HCHK
-    std::vector<StepParam> sp(4 * (size_t)B->nspec);
-    for (int r = 0; r < 4; ++r)
+    std::vector<StepParam> sp(kTuneSteps * (size_t)B->nspec);
+    for (int r = 0; r < kTuneSteps; ++r)
         for (int k = 0; k < B->nspec; ++k) sp[(size_t)r * B->nspec + k] = {B->members[k]->dt_stable, B->members[k]->dt_stable};
     HCHK(hipMemcpyAsync(B->steps, sp.data(), sizeof(StepParam) * sp.size(), hipMemcpyHostToDevice, B->stream));
     return PDM_OK;
@@ -681,7 +688,7 @@ int batch_autotune(pdm_handle* B)
     static const int kFams[] = {2, 4};
     double best = 1e300;
     int bt = 0, bw = 8, bf = 2;
-    int rc = ensure_chunk(B, 4);
+    int rc = ensure_chunk(B, kTuneSteps);
     if (rc) return rc;
     for (int t : kEdges) {
         rc = batch_layout(B, t);
@@ -696,16 +703,16 @@ int batch_autotune(pdm_handle* B)
             for (int f : kFams) {
                 B->waves = w;
                 B->fam_per_wave = f;
-                rc = enqueue_steps(B, 1);
+                rc = enqueue_steps(B, kTuneWarmSteps);
                 if (rc) return rc;
                 HCHK(hipEventRecord(B->ev0, B->stream));
-                rc = enqueue_steps(B, 3);
+                rc = enqueue_steps(B, kTuneSteps);
                 if (rc) return rc;
                 HCHK(hipEventRecord(B->ev1, B->stream));
                 HCHK(hipEventSynchronize(B->ev1));
                 float ms = 0.f;
                 HCHK(hipEventElapsedTime(&ms, B->ev0, B->ev1));
-                const double per = ms / 12.0;
+                const double per = ms / (4.0 * kTuneSteps);
                 if (B->opt.verbose)
                     fprintf(stderr, "pdm_batch_init: tune edge %d waves %d families %d: %.4f ms per stage\n", t, w, f,
                             per);
