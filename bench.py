@@ -79,7 +79,11 @@ def run_precision(args, spec, precision):
 
     text = pm.deck_text(spec)
     t0 = time.perf_counter()
-    prob = pm.Problem(text, precision=precision, graph_chunk=args.chunk, energies=1)
+    layout = {}
+    if args.layout:  # "edge,waves,families": fixed tile layout instead of the autotuner's
+        e, w, f = (int(v) for v in args.layout.split(","))
+        layout = dict(tile_edge=e, waves_per_tile=w, families_per_wave=f)
+    prob = pm.Problem(text, precision=precision, graph_chunk=args.chunk, energies=1, **layout)
     t_init = time.perf_counter() - t0
     if args.warmup:
         prob.step(args.warmup)
@@ -320,6 +324,7 @@ def main():
                          "Gaussian integration (--intg-point), FP64")
     ap.add_argument("--intg-point", type=int, default=2)
     ap.add_argument("--specimens", type=int, default=512)
+    ap.add_argument("--layout", default="", help="edge,waves,families: fixed tile layout (default: autotuned)")
     args = ap.parse_args()
 
     if args.gpus != 1:
