@@ -1,54 +1,59 @@
 #!/usr/bin/env python3
-"""Benchmark: bond-updates/s of the RK4 dynamic solve on one MI355X, plus the
-static-solve wall time.
+"""Benchmark: bond-updates/s of the RK4 dynamic solve on one MI355X, FP32-compensated
+against FP64, plus the static-solve wall time.
 
-RK4 workload (default): the 200 x 40 x 40 bar (320,000 nodes, dx = 1 mm,
-delta = 3.015 dx, 73,728,544 family entries), node jitter U(-0.01, 0.01) dx
-from seed 0, E = 72 GPa, rho = 2440, x = 0 face fixed, 1 MPa +x traction on
-the x = L layer, RK4 at 0.8 x the stable dt.  One step = 4 stages, each a
-full bond-force gather: 4 E bond-updates.
+Default workload ("slit"): the 200 x 40 x 40 bar (320,000 nodes, dx = 1 mm,
+delta = 3.015 dx, 73,728,544 family entries), node jitter U(-0.01, 0.01) dx from seed
+0, E = 72 GPa, rho = 2440, x = 0 layer fixed, 1 MPa +x traction on the x = L layer,
+RK4 at 0.8 x the stable dt, with one planar slit through the whole section at the bar
+centre, normal at 30 degrees to x in the x-y plane (it cuts 0.86 % of the bonds).  One
+step = 4 stages, each a full bond-force gather.  "bar" is the same bar without the slit;
+the default run also measures it (plain_bar) in FP64 and FP32C.
 
-Both paths run the same K steps after W warm-up steps: the FP64 path
-(reference arithmetic) and the FP32-compensated path.  `value` is the
-FP32-compensated rate; the line also carries the FP64 rate and the max
-relative displacement error of FP32C vs FP64 after the K + W steps.
-Inputs are resident in HBM before the timed region (pdm_init builds them
-on the GPU); the timed region is pdm_step_rk4 (hipGraph replays of the
-four fused stage kernels), synchronised on both sides.
+The paths run the same K steps after W warm-up steps: FP64 (reference arithmetic),
+FP32C (FP32 bond force, FP32 family sums) and FP32C2 (two-float family sums), all with
+FP64 RK4 state.  `value` is the FP32C rate; the line carries the FP64 and FP32C2 rates
+and each FP32 path's max relative displacement error against FP64 after the W + K steps.
+Inputs are resident in HBM before the timed region (pdm_init builds them on the GPU);
+the timed region is pdm_step_rk4: hipGraph replays of the four fused stage kernels per
+step, in chunks of gcd(W, K) steps (at most 50), so the warm-up has captured every
+graph the timed steps launch.
 
-Static solve: the same bar with one planar edge crack from the y = min face
-to the mid-plane, normal at 30 degrees to x (a crack through the whole
-section would leave the far half unsupported); matrix-free Jacobi-PCG to 1e-10
-relative residual, FP32 matvecs with FP64 refinement (and the FP64 solve for
-the error figure).  Wall seconds of pdm_solve_static.
+Static solve: the same bar with one planar edge crack from the y = min face to the
+mid-plane, normal at 30 degrees to x (the through-section slit leaves the far half
+unsupported, and the static problem singular); matrix-free PCG to 1e-10 relative
+residual in FP64, FP32 matvecs with FP64 residual replacement, and the FP64 solve for
+the error figure.  Wall seconds of pdm_solve_static.
 
-value counts bond-updates as the reference performs them: 4 per family entry per
-RK4 step (pdm_num_entries).  The gathers evaluate only the evaluated_entries
-(pdm_num_live_entries): a cut entry or one whose CCI beta is 0 contributes
+value counts bond-updates as the reference performs them: 4 per unbroken family entry
+per RK4 step (pdm_num_entries less the cut ones).  The gathers evaluate only the
+evaluated_entries (pdm_num_live_entries): an entry whose CCI beta is 0 contributes
 exactly 0 to every force and energy, and the body force still counts it.
 evaluated_bond_updates_per_s gives the rate of the evaluated ones.
 
-roofline: dominant kernel = the fused bond-force / RK4-stage kernel.  Its
-average launch time comes from HIP events recorded on the library's stream
-around the timed region (pdm_last_kernel_timing).  Algorithmic bytes per
-launch use the per-unit figures of the bond-update unit over the entries the
-kernel evaluates: 8 B per evaluated entry for FP32 coefficients (int32
-neighbour + FP32 coefficient; 12 B with FP64 coefficients) plus 264 B per
-node (trial u, X, Vbuf: 72 B, and the fused stage update, 8 vectors of 24 B).  The library's own format is smaller
-(uint16 LDS slot + one correction byte from which the FP32 kernels form the
-coefficient: 3 B per entry; uint16 slot + FP64 coefficient: 10 B); `format_bytes`
-gives that figure.  `traffic` is FETCH_SIZE + WRITE_SIZE per launch from rocprofv3
-counter runs on this workload, scaled by a plain copy kernel's readings, as
-recorded under profiles/ (null for any other workload).
+roofline: dominant kernel = the fused bond-force / RK4-stage kernel.  Its average
+launch time comes from HIP events recorded on the library's stream around the timed
+region (pdm_last_kernel_timing).  Algorithmic bytes per launch use the per-unit figures
+of the bond-update unit over the entries the kernel evaluates: 8 B per evaluated entry
+for FP32 coefficients (int32 neighbour + FP32 coefficient; 12 B with FP64
+coefficients) plus 264 B per node (trial u, X, Vbuf: 72 B, and the fused stage update,
+8 vectors of 24 B).  The library's own format is smaller (uint16 LDS slot + one
+correction byte from which the FP32 kernels form the coefficient: 3 B per entry; uint16
+slot + FP64 coefficient: 10 B); `format_bytes` gives that figure.  `traffic` is
+FETCH_SIZE + WRITE_SIZE per launch from rocprofv3 counter runs on this workload, scaled
+by a plain copy kernel's readings, as recorded under profiles/ (null for any other
+workload), and `measured_gbs` is that traffic over the launch time.
 
-cpu_baseline: the oracle (a serial restatement of the reference's RK4 loop)
-on the same bar, a few steps on 1 thread of this host (value, cores 1), then
-the next few steps with its node loop on all the cores given to this process
-(all_cores; OMP_NUM_THREADS, else the CPU count).
+cpu_baseline: the oracle (a serial C restatement of the reference's RK4 loop; the
+reference itself does not build here, so the figure is not calibrated against it) on the
+same bar, a few steps on 1 thread of this host (value, cores 1), then the next few steps
+with its node loop on all the cores given to this process (all_cores; OMP_NUM_THREADS,
+else the CPU count).
 """
 
 import argparse
 import json
+import math
 import os
 import re
 import sys
@@ -62,16 +67,35 @@ ENTRY_BYTES = {"fp32c": 8, "fp64": 12}        # int32 neighbour + coefficient
 FORMAT_ENTRY_BYTES = {"fp32c": 3, "fp64": 10}  # uint16 slot + correction byte / FP64 coefficient (this library's format)
 NODE_BYTES = 72 + 8 * 24                       # trial u, X, Vbuf + fused stage update (8 x 24 B)
 DEFAULT_DIMS = (200, 40, 40)
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_bar200x40x40_traffic.txt")
+TRAFFIC_FILES = {"slit": os.path.join(ROOT, "profiles", "k_stage_slitbar200x40x40_traffic.txt"),
+                 "bar": os.path.join(ROOT, "profiles", "k_stage_bar200x40x40_traffic.txt")}
+PRECISIONS = ("fp64", "fp32c", "fp32c2")
 
 
-def workload_spec(args, steps):
+def workload_spec(args, steps, workload=None):
     import pdmtel_amd as pm
     from pdmtel_amd.deck import plate_spec
 
-    if args.workload == "plate":
+    workload = workload or args.workload
+    if workload == "plate":
         return plate_spec(args.nx, args.ny, args.nz, jitter=0.01, seed=0, steps=steps)
-    return pm.bar_spec(args.nx, args.ny, args.nz, jitter=0.01, seed=0, steps=steps)
+    spec = pm.bar_spec(args.nx, args.ny, args.nz, jitter=0.01, seed=0, steps=steps)
+    if workload == "slit":
+        spec["slits"] = [pm.mid_slit(spec, 30.0)]
+    return spec
+
+
+def graph_chunk(args):
+    """Steps per captured graph: a divisor of both W and K (at most 50), so that the warm-up
+    captures the only graph the timed steps replay."""
+    c = math.gcd(args.warmup, args.steps) if args.warmup else args.steps
+    return max(1, min(args.chunk, c))
+
+
+def precision_code(name):
+    import pdmtel_amd as pm
+
+    return {"fp64": pm.PDM_FP64, "fp32c": pm.PDM_FP32C, "fp32c2": pm.PDM_FP32C2}[name]
 
 
 def run_precision(args, spec, precision):
@@ -83,7 +107,7 @@ def run_precision(args, spec, precision):
     if args.layout:  # "edge,waves,families": fixed tile layout instead of the autotuner's
         e, w, f = (int(v) for v in args.layout.split(","))
         layout = dict(tile_edge=e, waves_per_tile=w, families_per_wave=f)
-    prob = pm.Problem(text, precision=precision, graph_chunk=args.chunk, energies=1, **layout)
+    prob = pm.Problem(text, precision=precision, graph_chunk=graph_chunk(args), energies=1, **layout)
     t_init = time.perf_counter() - t0
     if args.warmup:
         prob.step(args.warmup)
@@ -93,8 +117,10 @@ def run_precision(args, spec, precision):
     assert done == args.steps, f"ran {done} of {args.steps} steps"
     ms_kernel, launches = prob.kernel_timing()
     U, _ = prob.state()
-    res = dict(n=prob.n, E=prob.E, E_live=prob.E_live, seconds=t1 - t0, init_seconds=t_init, kernel_ms=ms_kernel, launches=launches,
-               U=U, energy=prob.energy(), tuning=prob.tuning())
+    ncut = int(prob.csr()[2].sum())
+    res = dict(n=prob.n, E=prob.E, E_live=prob.E_live, E_unbroken=prob.E - ncut, seconds=t1 - t0,
+               init_seconds=t_init, kernel_ms=ms_kernel, launches=launches, U=U, energy=prob.energy(),
+               tuning=prob.tuning(), launch=prob.stage_launch())
     prob.close()
     return res
 
@@ -104,8 +130,8 @@ def run_static(args):
 
     import pdmtel_amd as pm
 
-    spec = workload_spec(args, 1)
-    if args.workload == "bar":
+    spec = workload_spec(args, 1, "bar" if args.workload in ("bar", "slit") else None)
+    if args.workload in ("bar", "slit"):
         spec["slits"] = [pm.mid_slit(spec, 30.0, edge=True)]
     text = pm.deck_text(spec)
     out = {}
@@ -137,7 +163,7 @@ def run_fgi(args):
     spec["cci"] = None
     spec["fgi"] = (1, n1)
     t0 = time.perf_counter()
-    prob = pm.Problem(pm.deck_text(spec), precision=pm.PDM_FP64, graph_chunk=args.chunk, energies=1)
+    prob = pm.Problem(pm.deck_text(spec), precision=pm.PDM_FP64, graph_chunk=graph_chunk(args), energies=1)
     t_init = time.perf_counter() - t0
     print(f"# fgi n={n1}: init {t_init:.2f} s, {prob.n} nodes, {prob.E} entries", file=sys.stderr, flush=True)
     if args.warmup:
@@ -190,7 +216,7 @@ def run_batch(args):
     for name, prec in (("fp32c", pm.PDM_FP32C), ("fp64", pm.PDM_FP64)):
         print(f"# batch {name}: building {len(decks)} specimens", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        b = pm.Batch(decks, precision=prec, graph_chunk=args.chunk, energies=1)
+        b = pm.Batch(decks, precision=prec, graph_chunk=graph_chunk(args), energies=1)
         t_init = time.perf_counter() - t0
         if args.warmup:
             b.step(args.warmup)
@@ -265,12 +291,13 @@ def cpu_baseline(spec, steps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
-    o = pyoracle.Oracle(spec, bond_method=1, nthreads=1)
+    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    o = pyoracle.Oracle(spec, bond_method=1, nthreads=cores)  # set-up is not timed
+    o.nthreads = 1
     t0 = time.perf_counter()
     o.run(steps)
     dt = time.perf_counter() - t0
     # the same restatement with its node loop on all host cores given to this process (OpenMP)
-    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     o.nthreads = cores
     t0 = time.perf_counter()
     o.run(steps)
@@ -279,11 +306,12 @@ def cpu_baseline(spec, steps):
                 value_all=4.0 * o.E * steps / dt_all, seconds_all=dt_all, cores_all=cores)
 
 
-def recorded_traffic(args, prec):
-    """FETCH_SIZE + WRITE_SIZE bytes per stage launch recorded for the default workload, else None."""
-    if args.workload != "bar" or (args.nx, args.ny, args.nz) != DEFAULT_DIMS or not os.path.exists(TRAFFIC_FILE):
+def recorded_traffic(args, prec, workload=None):
+    """FETCH_SIZE + WRITE_SIZE bytes per stage launch recorded for this workload, else None."""
+    path = TRAFFIC_FILES.get(workload or args.workload)
+    if path is None or (args.nx, args.ny, args.nz) != DEFAULT_DIMS or not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
+    for line in open(path):
         m = re.match(r"^traffic_bytes_per_launch\s+(\w+)\s+([0-9.eE+]+)", line)
         if m and m.group(1) == prec:
             return float(m.group(2))
@@ -313,13 +341,15 @@ def main():
     ap.add_argument("--ny", type=int, default=DEFAULT_DIMS[1])
     ap.add_argument("--nz", type=int, default=DEFAULT_DIMS[2])
     ap.add_argument("--chunk", type=int, default=50)
-    ap.add_argument("--precision", choices=["both", "fp64", "fp32c"], default="both")
+    ap.add_argument("--precision", choices=["all", "fp64", "fp32c", "fp32c2"], default="all")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-static", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true", help="slit workload: leave out the plain-bar figures")
     ap.add_argument("--calib-copy", type=int, default=0, help="run only a copy kernel of this many bytes")
-    ap.add_argument("--workload", choices=["bar", "batch", "plate", "fgi"], default="bar",
-                    help="bar: 200x40x40 bar (default); batch: random specimens; plate: --nx --ny --nz plate "
+    ap.add_argument("--workload", choices=["slit", "bar", "batch", "plate", "fgi"], default="slit",
+                    help="slit: 200x40x40 bar with a 30 deg through-section slit (default); bar: the same bar "
+                         "without it; batch: random specimens; plate: --nx --ny --nz plate "
                          "with two crossing slits (400 400 50 for the 8M-node plate); fgi: the bar with fixed "
                          "Gaussian integration (--intg-point), FP64")
     ap.add_argument("--intg-point", type=int, default=2)
@@ -343,34 +373,45 @@ def main():
 
     import numpy as np
 
-    import pdmtel_amd as pm
-
     spec = workload_spec(args, args.warmup + args.steps + 10)
+    precs = PRECISIONS if args.precision == "all" else (args.precision,)
     res = {}
-    if args.precision in ("both", "fp64"):
-        res["fp64"] = run_precision(args, spec, pm.PDM_FP64)
-        print(f"# fp64: {res['fp64']['seconds']:.3f} s for {args.steps} steps, "
-              f"kernel {res['fp64']['kernel_ms']:.4f} ms/launch, init {res['fp64']['init_seconds']:.2f} s",
+    for name in precs:
+        res[name] = run_precision(args, spec, precision_code(name))
+        r = res[name]
+        print(f"# {args.workload} {name}: {r['seconds']:.3f} s for {args.steps} steps, kernel {r['kernel_ms']:.4f} "
+              f"ms/launch, init {r['init_seconds']:.2f} s, layout {r['tuning']}, launch {r['launch']}",
               file=sys.stderr, flush=True)
-    if args.precision in ("both", "fp32c"):
-        res["fp32c"] = run_precision(args, spec, pm.PDM_FP32C)
-        print(f"# fp32c: {res['fp32c']['seconds']:.3f} s for {args.steps} steps, "
-              f"kernel {res['fp32c']['kernel_ms']:.4f} ms/launch", file=sys.stderr, flush=True)
-    head_prec = "fp32c" if "fp32c" in res else "fp64"
+    head_prec = "fp32c" if "fp32c" in res else precs[0]
     head = res[head_prec]
-    E, n, E_live = head["E"], head["n"], head["E_live"]
-    rate = 4.0 * E * args.steps / head["seconds"]
-    bytes_launch = E_live * ENTRY_BYTES[head_prec] + n * NODE_BYTES
-    format_bytes = E_live * FORMAT_ENTRY_BYTES[head_prec] + n * NODE_BYTES
-    kernel_s = head["kernel_ms"] * 1e-3
-    achieved = bytes_launch / kernel_s / 1e9 if kernel_s > 0 else 0.0
-    err = None
-    if "fp32c" in res and "fp64" in res:
-        U64, U32 = res["fp64"]["U"], res["fp32c"]["U"]
-        err = float(np.abs(U32 - U64).max() / np.abs(U64).max())
+    E, n, E_live, E_ub = head["E"], head["n"], head["E_live"], head["E_unbroken"]
+
+    def rate(r):
+        return 4.0 * r["E_unbroken"] * args.steps / r["seconds"]
+
+    def roof(name, r):
+        eb = ENTRY_BYTES["fp64" if name == "fp64" else "fp32c"]
+        fb = FORMAT_ENTRY_BYTES["fp64" if name == "fp64" else "fp32c"]
+        b = r["E_live"] * eb + r["n"] * NODE_BYTES
+        ks = r["kernel_ms"] * 1e-3
+        traffic = recorded_traffic(args, name)
+        ach = b / ks / 1e9 if ks > 0 else 0.0
+        return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": traffic, "measured_gbs": (traffic / ks / 1e9) if (traffic and ks > 0) else None,
+                "kernel": "k_stage (fused bond-force gather + RK4 stage)", "kernel_avg_ms": r["kernel_ms"],
+                "launches": r["launches"], "bytes_per_launch": b, "format_bytes": r["E_live"] * fb + r["n"] * NODE_BYTES}
+
+    def err_vs(name, ref="fp64"):
+        if name not in res or ref not in res:
+            return None
+        U64, U = res[ref]["U"], res[name]["U"]
+        return float(np.abs(U - U64).max() / np.abs(U64).max())
+
+    desc = {"slit": ", one through-section slit at the centre, normal 30 deg to x",
+            "bar": "", "plate": ", two crossing slits"}[args.workload]
     line = {
         "metric": "bond-updates/s (RK4), mixed vs FP64 at <=1e-6 rel err; static-solve wall s",
-        "value": rate,
+        "value": rate(head),
         "unit": "bond-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -379,52 +420,58 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if head_prec == "fp32c" else "f64",
+        "dtype": "f32+f64" if head_prec != "fp64" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload} {args.nx}x{args.ny}x{args.nz}, {n} nodes, {E} family entries, "
-                        f"delta=3.015dx, jitter 0.01dx seed 0"
-                        + (", two crossing slits" if args.workload == "plate" else "")
-                        + f", RK4 {args.steps} steps",
+            "workload": f"{'bar' if args.workload == 'slit' else args.workload} {args.nx}x{args.ny}x{args.nz}, "
+                        f"{n} nodes, {E} family entries ({E - E_ub} cut), delta=3.015dx, jitter 0.01dx seed 0"
+                        + desc + f", RK4 {args.steps} steps",
             "precision": head_prec,
             "nodes": n,
             "family_entries": E,
+            "unbroken_entries": E_ub,
             "evaluated_entries": E_live,
-            "layout": head["tuning"],
-            "layout_fp64": res["fp64"]["tuning"] if "fp64" in res else None,
+            "graph_chunk": graph_chunk(args),
+            "layout": {k: r["tuning"] for k, r in res.items()},
+            "stage_launch": {k: r["launch"] for k, r in res.items()},
         },
         "evaluated_bond_updates_per_s": 4.0 * E_live * args.steps / head["seconds"],
-        "fp64_value": (4.0 * E * args.steps / res["fp64"]["seconds"]) if "fp64" in res else None,
-        "fp32c_over_fp64": (res["fp64"]["seconds"] / res["fp32c"]["seconds"]) if len(res) == 2 else None,
-        "fp32c_max_rel_disp_err_vs_fp64": err,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": recorded_traffic(args, head_prec),
-            "kernel": "k_stage (fused bond-force gather + RK4 stage)",
-            "kernel_avg_ms": head["kernel_ms"],
-            "launches": head["launches"],
-            "bytes_per_launch": bytes_launch,
-            "format_bytes": format_bytes,
-        },
+        "fp64_value": rate(res["fp64"]) if "fp64" in res else None,
+        "fp32c_over_fp64": (res["fp64"]["seconds"] / res["fp32c"]["seconds"]) if {"fp32c", "fp64"} <= set(res) else None,
+        "fp32c_max_rel_disp_err_vs_fp64": err_vs("fp32c"),
+        "fp32c2_value": rate(res["fp32c2"]) if "fp32c2" in res else None,
+        "fp32c2_over_fp64": (res["fp64"]["seconds"] / res["fp32c2"]["seconds"]) if {"fp32c2", "fp64"} <= set(res) else None,
+        "fp32c2_max_rel_disp_err_vs_fp64": err_vs("fp32c2"),
+        "roofline": roof(head_prec, head),
         "cpu_baseline": None,
     }
-    if "fp64" in res and head_prec != "fp64":
-        k64 = res["fp64"]["kernel_ms"] * 1e-3
-        b64 = res["fp64"]["E_live"] * ENTRY_BYTES["fp64"] + n * NODE_BYTES
-        line["roofline_fp64"] = {"achieved": b64 / k64 / 1e9, "frac": b64 / k64 / 1e9 / HBM_PEAK_GBS,
-                                 "kernel_avg_ms": res["fp64"]["kernel_ms"], "bytes_per_launch": b64,
-                                 "traffic": recorded_traffic(args, "fp64")}
+    for name in res:
+        if name != head_prec:
+            line[f"roofline_{name}"] = roof(name, res[name])
+    if args.workload == "slit" and not args.skip_plain:
+        # the bar without the slit (the FP64 workload), same steps, FP64 and FP32C
+        pspec = workload_spec(args, args.warmup + args.steps + 10, "bar")
+        pr = {name: run_precision(args, pspec, precision_code(name)) for name in ("fp64", "fp32c")}
+        for name, r in pr.items():
+            print(f"# bar {name}: {r['seconds']:.3f} s for {args.steps} steps, kernel {r['kernel_ms']:.4f} ms/launch",
+                  file=sys.stderr, flush=True)
+        line["plain_bar"] = {
+            "workload": f"bar {args.nx}x{args.ny}x{args.nz} without the slit, {pr['fp64']['E']} family entries",
+            "fp64_value": rate(pr["fp64"]), "fp32c_value": rate(pr["fp32c"]),
+            "fp32c_over_fp64": pr["fp64"]["seconds"] / pr["fp32c"]["seconds"],
+            "fp32c_max_rel_disp_err_vs_fp64": float(np.abs(pr["fp32c"]["U"] - pr["fp64"]["U"]).max() /
+                                                    np.abs(pr["fp64"]["U"]).max()),
+            "roofline_fp64": roof("fp64", pr["fp64"]), "roofline_fp32c": roof("fp32c", pr["fp32c"]),
+        }
+        line["plain_bar"]["roofline_fp64"]["traffic"] = recorded_traffic(args, "fp64", "bar")
+        line["plain_bar"]["roofline_fp32c"]["traffic"] = recorded_traffic(args, "fp32c", "bar")
     if not args.skip_static:
         st, serr = run_static(args)
         line["static_solve"] = {
             "workload": (f"bar {args.nx}x{args.ny}x{args.nz} jitter 0.01dx seed 0, edge crack to the mid-plane, "
-                         "normal 30 deg to x, Jacobi-PCG to 1e-10 relative residual") if args.workload == "bar" else
+                         "normal 30 deg to x, PCG to 1e-10 relative residual") if args.workload in ("bar", "slit") else
                         (f"plate {args.nx}x{args.ny}x{args.nz} jitter 0.01dx seed 0, two crossing slits, "
-                         "Jacobi-PCG to 1e-10 relative residual"),
+                         "PCG to 1e-10 relative residual"),
             "wall_s": st["fp32c"]["wall"], "iterations": st["fp32c"]["iterations"],
             "refinements": st["fp32c"]["refinements"], "rel_residual": st["fp32c"]["rel_residual"],
             "fp64_wall_s": st["fp64"]["wall"], "fp64_iterations": st["fp64"]["iterations"],
@@ -437,7 +484,8 @@ def main():
             "unit": "bond-updates/s",
             "cores": 1,
             "kind": "port",
-            "sample": f"oracle serial restatement of the reference RK4 loop, same {cb['n']}-node bar, "
+            "sample": f"the oracle (serial C restatement of the reference RK4 loop; not calibrated against the "
+                      f"reference binary, which does not build here), same {cb['n']}-node workload, "
                       f"{cb['steps']} RK4 steps ({4 * cb['E'] * cb['steps']} bond-updates) in {cb['seconds']:.1f} s",
             "all_cores": {"value": cb["value_all"], "cores": cb["cores_all"],
                           "sample": f"the next {cb['steps']} steps with the node loop on {cb['cores_all']} OpenMP "

@@ -204,6 +204,12 @@ int pdm_write_tecplot(const pdm_handle* h, const char* path, int step);
 int pdm_get_tuning(const pdm_handle* h, int* tile_edge, int* waves_per_tile, int* families_per_wave,
                    double* stage_ms);
 
+/* The RK4 stage kernel the launches take (CCI decks): its register budget per lane (128, or 80
+ * for the FP32 operations on one-material specimens when the LDS holds a third tile per CU) and
+ * its LDS bytes per tile workgroup.  No counterpart in the reference; it reports the GPU
+ * launch choice. */
+int pdm_get_stage_launch(const pdm_handle* h, int* max_vgprs, int64_t* lds_bytes);
+
 /* Time of the last kernel launches, from HIP events on the handle's stream:
  * average duration in ms of the dominant (bond-force stage) kernel over the last
  * pdm_step_rk4 call, and the number of launches it averaged. */

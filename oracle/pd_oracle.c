@@ -396,7 +396,95 @@ static int cmp_i32(const void* a, const void* b)
 }
 
 /* InitBond, pdDataManager2.cpp:12-70 */
-static int init_bonds(orc_state* s, int method)
+/* Cell list with edge >= delta + 1.5 dx: every pair the reference can accept lies in
+ * the 27 cells around a node's own. */
+typedef struct {
+    double lo[3], h;
+    int64_t nc[3];
+    int64_t* head;   /* ncell + 1 */
+    int64_t* cid;    /* n */
+    int64_t* order;  /* n, nodes sorted by cell */
+} cell_grid;
+
+static void grid_build(const orc_state* s, cell_grid* g)
+{
+    const orc_problem* p = &s->p;
+    int64_t n = s->n;
+    double hi[3] = {-MEGA, -MEGA, -MEGA};
+    for (int c = 0; c < 3; ++c) g->lo[c] = MEGA;
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) {
+            double v = s->X[3 * i + c];
+            g->lo[c] = v < g->lo[c] ? v : g->lo[c];
+            hi[c] = v > hi[c] ? v : hi[c];
+        }
+    g->h = p->horizon + 1.5 * p->dx;
+    for (int c = 0; c < 3; ++c) g->nc[c] = (n ? (int64_t)floor((hi[c] - g->lo[c]) / g->h) : 0) + 1;
+    int64_t ncell = g->nc[0] * g->nc[1] * g->nc[2];
+    g->head = (int64_t*)calloc(ncell + 1, sizeof(int64_t));
+    g->cid = (int64_t*)malloc(sizeof(int64_t) * (n ? n : 1));
+    g->order = (int64_t*)malloc(sizeof(int64_t) * (n ? n : 1));
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t q[3];
+        for (int c = 0; c < 3; ++c) {
+            q[c] = (int64_t)floor((s->X[3 * i + c] - g->lo[c]) / g->h);
+            if (q[c] >= g->nc[c]) q[c] = g->nc[c] - 1;
+        }
+        g->cid[i] = q[0] + g->nc[0] * (q[1] + g->nc[1] * q[2]);
+        g->head[g->cid[i] + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; ++c) g->head[c + 1] += g->head[c];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (ncell + 1));
+    memcpy(fill, g->head, sizeof(int64_t) * (ncell + 1));
+    for (int64_t i = 0; i < n; ++i) g->order[fill[g->cid[i]]++] = i;
+    free(fill);
+}
+
+static void grid_free(cell_grid* g)
+{
+    free(g->head);
+    free(g->cid);
+    free(g->order);
+}
+
+/* The family of node i (ascending ids): every j != i for which the reference's pair test,
+ * evaluated on (lower id, higher id) as InitBond's i < j loop does, accepts the bond.
+ * *minb takes the minimum bond length over the accepted pairs with j > i. */
+static void gather_family(const orc_state* s, const cell_grid* g, int64_t i, ivec* out, double* minb)
+{
+    const orc_problem* p = &s->p;
+    int64_t c0 = g->cid[i];
+    int64_t q0 = c0 % g->nc[0], q1 = (c0 / g->nc[0]) % g->nc[1], q2 = c0 / (g->nc[0] * g->nc[1]);
+    for (int64_t a = q2 - 1; a <= q2 + 1; ++a) {
+        if (a < 0 || a >= g->nc[2]) continue;
+        for (int64_t b = q1 - 1; b <= q1 + 1; ++b) {
+            if (b < 0 || b >= g->nc[1]) continue;
+            for (int64_t c = q0 - 1; c <= q0 + 1; ++c) {
+                if (c < 0 || c >= g->nc[0]) continue;
+                int64_t cc = c + g->nc[0] * (b + g->nc[1] * a);
+                for (int64_t k = g->head[cc]; k < g->head[cc + 1]; ++k) {
+                    int64_t j = g->order[k];
+                    if (j == i) continue;
+                    int64_t lo_id = i < j ? i : j, hi_id = i < j ? j : i;
+                    double dmin, dmax;
+                    const double* xa = &s->X[3 * lo_id];
+                    const double* xb = &s->X[3 * hi_id];
+                    if (orc_pair_bonded(p, xa, xb, &dmin, &dmax)) {
+                        if (j > i && minb) {
+                            double d = sqrt(sq(xb[0] - xa[0]) + sq(xb[1] - xa[1]) + sq(xb[2] - xa[2]));
+                            *minb = (d < *minb) ? d : *minb;
+                        }
+                        ivec_push(out, (int32_t)j);
+                    }
+                }
+            }
+        }
+    }
+    qsort(out->v, out->cnt, sizeof(int32_t), cmp_i32);
+}
+
+/* InitBond, pdDataManager2.cpp:12-70 */
+static int init_bonds(orc_state* s, int method, int nthreads)
 {
     const orc_problem* p = &s->p;
     int64_t n = s->n;
@@ -415,64 +503,25 @@ static int init_bonds(orc_state* s, int method)
                 }
             }
     } else {
-        /* cell list with edge >= delta + 1.5 dx: every pair the reference can accept lies
-         * in the 27 neighbouring cells; each pair is tested as (lower id, higher id) */
-        double lo[3] = {MEGA, MEGA, MEGA}, hi[3] = {-MEGA, -MEGA, -MEGA};
-        for (int64_t i = 0; i < n; ++i)
-            for (int c = 0; c < 3; ++c) {
-                double v = s->X[3 * i + c];
-                lo[c] = v < lo[c] ? v : lo[c]; hi[c] = v > hi[c] ? v : hi[c];
-            }
-        double h = p->horizon + 1.5 * p->dx;
-        int64_t nc[3];
-        for (int c = 0; c < 3; ++c) nc[c] = (n ? (int64_t)floor((hi[c] - lo[c]) / h) : 0) + 1;
-        int64_t ncell = nc[0] * nc[1] * nc[2];
-        int64_t* head = (int64_t*)malloc(sizeof(int64_t) * (ncell + 1));
-        int64_t* cid = (int64_t*)malloc(sizeof(int64_t) * (n ? n : 1));
-        int64_t* order = (int64_t*)malloc(sizeof(int64_t) * (n ? n : 1));
-        memset(head, 0, sizeof(int64_t) * (ncell + 1));
-        for (int64_t i = 0; i < n; ++i) {
-            int64_t q[3];
-            for (int c = 0; c < 3; ++c) {
-                q[c] = (int64_t)floor((s->X[3 * i + c] - lo[c]) / h);
-                if (q[c] >= nc[c]) q[c] = nc[c] - 1;
-            }
-            cid[i] = q[0] + nc[0] * (q[1] + nc[1] * q[2]);
-            head[cid[i] + 1]++;
+        /* each node's family is gathered by that node alone, so the node loop runs in
+         * parallel; every pair is still tested as (lower id, higher id) */
+        cell_grid g;
+        grid_build(s, &g);
+#ifdef _OPENMP
+#pragma omp parallel if (nthreads != 1)
+#endif
+        {
+            double mloc = minb;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 256)
+#endif
+            for (int64_t i = 0; i < n; ++i) gather_family(s, &g, i, &fam[i], &mloc);
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+            minb = (mloc < minb) ? mloc : minb;
         }
-        for (int64_t c = 0; c < ncell; ++c) head[c + 1] += head[c];
-        int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (ncell + 1));
-        memcpy(fill, head, sizeof(int64_t) * (ncell + 1));
-        for (int64_t i = 0; i < n; ++i) order[fill[cid[i]]++] = i;
-        free(fill);
-        for (int64_t i = 0; i < n; ++i) {
-            int64_t c0 = cid[i];
-            int64_t q0 = c0 % nc[0], q1 = (c0 / nc[0]) % nc[1], q2 = c0 / (nc[0] * nc[1]);
-            for (int64_t a = q2 - 1; a <= q2 + 1; ++a) {
-                if (a < 0 || a >= nc[2]) continue;
-                for (int64_t b = q1 - 1; b <= q1 + 1; ++b) {
-                    if (b < 0 || b >= nc[1]) continue;
-                    for (int64_t c = q0 - 1; c <= q0 + 1; ++c) {
-                        if (c < 0 || c >= nc[0]) continue;
-                        int64_t cc = c + nc[0] * (b + nc[1] * a);
-                        for (int64_t k = head[cc]; k < head[cc + 1]; ++k) {
-                            int64_t j = order[k];
-                            if (j <= i) continue;
-                            double dmin, dmax;
-                            if (orc_pair_bonded(p, &s->X[3 * i], &s->X[3 * j], &dmin, &dmax)) {
-                                const double* xa = &s->X[3 * i]; const double* xb = &s->X[3 * j];
-                                double d = sqrt(sq(xb[0] - xa[0]) + sq(xb[1] - xa[1]) + sq(xb[2] - xa[2]));
-                                minb = (d < minb) ? d : minb;
-                                ivec_push(&fam[i], (int32_t)j);
-                                ivec_push(&fam[j], (int32_t)i);
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        for (int64_t i = 0; i < n; ++i) qsort(fam[i].v, fam[i].cnt, sizeof(int32_t), cmp_i32);
-        free(head); free(cid); free(order);
+        grid_free(&g);
     }
     s->min_bond = minb;
     s->rp = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
@@ -486,6 +535,9 @@ static int init_bonds(orc_state* s, int method)
         free(fam[i].v);
     }
     free(fam);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256) if (nthreads != 1)
+#endif
     for (int64_t i = 0; i < n; ++i)
         for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
             int64_t j = s->col[e];
@@ -513,7 +565,7 @@ static void init_material(orc_state* s)
 double orc_spring(const orc_state* s, int m) { return s->spring[m]; }
 
 /* CalcBdryEffFactor, pdDataManager2.cpp:1353-1477 */
-static double bdryeff_energy(const orc_state* s, int64_t i)
+static double bdryeff_energy(const orc_state* s, int64_t i, const int32_t* fam, int64_t cnt)
 {
     const orc_problem* p = &s->p;
     double e11 = 1.0e-4;
@@ -526,8 +578,8 @@ static double bdryeff_energy(const orc_state* s, int64_t i)
     const double* xi_ = &s->X[3 * i];
     /* slit cuts do not enter: the factor is that of the intact family (a slit is a
      * pre-crack introduced after the surface calibration) */
-    for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
-        const double* xj = &s->X[3 * (int64_t)s->col[e]];
+    for (int64_t e = 0; e < cnt; ++e) {
+        const double* xj = &s->X[3 * (int64_t)fam[e]];
         double xi1 = xj[0] - xi_[0], xi2 = xj[1] - xi_[1], xi3 = xj[2] - xi_[2];
         double eta1 = eps11 * xi1 + eps12 * xi2 + eps13 * xi3;
         double eta2 = eps21 * xi1 + eps22 * xi2 + eps23 * xi3;
@@ -558,14 +610,11 @@ static double bdryeff_type_one(const orc_state* s, double h, int m)
     return full / part;
 }
 
-/* InitBdryEffectFactor, pdDataManager.cpp:1340-1473 */
-static void init_bdryeff(orc_state* s)
+/* The largest family, counted on a virtual lattice around the origin
+ * (pdDataManager.cpp:1345-1387). */
+static int virtual_max_family(const orc_state* s)
 {
     const orc_problem* p = &s->p;
-    int64_t n = s->n;
-    s->be = (double*)malloc(sizeof(double) * (n ? n : 1));
-    for (int64_t i = 0; i < n; ++i) s->be[i] = 1.0;
-    /* max family from a virtual lattice around the origin (pdDataManager.cpp:1345-1387) */
     int vd = 2 * ((int)(p->horizon / p->dx) + 1);
     double o[3] = {0.0, 0.0, 0.0};
     int maxf = 0;
@@ -577,31 +626,51 @@ static void init_bdryeff(orc_state* s)
                 dist_to_cube(o, xj, p->dx, &dmin, &dmax);
                 if (dmin < p->horizon + s->tol) ++maxf;
             }
-    s->max_family = maxf;
-    if (!p->bdryeff_on) return;
+    return maxf;
+}
+
+/* The boundary-effect factor of node i with family fam[0..cnt) (pdDataManager.cpp:1389-1473):
+ * 1 for a full family, the type-one closed form in a single-surface band, else the energy
+ * ratio. */
+static double bdryeff_node(const orc_state* s, int64_t i, const int32_t* fam, int64_t cnt, int maxf)
+{
+    const orc_problem* p = &s->p;
     double x1L = -p->nx * 0.5 * p->dx + p->cx, x1H = p->nx * 0.5 * p->dx + p->cx;
     double x2L = -p->ny * 0.5 * p->dx + p->cy, x2H = p->ny * 0.5 * p->dx + p->cy;
     double x3L = -p->nz * 0.5 * p->dx + p->cz, x3H = p->nz * 0.5 * p->dx + p->cz;
     double ns = ((int)(p->horizon / p->dx)) * p->dx;
-    for (int64_t i = 0; i < n; ++i) {
-        double x1 = s->X[3 * i], x2 = s->X[3 * i + 1], x3 = s->X[3 * i + 2];
-        int m = s->mat[i];
-        if (s->rp[i + 1] - s->rp[i] == maxf) { s->be[i] = 1.0; continue; }
-        if (x1 >= x1H - ns && x1 <= x1H && x2 >= x2L + ns && x2 <= x2H - ns && x3 >= x3L + ns && x3 <= x3H - ns)
-            s->be[i] = bdryeff_type_one(s, x1H - x1, m);
-        else if (x1 >= x1L && x1 <= x1L + ns && x2 >= x2L + ns && x2 <= x2H - ns && x3 >= x3L + ns && x3 <= x3H - ns)
-            s->be[i] = bdryeff_type_one(s, x1 - x1L, m);
-        else if (x2 >= x2H - ns && x2 <= x2H && x1 >= x1L + ns && x1 <= x1H - ns && x3 >= x3L + ns && x3 <= x3H - ns)
-            s->be[i] = bdryeff_type_one(s, x2H - x2, m);
-        else if (x2 >= x2L && x2 <= x2L + ns && x1 >= x1L + ns && x1 <= x1H - ns && x3 >= x3L + ns && x3 <= x3H - ns)
-            s->be[i] = bdryeff_type_one(s, x2 - x2L, m);
-        else if (x3 >= x3H - ns && x3 <= x3H && x1 >= x1L + ns && x1 <= x1H - ns && x2 >= x2L + ns && x2 <= x2H - ns)
-            s->be[i] = bdryeff_type_one(s, x3H - x3, m);
-        else if (x3 >= x3L && x3 <= x3L + ns && x1 >= x1L + ns && x1 <= x1H - ns && x2 >= x2L + ns && x2 <= x2H - ns)
-            s->be[i] = bdryeff_type_one(s, x3 - x3L, m);
-        else
-            s->be[i] = bdryeff_energy(s, i);
-    }
+    double x1 = s->X[3 * i], x2 = s->X[3 * i + 1], x3 = s->X[3 * i + 2];
+    int m = s->mat[i];
+    if (cnt == maxf) return 1.0;
+    if (x1 >= x1H - ns && x1 <= x1H && x2 >= x2L + ns && x2 <= x2H - ns && x3 >= x3L + ns && x3 <= x3H - ns)
+        return bdryeff_type_one(s, x1H - x1, m);
+    if (x1 >= x1L && x1 <= x1L + ns && x2 >= x2L + ns && x2 <= x2H - ns && x3 >= x3L + ns && x3 <= x3H - ns)
+        return bdryeff_type_one(s, x1 - x1L, m);
+    if (x2 >= x2H - ns && x2 <= x2H && x1 >= x1L + ns && x1 <= x1H - ns && x3 >= x3L + ns && x3 <= x3H - ns)
+        return bdryeff_type_one(s, x2H - x2, m);
+    if (x2 >= x2L && x2 <= x2L + ns && x1 >= x1L + ns && x1 <= x1H - ns && x3 >= x3L + ns && x3 <= x3H - ns)
+        return bdryeff_type_one(s, x2 - x2L, m);
+    if (x3 >= x3H - ns && x3 <= x3H && x1 >= x1L + ns && x1 <= x1H - ns && x2 >= x2L + ns && x2 <= x2H - ns)
+        return bdryeff_type_one(s, x3H - x3, m);
+    if (x3 >= x3L && x3 <= x3L + ns && x1 >= x1L + ns && x1 <= x1H - ns && x2 >= x2L + ns && x2 <= x2H - ns)
+        return bdryeff_type_one(s, x3 - x3L, m);
+    return bdryeff_energy(s, i, fam, cnt);
+}
+
+/* InitBdryEffectFactor, pdDataManager.cpp:1340-1473 */
+static void init_bdryeff(orc_state* s, int nthreads)
+{
+    const orc_problem* p = &s->p;
+    int64_t n = s->n;
+    s->be = (double*)malloc(sizeof(double) * (n ? n : 1));
+    for (int64_t i = 0; i < n; ++i) s->be[i] = 1.0;
+    s->max_family = virtual_max_family(s);
+    if (!p->bdryeff_on) return;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256) if (nthreads != 1)
+#endif
+    for (int64_t i = 0; i < n; ++i)
+        s->be[i] = bdryeff_node(s, i, &s->col[s->rp[i]], s->rp[i + 1] - s->rp[i], s->max_family);
 }
 
 /* Traction as body force (extension): a TRACTION bc's value is a total force,
@@ -901,13 +970,15 @@ static void fgi_entry(const orc_state* s, int64_t i, int64_t j, const double* ui
 
 int orc_init(orc_state* s, int method, int nthreads)
 {
-    (void)nthreads;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
     int rc = init_nodes(s);
     if (rc) return rc;
-    rc = init_bonds(s, method);
+    rc = init_bonds(s, method, nthreads);
     if (rc) return rc;
     init_material(s);
-    init_bdryeff(s);
+    init_bdryeff(s, nthreads);
     init_traction(s);
     if (s->p.fgi_on && !s->p.cci_on) {
         rc = init_fgi(s);
@@ -948,14 +1019,15 @@ void orc_set_bdryeff(orc_state* s, const double* be) { memcpy(s->be, be, sizeof(
 
 /* The CCI branch of the family loop of CalcBondBasedMotionRK4 (pdDataManager.cpp:1795-1864),
  * for node i with the displacement of every node read from u. */
-static void family_force(const orc_state* s, int64_t i, const double* u, double* F, double* el)
+static void row_force(const orc_state* s, int64_t i, const int32_t* fam, const uint8_t* fcut, int64_t cnt,
+                      const double* u, double* F, double* el)
 {
     const orc_problem* p = &s->p;
     const orc_mat* mi = &p->mat[s->mat[i]];
     double acc[3] = {0.0, 0.0, 0.0}, ea = 0.0;
-    for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
-        if (s->cut[e]) continue;
-        int64_t j = s->col[e];
+    for (int64_t e = 0; e < cnt; ++e) {
+        if (fcut[e]) continue;
+        int64_t j = fam[e];
         const orc_mat* mj = &p->mat[s->mat[j]];
         double xi[3] = {s->X[3 * j] - s->X[3 * i], s->X[3 * j + 1] - s->X[3 * i + 1], s->X[3 * j + 2] - s->X[3 * i + 2]};
         double eta[3] = {u[3 * j] - u[3 * i], u[3 * j + 1] - u[3 * i + 1], u[3 * j + 2] - u[3 * i + 2]};
@@ -970,6 +1042,11 @@ static void family_force(const orc_state* s, int64_t i, const double* u, double*
     }
     F[0] = acc[0]; F[1] = acc[1]; F[2] = acc[2];
     if (el) *el = ea;
+}
+
+static void family_force(const orc_state* s, int64_t i, const double* u, double* F, double* el)
+{
+    row_force(s, i, &s->col[s->rp[i]], &s->cut[s->rp[i]], s->rp[i + 1] - s->rp[i], u, F, el);
 }
 
 void orc_forces(const orc_state* s, const double* u, double* F, double* elast, int nthreads)
@@ -1229,6 +1306,133 @@ void orc_static_system(const orc_state* s, double* mask, double* uc, double* b)
             }
         }
     }
+}
+
+/* Families, cut flags and boundary-effect factors of selected rows only, without building
+ * the whole CSR: InitNode, then for each row the cell-list family search of InitBond
+ * (pdDataManager2.cpp:12-70), the slit test and InitBdryEffectFactor's per-node factor
+ * (pdDataManager.cpp:1340-1473).  Rows are written in the given order: rp (nrows + 1),
+ * col / cut (cap entries), be (nrows).  Returns the entry count, -1 if cap is too small,
+ * or a negative init code. */
+int64_t orc_sample_rows(const orc_problem* p, const int64_t* rows, int64_t nrows, int64_t* rp, int32_t* col,
+                        uint8_t* cut, double* be, int64_t cap, int nthreads)
+{
+    orc_state* s = orc_create(p);
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    int rc = init_nodes(s);
+    if (rc) {
+        orc_destroy(s);
+        return rc;
+    }
+    init_material(s);
+    const int maxf = virtual_max_family(s);
+    cell_grid g;
+    grid_build(s, &g);
+    ivec* fam = (ivec*)calloc(nrows ? nrows : 1, sizeof(ivec));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) if (nthreads != 1)
+#endif
+    for (int64_t k = 0; k < nrows; ++k) {
+        int64_t i = rows[k];
+        if (i < 0 || i >= s->n) continue;
+        gather_family(s, &g, i, &fam[k], NULL);
+        be[k] = p->bdryeff_on ? bdryeff_node(s, i, fam[k].v, fam[k].cnt, maxf) : 1.0;
+    }
+    grid_free(&g);
+    int64_t tot = 0;
+    rp[0] = 0;
+    for (int64_t k = 0; k < nrows; ++k) {
+        rp[k + 1] = rp[k] + fam[k].cnt;
+        tot += fam[k].cnt;
+    }
+    if (tot <= cap) {
+        for (int64_t k = 0; k < nrows; ++k) {
+            int64_t i = rows[k];
+            for (int64_t e = 0; e < fam[k].cnt; ++e) {
+                int64_t j = fam[k].v[e];
+                int64_t a = i < j ? i : j, b = i < j ? j : i;
+                col[rp[k] + e] = (int32_t)j;
+                cut[rp[k] + e] = (uint8_t)orc_pair_cut(p, &s->X[3 * a], &s->X[3 * b]);
+            }
+        }
+    }
+    for (int64_t k = 0; k < nrows; ++k) free(fam[k].v);
+    free(fam);
+    orc_destroy(s);
+    return tot <= cap ? tot : -1;
+}
+
+/* Family force sums (CCI, no body force) of the given rows only, for a displacement field u
+ * of every node (3N): the family loop of CalcBondBasedMotionRK4 (pdDataManager.cpp:1795-1864)
+ * over each row's family from the cell-list search, with the bdryEff factors of the row and of
+ * its neighbours computed from their own families.  F: 3 nrows, el: nrows (may be NULL). */
+int orc_sample_forces(const orc_problem* p, const int64_t* rows, int64_t nrows, const double* u, double* F,
+                      double* el, int nthreads)
+{
+    orc_state* s = orc_create(p);
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    int rc = init_nodes(s);
+    if (rc) {
+        orc_destroy(s);
+        return rc;
+    }
+    init_material(s);
+    const int maxf = virtual_max_family(s);
+    cell_grid g;
+    grid_build(s, &g);
+    ivec* fam = (ivec*)calloc(nrows ? nrows : 1, sizeof(ivec));
+    uint8_t** fcut = (uint8_t**)calloc(nrows ? nrows : 1, sizeof(uint8_t*));
+    s->be = (double*)malloc(sizeof(double) * (s->n ? s->n : 1));
+    unsigned char* need = (unsigned char*)calloc(s->n ? s->n : 1, 1);
+    for (int64_t i = 0; i < s->n; ++i) s->be[i] = 1.0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) if (nthreads != 1)
+#endif
+    for (int64_t k = 0; k < nrows; ++k) {
+        int64_t i = rows[k];
+        gather_family(s, &g, i, &fam[k], NULL);
+        fcut[k] = (uint8_t*)malloc(fam[k].cnt ? fam[k].cnt : 1);
+        for (int64_t e = 0; e < fam[k].cnt; ++e) {
+            int64_t j = fam[k].v[e];
+            int64_t a = i < j ? i : j, b = i < j ? j : i;
+            fcut[k][e] = (uint8_t)orc_pair_cut(p, &s->X[3 * a], &s->X[3 * b]);
+        }
+    }
+    for (int64_t k = 0; k < nrows; ++k) {
+        need[rows[k]] = 1;
+        for (int64_t e = 0; e < fam[k].cnt; ++e) need[fam[k].v[e]] = 1;
+    }
+    if (p->bdryeff_on) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) if (nthreads != 1)
+#endif
+        for (int64_t i = 0; i < s->n; ++i) {
+            if (!need[i]) continue;
+            ivec f = {0, 0, NULL};
+            gather_family(s, &g, i, &f, NULL);
+            s->be[i] = bdryeff_node(s, i, f.v, f.cnt, maxf);
+            free(f.v);
+        }
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) if (nthreads != 1)
+#endif
+    for (int64_t k = 0; k < nrows; ++k)
+        row_force(s, rows[k], fam[k].v, fcut[k], fam[k].cnt, u, &F[3 * k], el ? &el[k] : NULL);
+    grid_free(&g);
+    for (int64_t k = 0; k < nrows; ++k) {
+        free(fam[k].v);
+        free(fcut[k]);
+    }
+    free(fam);
+    free(fcut);
+    free(need);
+    orc_destroy(s);
+    return 0;
 }
 
 void orc_get_state(const orc_state* s, double* U, double* V)

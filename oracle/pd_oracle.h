@@ -120,6 +120,17 @@ int64_t orc_num_contrib(const orc_state* s);
 void orc_get_gauss(const orc_state* s, double* Xg, double* Ug);
 void orc_get_mls(const orc_state* s, int64_t* cb_rp, int32_t* cb, double* phi);
 
+/* Families (ascending), cut flags and bdryEff factors of the given rows only (for problems
+ * too large to build whole): rp (nrows + 1), col / cut (cap entries), be (nrows), rows in the
+ * given order.  Returns the entry count, -1 if it exceeds cap, or a negative init code. */
+int64_t orc_sample_rows(const orc_problem* p, const int64_t* rows, int64_t nrows, int64_t* rp, int32_t* col,
+                        uint8_t* cut, double* be, int64_t cap, int nthreads);
+
+/* Family force sums (CCI, no body force) and energy densities of the given rows only, for a
+ * displacement field u of every node (3N): F (3 nrows), el (nrows, may be NULL). */
+int orc_sample_forces(const orc_problem* p, const int64_t* rows, int64_t nrows, const double* u, double* F,
+                      double* el, int nthreads);
+
 /* 0: reference pow(x, 2.0) via glibc (default); 1: correctly rounded square x*x */
 void orc_set_square_mode(int exact);
 

@@ -114,6 +114,10 @@ def lib() -> C.CDLL:
         L.orc_num_contrib.argtypes = [vp]
         L.orc_get_gauss.argtypes = [vp, dp, dp]
         L.orc_get_mls.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), dp]
+        L.orc_sample_forces.argtypes = [C.POINTER(Problem), C.POINTER(C.c_int64), C.c_int64, dp, dp, dp, C.c_int]
+        L.orc_sample_rows.restype = C.c_int64
+        L.orc_sample_rows.argtypes = [C.POINTER(Problem), C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_uint8), dp, C.c_int64, C.c_int]
         _lib = L
     return _lib
 
@@ -191,6 +195,40 @@ def problem_from_spec(spec: Dict[str, Any]) -> Problem:
             p.slit[i].p[k] = float(v)
     p.traction_body = int(spec.get("traction_body", 0))
     return p
+
+
+def sample_rows(spec: Dict[str, Any], rows, nthreads: int = 0):
+    """Families, cut flags and bdryEff of the given rows only (for problems too large to build
+    whole): (rp, col, cut, be), rows in the given order."""
+    p = problem_from_spec(spec)
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    nr = len(rows)
+    cap = 512 * max(nr, 1)
+    rp = np.zeros(nr + 1, dtype=np.int64)
+    col = np.zeros(cap, dtype=np.int32)
+    cut = np.zeros(cap, dtype=np.uint8)
+    be = np.zeros(max(nr, 1))
+    tot = lib().orc_sample_rows(C.byref(p), rows.ctypes.data_as(C.POINTER(C.c_int64)), nr,
+                                rp.ctypes.data_as(C.POINTER(C.c_int64)), col.ctypes.data_as(C.POINTER(C.c_int32)),
+                                cut.ctypes.data_as(C.POINTER(C.c_uint8)), _dp(be), cap, nthreads)
+    if tot < 0:
+        raise RuntimeError(f"orc_sample_rows failed ({tot})")
+    return rp, col[:tot].copy(), cut[:tot].copy(), be[:nr].copy()
+
+
+def sample_forces(spec: Dict[str, Any], rows, u, nthreads: int = 0):
+    """Family force sums and energy densities of the given rows only, for the displacement field
+    u of every node: (F (nrows, 3), el (nrows))."""
+    p = problem_from_spec(spec)
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+    F = np.zeros(3 * max(len(rows), 1))
+    el = np.zeros(max(len(rows), 1))
+    rc = lib().orc_sample_forces(C.byref(p), rows.ctypes.data_as(C.POINTER(C.c_int64)), len(rows), _dp(u), _dp(F),
+                                 _dp(el), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"orc_sample_forces failed ({rc})")
+    return F[: 3 * len(rows)].reshape(-1, 3), el[: len(rows)]
 
 
 def set_square_mode(exact: bool) -> None:

@@ -88,6 +88,7 @@ def lib() -> C.CDLL:
     L.pdm_last_kernel_timing.argtypes = [vp, dp, i64p]
     ip = C.POINTER(C.c_int)
     L.pdm_get_tuning.argtypes = [vp, ip, ip, ip, dp]
+    L.pdm_get_stage_launch.argtypes = [vp, ip, i64p]
     L.pdm_num_gauss_points.argtypes = [vp]
     L.pdm_num_live_gauss_pairs.restype = C.c_int64
     L.pdm_num_live_gauss_pairs.argtypes = [vp]
@@ -261,6 +262,12 @@ class Problem:
         t, w, f, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         self._chk(self.L.pdm_get_tuning(self.h, C.byref(t), C.byref(w), C.byref(f), C.byref(ms)))
         return dict(tile_edge=t.value, waves_per_tile=w.value, families_per_wave=f.value, tuned_stage_ms=ms.value)
+
+    def stage_launch(self) -> dict:
+        """Register budget (128, or 80) and LDS bytes per workgroup of the RK4 stage kernel."""
+        v, b = C.c_int(), C.c_int64()
+        self._chk(self.L.pdm_get_stage_launch(self.h, C.byref(v), C.byref(b)))
+        return dict(max_vgprs=v.value, lds_bytes=b.value)
 
     def solve_dynamic(self, out_dir: str) -> None:
         self._chk(self.L.pdm_solve_dynamic(self.h, out_dir.encode()))

@@ -145,15 +145,21 @@ int build_corr(pdm_handle* h)
 {
     if (h->opt.precision == PDM_FP64 || h->ntl == 0) return PDM_OK;
     int* cl = nullptr;
+    DevTemps temps(h->stream);
+    temps.add(&cl);
     HCHK(hipMalloc(&cl, sizeof(int)));
     HCHK(hipMemsetAsync(cl, 0, sizeof(int), h->stream));
     HCHK(launch_fly_corr(tile_view(h), h->u[0], h->corr, cl, h->stream));
     int clamped = 0;
     HCHK(hipStreamSynchronize(h->stream));
     HCHK(hipMemcpy(&clamped, cl, sizeof(int), hipMemcpyDeviceToHost));
-    HCHK(hipFree(cl));
     if (h->opt.verbose)
         fprintf(stderr, "pdm_init: %d family entries outside the 8-bit coefficient correction range\n", clamped);
+    // a clamped correction leaves that entry's FP32 coefficient off by more than 2^-27
+    // relative: refuse the layout rather than run with it
+    if (clamped)
+        return fail(h, PDM_E_LIMIT, std::to_string(clamped) +
+                                        " family entries need an FP32 coefficient correction outside the 8-bit range");
     return PDM_OK;
 }
 
@@ -226,6 +232,17 @@ int upload_state(pdm_handle* h, const double* U, const double* V)
 
 int alloc_parts(pdm_handle* h);
 
+// Release every captured chunk of steps.
+void drop_graphs(pdm_handle* h)
+{
+    for (int q = 0; q < pdm_handle::kGraphSlots; ++q) {
+        if (h->graph_exec[q]) (void)hipGraphExecDestroy(h->graph_exec[q]);
+        h->graph_exec[q] = nullptr;
+        h->graph_steps[q] = 0;
+    }
+    h->graph_next = 0;
+}
+
 int ensure_chunk(pdm_handle* h, int cap)
 {
     if (h->chunk_cap >= cap) return PDM_OK;
@@ -234,11 +251,7 @@ int ensure_chunk(pdm_handle* h, int cap)
     HCHK(hipMalloc(&h->steps, sizeof(StepParam) * cap * h->nspec));
     HCHK(hipMalloc(&h->energy, sizeof(double) * 2 * cap * h->nspec));
     h->chunk_cap = cap;
-    if (h->graph_exec) {
-        (void)hipGraphExecDestroy(h->graph_exec);
-        h->graph_exec = nullptr;
-        h->graph_steps = 0;
-    }
+    drop_graphs(h);  // the captured launches point at the old buffers
     return h->tiles ? alloc_parts(h) : PDM_OK;  // one row of tile partials per step
 }
 
@@ -276,20 +289,32 @@ int run_chunk(pdm_handle* h, const std::vector<StepParam>& sp)
     int rc = ensure_chunk(h, std::max(count, h->opt.graph_chunk));
     if (rc) return rc;
     HCHK(hipMemcpyAsync(h->steps, sp.data(), sizeof(StepParam) * sp.size(), hipMemcpyHostToDevice, h->stream));
-    bool graph = h->opt.use_graph && count == h->opt.graph_chunk;
-    if (graph && !h->graph_exec) {
-        hipGraph_t g;
-        HCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_steps(h, count);
-        hipError_t ce = hipStreamEndCapture(h->stream, &g);
-        if (rc) return rc;
-        HCHK(ce);
-        HCHK(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
-        HCHK(hipGraphDestroy(g));
-        h->graph_steps = count;
+    // every chunk runs as a captured graph (one per distinct step count, cached)
+    hipGraphExec_t exec = nullptr;
+    if (h->opt.use_graph) {
+        for (int q = 0; q < pdm_handle::kGraphSlots; ++q)
+            if (h->graph_exec[q] && h->graph_steps[q] == count) exec = h->graph_exec[q];
+        if (!exec) {
+            const int q = h->graph_next;
+            h->graph_next = (q + 1) % pdm_handle::kGraphSlots;
+            if (h->graph_exec[q]) (void)hipGraphExecDestroy(h->graph_exec[q]);
+            h->graph_exec[q] = nullptr;
+            h->graph_steps[q] = 0;
+            hipGraph_t g;
+            HCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_steps(h, count);
+            hipError_t ce = hipStreamEndCapture(h->stream, &g);
+            if (rc) return rc;
+            HCHK(ce);
+            hipError_t ie = hipGraphInstantiate(&h->graph_exec[q], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HCHK(ie);
+            h->graph_steps[q] = count;
+            exec = h->graph_exec[q];
+        }
     }
     HCHK(hipEventRecord(h->ev0, h->stream));
-    if (graph) HCHK(hipGraphLaunch(h->graph_exec, h->stream));
+    if (exec) HCHK(hipGraphLaunch(exec, h->stream));
     else {
         rc = enqueue_steps(h, count);
         if (rc) return rc;
@@ -780,7 +805,7 @@ int pdm_destroy(pdm_handle* h)
     if (!h) return PDM_E_ARG;
     (void)hipSetDevice(h->dev);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    drop_graphs(h);
     dfree(h->Xd); dfree(h->rp); dfree(h->col); dfree(h->cut); dfree(h->be); dfree(h->matd);
     dfree(h->permd); dfree(h->Xi); dfree(h->u[0]); dfree(h->u[1]); dfree(h->rpi); dfree(h->slot);
     dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->bk); dfree(h->corr); dfree(h->mati);
@@ -1105,6 +1130,17 @@ int pdm_get_tuning(const pdm_handle* h, int* tile_edge, int* waves_per_tile, int
     if (waves_per_tile) *waves_per_tile = h->waves;
     if (families_per_wave) *families_per_wave = h->fam_per_wave;
     if (stage_ms) *stage_ms = h->tuned ? h->tune_ms : 0.0;
+    return PDM_OK;
+}
+
+int pdm_get_stage_launch(const pdm_handle* h, int* max_vgprs, int64_t* lds_bytes)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (h->fgi_ng) return fail(h, PDM_E_MODEL, "fixed Gaussian decks step with their own kernels");
+    const size_t lds = stage_lds_bytes(tile_view(h), !h->uniform_mat, true, h->opt.precision != PDM_FP64);
+    if (max_vgprs) *max_vgprs = stage_vgpr_cap(tile_view(h), h->opt.precision, lds);
+    if (lds_bytes) *lds_bytes = (int64_t)lds;
     return PDM_OK;
 }
 

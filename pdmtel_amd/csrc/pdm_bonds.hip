@@ -82,6 +82,10 @@ int scan_exclusive(pdm_handle* h, const int64_t* in, int64_t* out, int64_t n)
     int64_t* sums_scan = nullptr;
     // scan n+1 elements with in[n] treated as 0: copy through a padded buffer
     int64_t* tmp = nullptr;
+    DevTemps temps(s);
+    temps.add(&tmp);
+    temps.add(&sums);
+    temps.add(&sums_scan);
     HIPCHK(hipMalloc(&tmp, sizeof(int64_t) * (n + 1)));
     HIPCHK(hipMemsetAsync(tmp + n, 0, sizeof(int64_t), s));
     if (n) HIPCHK(hipMemcpyAsync(tmp, in, sizeof(int64_t) * n, hipMemcpyDeviceToDevice, s));
@@ -97,9 +101,6 @@ int scan_exclusive(pdm_handle* h, const int64_t* in, int64_t* out, int64_t n)
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(hipFree(tmp));
-    HIPCHK(hipFree(sums));
-    HIPCHK(hipFree(sums_scan));
     return PDM_OK;
 }
 
@@ -347,6 +348,16 @@ int build_bonds(pdm_handle* h)
     int64_t *cell_id = nullptr, *cell_cnt = nullptr, *cell_start = nullptr, *cursor = nullptr, *cnt = nullptr;
     int32_t* cell_nodes = nullptr;
     unsigned long long* min_bits = nullptr;
+    int8_t* branch_d = nullptr;
+    DevTemps temps(s);  // the cell-list scratch, released on every return path
+    temps.add(&cell_id);
+    temps.add(&cell_cnt);
+    temps.add(&cell_start);
+    temps.add(&cursor);
+    temps.add(&cnt);
+    temps.add(&cell_nodes);
+    temps.add(&min_bits);
+    temps.add(&branch_d);
     HIPCHK(hipMalloc(&cell_id, sizeof(int64_t) * std::max<int64_t>(n, 1)));
     HIPCHK(hipMalloc(&cell_cnt, sizeof(int64_t) * ncell));
     HIPCHK(hipMalloc(&cell_start, sizeof(int64_t) * (ncell + 1)));
@@ -405,13 +416,6 @@ int build_bonds(pdm_handle* h)
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(hipFree(cell_id));
-    HIPCHK(hipFree(cell_cnt));
-    HIPCHK(hipFree(cell_start));
-    HIPCHK(hipFree(cursor));
-    HIPCHK(hipFree(cell_nodes));
-    HIPCHK(hipFree(cnt));
-    HIPCHK(hipFree(min_bits));
 
     // InitBdryEffectFactor branches (pdDataManager.cpp:1340-1473)
     h->max_family = count_max_family(d.horizon, d.dx, d.tol);
@@ -449,7 +453,6 @@ int build_bonds(pdm_handle* h)
                 branch[i] = 2;
         }
     }
-    int8_t* branch_d = nullptr;
     HIPCHK(hipMalloc(&h->be, sizeof(double) * std::max<int64_t>(n, 1)));
     HIPCHK(hipMalloc(&branch_d, std::max<int64_t>(n, 1)));
     HIPCHK(hipMemcpyAsync(h->be, beh.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
@@ -479,7 +482,6 @@ int build_bonds(pdm_handle* h)
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(hipFree(branch_d));
     return PDM_OK;
 }
 

@@ -175,6 +175,27 @@ struct FgiArgs {
     const uint32_t* pairs;    // (entry offset in the row) << 16 | point
 };
 
+// Device temporaries of a set-up routine, released on every return path once the
+// stream has drained: register each pointer (still null) before allocating it.
+struct DevTemps {
+    hipStream_t s;
+    std::vector<void**> p;
+    explicit DevTemps(hipStream_t st) : s(st) {}
+    template <class T>
+    void add(T** q) { p.push_back(reinterpret_cast<void**>(q)); }
+    ~DevTemps()
+    {
+        (void)hipStreamSynchronize(s);
+        for (void** q : p)
+            if (*q) {
+                (void)hipFree(*q);
+                *q = nullptr;
+            }
+    }
+    DevTemps(const DevTemps&) = delete;
+    DevTemps& operator=(const DevTemps&) = delete;
+};
+
 // Bond-build parameters.
 struct BondArgs {
     const d4* X;
@@ -283,9 +304,12 @@ struct pdm_handle {
     uint32_t* fgi_pairs = nullptr;
     int64_t fgi_npairs = 0;
 
-    // graph cache
-    hipGraphExec_t graph_exec = nullptr;
-    int graph_steps = 0;
+    // graph cache: the captured chunks of RK4 steps, keyed by their step count (a run of
+    // nsteps uses at most two: full chunks and the remainder)
+    static constexpr int kGraphSlots = 2;
+    hipGraphExec_t graph_exec[kGraphSlots] = {nullptr, nullptr};
+    int graph_steps[kGraphSlots] = {0, 0};
+    int graph_next = 0;             // slot the next capture replaces
 };
 
 namespace pdm {
@@ -336,5 +360,6 @@ hipError_t launch_fgi_pairs(const TileView& tv, const FgiArgs& f, bool fill, int
 // static solve (pdm_cg.cpp / pdm_cg.hip)
 int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_stats* stats);
 size_t stage_lds_bytes(const TileView& tv, bool with_ecrit, bool with_epi, bool f32);
+int stage_vgpr_cap(const TileView& tv, int precision, size_t lds);
 
 }  // namespace pdm

@@ -1577,10 +1577,7 @@ hipError_t launch_stage_t(const StageArgs& a, int step_idx, size_t lds, hipStrea
 template <int STAGE, int PREC, int NW, int NPW>
 hipError_t launch_stage_e(const StageArgs& a, int step_idx, size_t lds, hipStream_t s)
 {
-    // tiles per CU: 16 / NW waves at 4 per SIMD, 24 / NW at 6, LDS (dynamic + the static
-    // energy partials) permitting
-    const size_t per_tile = lds + sizeof(double) * 2 * NW;
-    const bool occ6 = PREC != PDM_FP64 && a.tv.uniform_mat && kLdsBytes / per_tile > (size_t)(16 / NW);
+    const bool occ6 = stage_vgpr_cap(a.tv, PREC, lds) == 80;
     if (a.energies) {
         if (a.tv.uniform_mat) {
             if constexpr (PREC != PDM_FP64)
@@ -1640,6 +1637,18 @@ hipError_t launch_gather_u(const TileView& tv, const d4* u, d4* y, double* elast
 }
 
 }  // namespace
+
+// Register budget of the stage kernel a launch takes.  Tiles per CU: 16 / NW waves at 4
+// per SIMD, 24 / NW at 6, LDS (dynamic + the static energy partials) permitting; the
+// 80-VGPR kernel (6 waves per SIMD) exists for the FP32 operations on uniform-material
+// tiles and is taken when the LDS holds more tiles per CU than the 128-VGPR one can use.
+int stage_vgpr_cap(const TileView& tv, int precision, size_t lds)
+{
+    const int nw = tv.waves == 4 ? 4 : 8;
+    const size_t per_tile = lds + sizeof(double) * 2 * nw;
+    const bool occ6 = precision != PDM_FP64 && tv.uniform_mat && kLdsBytes / per_tile > (size_t)(16 / nw);
+    return occ6 ? 80 : 128;
+}
 
 size_t stage_lds_bytes(const TileView& tv, bool with_ecrit, bool with_epi, bool f32)
 {

@@ -564,17 +564,15 @@ int build_tiles(pdm_handle* h)
     int64_t* halo_len = nullptr;
     int* err = nullptr;
     // temporaries are released on every return path (after the stream has drained)
-    struct Temps {
-        hipStream_t s;
-        std::vector<void**> p;
-        ~Temps()
-        {
-            (void)hipStreamSynchronize(s);
-            for (void** q : p)
-                if (*q) (void)hipFree(*q);
-        }
-    } temps{s, {(void**)&ipermd, (void**)&tileidd, (void**)&bcref, (void**)&ecm, (void**)&bei, (void**)&packed,
-                (void**)&halo_len, (void**)&err}};
+    DevTemps temps(s);
+    temps.add(&ipermd);
+    temps.add(&tileidd);
+    temps.add(&bcref);
+    temps.add(&ecm);
+    temps.add(&bei);
+    temps.add(&packed);
+    temps.add(&halo_len);
+    temps.add(&err);
     const int64_t n1 = std::max<int64_t>(n, 1), E1 = std::max<int64_t>(Ei, 1);
     HIPCHK(hipMalloc(&h->permd, sizeof(int32_t) * n1));
     HIPCHK(hipMalloc(&ipermd, sizeof(int32_t) * n1));

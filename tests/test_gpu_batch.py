@@ -29,7 +29,7 @@ def test_batch_equals_single(precision):
     decks = [pm.deck_text(s) for s in specs]
     b = pm.Batch(decks, precision=precision, graph_chunk=5, **LAYOUT)
     print(f"[batch] precision={precision} nodes={b.n} entries={sum(b.E)}", flush=True)
-    assert b.step(7) == 7  # one captured chunk of 5 steps, then 2 eager steps
+    assert b.step(7) == 7  # a captured chunk of 5 steps, then one of 2
     for k, d in enumerate(decks):
         p = pm.Problem(d, precision=precision, graph_chunk=5, **LAYOUT)
         done = p.step(7)

@@ -136,6 +136,34 @@ def test_brute_force_equals_cells_and_symmetry():
         assert np.all(np.diff(r) > 0)
 
 
+def test_sampled_rows_equal_whole_csr():
+    """The row sampler (families of selected nodes only, for problems too large to build whole)
+    returns exactly the rows, cut flags and bdryEff factors of the whole-problem oracle, which
+    the O(N^2) reference loop pins above; the whole-problem search also agrees for any thread
+    count."""
+    spec = small_bar(nx=14, ny=9, nz=7, jitter=0.01, seed=2, delta_over_dx=4.015)
+    spec["slits"] = [pm.mid_slit(spec, 30.0), dict(type=1, box=(0.0, 1.0, 0.004, 0.004, 0.0, 1.0))]
+    whole = pyoracle.Oracle(spec, bond_method=1)
+    serial = pyoracle.Oracle(spec, bond_method=1, nthreads=1)
+    rp, col, cut = whole.csr()
+    rps, cols, cuts = serial.csr()
+    assert np.array_equal(rp, rps) and np.array_equal(col, cols) and np.array_equal(cut, cuts)
+    assert np.array_equal(whole.bdryeff(), serial.bdryeff()) and whole.min_bond == serial.min_bond
+    be = whole.bdryeff()
+    rows = np.random.default_rng(1).permutation(whole.n)[:150]
+    rows = np.concatenate([rows, [0, whole.n - 1]])
+    srp, scol, scut, sbe = pyoracle.sample_rows(spec, rows)
+    assert cut.sum() > 0 and (be != 1.0).any()
+    for k, i in enumerate(rows):
+        assert np.array_equal(scol[srp[k]:srp[k + 1]], col[rp[i]:rp[i + 1]])
+        assert np.array_equal(scut[srp[k]:srp[k + 1]], cut[rp[i]:rp[i + 1]])
+    assert np.array_equal(sbe, be[rows])
+    u = np.random.default_rng(3).uniform(-1e-6, 1e-6, size=(whole.n, 3))
+    F, el = whole.forces(u, with_elast=True)
+    sF, sel = pyoracle.sample_forces(spec, rows, u)
+    assert np.array_equal(sF, F[rows]) and np.array_equal(sel, el[rows])
+
+
 # ---------------------------------------------------------------- material / beta / time
 def test_spring_constant_and_dt_closed_form():
     spec = small_bar()

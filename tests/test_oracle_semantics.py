@@ -1,0 +1,325 @@
+"""Pins the oracle's static system, energy totals and boundary-condition kinematics.
+
+Each check compares the oracle with a value fixed outside its code: a closed form of the
+reference's definitions evaluated by hand in the test, or the physics the model must
+reproduce (uniaxial stress: strain sigma / E and Poisson ratio 1/4 of bond-based
+peridynamics).  CPU only.
+"""
+
+import math
+
+import numpy as np
+import pytest
+from scipy.sparse.linalg import LinearOperator, cg
+
+import pdmtel_amd as pm
+import pyoracle
+
+DX = 1e-3
+E0, RHO = 72e9, 2440.0
+BIG = 1.0
+
+
+def box(x0, x1, y0=-BIG, y1=BIG, z0=-BIG, z1=BIG):
+    return (x0, x1, y0, y1, z0, z1)
+
+
+def time_factor(bc, t):
+    """pdBdryConditionDispl.cpp:15-27 (DISPL, DISPLGRAD, TRACTION) and pdBdryConditionVeloc.cpp:15-18."""
+    if bc["type"] == 2:
+        return 1.0 if t <= bc["t_end"] else 0.0
+    if bc["load"] == 1.0:
+        return t / bc["t_end"] if t <= bc["t_end"] else 1.0
+    return 1.0
+
+
+# ---------------------------------------------------------------- static system
+def test_static_system_rows_closed_form():
+    """Per component: free mask, prescribed value and load of every BC kind (the intent of
+    pdSolve::AssembleEquation, pdSolve.cpp:227-363): DISPL value * factor(T), VELOC value * T,
+    DISPLGRAD (G X) * factor(T), TRACTION force / nodes / dx^3 * factor(T) as a body force, and
+    gravity rho g once per unbroken family entry."""
+    spec = pm.bar_spec(8, 4, 4, steps=10)
+    T = 1e-3
+    spec["total_time"] = T
+    g = (3.0e3, -2.0e3, 1.0e3)
+    spec["materials"][0]["gravity"] = g
+    G = [1e-3, 2e-3, -1e-3, 0.0, 0.0, 0.0, 5e-4, 0.0, -2e-3]
+    spec["bcs"] = [
+        dict(type=1, flags=(1, 1, 1), value=(1e-6, 2e-6, -1e-6), t_end=2 * T, load=1.0),  # ramp: factor 1/2
+        dict(type=2, flags=(0, 1, 0), value=(0.0, 5e-7, 0.0), t_end=2 * T, load=1.0),      # VELOC: value T
+        dict(type=3, flags=(1, 0, 1), value=tuple(G), t_end=0.5 * T, load=2),             # step: factor 1
+        dict(type=4, flags=(0, 1, 0), value=(0.0, 3.0, 0.0), t_end=4 * T, load=1.0),      # ramp: factor 1/4
+    ]
+    L = 8 * DX
+    spec["boundary_spaces"] = [
+        dict(geo=1, ref=1, box=box(-BIG, 1.0 * DX)),
+        dict(geo=1, ref=2, box=box(3.0 * DX, 4.0 * DX)),
+        dict(geo=1, ref=3, box=box(L - 1.0 * DX, BIG)),
+        dict(geo=1, ref=4, box=box(5.0 * DX, 6.0 * DX)),
+    ]
+    o = pyoracle.Oracle(spec)
+    X, _, bc = o.nodes()
+    rp, col, cut = o.csr()
+    m, uc, b = o.static_system()
+    cnt4 = int((bc == 3).sum())
+    assert cnt4 > 0 and all((bc == k).sum() > 0 for k in range(4))
+    for i in range(o.n):
+        nb = int(rp[i + 1] - rp[i])
+        want_b = [nb * RHO * g[c] for c in range(3)]
+        want_m, want_u = [1.0, 1.0, 1.0], [0.0, 0.0, 0.0]
+        k = int(bc[i])
+        if k == 0:
+            want_m = [0.0, 0.0, 0.0]
+            want_u = [1e-6 * 0.5, 2e-6 * 0.5, -1e-6 * 0.5]
+        elif k == 1:
+            want_m[1], want_u[1] = 0.0, 5e-7 * T
+        elif k == 2:
+            want_m[0] = want_m[2] = 0.0
+            want_u[0] = G[0] * X[i, 0] + G[1] * X[i, 1] + G[2] * X[i, 2]
+            want_u[2] = G[6] * X[i, 0] + G[7] * X[i, 1] + G[8] * X[i, 2]
+        elif k == 3:
+            want_b[1] += 3.0 / cnt4 / DX ** 3 * 0.25
+        for c in range(3):
+            assert m[i, c] == want_m[c], (i, c)
+            assert math.isclose(uc[i, c], want_u[c], rel_tol=1e-15, abs_tol=0.0), (i, c)
+            assert math.isclose(b[i, c], want_b[c], rel_tol=1e-14, abs_tol=0.0), (i, c)
+
+
+def solve_static(o):
+    """The oracle's static system solved by CG on its matrix-free tangent (free components)."""
+    m, uc, b = o.static_system()
+    free = m.reshape(-1) > 0
+    ucf = uc.reshape(-1)
+    rhs = b.reshape(-1) - o.tangent(ucf).reshape(-1)
+
+    def mv(x):
+        full = np.zeros_like(ucf)
+        full[free] = x
+        return o.tangent(full).reshape(-1)[free]
+
+    op = LinearOperator((int(free.sum()), int(free.sum())), matvec=mv)
+    x, info = cg(op, rhs[free], rtol=1e-12, maxiter=20000)
+    assert info == 0
+    u = ucf.copy()
+    u[free] = x
+    return u.reshape(-1, 3)
+
+
+def middle_slopes(X, U, L):
+    """Least-squares du_x/dx and du_y/dy, du_z/dz over the nodes of the middle third."""
+    sel = (X[:, 0] > L / 3) & (X[:, 0] < 2 * L / 3)
+    A = np.c_[np.ones(sel.sum()), X[sel]]
+    cx = np.linalg.lstsq(A, U[sel, 0], rcond=None)[0]
+    cy = np.linalg.lstsq(A, U[sel, 1], rcond=None)[0]
+    cz = np.linalg.lstsq(A, U[sel, 2], rcond=None)[0]
+    return cx[1], cy[2], cz[3]
+
+
+def lattice_stiffness_ratio(delta_over_dx):
+    """p_grid / p_inf of a full lattice family under isotropic strain: the discrete CCI sum
+    sum_j 1/2 (1/2 (c/r)(eps r)^2) beta dx^3 over the lattice offsets the bond criterion
+    accepts (point-to-cube distance < delta), over the continuum 3 E eps^2, written out
+    here from the definitions (c = 12 E / (pi delta^4), modified beta)."""
+    d = delta_over_dx
+    c = 12.0 / (math.pi * d ** 4)  # E = dx = 1
+    eps = 1e-4
+    tot = 0.0
+    R = range(-int(d) - 2, int(d) + 3)
+    for i in R:
+        for j in R:
+            for k in R:
+                if i == j == k == 0:
+                    continue
+                r = math.sqrt(i * i + j * j + k * k)
+                q = [max(abs(a) - 0.5, 0.0) for a in (i, j, k)]
+                if math.sqrt(sum(x * x for x in q)) >= d:
+                    continue
+                beta = min(1.0, max(0.0, d + 0.5 - r))
+                tot += 0.5 * (0.5 * (c / r) * (eps * r) ** 2) * beta
+    return tot / (3.0 * eps * eps)
+
+
+def test_static_uniaxial_bar_strain_and_poisson_ratio():
+    """A bar fixed at x = 0 and pulled with 1 MPa at x = L.  The lattice CCI sum is 5.9 %
+    stiffer than the continuum (lattice_stiffness_ratio), so an infinite body would strain
+    by sigma / E_eff with E_eff = 1.059 E.  On a 14 x 14 section every node lies within a few
+    horizons of a surface: without the boundary-effect factors the surface layers are too
+    soft, with them too stiff under uniaxial stress (the factor is calibrated on isotropic
+    strain), so the middle-third strains of the two runs bracket sigma / E_eff, each within
+    15 %; the lateral strains are -nu exx with nu = 1/4 (bond-based peridynamics); the
+    reaction at the fixed layer balances the applied force sigma A; the response is
+    linear in the load."""
+    sigma = 1e6
+    nyz = 14
+    ratio = lattice_stiffness_ratio(3.015)
+    assert abs(ratio - 1.059) < 1e-3
+    eps_eff = sigma / (E0 * ratio)
+    res = {}
+    for on in (1, 0):
+        spec = pm.bar_spec(30, nyz, nyz, steps=1, traction=sigma, bdryeff_on=on)
+        o = pyoracle.Oracle(spec)
+        X, _, bc = o.nodes()
+        U = solve_static(o)
+        exx, eyy, ezz = middle_slopes(X, U, 30 * DX)
+        print(f"bdryEff {on}: exx / (sigma/E_eff) = {exx / eps_eff:.4f}, nu = {-eyy / exx:.4f} {-ezz / exx:.4f}")
+        res[on] = exx
+        assert abs(exx / eps_eff - 1.0) < 0.15
+        assert abs(-eyy / exx - 0.25) < 0.01 and abs(-ezz / exx - 0.25) < 0.01
+        # reaction of the fixed layer: internal forces sum to zero, so it balances the load
+        Ku = o.tangent(U)
+        _, _, b = o.static_system()
+        area = (nyz * DX) ** 2
+        vol = DX ** 3  # b and K u are force densities (per node volume)
+        assert math.isclose(b[:, 0].sum() * vol, sigma * area, rel_tol=1e-12)
+        assert math.isclose(Ku[bc == 0, 0].sum() * vol, -sigma * area, rel_tol=1e-6)
+    assert res[1] < eps_eff < res[0]
+    # half the load, half the response
+    spec2 = pm.bar_spec(30, nyz, nyz, steps=1, traction=sigma / 2)
+    U2 = solve_static(pyoracle.Oracle(spec2))
+    spec1 = pm.bar_spec(30, nyz, nyz, steps=1, traction=sigma)
+    U1 = solve_static(pyoracle.Oracle(spec1))
+    assert np.allclose(U2, U1 / 2, rtol=0, atol=1e-9 * np.abs(U1).max())
+
+
+def test_static_prescribed_ends_veloc_displ_displgrad():
+    """The same end displacement d prescribed three ways gives one solution: DISPL value d
+    (step), VELOC value d / T (prescribed value * total_time), DISPLGRAD G with G_xx = d / x_L
+    on the x = L layer (its x = 0 layer fixed as DISPL); the interior strain is then close to
+    d / (x_L - x_0) and lateral strains -eps / 4."""
+    T = 2e-3
+    d = 2e-6
+    spec = pm.bar_spec(24, 8, 8, steps=1)
+    spec["total_time"] = T
+    spec["traction_body"] = 0
+    L = 24 * DX
+    xl = L - 0.5 * DX
+    fixed = dict(type=1, flags=(1, 1, 1), value=(0.0, 0.0, 0.0), t_end=1.0, load=2)
+    variants = {
+        "displ": dict(type=1, flags=(1, 0, 0), value=(d, 0.0, 0.0), t_end=1.0, load=2),
+        "veloc": dict(type=2, flags=(1, 0, 0), value=(d / T, 0.0, 0.0), t_end=1.0, load=2),
+        "displgrad": dict(type=3, flags=(1, 0, 0), value=(d / xl, 0, 0, 0, 0, 0, 0, 0, 0), t_end=1.0, load=2),
+    }
+    sols = {}
+    for name, bcx in variants.items():
+        s = dict(spec)
+        s["bcs"] = [fixed, bcx]
+        o = pyoracle.Oracle(s)
+        X, _, bc = o.nodes()
+        m, uc, _ = o.static_system()
+        end = bc == 1
+        assert np.allclose(uc[end, 0], d, rtol=1e-13, atol=0) and np.all(m[end, 0] == 0)
+        sols[name] = solve_static(o)
+    for name in ("veloc", "displgrad"):
+        assert np.allclose(sols[name], sols["displ"], rtol=0, atol=1e-12 * d)
+    exx, eyy, ezz = middle_slopes(X, sols["displ"], L)
+    eps = d / (xl - 0.5 * DX)
+    print(f"exx={exx:.4e} (d/L {eps:.4e}) nu={-eyy / exx:.4f}")
+    assert abs(exx / eps - 1.0) < 0.10
+    assert abs(-eyy / exx - 0.25) < 0.03
+
+
+# ---------------------------------------------------------------- energies
+def two_node_spec(**kw):
+    spec = pm.bar_spec(2, 1, 1, delta_over_dx=1.2, steps=20, bdryeff_on=0)
+    spec["bcs"] = []
+    spec["boundary_spaces"] = []
+    spec["traction_body"] = 0
+    spec.update(kw)
+    return spec
+
+
+def two_node_step(U, V, X, dt, c, beta, vol, damping=0.0):
+    """One step of the reference's RK4 (pdDataManager.cpp:1994-2134) for two bonded free nodes,
+    written out: velocity buffer accumulated over stages of dt/2, dt/2, dt, dt; stage trial
+    displacements U + k1/2, U + k2/2, U + k3; U += (k1 + 2 k2 + 2 k3 + k4) / 6; per node and
+    stage the bond energy density 1/2 (1/2 beta (c/r) u^2) (CalcBondForceCCI); returns the
+    new U, V and the step's KE = sum 1/2 rho V^2 dx^3 and EE = dx^3 sum of the four stages'
+    energy densities."""
+    xi = X[1] - X[0]
+    r = math.sqrt(xi @ xi)
+
+    def forces(u):
+        eta = u[1] - u[0]
+        p = math.sqrt((xi + eta) @ (xi + eta))
+        st = (p - r) / r
+        f = vol * beta * c * st * (xi + eta) / p   # on node 0, toward node 1 when stretched
+        el = 0.5 * (0.5 * beta * (c / r) * (p - r) ** 2)
+        return np.array([f, -f]), el
+
+    Vb = V.copy()
+    ee = 0.0
+    ks = []
+    trial = U.copy()
+    for s, dts in enumerate((0.5 * dt, 0.5 * dt, dt, dt)):
+        F, el = forces(trial)
+        ee += 2 * el  # both nodes hold the bond
+        Vb = Vb + F * dts / RHO
+        k = Vb * dts
+        ks.append(k)
+        trial = U + (k / 2 if s < 2 else k)
+    k1, k2, k3, k4 = ks
+    Un = U + (k1 + 2 * k2 + 2 * k3 + k4) / 6 - dt * damping * V * dt / RHO
+    Vn = Vb - damping * V * dt / RHO
+    ke = float(np.sum(0.5 * RHO * (Vn * Vn).sum(axis=1) * vol))
+    return Un, Vn, ke, ee * vol
+
+
+@pytest.mark.parametrize("damping", [0.0, 0.3])
+def test_energy_totals_and_damping_two_nodes(damping):
+    """KE and EE of each step, and the damping terms -dt^2 gamma V / rho on U and -dt gamma V /
+    rho on V, against the two-node recurrence written out above."""
+    spec = two_node_spec(damping=damping)
+    o = pyoracle.Oracle(spec)
+    assert o.n == 2 and o.E == 2
+    X, _, _ = o.nodes()
+    delta = spec["horizon"]
+    c = 12.0 * E0 / (math.pi * delta ** 4)
+    assert math.isclose(o.spring(), c, rel_tol=1e-14)
+    beta = min(1.0, max(0.0, (delta + 0.5 * DX - DX) / DX))  # modified CCI, r = dx
+    V = np.array([[-3.0, 0.5, 0.0], [2.0, -0.5, 1.0]])
+    U = np.zeros((2, 3))
+    o.set_state(U, V)
+    for _ in range(4):
+        U, V, ke, ee = two_node_step(U, V, X, o.dt, c, beta, DX ** 3, damping)
+        o.run(1)
+        Uo, Vo = o.state()
+        assert np.allclose(Uo, U, rtol=1e-12, atol=1e-14 * np.abs(U).max())
+        assert np.allclose(Vo, V, rtol=1e-12, atol=1e-14 * np.abs(V).max())
+        assert math.isclose(o.ke, ke, rel_tol=1e-12) and math.isclose(o.ee, ee, rel_tol=1e-10)
+    assert ee > 0 and ke > 0
+
+
+# ---------------------------------------------------------------- BC kinematics
+def test_veloc_and_displgrad_kinematics():
+    """UpdateNodeStatusWithBC (pdDataManager.cpp:2497-2602) after every stage: a VELOC node's
+    flagged velocity is value * factor(t) (1 up to t_end, 0 after), so its displacement grows
+    by 0.75 dt v per step (the reference's stage weights); a DISPLGRAD node's flagged
+    displacement is (G X) * factor(t) (ramp t / t_end) with zero velocity."""
+    spec = two_node_spec()
+    dt = pyoracle.Oracle(spec).dt
+    spec["total_time"] = 100 * dt
+    Gm = [1e-3, 2e-3, 0.0, 0.0, 0.0, 0.0, 5e-4, 0.0, -1e-3]
+    bv = dict(type=2, flags=(1, 1, 0), value=(0.3, -0.2, 0.0), t_end=2.5 * dt, load=1.0)
+    bg = dict(type=3, flags=(1, 0, 1), value=tuple(Gm), t_end=4.0 * dt, load=1.0)
+    spec["bcs"] = [bv, bg]
+    spec["boundary_spaces"] = [dict(geo=1, ref=1, box=box(-BIG, 1.0 * DX)),
+                               dict(geo=1, ref=2, box=box(1.0 * DX, BIG))]
+    o = pyoracle.Oracle(spec)
+    X, _, bc = o.nodes()
+    assert list(bc) == [0, 1]
+    ux0 = np.zeros(2)
+    for step in range(1, 7):
+        o.run(1)
+        t = o.t
+        assert math.isclose(t, step * dt, rel_tol=1e-14)
+        U, V = o.state()
+        fv = time_factor(bv, t)
+        ux0 += 0.75 * dt * np.array([0.3, -0.2]) * fv
+        assert np.allclose(V[0, :2], np.array([0.3, -0.2]) * fv, rtol=1e-15, atol=0)
+        assert np.allclose(U[0, :2], ux0, rtol=1e-13, atol=0)
+        fg = time_factor(bg, t)
+        assert math.isclose(U[1, 0], (Gm[0] * X[1, 0] + Gm[1] * X[1, 1] + Gm[2] * X[1, 2]) * fg, rel_tol=1e-14)
+        assert math.isclose(U[1, 2], (Gm[6] * X[1, 0] + Gm[7] * X[1, 1] + Gm[8] * X[1, 2]) * fg, rel_tol=1e-14)
+        assert V[1, 0] == 0.0 and V[1, 2] == 0.0
+    assert fv == 0.0 and fg == 1.0  # both ends of the factors were crossed
