@@ -150,6 +150,17 @@ int pdm_get_timestep(const pdm_handle* h, double* dt, double* min_bond_length);
  * (CalcIntgPointDisplRK4's non-BC branch, pdDataManager.cpp:2426-2479). */
 int pdm_compute_forces(pdm_handle* h, const double* u, double* F, double* elast);
 
+/* pdm_compute_forces for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops): u (3N), F (3N) and elast (N, may be NULL) are DEVICE pointers in the
+ * same reference-order layout, and the call makes no device allocation: its internal-order
+ * copies live in a per-GPU scratch array of the library sized for 524,288 nodes (36 MiB on each
+ * GPU used, held until the process ends; larger problems get PDM_E_LIMIT).  The work is
+ * enqueued on the handle's stream; with sync != 0 the call waits for it, else the caller
+ * orders it (e.g. captures the stream).  Calls on one GPU share its scratch: calls that
+ * overlap on one GPU race.  CCI decks only.  No counterpart in the reference: it serves the
+ * callers that cannot allocate. */
+int pdm_compute_forces_noalloc(pdm_handle* h, const double* u, double* F, double* elast, int sync);
+
 /* Fixed Gaussian integration: Gauss points per node (n^3; 0 for CCI), and their positions
  * and current displacements, [node][point][xyz] in reference node order with points in
  * SetGaussianPoint's order (pdDataManager2.cpp:72-104; pdNode::GetGaussianPoint and the

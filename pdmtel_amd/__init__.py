@@ -77,6 +77,7 @@ def lib() -> C.CDLL:
     L.pdm_get_timestep.argtypes = [vp, dp, dp]
     L.pdm_compute_forces.argtypes = [vp, dp, dp, dp]
     L.pdm_apply_tangent.argtypes = [vp, dp, dp]
+    L.pdm_compute_forces_noalloc.argtypes = [vp, vp, vp, vp, C.c_int]
     L.pdm_set_state.argtypes = [vp, dp, dp]
     L.pdm_get_state.argtypes = [vp, dp, dp]
     L.pdm_step_rk4.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
@@ -203,6 +204,12 @@ class Problem:
         el = np.zeros(self.n)
         self._chk(self.L.pdm_compute_forces(self.h, _dp(u), _dp(F), _dp(el)))
         return (F.reshape(-1, 3), el) if with_elast else F.reshape(-1, 3)
+
+    def forces_noalloc(self, u_dev: int, F_dev: int, elast_dev: int = 0, sync: bool = True) -> None:
+        """pdm_compute_forces_noalloc on device buffers given by address (reference-order 3N u
+        and F, N energy densities or 0): no device allocation in the call."""
+        self._chk(self.L.pdm_compute_forces_noalloc(self.h, C.c_void_p(u_dev), C.c_void_p(F_dev),
+                                                     C.c_void_p(elast_dev or None), 1 if sync else 0))
 
     @property
     def live_gauss_pairs(self) -> int:

@@ -406,6 +406,7 @@ int init_device(pdm_handle* h)
         HCHK(hipEventCreate(&h->ev0));
         HCHK(hipEventCreate(&h->ev1));
     }
+    HCHK(noalloc_scratch_setup(h->dev));  // once per device: the no-allocation entry point's scratch
     return PDM_OK;
 }
 
@@ -992,6 +993,22 @@ int pdm_compute_forces(pdm_handle* h, const double* u, double* F, double* elast)
     HCHK(hipFree(ud));
     HCHK(hipFree(Fd));
     HCHK(hipFree(ed));
+    return PDM_OK;
+}
+
+int pdm_compute_forces_noalloc(pdm_handle* h, const double* u, double* F, double* elast, int sync)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (!u || !F) return fail(h, PDM_E_ARG, "null u or F");
+    if (h->fgi_ng) return fail(h, PDM_E_MODEL, "pdm_compute_forces_noalloc takes CCI decks only");
+    if (h->n > kNoAllocMaxNodes)
+        return fail(h, PDM_E_LIMIT, std::to_string(h->n) + " nodes exceed the " + std::to_string(kNoAllocMaxNodes) +
+                                        " the no-allocation scratch holds; use pdm_compute_forces");
+    if (h->n == 0) return PDM_OK;
+    HCHK(hipSetDevice(h->dev));
+    HCHK(noalloc_forces(h, tile_view(h), u, F, elast));
+    if (sync) HCHK(hipStreamSynchronize(h->stream));
     return PDM_OK;
 }
 

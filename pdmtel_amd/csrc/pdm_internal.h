@@ -47,6 +47,7 @@ constexpr size_t kLdsBytes = 160 * 1024;  // LDS per CU on gfx950
 constexpr int kMaxFlags = 8192;      // nodes in a tile's neighbourhood (halo build)
 constexpr int kMaxHalo = 2560;       // halo entries staged in LDS: 7 x 8 B x 2560 + 32 B x 512 < 160 KB
 constexpr int kRowAlign = 8;          // internal rows padded to whole blocks of 8 entries
+constexpr int64_t kNoAllocMaxNodes = 1 << 19;  // nodes pdm_compute_forces_noalloc's scratch holds
 
 struct alignas(32) d4 {
     double x, y, z, w;
@@ -356,6 +357,11 @@ hipError_t launch_fgi_forces(const StageArgs& a, const FgiArgs& f, const d4* u, 
 // one (ent); fill = true writes them at f.pair_ptr.
 hipError_t launch_fgi_pairs(const TileView& tv, const FgiArgs& f, bool fill, int64_t* cnt, int64_t* ent,
                             uint32_t* pairs, hipStream_t s);
+
+// force sums with no allocation in the call (pdm_noalloc.hip): the per-device scratch
+// address is looked up once per device at set-up; noalloc_forces enqueues on h->stream
+hipError_t noalloc_scratch_setup(int dev);
+hipError_t noalloc_forces(const pdm_handle* h, const TileView& tv, const double* u, double* F, double* elast);
 
 // static solve (pdm_cg.cpp / pdm_cg.hip)
 int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_stats* stats);
