@@ -363,7 +363,7 @@ void free_tiles(pdm_handle* h)
     dfree(h->permd); dfree(h->Xi); dfree(h->u[0]); dfree(h->u[1]); dfree(h->rpi); dfree(h->slot);
     dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->bk); dfree(h->corr); dfree(h->mati);
     dfree(h->bci); dfree(h->nb); dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr);
-    dfree(h->halo); dfree(h->part);
+    dfree(h->halo); dfree(h->part); dfree(h->tc64); dfree(h->tc32);
     h->part_cap = 0;
     h->ntl = 0;
     h->ntiles = 0;
@@ -760,6 +760,26 @@ int batch_autotune(pdm_handle* B)
 
 }  // namespace
 
+namespace pdm {
+
+int tangent_view(pdm_handle* h, TileView& tv)
+{
+    tv = tile_view(h);
+    if (h->ntl == 0) return PDM_OK;
+    const bool f32 = h->opt.precision != PDM_FP64;
+    if (!h->tc64 || (f32 && !h->tc32)) {
+        if (!h->coef64) return fail(h, PDM_E_STATE, "the tangent needs the FP64 force coefficients");
+        if (!h->tc64) HCHK(hipMalloc(&h->tc64, sizeof(double) * std::max<int64_t>(h->Ei, 1)));
+        if (f32 && !h->tc32) HCHK(hipMalloc(&h->tc32, sizeof(float) * std::max<int64_t>(h->Ei, 1)));
+        HCHK(launch_tangent_coef(tv, h->tc64, f32 ? h->tc32 : nullptr, h->stream));
+    }
+    tv.coef64 = h->tc64;
+    tv.coef32 = h->tc32;
+    return PDM_OK;
+}
+
+}  // namespace pdm
+
 extern "C" {
 
 int pdm_options_default(pdm_options* opt)
@@ -812,7 +832,7 @@ int pdm_destroy(pdm_handle* h)
     dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->bk); dfree(h->corr); dfree(h->mati);
     dfree(h->bci); dfree(h->nb); dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr);
     dfree(h->halo); dfree(h->U); dfree(h->V); dfree(h->Vb); dfree(h->k1); dfree(h->k23);
-    dfree(h->elast); dfree(h->part); dfree(h->energy); dfree(h->steps);
+    dfree(h->elast); dfree(h->part); dfree(h->energy); dfree(h->steps); dfree(h->tc64); dfree(h->tc32);
     dfree(h->specs_d); dfree(h->mats_d); dfree(h->bcs_d); dfree(h->tile_spec_d); dfree(h->spec_tile_ptr_d);
     free_fgi(h);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -1061,7 +1081,10 @@ int pdm_apply_tangent(pdm_handle* h, const double* u, double* y)
     HCHK(hipMalloc(&ud, sizeof(d4) * n));
     HCHK(hipMalloc(&yd, sizeof(d4) * n));
     HCHK(hipMemcpy(ud, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
-    HCHK(launch_tangent(h->opt.precision, tile_view(h), ud, yd, h->stream));
+    TileView tv;
+    rc = tangent_view(h, tv);
+    if (rc) return rc;
+    HCHK(launch_tangent(h->opt.precision, tv, ud, yd, h->stream));
     HCHK(hipStreamSynchronize(h->stream));
     HCHK(hipMemcpy(buf.data(), yd, sizeof(d4) * n, hipMemcpyDeviceToHost));
     from_internal(h, buf, y);

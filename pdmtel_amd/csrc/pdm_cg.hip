@@ -19,6 +19,7 @@
 // replaces the recurrence residual, the search direction kept ("reliable update"
 // residual replacement): the stopping test is on the FP64 residual, and FP32 matvecs
 // converge to the FP64 tolerance without restarts.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -43,7 +44,7 @@ constexpr int kVB = 256;          // vector-kernel block
 constexpr int kBlockIters = 25;   // CG iterations per captured graph
 
 struct Scal {
-    double rho, rho_new, pq, rr, rr0;
+    double rho, rho_new, pq, rr, rr0, beta;
 };
 
 __device__ __forceinline__ double dot3(const d4& a, const d4& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
@@ -108,15 +109,6 @@ __global__ __launch_bounds__(kVB) void k_pcg_start(int64_t n, const d4* __restri
     block_sum2(rz, rr, part);
 }
 
-// partial p . q
-__global__ __launch_bounds__(kVB) void k_pq(int64_t n, const d4* __restrict__ p, const d4* __restrict__ q, double* part)
-{
-    int64_t i = (int64_t)blockIdx.x * kVB + threadIdx.x;
-    double pq = 0.0;
-    if (i < n) pq = dot3(p[i], q[i]);
-    block_sum2(pq, 0.0, part);
-}
-
 // alpha = rho / pq; d += alpha p; r -= alpha m q; z = m r / diag; partial r.z, |r|^2.
 __global__ __launch_bounds__(kVB) void k_pcg_update(int64_t n, const Scal* __restrict__ sc, const d4* __restrict__ p,
                                                     const d4* __restrict__ q, const d4* __restrict__ dg,
@@ -161,12 +153,12 @@ __global__ __launch_bounds__(kVB) void k_pcg_refresh(int64_t n, const d4* __rest
     block_sum2(rz, rr, part);
 }
 
-// beta = rho_new / rho; p = z + beta p.
+// p = z + beta p (beta = rho_new / rho, set by the reduction that produced rho_new).
 __global__ __launch_bounds__(kVB) void k_pcg_dir(int64_t n, const Scal* __restrict__ sc, const d4* __restrict__ z,
                                                  d4* __restrict__ p)
 {
     int64_t i = (int64_t)blockIdx.x * kVB + threadIdx.x;
-    const double beta = (sc->rho != 0.0) ? sc->rho_new / sc->rho : 0.0;
+    const double beta = sc->beta;
     if (i < n) {
         d4 zi = z[i], pi = p[i];
         p[i] = d4{zi.x + beta * pi.x, zi.y + beta * pi.y, zi.z + beta * pi.z, 0.0};
@@ -192,8 +184,8 @@ __global__ __launch_bounds__(kVB) void k_axpy_out(int64_t n, d4* __restrict__ x,
 }
 
 // Reduce partial pairs and store into the scalar block.  mode 0: rr0 = sum a;
-// 1: rho = sum a, rr = sum b; 2: pq = sum a; 3: rho_new = sum a, rr = sum b, then the
-// direction kernel reads rho_new / rho and the next update rolls rho.
+// 1: rho = sum a, rr = sum b; 2: pq = sum a; 3: rho_new = sum a, rr = sum b, beta =
+// rho_new / rho for the direction update, and rho rolls to rho_new.
 __global__ __launch_bounds__(kVB) void k_reduce_scal(const double* __restrict__ part, int64_t nb, Scal* sc, int mode)
 {
     __shared__ double sa[kVB], sb[kVB];
@@ -216,11 +208,14 @@ __global__ __launch_bounds__(kVB) void k_reduce_scal(const double* __restrict__ 
         if (mode == 0) sc->rr0 = sa[0];
         else if (mode == 1) { sc->rho = sa[0]; sc->rr = sb[0]; }
         else if (mode == 2) sc->pq = sa[0];
-        else { sc->rho_new = sa[0]; sc->rr = sb[0]; }
+        else {
+            sc->rho_new = sa[0];
+            sc->rr = sb[0];
+            sc->beta = (sc->rho != 0.0) ? sa[0] / sc->rho : 0.0;
+            sc->rho = sa[0];
+        }
     }
 }
-
-__global__ void k_roll_rho(Scal* sc) { sc->rho = sc->rho_new; }
 
 struct Work {
     d4 *m = nullptr, *uc = nullptr, *b = nullptr, *dg = nullptr, *rhs = nullptr, *x = nullptr, *r = nullptr,
@@ -288,7 +283,8 @@ int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_s
         if (e_ != hipSuccess) return fail(PDM_E_HIP, std::string("HIP error ") + hipGetErrorString(e_) + " at " #x); \
     } while (0)
     for (d4** v : {&w.m, &w.uc, &w.b, &w.dg, &w.rhs, &w.x, &w.r, &w.z, &w.p, &w.q, &w.d, &w.zero}) WCHK(hipMalloc(v, vb));
-    WCHK(hipMalloc(&w.part, sizeof(double) * 2 * nblk));
+    const int64_t ntl = h->ntl;
+    WCHK(hipMalloc(&w.part, sizeof(double) * 2 * std::max<int64_t>(nblk, std::max<int64_t>(ntl, 1))));
     WCHK(hipMalloc(&w.sc, sizeof(Scal)));
     WCHK(hipMemcpyAsync(w.m, mh.data(), vb, hipMemcpyHostToDevice, s));
     WCHK(hipMemcpyAsync(w.uc, uch.data(), vb, hipMemcpyHostToDevice, s));
@@ -296,7 +292,14 @@ int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_s
     WCHK(hipMemsetAsync(w.zero, 0, vb, s));
     WCHK(hipMemsetAsync(w.x, 0, vb, s));
     WCHK(hipMemsetAsync(w.sc, 0, sizeof(Scal), s));
-    const TileView tv = tile_view(h);
+    TileView tv;
+    {
+        const int rc = tangent_view(h, tv);  // the operator's per-entry weights coef / r^3
+        if (rc) {
+            w.release();
+            return rc;
+        }
+    }
     const dim3 g((unsigned)nblk), bk(kVB);
     // Jacobi diagonal; a free component with no bonds keeps diagonal 1 (its row is empty)
     WCHK(launch_tangent_diag(tv, w.zero, w.dg, s));
@@ -328,14 +331,13 @@ int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_s
     // one captured block of kBlockIters PCG iterations
     auto enqueue_block = [&]() -> hipError_t {
         for (int k = 0; k < kBlockIters; ++k) {
-            hipError_t e = launch_tangent(inner_prec, tv, w.p, w.q, s);
+            // q = K p, with each tile's share of p . q from the same kernel
+            hipError_t e = launch_tangent(inner_prec, tv, w.p, w.q, s, w.part);
             if (e != hipSuccess) return e;
-            k_pq<<<g, bk, 0, s>>>(n, w.p, w.q, w.part);
-            k_reduce_scal<<<1, kVB, 0, s>>>(w.part, nblk, w.sc, 2);
+            k_reduce_scal<<<1, kVB, 0, s>>>(w.part, ntl, w.sc, 2);
             k_pcg_update<<<g, bk, 0, s>>>(n, w.sc, w.p, w.q, w.dg, w.m, w.d, w.r, w.z, w.part);
             k_reduce_scal<<<1, kVB, 0, s>>>(w.part, nblk, w.sc, 3);
             k_pcg_dir<<<g, bk, 0, s>>>(n, w.sc, w.z, w.p);
-            k_roll_rho<<<1, 1, 0, s>>>(w.sc);
         }
         return hipGetLastError();
     };

@@ -301,6 +301,8 @@ struct pdm_handle {
     int64_t* cb_ptr = nullptr;         // MLS contributing nodes (n > 1)
     int32_t* cb = nullptr;
     double* phi = nullptr;
+    double* tc64 = nullptr;            // tangent weights coef / r^3 per internal entry (static solve)
+    float* tc32 = nullptr;             // the same rounded to FP32 (FP32 precisions' CG matvecs)
     int64_t* fgi_pair_ptr = nullptr;   // [N + 1] live (entry, Gauss point) pairs per internal row
     uint32_t* fgi_pairs = nullptr;
     int64_t fgi_npairs = 0;
@@ -340,8 +342,17 @@ hipError_t fill_i32(int32_t* dst, int64_t n, int32_t v, hipStream_t s);
 hipError_t launch_energy_steps(const double* part, int64_t ntl, const int64_t* spec_tile_ptr, int nspec, int nsteps,
                                double* out, hipStream_t s);
 hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, double* elast, hipStream_t s);
-hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s);
+// y = K u; with dot set, also each tile's share of u . y at dot[2 k] (k the launch index,
+// dot[2 k + 1] = 0), in the layout of the CG reductions' partial pairs.
+hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s,
+                          double* dot = nullptr);
 hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s);
+// Tangent weights coef / r^3 per internal entry in FP64 (tc64) and rounded to FP32 (tc32, may
+// be null); the tangent operations stream them through the view's coef64 / coef32.
+hipError_t launch_tangent_coef(const TileView& tv, double* tc64, float* tc32, hipStream_t s);
+// The tile view of the tangent operator: coef64 / coef32 replaced by the tangent weights,
+// built on first use and kept with the tile layout.
+int tangent_view(pdm_handle* h, TileView& tv);
 // Per-entry beta corrections of the FP32 force kernels (from coef64); *clamped counts the
 // entries whose correction left the 8-bit range.
 hipError_t launch_fly_corr(const TileView& tv, const d4* u, uint8_t* corr, int* clamped, hipStream_t s);

@@ -383,6 +383,25 @@ struct SelfF {
 // of (fx, fy, fz, energy density).  Energy: acc.el sums coef * s * u; the bond energy
 // bdryEff * beta * c * u^2 / (4 r) is that times 1 / (4 dx^3), applied once per family.
 
+// Correctly rounded FP64 square root for the bond lengths: v_rsq_f64 refined by the same
+// Newton-Raphson sequence the compiler emits for sqrt(), without the input scaling and the
+// zero / infinity selects that sequence spends on operands outside [2^-767, 2^1000] (one
+// rarely taken branch falls back to sqrt() there).  r and p must equal the reference's
+// libm square roots to the bit: u = p - r cancels all but a few of their digits.
+__device__ __forceinline__ double sqrt_rn(double x)
+{
+    if (!(x >= 0x1p-767 && x <= 0x1p+1000)) return sqrt(x);
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+
 struct OpF64 {  // PDM_FP64, reference arithmetic
     using Coef = double;
     using SelfT = Self;
@@ -395,22 +414,29 @@ struct OpF64 {  // PDM_FP64, reference arithmetic
         const Rec r = l.rec[s];
         double xi1 = r.x - me.x, xi2 = r.y - me.y, xi3 = r.z - me.z;
         double et1 = r.u - me.u, et2 = r.v - me.v, et3 = r.w - me.w;
-        double rr = sqrt(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
+        double rr = sqrt_rn(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
         double a1 = xi1 + et1, a2 = xi2 + et2, a3 = xi3 + et3;
-        double p = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+        double p = sqrt_rn(a1 * a1 + a2 * a2 + a3 * a3);
         double u = p - rr;
-        double st = u / rr;
+        // r and p, hence u, are the reference's to the bit; its two divisions (u / r and
+        // f / p) share one reciprocal here, 1/r = p / (r p) and 1/p = r / (r p), which moves
+        // each bond force by a few ulp of itself
+        const double inv = 1.0 / (rr * p);
+        double st = u * (p * inv);
         double ecrit = me.ecrit;
         if (!UE) {
             double ej = l.ec[s];
             ecrit = (ej < me.ecrit) ? ej : me.ecrit;
         }
-        const bool ok = u <= 0.0 || (u > 0.0 && st <= ecrit);
+        // u <= 0 || (u > 0 && st <= ecrit), evaluated without branches (st has u's sign; a NaN
+        // stretch fails both tests, as in the reference)
+        const bool ok = (u <= 0.0) | (st <= ecrit);
         const double cs = ok ? c * st : 0.0;
-        a.f0 += (cs * a1) / p;
-        a.f1 += (cs * a2) / p;
-        a.f2 += (cs * a3) / p;
-        if (ENERGY) a.el += cs * u;
+        const double g = cs * (rr * inv);
+        a.f0 = __builtin_fma(g, a1, a.f0);
+        a.f1 = __builtin_fma(g, a2, a.f1);
+        a.f2 = __builtin_fma(g, a3, a.f2);
+        if (ENERGY) a.el = __builtin_fma(cs, u, a.el);
     }
     template <int LPN>
     __device__ static void reduce(Acc& a)
@@ -578,24 +604,31 @@ struct OpF32TF {  // PDM_FP32C2: FP32 bond force, two-float family sums
     }
 };
 
-// Tangent of the bond force at eta = 0: y_i = sum_j coef_ij xi (xi . (u_i - u_j)) / r^3.
-struct OpT64 {
-    using Coef = double;
+// Tangent of the bond force at eta = 0: y_i = sum_j w_ij xi (xi . (u_i - u_j)), with the
+// per-entry tangent weight w = coef / r^3 streamed in place of the force coefficient
+// (k_tangent_coef builds it once: the operator is fixed, so no root or division per bond).
+// W = double: the FP64 operator; W = float: the weights stored in FP32 (a matvec streams 6
+// instead of 10 bytes per entry), the arithmetic in FP64.  FP32 rounding of the weights
+// leaves the operator symmetric and fixed, so CG keeps its FP64 convergence up to the small
+// operator difference; FP32 arithmetic instead adds rounding noise to every product, which
+// slows CG's convergence 2-3 fold.
+template <class W>
+struct OpTangent {
+    using Coef = W;
     using SelfT = Self;
     struct Acc {
         double f0, f1, f2, el;
     };
     template <bool ENERGY, bool UE>
-    __device__ static void bond(const Lds& l, unsigned s, double c, const Self& me, Acc& a)
+    __device__ static void bond(const Lds& l, unsigned s, W c, const Self& me, Acc& a)
     {
         const Rec r = l.rec[s];
-        double x1 = r.x - me.x, x2 = r.y - me.y, x3 = r.z - me.z;
-        double d0 = me.u - r.u, d1 = me.v - r.v, d2 = me.w - r.w;
-        double r2 = x1 * x1 + x2 * x2 + x3 * x3;
-        double proj = (x1 * d0 + x2 * d1 + x3 * d2) * c / (r2 * sqrt(r2));
-        a.f0 += proj * x1;
-        a.f1 += proj * x2;
-        a.f2 += proj * x3;
+        const double x1 = r.x - me.x, x2 = r.y - me.y, x3 = r.z - me.z;
+        const double d0 = me.u - r.u, d1 = me.v - r.v, d2 = me.w - r.w;
+        const double proj = __builtin_fma(x3, d2, __builtin_fma(x2, d1, x1 * d0)) * (double)c;
+        a.f0 = __builtin_fma(proj, x1, a.f0);
+        a.f1 = __builtin_fma(proj, x2, a.f1);
+        a.f2 = __builtin_fma(proj, x3, a.f2);
     }
     template <int LPN>
     __device__ static void reduce(Acc& a)
@@ -612,53 +645,11 @@ struct OpT64 {
         out[3] = 0.0;
     }
 };
+using OpT64 = OpTangent<double>;
+using OpT32 = OpTangent<float>;
 
-struct OpT32 {
-    using Coef = float;
-    using SelfT = SelfF;
-    struct Acc {
-        double f0, f1, f2, el;
-        float p0, p1, p2;
-    };
-    __device__ static void flush(Acc& a)
-    {
-        a.f0 += (double)a.p0;
-        a.f1 += (double)a.p1;
-        a.f2 += (double)a.p2;
-        a.p0 = a.p1 = a.p2 = 0.0f;
-    }
-    template <bool ENERGY, bool UE>
-    __device__ static void bond(const Lds& l, unsigned s, float c, const SelfF& me, Acc& a)
-    {
-        const RecF& rec = l.recf[s];
-        const u32x4 ra = rec.a;
-        const f64x2 rb = rec.u;
-        const float x1 = (float)((int)ra.x - me.x), x2 = (float)((int)ra.y - me.y), x3 = (float)((int)ra.z - me.z);
-        const float d0 = (float)(me.u - rb.x), d1 = (float)(me.v - rb.y), d2 = (float)(me.w - rec.w.x);
-        const float r2 = __builtin_fmaf(x3, x3, __builtin_fmaf(x2, x2, x1 * x1));
-        const float rinv = __builtin_amdgcn_rsqf(r2);
-        const float proj = __builtin_fmaf(x3, d2, __builtin_fmaf(x2, d1, x1 * d0)) * c * (rinv * rinv * rinv);
-        a.p0 = __builtin_fmaf(proj, x1, a.p0);
-        a.p1 = __builtin_fmaf(proj, x2, a.p1);
-        a.p2 = __builtin_fmaf(proj, x3, a.p2);
-    }
-    template <int LPN>
-    __device__ static void reduce(Acc& a)
-    {
-        a.f0 = group_sum<LPN>(a.f0);
-        a.f1 = group_sum<LPN>(a.f1);
-        a.f2 = group_sum<LPN>(a.f2);
-    }
-    __device__ static void store(const Acc& a, double, double* out)
-    {
-        out[0] = a.f0;
-        out[1] = a.f1;
-        out[2] = a.f2;
-        out[3] = 0.0;
-    }
-};
-
-// Diagonal of the tangent operator (Jacobi preconditioner): d_c = sum_j coef_ij xi_c^2 / r^3.
+// Diagonal of the tangent operator (Jacobi preconditioner): d_c = sum_j w_ij xi_c^2, w the
+// tangent weight coef / r^3.
 struct OpD64 {
     using Coef = double;
     using SelfT = Self;
@@ -669,17 +660,15 @@ struct OpD64 {
     __device__ static void bond(const Lds& l, unsigned s, double c, const Self& me, Acc& a)
     {
         const Rec r = l.rec[s];
-        double x1 = r.x - me.x, x2 = r.y - me.y, x3 = r.z - me.z;
-        double r2 = x1 * x1 + x2 * x2 + x3 * x3;
-        double w = c / (r2 * sqrt(r2));
-        a.f0 += w * x1 * x1;
-        a.f1 += w * x2 * x2;
-        a.f2 += w * x3 * x3;
+        const double x1 = r.x - me.x, x2 = r.y - me.y, x3 = r.z - me.z;
+        a.f0 = __builtin_fma(c * x1, x1, a.f0);
+        a.f1 = __builtin_fma(c * x2, x2, a.f1);
+        a.f2 = __builtin_fma(c * x3, x3, a.f2);
     }
     template <int LPN>
     __device__ static void reduce(Acc& a)
     {
-        OpT64::reduce<LPN>(reinterpret_cast<OpT64::Acc&>(a));
+        OpT64::template reduce<LPN>(reinterpret_cast<OpT64::Acc&>(a));
     }
     __device__ static void store(const Acc& a, double, double* out)
     {
@@ -1225,10 +1214,13 @@ __global__ __launch_bounds__(256) void k_reduce_steps(const double* __restrict__
 }
 
 // Family sums (force or tangent) for every node, no state update: y = (fx, fy, fz), energy.
+// With dot set, also the tile's share of u . y (CG's p . Kp) into dot[2 k] (dot[2 k + 1] = 0),
+// k the launch index.
 template <class OP, bool UE, int NW, int NPW>
 __global__ __launch_bounds__(64 * NW) void k_gather(TileView tv, const d4* __restrict__ u, d4* __restrict__ y,
-                                                    double* __restrict__ elast)
+                                                    double* __restrict__ elast, double* __restrict__ dot)
 {
+    __shared__ double sh_dot[NW];
     extern __shared__ double L[];
     const int64_t k = launch_index(tv.xcd_remap);
     if (k >= tv.ntl) return;
@@ -1244,10 +1236,60 @@ __global__ __launch_bounds__(64 * NW) void k_gather(TileView tv, const d4* __res
     __syncthreads();
     tile_families<OP, true, UE, NW, NPW>(tv, sp, l, nt);
     __syncthreads();
+    double pq = 0.0;
     for (int loc = threadIdx.x; loc < nt; loc += 64 * NW) {
         const double* res = l.res + 4 * loc;
         y[a0 + loc] = d4{res[0], res[1], res[2], 0.0};
         if (elast) elast[a0 + loc] = res[3];
+        if (dot) {
+            const d4 v = u[a0 + loc];
+            pq += v.x * res[0] + v.y * res[1] + v.z * res[2];
+        }
+    }
+    if (dot) {
+        pq = wave_sum(pq);
+        if ((threadIdx.x & 63) == 0) sh_dot[threadIdx.x >> 6] = pq;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < NW; ++w) t += sh_dot[w];
+            dot[2 * k] = t;
+            dot[2 * k + 1] = 0.0;
+        }
+    }
+}
+
+// Tangent weights w = coef / r^3 of every internal entry, r from the FP64 positions, one
+// workgroup per tile: tc64 in FP64, tc32 (may be null) rounded to FP32.  Entries with
+// coefficient 0 (row padding) get 0.
+__global__ __launch_bounds__(256) void k_tangent_coef(TileView tv, double* __restrict__ tc64,
+                                                      float* __restrict__ tc32)
+{
+    const int64_t k = blockIdx.x;
+    const int32_t tid = tv.tiles[k];
+    const int64_t a0 = tv.tile_start[tid];
+    const int nt = (int)(tv.tile_start[tid + 1] - a0);
+    const int64_t h0 = tv.halo_ptr[k];
+    const int64_t e0 = tv.rpi[a0], e1 = tv.rpi[a0 + nt];
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+        const double c = tv.coef64[e];
+        double w = 0.0;
+        if (c != 0.0) {
+            // the row of e: the last tile node whose row starts at or before e
+            int lo = 0, hi = nt - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (tv.rpi[a0 + mid] <= e) lo = mid;
+                else hi = mid - 1;
+            }
+            const d4 xi = tv.X[a0 + lo];
+            const d4 xj = tv.X[tv.halo[h0 + tv.slot[e]]];
+            const double x1 = xj.x - xi.x, x2 = xj.y - xi.y, x3 = xj.z - xi.z;
+            const double r2 = x1 * x1 + x2 * x2 + x3 * x3;
+            w = c / (r2 * sqrt(r2));
+        }
+        tc64[e] = w;
+        if (tc32) tc32[e] = (float)w;
     }
 }
 
@@ -1326,18 +1368,22 @@ __device__ __forceinline__ void fgi_family(const TileView& tv, const FgiArgs& f,
         const d4 ug = f.gin[(int64_t)j * ng + g];
         const double xi1 = (t[0] + xj.x) - xi.x, xi2 = (t[1] + xj.y) - xi.y, xi3 = (t[2] + xj.z) - xi.z;
         const double a1 = xi1 + (ug.x - ui.x), a2 = xi2 + (ug.y - ui.y), a3 = xi3 + (ug.z - ui.z);
-        const double r = sqrt(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
-        const double p = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+        const double r = sqrt_rn(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
+        const double p = sqrt_rn(a1 * a1 + a2 * a2 + a3 * a3);
         const double beta = g_beta(r, f.horizon, t[4], false);
         const double u = p - r;
-        const double st = u / r;
-        if (u <= 0.0 || (u > 0.0 && st <= ecrit)) {
-            const double c = bij * beta * spring * st;
-            f0 += t[3] * ((c * a1) / p);
-            f1 += t[3] * ((c * a2) / p);
-            f2 += t[3] * ((c * a3) / p);
-            el += bij * 0.5 * (0.5 * beta * (spring / r) * u * u) * t[5];
-        }
+        // one reciprocal for the divisions by r and p: 1/r = p / (r p), 1/p = r / (r p)
+        const double inv = 1.0 / (r * p);
+        const double rinv = p * inv;
+        const double st = u * rinv;
+        // u <= 0 || (u > 0 && st <= ecrit), without branches
+        const bool ok = (u <= 0.0) | (st <= ecrit);
+        const double c = ok ? bij * beta * spring * st : 0.0;
+        const double w = t[3] * (c * (r * inv));
+        f0 = __builtin_fma(w, a1, f0);
+        f1 = __builtin_fma(w, a2, f1);
+        f2 = __builtin_fma(w, a3, f2);
+        el += ok ? bij * 0.5 * (0.5 * beta * (spring * rinv) * u * u) * t[5] : 0.0;
     }
     acc[0] = wave_sum(f0);
     acc[1] = wave_sum(f1);
@@ -1620,20 +1666,22 @@ hipError_t launch_stage_p(int stage, const StageArgs& a, int step_idx, size_t ld
 }
 
 template <class OP, bool UE>
-hipError_t launch_gather_t(const TileView& tv, const d4* u, d4* y, double* elast, size_t lds, hipStream_t s)
+hipError_t launch_gather_t(const TileView& tv, const d4* u, d4* y, double* elast, size_t lds, hipStream_t s,
+                           double* dot = nullptr)
 {
     constexpr int NW = 8, NPW = 4;
     hipError_t e = set_lds<k_gather<OP, UE, NW, NPW>>(lds);
     if (e != hipSuccess) return e;
-    k_gather<OP, UE, NW, NPW><<<dim3((unsigned)padded_grid(tv.ntl)), dim3(64 * NW), lds, s>>>(tv, u, y, elast);
+    k_gather<OP, UE, NW, NPW><<<dim3((unsigned)padded_grid(tv.ntl)), dim3(64 * NW), lds, s>>>(tv, u, y, elast, dot);
     return hipGetLastError();
 }
 
 template <class OP>
-hipError_t launch_gather_u(const TileView& tv, const d4* u, d4* y, double* elast, size_t lds, hipStream_t s)
+hipError_t launch_gather_u(const TileView& tv, const d4* u, d4* y, double* elast, size_t lds, hipStream_t s,
+                           double* dot = nullptr)
 {
-    if (tv.uniform_mat) return launch_gather_t<OP, true>(tv, u, y, elast, lds, s);
-    return launch_gather_t<OP, false>(tv, u, y, elast, lds, s);
+    if (tv.uniform_mat) return launch_gather_t<OP, true>(tv, u, y, elast, lds, s, dot);
+    return launch_gather_t<OP, false>(tv, u, y, elast, lds, s, dot);
 }
 
 }  // namespace
@@ -1752,18 +1800,25 @@ hipError_t launch_fly_corr(const TileView& tv, const d4* u, uint8_t* corr, int* 
     return hipGetLastError();
 }
 
+hipError_t launch_tangent_coef(const TileView& tv, double* tc64, float* tc32, hipStream_t s)
+{
+    if (tv.ntl == 0) return hipSuccess;
+    k_tangent_coef<<<dim3((unsigned)tv.ntl), dim3(256), 0, s>>>(tv, tc64, tc32);
+    return hipGetLastError();
+}
+
 hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s)
 {
     if (tv.ntl == 0) return hipSuccess;
     return launch_gather_u<OpD64>(tv, u, y, nullptr, stage_lds_bytes(tv, !tv.uniform_mat, false, false), s);
 }
 
-hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s)
+hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s, double* dot)
 {
     if (tv.ntl == 0) return hipSuccess;
-    size_t lds = stage_lds_bytes(tv, !tv.uniform_mat, false, precision != PDM_FP64);
-    if (precision == PDM_FP64) return launch_gather_u<OpT64>(tv, u, y, nullptr, lds, s);
-    return launch_gather_u<OpT32>(tv, u, y, nullptr, lds, s);
+    size_t lds = stage_lds_bytes(tv, !tv.uniform_mat, false, false);  // FP64 records in every precision
+    if (precision == PDM_FP64) return launch_gather_u<OpT64>(tv, u, y, nullptr, lds, s, dot);
+    return launch_gather_u<OpT32>(tv, u, y, nullptr, lds, s, dot);
 }
 
 }  // namespace pdm
