@@ -161,6 +161,17 @@ int pdm_compute_forces(pdm_handle* h, const double* u, double* F, double* elast)
  * callers that cannot allocate. */
 int pdm_compute_forces_noalloc(pdm_handle* h, const double* u, double* F, double* elast, int sync);
 
+/* Areal force density at the given nodes (reference node ids): the post-processing of
+ * pdSolve::PostAFD / pdDataManager::CalcArealForceDensity (pdDataManager3.cpp:442-620).  For
+ * normal e_n, the collinear lattice nodes x_s - (1 + k) dx e_n (k < int(delta / dx), exact
+ * coordinate equality) each add ds = delta / int(delta / dx) times the CCI bond forces of
+ * their family members in the half space (x_j - x_s) . e_n >= 0; sigma[9 t + 3 n + c].
+ * From the current displacement U (pdm_set_state loads a dump's); bdryeff (N, may be NULL)
+ * replaces the boundary-effect factors, as PostAFD takes them from the dump.  Bonds cut by a
+ * slit (extension) carry no force.  CCI decks. */
+int pdm_areal_force_density(pdm_handle* h, const int64_t* nodes, int64_t count, const double* bdryeff,
+                            double* sigma);
+
 /* Fixed Gaussian integration: Gauss points per node (n^3; 0 for CCI), and their positions
  * and current displacements, [node][point][xyz] in reference node order with points in
  * SetGaussianPoint's order (pdDataManager2.cpp:72-104; pdNode::GetGaussianPoint and the

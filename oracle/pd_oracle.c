@@ -1435,6 +1435,69 @@ int orc_sample_forces(const orc_problem* p, const int64_t* rows, int64_t nrows, 
     return 0;
 }
 
+/* CalcArealForceDensity + CalcForceOfPlusSpace (pdDataManager3.cpp:442-620), CCI branch, for
+ * the current displacement U: the collinear nodes by exact coordinate equality, every node j
+ * tested against the collinear node i with the rough 2 dx prefilter and CalcDistanceToCube
+ * (i, j) < delta + tol, the half space (x_j - x_s) . e_n >= 0, j != i, the bond overload of
+ * CalcBondForceCCI (pdDataManager2.cpp:1087-1154).  be (N, may be NULL) replaces bdryEff. */
+void orc_areal_force_density(const orc_state* s, const int64_t* nodes, int64_t count, const double* be,
+                             double* sigma)
+{
+    const orc_problem* p = &s->p;
+    const double* bef = be ? be : s->be;
+    const int ns = (int)(p->horizon / p->dx);
+    const double ds = p->horizon / ns;
+    const double tol = s->tol, dx = p->dx;
+    for (int64_t t = 0; t < count; ++t) {
+        const int64_t src = nodes[t];
+        const double* xs = &s->X[3 * src];
+        double sig[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int n = 0; n < 3; ++n) {
+            /* collinear nodes in node-id order, as the reference collects them */
+            for (int64_t i = 0; i < s->n; ++i) {
+                const double* xi = &s->X[3 * i];
+                int hit = 0;
+                for (int is = 0; is != ns; ++is) {
+                    if (n == 0 && xi[0] == xs[0] - dx * (1 + is) && xi[1] == xs[1] && xi[2] == xs[2]) hit = 1;
+                    if (n == 1 && xi[0] == xs[0] && xi[1] == xs[1] - dx * (1 + is) && xi[2] == xs[2]) hit = 1;
+                    if (n == 2 && xi[0] == xs[0] && xi[1] == xs[1] && xi[2] == xs[2] - dx * (1 + is)) hit = 1;
+                }
+                if (!hit) continue;
+                double f[3] = {0.0, 0.0, 0.0};
+                const orc_mat* mi = &p->mat[s->mat[i]];
+                for (int64_t j = 0; j < s->n; ++j) {
+                    const double* xj = &s->X[3 * j];
+                    if (!(xj[n] - xs[n] >= 0.0) || j == i) continue;
+                    double d = sqrt(sq(xj[0] - xi[0]) + sq(xj[1] - xi[1]) + sq(xj[2] - xi[2]));
+                    if (d > p->horizon + 2.0 * dx) continue;
+                    double dmin, dmax;
+                    dist_to_cube(xi, xj, dx, &dmin, &dmax);
+                    if (!(dmin < p->horizon + tol)) continue;
+                    int a = i < j ? (int)i : (int)j, b = i < j ? (int)j : (int)i;
+                    if (orc_pair_cut(p, &s->X[3 * a], &s->X[3 * b])) continue; /* slit extension */
+                    const orc_mat* mj = &p->mat[s->mat[j]];
+                    double xi3[3] = {xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]};
+                    double eta[3] = {s->U[3 * j] - s->U[3 * i], s->U[3 * j + 1] - s->U[3 * i + 1],
+                                     s->U[3 * j + 2] - s->U[3 * i + 2]};
+                    double ecrit = (mj->ecrit < mi->ecrit) ? mj->ecrit : mi->ecrit;
+                    double ci = s->spring[s->mat[i]], cj = s->spring[s->mat[j]];
+                    double spring = (cj < ci) ? cj : ci;
+                    double bij = 0.5 * (bef[i] + bef[j]);
+                    double fb[3], el;
+                    orc_bond_force_cci(p, xi3, eta, ecrit, spring, bij, 2.0, fb, &el);
+                    f[0] += fb[0];
+                    f[1] += fb[1];
+                    f[2] += fb[2];
+                }
+                sig[3 * n] += f[0] * ds;
+                sig[3 * n + 1] += f[1] * ds;
+                sig[3 * n + 2] += f[2] * ds;
+            }
+        }
+        memcpy(&sigma[9 * t], sig, sizeof sig);
+    }
+}
+
 void orc_get_state(const orc_state* s, double* U, double* V)
 {
     if (U) memcpy(U, s->U, sizeof(double) * 3 * s->n);

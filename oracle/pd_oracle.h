@@ -131,6 +131,11 @@ int64_t orc_sample_rows(const orc_problem* p, const int64_t* rows, int64_t nrows
 int orc_sample_forces(const orc_problem* p, const int64_t* rows, int64_t nrows, const double* u, double* F,
                       double* el, int nthreads);
 
+/* Areal force density (CalcArealForceDensity, CCI) at the given nodes from the current U:
+ * sigma[9 t + 3 n + c]; be (N, may be NULL) replaces the boundary-effect factors. */
+void orc_areal_force_density(const orc_state* s, const int64_t* nodes, int64_t count, const double* be,
+                             double* sigma);
+
 /* 0: reference pow(x, 2.0) via glibc (default); 1: correctly rounded square x*x */
 void orc_set_square_mode(int exact);
 

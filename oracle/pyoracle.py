@@ -114,6 +114,7 @@ def lib() -> C.CDLL:
         L.orc_num_contrib.argtypes = [vp]
         L.orc_get_gauss.argtypes = [vp, dp, dp]
         L.orc_get_mls.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), dp]
+        L.orc_areal_force_density.argtypes = [vp, C.POINTER(C.c_int64), C.c_int64, dp, dp]
         L.orc_sample_forces.argtypes = [C.POINTER(Problem), C.POINTER(C.c_int64), C.c_int64, dp, dp, dp, C.c_int]
         L.orc_sample_rows.restype = C.c_int64
         L.orc_sample_rows.argtypes = [C.POINTER(Problem), C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64),
@@ -318,6 +319,15 @@ class Oracle:
         U = None if U is None else np.ascontiguousarray(U, dtype=np.float64).reshape(-1)
         V = None if V is None else np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
         self.L.orc_set_state(self.h, _dp(U) if U is not None else None, _dp(V) if V is not None else None)
+
+    def areal_force_density(self, nodes, be=None):
+        """Areal force density (9 components per node, row n = plane normal) from the current U."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+        sig = np.zeros(9 * max(len(nodes), 1))
+        bep = None if be is None else np.ascontiguousarray(be, dtype=np.float64)
+        self.L.orc_areal_force_density(self.h, nodes.ctypes.data_as(C.POINTER(C.c_int64)), len(nodes),
+                                       _dp(bep) if bep is not None else None, _dp(sig))
+        return sig[: 9 * len(nodes)].reshape(-1, 3, 3)
 
     def gauss(self):
         """Gauss-point positions and displacements, each (N, n^3, 3); None for CCI."""

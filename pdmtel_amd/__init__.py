@@ -78,6 +78,7 @@ def lib() -> C.CDLL:
     L.pdm_compute_forces.argtypes = [vp, dp, dp, dp]
     L.pdm_apply_tangent.argtypes = [vp, dp, dp]
     L.pdm_compute_forces_noalloc.argtypes = [vp, vp, vp, vp, C.c_int]
+    L.pdm_areal_force_density.argtypes = [vp, i64p, C.c_int64, dp, dp]
     L.pdm_set_state.argtypes = [vp, dp, dp]
     L.pdm_get_state.argtypes = [vp, dp, dp]
     L.pdm_step_rk4.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
@@ -204,6 +205,15 @@ class Problem:
         el = np.zeros(self.n)
         self._chk(self.L.pdm_compute_forces(self.h, _dp(u), _dp(F), _dp(el)))
         return (F.reshape(-1, 3), el) if with_elast else F.reshape(-1, 3)
+
+    def areal_force_density(self, nodes, bdryeff: Optional[np.ndarray] = None) -> np.ndarray:
+        """PostAFD's stress at the given nodes from the current U: (count, 3, 3), row = plane normal."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+        sig = np.zeros(9 * max(len(nodes), 1))
+        be = None if bdryeff is None else np.ascontiguousarray(bdryeff, dtype=np.float64)
+        self._chk(self.L.pdm_areal_force_density(self.h, nodes.ctypes.data_as(C.POINTER(C.c_int64)), len(nodes),
+                                                  _dp(be) if be is not None else None, _dp(sig)))
+        return sig[: 9 * len(nodes)].reshape(-1, 3, 3)
 
     def forces_noalloc(self, u_dev: int, F_dev: int, elast_dev: int = 0, sync: bool = True) -> None:
         """pdm_compute_forces_noalloc on device buffers given by address (reference-order 3N u

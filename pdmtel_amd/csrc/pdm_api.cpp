@@ -7,6 +7,7 @@
 #include <fstream>
 #include <iomanip>
 #include <sstream>
+#include <unordered_map>
 
 #include "pdm_internal.h"
 
@@ -1029,6 +1030,106 @@ int pdm_compute_forces_noalloc(pdm_handle* h, const double* u, double* F, double
     HCHK(hipSetDevice(h->dev));
     HCHK(noalloc_forces(h, tile_view(h), u, F, elast));
     if (sync) HCHK(hipStreamSynchronize(h->stream));
+    return PDM_OK;
+}
+
+int pdm_areal_force_density(pdm_handle* h, const int64_t* nodes, int64_t count, const double* bdryeff,
+                            double* sigma)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (count < 0 || (count > 0 && (!nodes || !sigma))) return fail(h, PDM_E_ARG, "null nodes or sigma");
+    if (h->fgi_ng) return fail(h, PDM_E_MODEL, "the areal force density is built for CCI decks only");
+    if (count == 0) return PDM_OK;
+    const Deck& d = h->deck;
+    const int64_t n = h->n;
+    for (int64_t t = 0; t < count; ++t)
+        if (nodes[t] < 0 || nodes[t] >= n) return fail(h, PDM_E_ARG, "node id out of range");
+    const int ns = (int)(d.horizon / d.dx);
+    if (ns < 1) return fail(h, PDM_E_MODEL, "the horizon is shorter than one grid spacing");
+    // collinear lattice nodes by exact coordinate equality (CalcArealForceDensity,
+    // pdDataManager3.cpp:463-481): x_s - dx (1 + k) along each axis
+    struct Key {
+        uint64_t a, b, c;
+        bool operator==(const Key& o) const { return a == o.a && b == o.b && c == o.c; }
+    };
+    struct KeyHash {
+        size_t operator()(const Key& k) const
+        {
+            uint64_t x = k.a * 0x9E3779B97F4A7C15ULL ^ (k.b + 0x632BE59BD9B4E019ULL + (k.a << 6));
+            x ^= k.c * 0xC2B2AE3D27D4EB4FULL + (x >> 29);
+            return (size_t)(x ^ (x >> 32));
+        }
+    };
+    auto key = [](double x, double y, double z) {
+        Key k;
+        std::memcpy(&k.a, &x, 8);
+        std::memcpy(&k.b, &y, 8);
+        std::memcpy(&k.c, &z, 8);
+        return k;
+    };
+    std::unordered_map<Key, int32_t, KeyHash> where;
+    where.reserve((size_t)n * 2);
+    for (int64_t i = n - 1; i >= 0; --i) where[key(h->X[3 * i], h->X[3 * i + 1], h->X[3 * i + 2])] = (int32_t)i;
+    std::vector<int32_t> colin((size_t)count * 3 * ns, -1);
+    for (int64_t t = 0; t < count; ++t) {
+        const double* xs = &h->X[3 * nodes[t]];
+        for (int q = 0; q < 3; ++q)
+            for (int k = 0; k < ns; ++k) {
+                double y[3] = {xs[0], xs[1], xs[2]};
+                y[q] = xs[q] - d.dx * (1 + k);
+                auto it = where.find(key(y[0], y[1], y[2]));
+                if (it != where.end()) colin[((size_t)t * 3 + q) * ns + k] = it->second;
+            }
+    }
+    // displacements in reference order (and the caller's boundary-effect factors)
+    std::vector<double> U(3 * (size_t)n);
+    rc = pdm_get_state(h, U.data(), nullptr);
+    if (rc) return rc;
+    std::vector<d4> Ud(n);
+    for (int64_t i = 0; i < n; ++i) Ud[i] = d4{U[3 * i], U[3 * i + 1], U[3 * i + 2], 0.0};
+    d4* Udev = nullptr;
+    double *bed = nullptr, *sig = nullptr;
+    int64_t* srcd = nullptr;
+    int32_t* cold = nullptr;
+    DevTemps temps(h->stream);
+    temps.add(&Udev);
+    temps.add(&bed);
+    temps.add(&sig);
+    temps.add(&srcd);
+    temps.add(&cold);
+    HCHK(hipMalloc(&Udev, sizeof(d4) * n));
+    HCHK(hipMalloc(&sig, sizeof(double) * 9 * count));
+    HCHK(hipMalloc(&srcd, sizeof(int64_t) * count));
+    HCHK(hipMalloc(&cold, sizeof(int32_t) * colin.size()));
+    HCHK(hipMemcpyAsync(Udev, Ud.data(), sizeof(d4) * n, hipMemcpyHostToDevice, h->stream));
+    HCHK(hipMemcpyAsync(srcd, nodes, sizeof(int64_t) * count, hipMemcpyHostToDevice, h->stream));
+    HCHK(hipMemcpyAsync(cold, colin.data(), sizeof(int32_t) * colin.size(), hipMemcpyHostToDevice, h->stream));
+    if (bdryeff) {
+        HCHK(hipMalloc(&bed, sizeof(double) * n));
+        HCHK(hipMemcpyAsync(bed, bdryeff, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    }
+    AfdArgs a{};
+    a.X = h->Xd;
+    a.U = Udev;
+    a.rp = h->rp;
+    a.col = h->col;
+    a.cut = h->cut;
+    a.mat = h->matd;
+    a.be = bdryeff ? bed : h->be;
+    for (size_t m = 0; m < d.mats.size() && m < (size_t)kMaxMat; ++m) {
+        a.ecrit[m] = d.mats[m].ecrit;
+        a.spring[m] = h->spring[m];
+    }
+    a.horizon = d.horizon;
+    a.dx = d.dx;
+    a.vol = std::pow(d.dx, 3.0);
+    a.ns = ns;
+    a.ds = d.horizon / ns;
+    a.original_cci = (d.cci_on && !d.cci_mdf) ? 1 : 0;
+    HCHK(launch_afd(a, srcd, count, cold, sig, h->stream));
+    HCHK(hipMemcpyAsync(sigma, sig, sizeof(double) * 9 * count, hipMemcpyDeviceToHost, h->stream));
+    HCHK(hipStreamSynchronize(h->stream));
     return PDM_OK;
 }
 

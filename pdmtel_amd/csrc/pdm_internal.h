@@ -197,6 +197,21 @@ struct DevTemps {
     DevTemps& operator=(const DevTemps&) = delete;
 };
 
+// Areal force density (pdm_post.hip): reference-order node data and the bond-force constants.
+struct AfdArgs {
+    const d4* X;            // positions
+    const d4* U;            // displacements
+    const int64_t* rp;      // reference-order CSR
+    const int32_t* col;
+    const uint8_t* cut;
+    const int32_t* mat;
+    const double* be;       // boundary-effect factors
+    double ecrit[kMaxMat], spring[kMaxMat];
+    double horizon, dx, vol, ds;
+    int ns;                 // collinear nodes per direction, int(delta / dx)
+    int original_cci;
+};
+
 // Bond-build parameters.
 struct BondArgs {
     const d4* X;
@@ -373,6 +388,10 @@ hipError_t launch_fgi_pairs(const TileView& tv, const FgiArgs& f, bool fill, int
 // address is looked up once per device at set-up; noalloc_forces enqueues on h->stream
 hipError_t noalloc_scratch_setup(int dev);
 hipError_t noalloc_forces(const pdm_handle* h, const TileView& tv, const double* u, double* F, double* elast);
+
+// areal force density: sigma[9 t ..] of source node src[t], collinear ids [t][3][ns]
+hipError_t launch_afd(const AfdArgs& a, const int64_t* src, int64_t nsrc, const int32_t* colinear, double* sigma,
+                      hipStream_t s);
 
 // static solve (pdm_cg.cpp / pdm_cg.hip)
 int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_stats* stats);

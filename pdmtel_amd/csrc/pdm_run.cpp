@@ -1,11 +1,13 @@
 // pdm_run -- command-line driver replacing the reference's main.cpp menu
 // (main.cpp:15-85): option 1 runs InitialProblem + SolveDynamic, option 2
-// InitialProblem + SolveStatic, anything else quits with the reference's
-// message.  The working directory (which holds pds_input.txt) is argv[1]
-// instead of the reference's stdin prompt (pdDataManager3.cpp:1503-1541);
-// the option is argv[2], or read from stdin as the reference's SelectMenu does.
+// InitialProblem + SolveStatic, option 4 InitialProblem + PostAFD, option 5
+// InitialProblem + PostFormatData (both read pds_tecplot_<step>.dat and ask on
+// stdin), anything else quits with the reference's message.  The working
+// directory (which holds pds_input.txt) is argv[1] instead of the reference's
+// stdin prompt (pdDataManager3.cpp:1503-1541); the option is argv[2], or read
+// from stdin as the reference's SelectMenu does.
 //
-//   pdm_run <wkdir> [1|2] [--fp32c] [--fp32c2] [--device N]
+//   pdm_run <wkdir> [1|2|4|5] [--fp32c] [--fp32c2] [--device N]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -23,6 +25,8 @@ int select_menu()
     std::cout << " Please choose an option to run the program: \n"
               << "   - Solve the problem using dynamic solution (1) \n"
               << "   - Solve the problem using static solution (2) \n"
+              << "   - Post: Read output file and calculate AFD on given node (4) \n"
+              << "   - Post: Convert output file to formatted data file (5) \n"
               << "   - Any other numbers to exit " << std::endl
               << " Your input: ";
     int i = 0;
@@ -35,7 +39,7 @@ int select_menu()
 int main(int argc, char** argv)
 {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s <wkdir> [1|2] [--fp32c|--fp32c2] [--device N]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <wkdir> [1|2|4|5] [--fp32c|--fp32c2] [--device N]\n", argv[0]);
         return 2;
     }
     pdm_options opt;
@@ -63,6 +67,12 @@ int main(int argc, char** argv)
               << " refinement sweeps, relative residual " << st.rel_residual << " **" << std::endl;
             f.close();
             pdmtel::pdSolve::WriteTecplot(dat, 1);
+        } else if (option == 4) {
+            pdmtel::pdSolve::InitialProblem(dat);
+            pdmtel::pdSolve::PostAFD(dat, std::cin, std::cout);
+        } else if (option == 5) {
+            pdmtel::pdSolve::InitialProblem(dat);
+            pdmtel::pdSolve::PostFormatData(dat, std::cin, std::cout);
         } else {
             std::ofstream f(out.c_str(), std::ios::app);
             f << std::endl << "user quits without doing any calculation!" << std::endl;

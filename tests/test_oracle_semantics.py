@@ -323,3 +323,77 @@ def test_veloc_and_displgrad_kinematics():
         assert math.isclose(U[1, 2], (Gm[6] * X[1, 0] + Gm[7] * X[1, 1] + Gm[8] * X[1, 2]) * fg, rel_tol=1e-14)
         assert V[1, 0] == 0.0 and V[1, 2] == 0.0
     assert fv == 0.0 and fg == 1.0  # both ends of the factors were crossed
+
+
+# ---------------------------------------------------------------- areal force density
+def afd_lattice_sum(G, delta_over_dx, dx, E, hits):
+    """The areal force density of CalcArealForceDensity at an interior lattice node under the
+    uniform small strain G, as a lattice sum written out from its definition: for normal e_n the
+    collinear nodes at -k dx e_n (k = 1..ns, ns = int(delta / dx)) that the reference finds
+    (hits[n][k - 1]: it compares coordinates x_s - dx (1 + is) for exact equality, so a node
+    whose coordinate that subtraction does not reproduce to the bit is left out), each with the
+    bonds to lattice offsets a of its family (point-to-cube distance < delta) whose end lies in
+    the half space a_n >= k, each carrying the linearised CCI force c dx^3 beta(r) (a^ . G a^) a^;
+    summed with weight ds = delta / ns."""
+    d = delta_over_dx
+    ns = int(d)
+    ds = d * dx / ns
+    c = 12.0 * E / (math.pi * (d * dx) ** 4)
+    R = range(-int(d) - 2, int(d) + 3)
+    offs = []
+    for i in R:
+        for j in R:
+            for k in R:
+                if i == j == k == 0:
+                    continue
+                q = [max(abs(v) - 0.5, 0.0) for v in (i, j, k)]
+                if math.sqrt(sum(x * x for x in q)) < d:
+                    offs.append((i, j, k))
+    sig = np.zeros((3, 3))
+    for n in range(3):
+        for kk in range(1, ns + 1):
+            if not hits[n][kk - 1]:
+                continue
+            for a in offs:
+                if a[n] < kk:
+                    continue
+                av = np.array(a, dtype=float)
+                r = np.linalg.norm(av)
+                ah = av / r
+                beta = min(1.0, max(0.0, d + 0.5 - r))
+                sig[n] += ds * c * dx ** 3 * beta * (ah @ G @ ah) * ah
+    return sig
+
+
+@pytest.mark.parametrize("strain", ["uniaxial", "shear", "isotropic"])
+def test_areal_force_density_interior_lattice_sum(strain):
+    """PostAFD's stress at the centre node of a 16^3 lattice block under a uniform strain equals
+    the lattice sum of its definition (the oracle searches nodes, tests every pair and evaluates
+    the nonlinear bond force; the sum uses offsets and the linearised force), to O(strain).  At
+    this node some collinear coordinates do not round back to the lattice's, and the reference's
+    exact comparison skips those nodes: the sum follows the same rule."""
+    eps = 1e-5
+    G = {"uniaxial": np.diag([eps, 0.0, 0.0]),
+         "shear": np.array([[0.0, eps, 0.0], [eps, 0.0, 0.0], [0.0, 0.0, 0.0]]),
+         "isotropic": eps * np.eye(3)}[strain]
+    spec = pm.bar_spec(16, 16, 16)
+    spec["bcs"], spec["boundary_spaces"], spec["traction_body"] = [], [], 0
+    o = pyoracle.Oracle(spec)
+    X, _, _ = o.nodes()
+    ctr = int(np.argmin(((X - X.mean(axis=0)) ** 2).sum(axis=1)))
+    assert o.bdryeff()[ctr] == 1.0
+    o.set_state(X @ G.T, None)
+    sig = o.areal_force_density([ctr])[0]
+    # which collinear nodes the exact-equality search finds (the same IEEE expressions)
+    hits = [[any(np.array_equal(np.where(np.arange(3) == n, X[ctr] - DX * (1 + k), X[ctr]), x) for x in X)
+             for k in range(3)] for n in range(3)]
+    assert any(any(h) for h in hits) and not all(all(h) for h in hits)
+    want = afd_lattice_sum(G, 3.015, DX, E0, hits)
+    scale = np.abs(want).max()
+    print(strain, sig / (E0 * eps), want / (E0 * eps))
+    assert np.abs(sig - want).max() <= 1e-4 * scale
+    # zero field, zero stress; the response is odd in the strain
+    o.set_state(-(X @ G.T), None)
+    assert np.allclose(o.areal_force_density([ctr])[0], -sig, rtol=0, atol=1e-4 * scale)  # to O(strain)
+    o.set_state(np.zeros_like(X), None)
+    assert np.all(o.areal_force_density([ctr])[0] == 0.0)
