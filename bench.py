@@ -86,10 +86,13 @@ def workload_spec(args, steps, workload=None):
 
 
 def graph_chunk(args):
-    """Steps per captured graph: a divisor of both W and K (at most 50), so that the warm-up
-    captures the only graph the timed steps replay."""
-    c = math.gcd(args.warmup, args.steps) if args.warmup else args.steps
-    return max(1, min(args.chunk, c))
+    """Steps per captured graph: all K timed steps in one graph up to 200 steps, else the largest
+    divisor of K not above 200.  The graph of that length is captured before the warm-up
+    (prepare), so the timed steps replay it with no capture or host sync between chunks."""
+    k = args.steps
+    if k <= 200:
+        return max(1, k)
+    return next(c for c in range(200, 0, -1) if k % c == 0)
 
 
 def precision_code(name):
@@ -108,6 +111,7 @@ def run_precision(args, spec, precision):
         e, w, f = (int(v) for v in args.layout.split(","))
         layout = dict(tile_edge=e, waves_per_tile=w, families_per_wave=f)
     prob = pm.Problem(text, precision=precision, graph_chunk=graph_chunk(args), energies=1, **layout)
+    prob.prepare(graph_chunk(args))
     t_init = time.perf_counter() - t0
     if args.warmup:
         prob.step(args.warmup)
@@ -164,6 +168,7 @@ def run_fgi(args):
     spec["fgi"] = (1, n1)
     t0 = time.perf_counter()
     prob = pm.Problem(pm.deck_text(spec), precision=pm.PDM_FP64, graph_chunk=graph_chunk(args), energies=1)
+    prob.prepare(graph_chunk(args))
     t_init = time.perf_counter() - t0
     print(f"# fgi n={n1}: init {t_init:.2f} s, {prob.n} nodes, {prob.E} entries", file=sys.stderr, flush=True)
     if args.warmup:
@@ -217,6 +222,7 @@ def run_batch(args):
         print(f"# batch {name}: building {len(decks)} specimens", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         b = pm.Batch(decks, precision=prec, graph_chunk=graph_chunk(args), energies=1)
+        b.prepare(graph_chunk(args))
         t_init = time.perf_counter() - t0
         if args.warmup:
             b.step(args.warmup)

@@ -195,6 +195,12 @@ int pdm_get_state(const pdm_handle* h, double* U, double* V);
  * the count in *done) when total_step or total_time is reached. */
 int pdm_step_rk4(pdm_handle* h, int nsteps, int* done);
 
+/* Capture and cache the HIP graph of a `steps`-step chunk without running it, so that a later
+ * pdm_step_rk4 whose chunks have that length (options.graph_chunk >= steps) replays it with no
+ * capture on its path; two chunk lengths stay cached per handle.  No counterpart in the
+ * reference (which launches no graphs); for timed loops and serving loops. */
+int pdm_prepare_steps(pdm_handle* h, int steps);
+
 /* Current time, step count, kinetic and elastic energy of the last step
  * (GetKinetTotal / GetElastStep, pdDataManager3.cpp:1415-1428). */
 int pdm_get_clock(const pdm_handle* h, double* t, int* step, double* dt_next);
@@ -264,6 +270,8 @@ int64_t pdm_batch_num_live_entries(const pdm_batch* b, int spec);  /* as pdm_num
 /* Advance every specimen by up to nsteps steps of its own SolveDynamic loop; *done =
  * the largest count any specimen took. */
 int pdm_batch_step_rk4(pdm_batch* b, int nsteps, int* done);
+/* pdm_prepare_steps for the batch's launches (no counterpart in the reference). */
+int pdm_batch_prepare_steps(pdm_batch* b, int steps);
 
 /* Per-specimen state (3N, reference node order), clock and energies, as the single-
  * specimen calls. */

@@ -82,6 +82,7 @@ def lib() -> C.CDLL:
     L.pdm_set_state.argtypes = [vp, dp, dp]
     L.pdm_get_state.argtypes = [vp, dp, dp]
     L.pdm_step_rk4.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    L.pdm_prepare_steps.argtypes = [vp, C.c_int]
     L.pdm_get_clock.argtypes = [vp, dp, C.POINTER(C.c_int), dp]
     L.pdm_get_energy.argtypes = [vp, dp, dp]
     L.pdm_solve_dynamic.argtypes = [vp, C.c_char_p]
@@ -107,6 +108,7 @@ def lib() -> C.CDLL:
         getattr(L, name).restype = C.c_int64
         getattr(L, name).argtypes = [vp, C.c_int]
     L.pdm_batch_step_rk4.argtypes = [vp, C.c_int, ip]
+    L.pdm_batch_prepare_steps.argtypes = [vp, C.c_int]
     L.pdm_batch_get_state.argtypes = [vp, C.c_int, dp, dp]
     L.pdm_batch_set_state.argtypes = [vp, C.c_int, dp, dp]
     L.pdm_batch_get_clock.argtypes = [vp, C.c_int, dp, ip, dp]
@@ -259,6 +261,10 @@ class Problem:
         self._chk(self.L.pdm_step_rk4(self.h, int(nsteps), C.byref(done)))
         return done.value
 
+    def prepare(self, nsteps: int) -> None:
+        """Capture the graph of an nsteps-step chunk now, so stepping that many replays it."""
+        self._chk(self.L.pdm_prepare_steps(self.h, int(nsteps)))
+
     def clock(self) -> Tuple[float, int, float]:
         t, s, dt = C.c_double(), C.c_int(), C.c_double()
         self._chk(self.L.pdm_get_clock(self.h, C.byref(t), C.byref(s), C.byref(dt)))
@@ -338,6 +344,10 @@ class Batch:
         done = C.c_int()
         self._chk(self.L.pdm_batch_step_rk4(self.h, int(nsteps), C.byref(done)))
         return done.value
+
+    def prepare(self, nsteps: int) -> None:
+        """Capture the graph of an nsteps-step chunk now, so stepping that many replays it."""
+        self._chk(self.L.pdm_batch_prepare_steps(self.h, int(nsteps)))
 
     def state(self, spec: int) -> Tuple[np.ndarray, np.ndarray]:
         U = np.zeros(3 * self.n[spec])

@@ -282,6 +282,34 @@ int enqueue_steps(pdm_handle* h, int count)
     return PDM_OK;
 }
 
+// The graph of a `count`-step chunk: from the cache (one per distinct step count, kGraphSlots
+// of them), else captured from enqueue_steps and instantiated.  The chunk buffers must already
+// hold `count` steps.  A captured graph reads its step parameters from h->steps at replay.
+int chunk_graph(pdm_handle* h, int count, hipGraphExec_t* out)
+{
+    *out = nullptr;
+    for (int q = 0; q < pdm_handle::kGraphSlots; ++q)
+        if (h->graph_exec[q] && h->graph_steps[q] == count) *out = h->graph_exec[q];
+    if (*out) return PDM_OK;
+    const int q = h->graph_next;
+    h->graph_next = (q + 1) % pdm_handle::kGraphSlots;
+    if (h->graph_exec[q]) (void)hipGraphExecDestroy(h->graph_exec[q]);
+    h->graph_exec[q] = nullptr;
+    h->graph_steps[q] = 0;
+    hipGraph_t g;
+    HCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_steps(h, count);
+    hipError_t ce = hipStreamEndCapture(h->stream, &g);
+    if (rc) return rc;
+    HCHK(ce);
+    hipError_t ie = hipGraphInstantiate(&h->graph_exec[q], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HCHK(ie);
+    h->graph_steps[q] = count;
+    *out = h->graph_exec[q];
+    return PDM_OK;
+}
+
 // Run `count` steps; sp holds count rows of nspec StepParams ([step][specimen]).
 int run_chunk(pdm_handle* h, const std::vector<StepParam>& sp)
 {
@@ -290,29 +318,11 @@ int run_chunk(pdm_handle* h, const std::vector<StepParam>& sp)
     int rc = ensure_chunk(h, std::max(count, h->opt.graph_chunk));
     if (rc) return rc;
     HCHK(hipMemcpyAsync(h->steps, sp.data(), sizeof(StepParam) * sp.size(), hipMemcpyHostToDevice, h->stream));
-    // every chunk runs as a captured graph (one per distinct step count, cached)
+    // every chunk runs as a captured graph
     hipGraphExec_t exec = nullptr;
     if (h->opt.use_graph) {
-        for (int q = 0; q < pdm_handle::kGraphSlots; ++q)
-            if (h->graph_exec[q] && h->graph_steps[q] == count) exec = h->graph_exec[q];
-        if (!exec) {
-            const int q = h->graph_next;
-            h->graph_next = (q + 1) % pdm_handle::kGraphSlots;
-            if (h->graph_exec[q]) (void)hipGraphExecDestroy(h->graph_exec[q]);
-            h->graph_exec[q] = nullptr;
-            h->graph_steps[q] = 0;
-            hipGraph_t g;
-            HCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            rc = enqueue_steps(h, count);
-            hipError_t ce = hipStreamEndCapture(h->stream, &g);
-            if (rc) return rc;
-            HCHK(ce);
-            hipError_t ie = hipGraphInstantiate(&h->graph_exec[q], g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            HCHK(ie);
-            h->graph_steps[q] = count;
-            exec = h->graph_exec[q];
-        }
+        rc = chunk_graph(h, count, &exec);
+        if (rc) return rc;
     }
     HCHK(hipEventRecord(h->ev0, h->stream));
     if (exec) HCHK(hipGraphLaunch(exec, h->stream));
@@ -1243,6 +1253,19 @@ int pdm_step_rk4(pdm_handle* h, int nsteps, int* done)
     return PDM_OK;
 }
 
+int pdm_prepare_steps(pdm_handle* h, int steps)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (steps <= 0) return fail(h, PDM_E_ARG, "step count must be positive");
+    if (!h->opt.use_graph) return PDM_OK;
+    HCHK(hipSetDevice(h->dev));
+    rc = ensure_chunk(h, std::max(steps, h->opt.graph_chunk));
+    if (rc) return rc;
+    hipGraphExec_t exec = nullptr;
+    return chunk_graph(h, steps, &exec);
+}
+
 int pdm_get_clock(const pdm_handle* h, double* t, int* step, double* dt_next)
 {
     int rc = require_init(h);
@@ -1568,6 +1591,12 @@ int64_t pdm_batch_num_live_entries(const pdm_batch* b, int spec)
     if (!b || !b->h) return -1;
     if (spec < 0) return b->h->Elive;
     return (spec < b->h->nspec) ? b->h->members[spec]->Elive : -1;
+}
+
+int pdm_batch_prepare_steps(pdm_batch* b, int steps)
+{
+    if (!b || !b->h) return PDM_E_ARG;
+    return pdm_prepare_steps(b->h, steps);
 }
 
 int pdm_batch_step_rk4(pdm_batch* b, int nsteps, int* done)

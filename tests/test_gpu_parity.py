@@ -169,6 +169,24 @@ def test_rk4_graph_matches_eager():
         assert np.array_equal(Ua, Ub) and np.array_equal(Va, Vb)
 
 
+def test_rk4_prepared_graph_matches_eager():
+    """pdm_prepare_steps captures a 7-step chunk before any step runs; chunks of 3 (captured on
+    the way) and 7 (replayed) then give what eager launches give, bit for bit, and the clock
+    and energies agree."""
+    spec = make("bar_jitter", steps=40)
+    a = pm.Problem(pm.deck_text(spec), use_graph=1, graph_chunk=7, energies=1)
+    b = pm.Problem(pm.deck_text(spec), use_graph=0, energies=1)
+    a.prepare(7)
+    with pytest.raises(pm.PdmError):
+        a.prepare(0)
+    for n in (3, 7, 7, 3):
+        assert a.step(n) == n and b.step(n) == n
+        Ua, Va = a.state()
+        Ub, Vb = b.state()
+        assert np.array_equal(Ua, Ub) and np.array_equal(Va, Vb)
+        assert a.clock() == b.clock() and a.energy() == b.energy()
+
+
 @pytest.mark.parametrize("case", ["bar_jitter", "bar_delta4", "bar_cci_original", "slits"])
 @pytest.mark.parametrize("precision", [pm.PDM_FP32C, pm.PDM_FP32C2])
 def test_forces_fp32(case, precision):
