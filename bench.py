@@ -87,9 +87,12 @@ def workload_spec(args, steps, workload=None):
 
 def graph_chunk(args):
     """Steps per captured graph: all K timed steps in one graph up to 200 steps, else the largest
-    divisor of K not above 200.  The graph of that length is captured before the warm-up
-    (prepare), so the timed steps replay it with no capture or host sync between chunks."""
+    divisor of K not above 200 (or --chunk when given and dividing K).  The graph of that length
+    is captured before the warm-up (prepare), so the timed steps replay it with no capture or
+    host sync between chunks."""
     k = args.steps
+    if args.chunk > 0 and k % args.chunk == 0:
+        return args.chunk
     if k <= 200:
         return max(1, k)
     return next(c for c in range(200, 0, -1) if k % c == 0)
@@ -244,7 +247,7 @@ def run_batch(args):
     # is relative to max |U| at the last step, which is ill-conditioned for a specimen that
     # oscillates and is near a zero crossing then; the peak over the run is the amplitude.
     print("# batch: untimed FP64 rerun for the displacement amplitudes", file=sys.stderr, flush=True)
-    b = pm.Batch(decks, precision=pm.PDM_FP64, graph_chunk=args.chunk, energies=1)
+    b = pm.Batch(decks, precision=pm.PDM_FP64, graph_chunk=50, energies=1)
     peak = np.zeros(b.nspec)
     total, done = args.warmup + args.steps, 0
     while done < total:
@@ -346,7 +349,8 @@ def main():
     ap.add_argument("--nx", type=int, default=DEFAULT_DIMS[0])
     ap.add_argument("--ny", type=int, default=DEFAULT_DIMS[1])
     ap.add_argument("--nz", type=int, default=DEFAULT_DIMS[2])
-    ap.add_argument("--chunk", type=int, default=50)
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="steps per graph (a divisor of --steps; default: automatic, up to 200)")
     ap.add_argument("--precision", choices=["all", "fp64", "fp32c", "fp32c2"], default="all")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--skip-cpu", action="store_true")

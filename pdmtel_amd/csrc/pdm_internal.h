@@ -94,6 +94,8 @@ struct SpecDev {
     double damping;       // damping_coeff
     double vol;           // pow(dx, 3): node volume for the energies
     double kc0;           // dx^3 c / 2 of the specimen's materials (when uniform_mat)
+    double ivc4;          // 0.25 / dx^3: energy-density scale of the FP64 family sums
+    double ivc4_rec;      // 0.25 / dx^3 / fscale: energy-density scale of the FP32 record sums
     // beta(r) of the FP32 records (r in record units, S m), offset by the correction bias:
     // med3(beta0 - r beta1, -128 2^-26, 1 - 128 2^-26)
     float beta0, beta1;

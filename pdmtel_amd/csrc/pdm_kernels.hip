@@ -881,7 +881,8 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev&
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane / LPN, sl = lane % LPN;
     // FP32 records are scaled by S: the energy sum coef s (s r S) carries one factor S
-    const double ivc4 = 0.25 * sp.inv_vol_cube * (kF32Rec<OP> ? 1.0 / sp.fscale : 1.0);
+    // (the host forms both scales once, so they stay in scalar registers)
+    const double ivc4 = kF32Rec<OP> ? sp.ivc4_rec : sp.ivc4;
     int first = wave * NPW;
     if (first >= nt) return;
     int loc = first + g;

@@ -204,6 +204,8 @@ void fill_spec_tables(const pdm_handle* h, SpecDev& sp, MatDev* mats, BCDev* bcs
     sp.ecrit0 = d.mats.empty() ? 0.0 : d.mats[0].ecrit;
     sp.inv_vol_cube = 1.0 / (d.dx * d.dx * d.dx);
     sp.fscale = h->fscale;
+    sp.ivc4 = 0.25 * sp.inv_vol_cube;
+    sp.ivc4_rec = 0.25 * sp.inv_vol_cube * (1.0 / sp.fscale);
     sp.damping = d.damping;
     sp.vol = std::pow(d.dx, 3.0);
     sp.kc0 = h->spring.empty() ? 0.0 : 0.5 * (d.dx * d.dx * d.dx) * h->spring[0];
