@@ -1364,13 +1364,33 @@ int pdm_solve_dynamic(pdm_handle* h, const char* out_dir)
     outf << std::endl << "** Dynamic solution: starting from cycle 1 **" << std::endl;
     const Deck& d = h->deck;
     int dump = std::max(1, d.dump);
+    // WriteEachStep (pdSolve.cpp:751-774): every step, the nodes at exactly (dx/2, dx/2, dx/2),
+    // printed as pdNode::Print (pdNode.cpp:418-433) after the step and the time, in the step
+    // file's default number format
+    struct Watch {
+        int64_t ref, in;
+    };
+    std::vector<Watch> watch;
+    const double half = 0.5 * d.dx;
+    for (int64_t i = 0; i < h->n; ++i)
+        if (h->X[3 * i] == half && h->X[3 * i + 1] == half && h->X[3 * i + 2] == half) watch.push_back({i, -1});
+    std::vector<double> be;
+    if (!watch.empty()) {
+        for (int64_t a = 0; a < h->n; ++a)
+            for (Watch& w : watch)
+                if (h->perm[a] == w.ref) w.in = a;
+        be.resize(h->n);
+        rc = pdm_get_bdryeff(h, be.data());
+        if (rc) return rc;
+    }
     std::vector<StepParam> sp;
     std::vector<double> en;
-    // break chunks at steps that write a dump (1, every dump, last)
+    // break chunks at steps that write a dump (1, every dump, last), and after every step
+    // while nodes are watched
     for (;;) {
         int next = h->step + 1;
         int want = 1;
-        if (next != 1) {
+        if (next != 1 && watch.empty()) {
             int to_dump = dump - (h->step % dump);
             want = std::min(to_dump, h->opt.graph_chunk);
         }
@@ -1388,6 +1408,15 @@ int pdm_solve_dynamic(pdm_handle* h, const char* out_dir)
             int iter = before + k + 1;
             outf << iter << ": dt= " << sp[k].dt << " t= " << sp[k].t << " KE= " << (h->opt.energies ? en[2 * k] : 0.0)
                  << " EE= " << (h->opt.energies ? en[2 * k + 1] : 0.0) << std::endl;
+        }
+        for (const Watch& w : watch) {  // one step per chunk here: the state after step h->step
+            d4 u, v;
+            HCHK(hipMemcpy(&u, h->U + w.in, sizeof(d4), hipMemcpyDeviceToHost));
+            HCHK(hipMemcpy(&v, h->V + w.in, sizeof(d4), hipMemcpyDeviceToHost));
+            const double* x = &h->X[3 * w.ref];
+            stepf << h->step << "\t" << sp[cnt - 1].t << "\t" << w.ref << "\t" << x[0] << "\t" << x[1] << "\t" << x[2]
+                  << "\t" << u.x << "\t" << u.y << "\t" << u.z << "\t" << v.x << "\t" << v.y << "\t" << v.z << "\t"
+                  << h->mat[w.ref] << "\t" << h->bc[w.ref] << "\t" << be[w.ref] << "\t" << 0.0 << std::endl;
         }
         int iter = h->step;
         if (iter == 1 || iter == h->last_step || iter % dump == 0) {

@@ -63,3 +63,65 @@ def test_cli_static_run(tmp_path):
     Ud = orc.solve_static_dense()
     T = read_tecplot(tmp_path / "pds_tecplot_1.dat")
     assert np.allclose(T[:, 4:7], Ud, rtol=2e-5, atol=2e-5 * np.abs(Ud).max())
+
+
+def test_solve_dynamic_output_files(tmp_path):
+    """pds_output.dat and pds_steps.dat of pdm_solve_dynamic against the reference writers
+    (pdSolve.cpp:24-130, 751-774): a per-step line "iter: dt= t= KE= EE=" in scientific
+    notation with 5 digits, and for the node at exactly (dx/2, dx/2, dx/2) a per-step line
+    "step time id x1 x2 x3 u1 u2 u3 v1 v2 v3 mat bc bdryEff dmg" in the step file's default
+    number format (6 significant digits, %g), values against the oracle's trajectory.  The
+    watched node is a free traction node here (the bar is held at x = L and pulled at x = 0)."""
+    import re
+
+    dx = 1e-3
+    spec = pm.bar_spec(8, 6, 6, steps=6)  # its first node is at exactly dx/2 (not so for nx = 10)
+    L, big = 8 * dx, 1e3
+    spec["bcs"][1]["value"] = (-spec["bcs"][1]["value"][0], 0.0, 0.0)
+    spec["boundary_spaces"] = [dict(geo=1, ref=1, box=(L - 1.0 * dx, big, -big, big, -big, big)),
+                               dict(geo=1, ref=2, box=(-big, 1.0 * dx, -big, big, -big, big))]
+    prob = pm.Problem(pm.deck_text(spec), energies=1)
+    X, mat, bc = prob.nodes()
+    watched = np.nonzero((X[:, 0] == 0.5 * dx) & (X[:, 1] == 0.5 * dx) & (X[:, 2] == 0.5 * dx))[0]
+    assert len(watched) == 1 and bc[watched[0]] == 1
+    w = int(watched[0])
+    be = prob.bdryeff()
+    prob.solve_dynamic(str(tmp_path))
+    orc = pyoracle.Oracle(spec, bond_method=1)
+    ref = []
+    for _ in range(6):
+        orc.run(1)
+        U, V = orc.state()
+        ref.append((orc.t, orc.ke, orc.ee, U[w].copy(), V[w].copy()))
+
+    sci = re.compile(r"^-?\d\.\d{5}e[+-]\d\d$")
+    out = (tmp_path / "pds_output.dat").read_text().splitlines()
+    k = out.index("** Dynamic solution: starting from cycle 1 **")
+    lines = [l for l in out[k + 1:] if re.match(r"^\d+: dt= ", l)]
+    assert len(lines) == 6
+    for n, l in enumerate(lines, 1):
+        m = re.match(r"^(\d+): dt= (\S+) t= (\S+) KE= (\S+) EE= (\S+)$", l)
+        assert m and int(m.group(1)) == n, l
+        assert all(sci.match(m.group(g)) for g in range(2, 6)), l
+        t, ke, ee = (float(m.group(g)) for g in (3, 4, 5))
+        tr, ker, eer = ref[n - 1][:3]
+        assert abs(t - tr) <= 1e-5 * tr and abs(ke - ker) <= 1e-5 * abs(ker) and abs(ee - eer) <= 1e-5 * abs(eer)
+
+    steps = (tmp_path / "pds_steps.dat").read_text().splitlines()
+    assert steps[0] == "Data at given node for each time step output mode"
+    assert steps[1] == ""
+    assert steps[2] == "Step Time ID X1 X2 X3 U1 U2 U3 V1 V2 V3 MatID BCID bdryEff DmgFrac "
+    assert len(steps) == 3 + 6
+    for n, l in enumerate(steps[3:], 1):
+        f = l.split("\t")
+        assert len(f) == 16, l
+        for tok in f[1:]:  # default stream format: what %g prints for the value read back
+            assert tok == f"{float(tok):g}", (tok, l)
+        t, U, V = ref[n - 1][0], ref[n - 1][3], ref[n - 1][4]
+        assert int(f[0]) == n and int(f[2]) == w and int(f[12]) == mat[w] and int(f[13]) == 1
+        assert np.allclose([float(v) for v in f[3:6]], X[w], rtol=1e-5, atol=0)
+        vals = np.array([float(v) for v in f[6:12]])
+        want = np.r_[U, V]
+        assert np.all(np.abs(vals - want) <= 1e-5 * np.abs(want) + 1e-9 * np.abs(want).max()), (vals, want)
+        assert abs(float(f[1]) - t) <= 1e-5 * t and abs(float(f[14]) - be[w]) <= 1e-5 * be[w] and float(f[15]) == 0.0
+    print(f"[output files] 6 steps of pds_output.dat and pds_steps.dat (node {w}) match", flush=True)
