@@ -90,7 +90,8 @@ int pdm_options_default(pdm_options* opt);
  * ReadKeyword/FindKeyword (pdDataManager.h:270-421) and SetWkDir's input
  * check (pdDataManager3.cpp:1503-1541).  Integration: CCI (cci_on 1) in any precision;
  * fixed Gaussian integration (cci_on 0, fgi_on 1, intg_point 1..6 or 8) in PDM_FP64;
- * adaptive integration (ai_on) is refused with PDM_E_MODEL. */
+ * adaptive integration (cci_on 0, fgi_on 0, ai_on 1 mls, intg_point 1..5) in PDM_FP64, without
+ * error control (error_control_on 1 is refused with PDM_E_MODEL). */
 int pdm_create(const char* input_path, const pdm_options* opt, pdm_handle** out);
 
 /* Same, from deck text in memory. */
@@ -183,6 +184,16 @@ int pdm_get_gauss(const pdm_handle* h, double* Xg, double* Ug);
  * to the force and the energy (pdDataManager.cpp:1918-1957 scales both by beta).  No
  * reference counterpart (the reference evaluates every pair); for throughput accounting. */
 int64_t pdm_num_live_gauss_pairs(const pdm_handle* h);
+
+/* Adaptive integration (ai_on 1): the trapezoidal points of every family entry, node j's
+ * cell clipped to node i's horizon sphere (CalcBondForceIntg, pdDataManager2.cpp:1222-1291;
+ * InitIntegrationPoint's AI branch, :257-297; pdIntgManager.cpp:49-451, 523-3482), 0 for the
+ * other modes.  pdm_get_ai_points: per point its entry's node i and node j (reference ids) and
+ * x, y, z relative to node i and the weight (xw, 4 per point); the points of an entry are
+ * contiguous, in x (outer), y, z (inner) order; entries are in no particular order.  Any
+ * pointer may be NULL. */
+int64_t pdm_num_ai_points(const pdm_handle* h);
+int pdm_get_ai_points(const pdm_handle* h, int32_t* node, int32_t* nbr, double* xw);
 
 /* Set / get the dynamic state U, V (3N each, host; NULL skips). */
 int pdm_set_state(pdm_handle* h, const double* U, const double* V);

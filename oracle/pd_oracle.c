@@ -99,6 +99,12 @@ struct orc_state {
     int32_t* cb;
     double* phi;   /* ng * cb_rp[N]: node i's block [g][k] at ng * cb_rp[i] */
     double* G;     /* 3 N ng: Gauss-point displacements */
+    /* adaptive integration (ai_on): trapezoidal points of every entry, in family coordinates */
+    int ai;
+    int64_t* ai_ep;  /* E+1: points of each entry (reference CSR order) */
+    double* ai_xw;   /* 4 P: x, y, z relative to node i, weight */
+    int64_t* ai_po;  /* P+1: MLS coefficients of each point over node j's contributing nodes */
+    double* ai_phi;
     /* dynamic state */
     double *U, *V, *Vb, *elast;
     double t, dt_cur, dt_stable, ke, ee;
@@ -330,6 +336,7 @@ void orc_destroy(orc_state* s)
     free(s->be); free(s->spring); free(s->trac);
     free(s->U); free(s->V); free(s->Vb); free(s->elast);
     free(s->gofs); free(s->gw); free(s->cb_rp); free(s->cb); free(s->phi); free(s->G);
+    free(s->ai_ep); free(s->ai_xw); free(s->ai_po); free(s->ai_phi);
     free(s);
 }
 
@@ -810,6 +817,69 @@ static int gj_solve(double A[10][10], double* b, double* x)
     return 0;
 }
 
+/* MLS contributing nodes of every node (InitIntegrationPoint, pdDataManager2.cpp:106-160):
+ * the node itself, then its family within rw, or within 1.5 rw when that leaves fewer than
+ * 10.  Returns the total list length. */
+static int64_t init_contrib(orc_state* s)
+{
+    const orc_problem* p = &s->p;
+    int64_t n = s->n;
+    s->cb_rp = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
+    s->cb_rp[0] = 0;
+    ivec lst = {0, 0, NULL};
+    for (int64_t i = 0; i < n; ++i) {
+        const double* xi = &s->X[3 * i];
+        for (int pass = 0; pass < 2; ++pass) {
+            double rad = pass ? p->rw * 1.5 : p->rw;
+            int64_t start = lst.cnt;
+            ivec_push(&lst, (int32_t)i);
+            for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
+                const double* xj = &s->X[3 * s->col[e]];
+                double d = sqrt(sq(xj[0] - xi[0]) + sq(xj[1] - xi[1]) + sq(xj[2] - xi[2]));
+                if (d <= rad) ivec_push(&lst, s->col[e]);
+            }
+            if (lst.cnt - start >= 10 || pass == 1) break;
+            lst.cnt = start;
+        }
+        s->cb_rp[i + 1] = lst.cnt;
+    }
+    s->cb = lst.v;
+    return lst.cnt;
+}
+
+/* MLS coefficients at the point xg of node o's contributing nodes, the basis relative to node
+ * o in units of dx (InitMLSCoeff, pdIntgManager2.cpp:1879-1930; GetMLSCoeff's FGI twin,
+ * pdDataManager2.cpp:177-254): phi_k = w_k p_k^T A^-1 p(xg), A = sum_k w_k p_k p_k^T.
+ * -1 if A is singular. */
+static int mls_coef_at(const orc_state* s, int64_t o, const double* xg, double* out)
+{
+    const orc_problem* p = &s->p;
+    const double* xo = &s->X[3 * o];
+    int64_t c0 = s->cb_rp[o], m = s->cb_rp[o + 1] - c0;
+    double A[10][10], bb[10], sol[10], pg[10];
+    memset(A, 0, sizeof A);
+    for (int64_t k = 0; k < m; ++k) {
+        const double* xk = &s->X[3 * s->cb[c0 + k]];
+        double pk[10];
+        mls_basis((xk[0] - xo[0]) / p->dx, (xk[1] - xo[1]) / p->dx, (xk[2] - xo[2]) / p->dx, pk);
+        double wk = mls_weight(xk, xg, p->rw);
+        for (int r = 0; r < 10; ++r)
+            for (int c = 0; c < 10; ++c) A[r][c] += wk * pk[r] * pk[c];
+    }
+    mls_basis((xg[0] - xo[0]) / p->dx, (xg[1] - xo[1]) / p->dx, (xg[2] - xo[2]) / p->dx, pg);
+    memcpy(bb, pg, sizeof bb);
+    if (gj_solve(A, bb, sol)) return -1;
+    for (int64_t k = 0; k < m; ++k) {
+        const double* xk = &s->X[3 * s->cb[c0 + k]];
+        double pk[10];
+        mls_basis((xk[0] - xo[0]) / p->dx, (xk[1] - xo[1]) / p->dx, (xk[2] - xo[2]) / p->dx, pk);
+        double v = 0.0;
+        for (int r = 0; r < 10; ++r) v += pk[r] * sol[r];
+        out[k] = mls_weight(xk, xg, p->rw) * v;
+    }
+    return 0;
+}
+
 /* SetGaussianPoint and InitIntegrationPoint's FGI branch (pdDataManager2.cpp:72-104,
  * 106-256): n^3 points per node (x index outermost, z innermost) at x_i + (dx/2) a, zero
  * displacement; for n > 1 the MLS contributing nodes (the node itself, then its family
@@ -837,55 +907,16 @@ static int init_fgi(orc_state* s)
             }
     s->G = (double*)calloc(3 * (size_t)(n ? n : 1) * ng, sizeof(double));
     if (n1 == 1) return 0;
-    s->cb_rp = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
-    s->cb_rp[0] = 0;
-    ivec lst = {0, 0, NULL};
-    for (int64_t i = 0; i < n; ++i) {
-        const double* xi = &s->X[3 * i];
-        for (int pass = 0; pass < 2; ++pass) {
-            double rad = pass ? p->rw * 1.5 : p->rw;
-            int64_t start = lst.cnt;
-            ivec_push(&lst, (int32_t)i);
-            for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
-                const double* xj = &s->X[3 * s->col[e]];
-                double d = sqrt(sq(xj[0] - xi[0]) + sq(xj[1] - xi[1]) + sq(xj[2] - xi[2]));
-                if (d <= rad) ivec_push(&lst, s->col[e]);
-            }
-            if (lst.cnt - start >= 10 || pass == 1) break;
-            lst.cnt = start;
-        }
-        s->cb_rp[i + 1] = lst.cnt;
-    }
-    s->cb = lst.v;
-    s->phi = (double*)malloc(sizeof(double) * (size_t)ng * (size_t)(lst.cnt ? lst.cnt : 1));
+    int64_t total = init_contrib(s);
+    s->phi = (double*)malloc(sizeof(double) * (size_t)ng * (size_t)(total ? total : 1));
     int bad = 0;
     for (int64_t i = 0; i < n && !bad; ++i) {
         const double* xi = &s->X[3 * i];
         int64_t c0 = s->cb_rp[i], m = s->cb_rp[i + 1] - c0;
         double* ph = s->phi + (size_t)ng * c0;
-        for (int g = 0; g < ng; ++g) {
+        for (int g = 0; g < ng && !bad; ++g) {
             double xg[3] = {s->gofs[3 * g] + xi[0], s->gofs[3 * g + 1] + xi[1], s->gofs[3 * g + 2] + xi[2]};
-            double A[10][10], bb[10], sol[10], pg[10];
-            memset(A, 0, sizeof A);
-            for (int64_t k = 0; k < m; ++k) {
-                const double* xk = &s->X[3 * s->cb[c0 + k]];
-                double pk[10];
-                mls_basis((xk[0] - xi[0]) / p->dx, (xk[1] - xi[1]) / p->dx, (xk[2] - xi[2]) / p->dx, pk);
-                double wk = mls_weight(xk, xg, p->rw);
-                for (int r = 0; r < 10; ++r)
-                    for (int c = 0; c < 10; ++c) A[r][c] += wk * pk[r] * pk[c];
-            }
-            mls_basis((xg[0] - xi[0]) / p->dx, (xg[1] - xi[1]) / p->dx, (xg[2] - xi[2]) / p->dx, pg);
-            memcpy(bb, pg, sizeof bb);
-            if (gj_solve(A, bb, sol)) { bad = 1; break; }
-            for (int64_t k = 0; k < m; ++k) {
-                const double* xk = &s->X[3 * s->cb[c0 + k]];
-                double pk[10];
-                mls_basis((xk[0] - xi[0]) / p->dx, (xk[1] - xi[1]) / p->dx, (xk[2] - xi[2]) / p->dx, pk);
-                double v = 0.0;
-                for (int r = 0; r < 10; ++r) v += pk[r] * sol[r];
-                ph[(size_t)g * m + k] = mls_weight(xk, xg, p->rw) * v;
-            }
+            bad = mls_coef_at(s, i, xg, ph + (size_t)g * m) != 0;
         }
     }
     return bad ? -7 : 0;
@@ -968,6 +999,247 @@ static void fgi_entry(const orc_state* s, int64_t i, int64_t j, const double* ui
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* Adaptive integration (ai_on 1): CalcBondForceIntg (pdDataManager2.cpp:1222-1291) integrates
+ * the bond force over node j's cell clipped to node i's horizon sphere with nested trapezoidal
+ * rules (CalcIntg3D / CalcIntgX1 / CalcIntgX2, pdIntgManager.cpp:49-280; CalcTrapzdQuad,
+ * :347-451): intg_point n gives 2^(n-1) + 1 points per direction, end points weighted 1/2.
+ * The cell is taken whole when the bond's longest distance is inside the horizon
+ * (CalcConfigurationType3, :3471-3482), and nothing when its shortest is not.  A partial cell
+ * is integrated in x (outer), y, z (inner) between the cell faces and the sphere: the limits of
+ * each direction are the face or the sphere, whichever is nearer, at the outer coordinates.
+ * (The reference enumerates the sphere/cube configurations instead -- 5 subtypes with a node
+ * on an axis, 39 cases otherwise, :523-3469 -- and returns 0 for a configuration outside its
+ * list; here one rule covers every configuration.)  An interval shorter than 1e-10 adds
+ * nothing, as in the reference.  A point's weight is the product of its three 1-D weights. */
+static int ai_cell_points(const orc_problem* p, const double* c, double dmin, double dmax, double* xw, int cap)
+{
+    double delta = p->horizon, h = 0.5 * p->dx;
+    if (delta <= dmin) return 0;
+    int full = delta >= dmax;
+    int nseg = 1 << (p->intg_point - 1);
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) { lo[k] = c[k] - h; hi[k] = c[k] + h; }
+    double a1 = lo[0], b1 = hi[0];
+    if (!full) { a1 = fmax(a1, -delta); b1 = fmin(b1, delta); }
+    if (b1 - a1 <= 1.0e-10) return 0;
+    double H1 = (b1 - a1) / nseg;
+    int cnt = 0;
+    for (int ia = 0; ia <= nseg; ++ia) {
+        double x = ((nseg - ia) * a1 + ia * b1) / nseg;
+        double w1 = H1 * ((ia == 0 || ia == nseg) ? 0.5 : 1.0);
+        double a2 = lo[1], b2 = hi[1];
+        if (!full) {
+            double r1 = sqrt(fmax(delta * delta - x * x, 0.0));
+            a2 = fmax(a2, -r1); b2 = fmin(b2, r1);
+        }
+        if (b2 - a2 <= 1.0e-10) continue;
+        double H2 = (b2 - a2) / nseg;
+        for (int ib = 0; ib <= nseg; ++ib) {
+            double y = ((nseg - ib) * a2 + ib * b2) / nseg;
+            double w2 = H2 * ((ib == 0 || ib == nseg) ? 0.5 : 1.0);
+            double a3 = lo[2], b3 = hi[2];
+            if (!full) {
+                double r2 = sqrt(fmax(delta * delta - x * x - y * y, 0.0));
+                a3 = fmax(a3, -r2); b3 = fmin(b3, r2);
+            }
+            if (b3 - a3 <= 1.0e-10) continue;
+            double H3 = (b3 - a3) / nseg;
+            for (int ic = 0; ic <= nseg; ++ic) {
+                double z = ((nseg - ic) * a3 + ic * b3) / nseg;
+                double w3 = H3 * ((ic == 0 || ic == nseg) ? 0.5 : 1.0);
+                if (cnt < cap) {
+                    xw[4 * cnt] = x; xw[4 * cnt + 1] = y; xw[4 * cnt + 2] = z;
+                    xw[4 * cnt + 3] = w1 * w2 * w3;
+                }
+                ++cnt;
+            }
+        }
+    }
+    return cnt;
+}
+
+int orc_ai_cell_points(const orc_problem* p, const double* c, double dmin, double dmax, double* xw, int cap)
+{
+    return ai_cell_points(p, c, dmin, dmax, xw, cap);
+}
+
+/* The bond's shortest and longest distances as InitBond stores them (pdDataManager2.cpp:33-53):
+ * from the lower-id node to the higher-id node's cell, shared by both directions. */
+static void bond_dists(const orc_state* s, int64_t i, int64_t j, double* dmin, double* dmax)
+{
+    int64_t a = (i < j) ? i : j, b = (i < j) ? j : i;
+    dist_to_cube(&s->X[3 * a], &s->X[3 * b], s->p.dx, dmin, dmax);
+}
+
+/* Points of every uncut entry (reference CSR order), then with mls_on the MLS coefficients of
+ * each point over node j's contributing nodes (InitIntegrationPoint's AI branch,
+ * pdDataManager2.cpp:257-297, which runs the same integration with a unit integrand to place
+ * them).  Without MLS every node must carry a DISPLGRAD condition: the integrand prescribes the
+ * points' relative displacement from it and exits otherwise (pdIntgManager2.cpp:187-211). */
+static int init_ai(orc_state* s)
+{
+    const orc_problem* p = &s->p;
+    int64_t n = s->n, E = s->rp[n];
+    if (p->intg_point < 1 || p->intg_point > 5) return -8;
+    if (!p->mls_on)
+        for (int64_t i = 0; i < n; ++i)
+            if (s->bc[i] < 0 || p->bc[s->bc[i]].type != 3) return -9;
+    int nmax = (1 << (p->intg_point - 1)) + 1;
+    nmax = nmax * nmax * nmax;
+    s->ai = 1;
+    s->ai_ep = (int64_t*)malloc(sizeof(int64_t) * (E + 1));
+    s->ai_ep[0] = 0;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
+            int cnt = 0;
+            if (!s->cut[e]) {
+                int64_t j = s->col[e];
+                double dmin, dmax, c[3];
+                bond_dists(s, i, j, &dmin, &dmax);
+                for (int k = 0; k < 3; ++k) c[k] = s->X[3 * j + k] - s->X[3 * i + k];
+                cnt = ai_cell_points(p, c, dmin, dmax, NULL, 0);
+            }
+            s->ai_ep[e + 1] = s->ai_ep[e] + cnt;
+        }
+    int64_t P = s->ai_ep[E];
+    s->ai_xw = (double*)malloc(sizeof(double) * 4 * (size_t)(P ? P : 1));
+    if (!s->ai_xw) return -10;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
+            if (s->ai_ep[e + 1] == s->ai_ep[e]) continue;
+            int64_t j = s->col[e];
+            double dmin, dmax, c[3];
+            bond_dists(s, i, j, &dmin, &dmax);
+            for (int k = 0; k < 3; ++k) c[k] = s->X[3 * j + k] - s->X[3 * i + k];
+            ai_cell_points(p, c, dmin, dmax, s->ai_xw + 4 * s->ai_ep[e], nmax);
+        }
+    if (!p->mls_on) return 0;
+    init_contrib(s);
+    s->ai_po = (int64_t*)malloc(sizeof(int64_t) * (P + 1));
+    s->ai_po[0] = 0;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
+            int64_t j = s->col[e];
+            int64_t m = s->cb_rp[j + 1] - s->cb_rp[j];
+            for (int64_t q = s->ai_ep[e]; q < s->ai_ep[e + 1]; ++q) s->ai_po[q + 1] = s->ai_po[q] + m;
+        }
+    s->ai_phi = (double*)malloc(sizeof(double) * (size_t)(s->ai_po[P] ? s->ai_po[P] : 1));
+    if (!s->ai_phi) return -10;
+    int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : bad)
+#endif
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t e = s->rp[i]; e < s->rp[i + 1] && !bad; ++e) {
+            int64_t j = s->col[e];
+            for (int64_t q = s->ai_ep[e]; q < s->ai_ep[e + 1] && !bad; ++q) {
+                /* the point in global coordinates (InitMLSCoeff, pdIntgManager2.cpp:1891-1893) */
+                double xg[3];
+                for (int k = 0; k < 3; ++k) xg[k] = s->ai_xw[4 * q + k] + s->X[3 * i + k];
+                bad = mls_coef_at(s, j, xg, s->ai_phi + s->ai_po[q]) != 0;
+            }
+        }
+    return bad ? -7 : 0;
+}
+
+/* CalcBondBasedForce (pdIntgManager2.cpp:262-309) at one point xw (x, y, z relative to node i,
+ * weight) with beta = 1, added with the point's weight; energy scaled by weight / vol. */
+static void ai_point_force(const double* xw, const double* eta, double ecrit, double spring, double bij, double vol,
+                           double* F, double* el)
+{
+    double xi1 = xw[0], xi2 = xw[1], xi3 = xw[2];
+    double r = sqrt(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
+    double pp = sqrt(sq(xi1 + eta[0]) + sq(xi2 + eta[1]) + sq(xi3 + eta[2]));
+    double u = pp - r;
+    double stretch = u / r;
+    if (u <= 0.0 || (u > 0.0 && stretch <= ecrit)) {
+        double a = bij * 1.0 * spring * stretch;
+        F[0] += xw[3] * (a * (xi1 + eta[0]) / pp);
+        F[1] += xw[3] * (a * (xi2 + eta[1]) / pp);
+        F[2] += xw[3] * (a * (xi3 + eta[2]) / pp);
+        *el += bij * 0.5 * (0.5 * 1.0 * (spring / r) * u * u) * (xw[3] / vol);
+    }
+}
+
+/* The AI branch of CalcBondBasedMotionRK4's family loop (pdDataManager.cpp:1888-1916) for entry
+ * e = (i, j): CalcBondBasedForce (pdIntgManager2.cpp:92-310) at every point, beta = 1, summed
+ * with the points' weights.  The point's displacement: with MLS, a BC node j's DISPL / DISPLGRAD
+ * value at time t in its flagged components (0 in the others; mode 2 skips this), else the MLS
+ * interpolation of the contributing nodes' trial displacements `trial`; without MLS the
+ * relative displacement is G_j xi (node j's DISPLGRAD gradient).  The elastic energy density
+ * of a point is scaled by its share of a cell, weight / dx^3 (the reference adds it unscaled,
+ * pdIntgManager2.cpp:287-292). */
+static void ai_entry(const orc_state* s, int64_t i, int64_t e, const double* ui, const double* trial, double t,
+                     int mode, double* F, double* el)
+{
+    const orc_problem* p = &s->p;
+    int64_t j = s->col[e];
+    const orc_mat* mi = &p->mat[s->mat[i]];
+    const orc_mat* mj = &p->mat[s->mat[j]];
+    double ecrit = (mj->ecrit < mi->ecrit) ? mj->ecrit : mi->ecrit;
+    double ci = s->spring[s->mat[i]], cj = s->spring[s->mat[j]];
+    double spring = (cj < ci) ? cj : ci;
+    double bij = 0.5 * (s->be[i] + s->be[j]);
+    double vol = p->dx * p->dx * p->dx;
+    const orc_bc* bcj = (s->bc[j] >= 0) ? &p->bc[s->bc[j]] : NULL;
+    int prescribed = p->mls_on && mode != 2 && bc_node(s, j);
+    double fac = prescribed ? orc_time_factor(bcj, t) : 0.0;
+    F[0] = F[1] = F[2] = 0.0;
+    for (int64_t q = s->ai_ep[e]; q < s->ai_ep[e + 1]; ++q) {
+        const double* xw = &s->ai_xw[4 * q];
+        double xi1 = xw[0], xi2 = xw[1], xi3 = xw[2];
+        double eta[3];
+        if (!p->mls_on) {
+            const double* G = bcj->v;
+            eta[0] = G[0] * xi1 + G[1] * xi2 + G[2] * xi3;
+            eta[1] = G[3] * xi1 + G[4] * xi2 + G[5] * xi3;
+            eta[2] = G[6] * xi1 + G[7] * xi2 + G[8] * xi3;
+        } else {
+            double uj[3] = {0.0, 0.0, 0.0};
+            if (prescribed) {
+                double xg[3] = {xi1 + s->X[3 * i], xi2 + s->X[3 * i + 1], xi3 + s->X[3 * i + 2]};
+                for (int c = 0; c < 3; ++c) {
+                    if (bcj->f[c] != 1) continue;
+                    if (bcj->type == 1) uj[c] = bcj->v[c] * fac;
+                    else if (bcj->type == 3)
+                        uj[c] = (bcj->v[3 * c] * xg[0] + bcj->v[3 * c + 1] * xg[1] + bcj->v[3 * c + 2] * xg[2]) * fac;
+                }
+            } else {
+                int64_t c0 = s->cb_rp[j], m = s->cb_rp[j + 1] - c0;
+                const double* ph = s->ai_phi + s->ai_po[q];
+                for (int64_t k = 0; k < m; ++k) {
+                    int64_t o = s->cb[c0 + k];
+                    uj[0] += trial[3 * o] * ph[k];
+                    uj[1] += trial[3 * o + 1] * ph[k];
+                    uj[2] += trial[3 * o + 2] * ph[k];
+                }
+            }
+            for (int c = 0; c < 3; ++c) eta[c] = uj[c] - ui[c];
+        }
+        ai_point_force(xw, eta, ecrit, spring, bij, vol, F, el);
+    }
+}
+
+int orc_ai_cell_force(const orc_problem* p, const double* c, double dmin, double dmax, const double* G, double bij,
+                      double spring, double ecrit, double* F, double* el)
+{
+    int cnt = ai_cell_points(p, c, dmin, dmax, NULL, 0);
+    double* xw = (double*)malloc(sizeof(double) * 4 * (size_t)(cnt ? cnt : 1));
+    ai_cell_points(p, c, dmin, dmax, xw, cnt);
+    double vol = p->dx * p->dx * p->dx;
+    F[0] = F[1] = F[2] = 0.0;
+    *el = 0.0;
+    for (int q = 0; q < cnt; ++q) {
+        const double* x = &xw[4 * q];
+        double eta[3] = {G[0] * x[0] + G[1] * x[1] + G[2] * x[2], G[3] * x[0] + G[4] * x[1] + G[5] * x[2],
+                         G[6] * x[0] + G[7] * x[1] + G[8] * x[2]};
+        ai_point_force(x, eta, ecrit, spring, bij, vol, F, el);
+    }
+    free(xw);
+    return cnt;
+}
+
 int orc_init(orc_state* s, int method, int nthreads)
 {
 #ifdef _OPENMP
@@ -982,6 +1254,9 @@ int orc_init(orc_state* s, int method, int nthreads)
     init_traction(s);
     if (s->p.fgi_on && !s->p.cci_on) {
         rc = init_fgi(s);
+        if (rc) return rc;
+    } else if (s->p.ai_on && !s->p.cci_on) {
+        rc = init_ai(s);
         if (rc) return rc;
     }
     int64_t n = s->n;
@@ -1051,6 +1326,25 @@ static void family_force(const orc_state* s, int64_t i, const double* u, double*
 
 void orc_forces(const orc_state* s, const double* u, double* F, double* elast, int nthreads)
 {
+    if (s->ai) {
+        /* AI: every node's points interpolated from u (no BC values) */
+#ifdef _OPENMP
+        if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16) if (nthreads != 1)
+#endif
+        for (int64_t i = 0; i < s->n; ++i) {
+            double acc[3] = {0.0, 0.0, 0.0}, ea = 0.0;
+            for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
+                if (s->cut[e]) continue;
+                double f[3];
+                ai_entry(s, i, e, &u[3 * i], u, 0.0, 2, f, &ea);
+                acc[0] += f[0]; acc[1] += f[1]; acc[2] += f[2];
+            }
+            for (int c = 0; c < 3; ++c) F[3 * i + c] = acc[c];
+            if (elast) elast[i] = ea;
+        }
+        return;
+    }
     if (s->ng) {
         /* FGI: every node's Gauss points interpolated from u (no BC values), in a scratch
          * copy so that the stepping state keeps its own */
@@ -1161,6 +1455,14 @@ static void rk4_stage(orc_state* s, double dts, double t, const double* trial, d
         for (int64_t e = s->rp[i]; e < s->rp[i + 1]; ++e) {
             if (s->cut[e]) continue;
             int64_t j = s->col[e];
+            if (s->ai) {
+                /* adaptive integration: the entry's points, displacements from this stage's trial */
+                double f[3], el = 0.0;
+                ai_entry(s, i, e, ui, trial, t, 0, f, &el);
+                s->elast[i] += el;
+                for (int c = 0; c < 3; ++c) s->Vb[3 * i + c] += (f[c] + bf[c]) * dts / denst;
+                continue;
+            }
             if (s->ng) {
                 /* fixed Gaussian integration: node j's Gauss points */
                 double f[3], el = 0.0;
@@ -1527,11 +1829,26 @@ void orc_get_gauss(const orc_state* s, double* Xg, double* Ug)
                 if (Ug) Ug[q] = s->G[q];
             }
 }
+int64_t orc_num_ai_points(const orc_state* s) { return s->ai ? s->ai_ep[s->rp[s->n]] : 0; }
+void orc_get_ai_points(const orc_state* s, int64_t* ent_ptr, double* xw)
+{
+    if (!s->ai) return;
+    int64_t E = s->rp[s->n];
+    if (ent_ptr) memcpy(ent_ptr, s->ai_ep, sizeof(int64_t) * (E + 1));
+    if (xw) memcpy(xw, s->ai_xw, sizeof(double) * 4 * s->ai_ep[E]);
+}
+void orc_get_ai_mls(const orc_state* s, int64_t* po, double* phi)
+{
+    if (!s->ai_po) return;
+    int64_t P = s->ai_ep[s->rp[s->n]];
+    if (po) memcpy(po, s->ai_po, sizeof(int64_t) * (P + 1));
+    if (phi) memcpy(phi, s->ai_phi, sizeof(double) * s->ai_po[P]);
+}
 void orc_get_mls(const orc_state* s, int64_t* cb_rp, int32_t* cb, double* phi)
 {
     if (!s->cb_rp) return;
     int64_t m = s->cb_rp[s->n];
     if (cb_rp) memcpy(cb_rp, s->cb_rp, sizeof(int64_t) * (s->n + 1));
     if (cb) memcpy(cb, s->cb, sizeof(int32_t) * m);
-    if (phi) memcpy(phi, s->phi, sizeof(double) * (size_t)s->ng * m);
+    if (phi && s->phi) memcpy(phi, s->phi, sizeof(double) * (size_t)s->ng * m);
 }

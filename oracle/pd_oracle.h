@@ -73,6 +73,8 @@ typedef struct {
     uint64_t seed;            /* SplitMix64 seed for the jitter */
     int nslit; orc_slit slit[ORC_MAX_SLIT];
     int traction_body;        /* 1: TRACTION BCs act as a body force (total force / face nodes) */
+    int ai_on;                /* adaptive integration (cci_on 0, fgi_on 0, ai_on 1 mls_on): intg_point is
+                                 the trapezoidal index, 2^(n-1)+1 points per direction */
 } orc_problem;
 
 typedef struct orc_state orc_state;
@@ -119,6 +121,23 @@ int orc_num_gauss(const orc_state* s);
 int64_t orc_num_contrib(const orc_state* s);
 void orc_get_gauss(const orc_state* s, double* Xg, double* Ug);
 void orc_get_mls(const orc_state* s, int64_t* cb_rp, int32_t* cb, double* phi);
+
+/* Adaptive integration (ai_on): trapezoidal points of each family entry over node j's cell
+ * clipped to node i's horizon, in family coordinates (node i at the origin).
+ * orc_ai_cell_points: the points (x, y, z, weight; 4 per point) of one cell whose centre is c
+ * (X_j - X_i) for the bond distances dmin / dmax; returns the count (points beyond cap are
+ * counted, not written).
+ * orc_get_ai_points: ent_ptr (E + 1, reference CSR entry order) and the points (4 P).
+ * orc_get_ai_mls: the MLS coefficients of each point over node j's contributing nodes
+ * (orc_get_mls's cb_rp / cb), point q's at phi[po[q] .. po[q + 1]). */
+int orc_ai_cell_points(const orc_problem* p, const double* c, double dmin, double dmax, double* xw, int cap);
+/* The integrated bond force of one such cell for the relative displacement eta = G xi (G 3x3
+ * row-major), with bdryEff_ij, spring constant and ecrit given; returns the point count. */
+int orc_ai_cell_force(const orc_problem* p, const double* c, double dmin, double dmax, const double* G, double bij,
+                      double spring, double ecrit, double* F, double* el);
+int64_t orc_num_ai_points(const orc_state* s);
+void orc_get_ai_points(const orc_state* s, int64_t* ent_ptr, double* xw);
+void orc_get_ai_mls(const orc_state* s, int64_t* po, double* phi);
 
 /* Families (ascending), cut flags and bdryEff factors of the given rows only (for problems
  * too large to build whole): rp (nrows + 1), col / cut (cap entries), be (nrows), rows in the

@@ -57,6 +57,7 @@ class Problem(C.Structure):
         ("jitter", C.c_double), ("seed", C.c_uint64),
         ("nslit", C.c_int), ("slit", _Slit * MAX_SLIT),
         ("traction_body", C.c_int),
+        ("ai_on", C.c_int),
     ]
 
 
@@ -114,6 +115,13 @@ def lib() -> C.CDLL:
         L.orc_num_contrib.argtypes = [vp]
         L.orc_get_gauss.argtypes = [vp, dp, dp]
         L.orc_get_mls.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32), dp]
+        L.orc_ai_cell_points.argtypes = [C.POINTER(Problem), dp, C.c_double, C.c_double, dp, C.c_int]
+        L.orc_ai_cell_force.argtypes = [C.POINTER(Problem), dp, C.c_double, C.c_double, dp, C.c_double, C.c_double,
+                                        C.c_double, dp, dp]
+        L.orc_num_ai_points.restype = C.c_int64
+        L.orc_num_ai_points.argtypes = [vp]
+        L.orc_get_ai_points.argtypes = [vp, C.POINTER(C.c_int64), dp]
+        L.orc_get_ai_mls.argtypes = [vp, C.POINTER(C.c_int64), dp]
         L.orc_areal_force_density.argtypes = [vp, C.POINTER(C.c_int64), C.c_int64, dp, dp]
         L.orc_sample_forces.argtypes = [C.POINTER(Problem), C.POINTER(C.c_int64), C.c_int64, dp, dp, dp, C.c_int]
         L.orc_sample_rows.restype = C.c_int64
@@ -179,6 +187,10 @@ def problem_from_spec(spec: Dict[str, Any]) -> Problem:
     cci = spec.get("cci", (1, 1))
     if cci is not None and cci[0]:
         p.cci_on, p.cci_mdf = 1, int(cci[1])
+    elif spec.get("ai") is not None:
+        ai = spec["ai"]
+        p.ai_on, p.mls_on = 1, int(ai[1])
+        p.intg_point = int(ai[2]) if len(ai) > 2 and ai[2] is not None else 3
     else:
         fgi = spec["fgi"]
         p.fgi_on, p.intg_point = 1, int(fgi[1])
@@ -247,6 +259,40 @@ def gauss_rule(n: int):
     if lib().orc_gauss_rule(int(n), _dp(a), _dp(w)) != 0:
         raise ValueError(f"no Gauss rule of order {n}")
     return a[:n].copy(), w[:n].copy()
+
+
+def ai_cell_points(spec: Dict[str, Any], c, dmin: float, dmax: float) -> np.ndarray:
+    """The adaptive-integration points (x, y, z, weight) of one cell with centre c relative to
+    node i, for the bond distances dmin / dmax; shape (P, 4)."""
+    p = problem_from_spec(spec)
+    cc = np.ascontiguousarray(c, dtype=np.float64)
+    cnt = lib().orc_ai_cell_points(C.byref(p), _dp(cc), float(dmin), float(dmax), None, 0)
+    xw = np.zeros(4 * max(cnt, 1))
+    lib().orc_ai_cell_points(C.byref(p), _dp(cc), float(dmin), float(dmax), _dp(xw), cnt)
+    return xw[: 4 * cnt].reshape(-1, 4)
+
+
+def ai_cell_force(spec: Dict[str, Any], c, dmin: float, dmax: float, G, bij: float = 1.0, spring: float = 1.0,
+                  ecrit: float = 1e10):
+    """Integrated bond force (3) and energy density of one AI cell for eta = G xi."""
+    p = problem_from_spec(spec)
+    cc = np.ascontiguousarray(c, dtype=np.float64)
+    g = np.ascontiguousarray(G, dtype=np.float64).reshape(-1)
+    F = np.zeros(3)
+    el = C.c_double()
+    lib().orc_ai_cell_force(C.byref(p), _dp(cc), float(dmin), float(dmax), _dp(g), float(bij), float(spring),
+                            float(ecrit), _dp(F), C.byref(el))
+    return F, el.value
+
+
+def pair_dists(spec: Dict[str, Any], xi, xj):
+    """(bonded, dmin, dmax) of the pair as InitBond computes it (xi the lower id)."""
+    p = problem_from_spec(spec)
+    a = np.ascontiguousarray(xi, dtype=np.float64)
+    b = np.ascontiguousarray(xj, dtype=np.float64)
+    dmin, dmax = C.c_double(), C.c_double()
+    ok = lib().orc_pair_bonded(C.byref(p), _dp(a), _dp(b), C.byref(dmin), C.byref(dmax))
+    return bool(ok), dmin.value, dmax.value
 
 
 class Oracle:
@@ -351,6 +397,27 @@ class Oracle:
         self.L.orc_get_mls(self.h, rp.ctypes.data_as(C.POINTER(C.c_int64)),
                            cb.ctypes.data_as(C.POINTER(C.c_int32)), _dp(phi))
         return rp, cb, phi
+
+    def ai_points(self):
+        """Adaptive-integration points: (ent_ptr (E + 1), xw (P, 4)) in reference entry order;
+        None without ai_on."""
+        P = int(self.L.orc_num_ai_points(self.h))
+        if P == 0:
+            return None
+        ep = np.zeros(self.E + 1, dtype=np.int64)
+        xw = np.zeros(4 * P)
+        self.L.orc_get_ai_points(self.h, ep.ctypes.data_as(C.POINTER(C.c_int64)), _dp(xw))
+        return ep, xw.reshape(-1, 4)
+
+    def ai_mls(self):
+        """MLS coefficients of the AI points: (po (P + 1), phi); point q's over node j's
+        contributing nodes (mls()'s cb_rp / cb) at phi[po[q]:po[q + 1]]."""
+        P = int(self.L.orc_num_ai_points(self.h))
+        po = np.zeros(P + 1, dtype=np.int64)
+        self.L.orc_get_ai_mls(self.h, po.ctypes.data_as(C.POINTER(C.c_int64)), None)
+        phi = np.zeros(max(int(po[-1]), 1))
+        self.L.orc_get_ai_mls(self.h, po.ctypes.data_as(C.POINTER(C.c_int64)), _dp(phi))
+        return po, phi[: int(po[-1])]
 
     def static_system(self):
         m, uc, b = np.zeros(3 * self.n), np.zeros(3 * self.n), np.zeros(3 * self.n)

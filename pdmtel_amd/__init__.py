@@ -96,6 +96,9 @@ def lib() -> C.CDLL:
     L.pdm_num_live_gauss_pairs.restype = C.c_int64
     L.pdm_num_live_gauss_pairs.argtypes = [vp]
     L.pdm_get_gauss.argtypes = [vp, dp, dp]
+    L.pdm_num_ai_points.restype = C.c_int64
+    L.pdm_num_ai_points.argtypes = [vp]
+    L.pdm_get_ai_points.argtypes = [vp, i32p, i32p, dp]
     cpp = C.POINTER(C.c_char_p)
     L.pdm_batch_create.argtypes = [cpp, C.c_int, C.POINTER(Options), C.POINTER(vp)]
     L.pdm_batch_create_from_text.argtypes = [cpp, C.c_int, C.POINTER(Options), C.POINTER(vp)]
@@ -237,6 +240,19 @@ class Problem:
         Ug = np.zeros(3 * ng * self.n)
         self._chk(self.L.pdm_get_gauss(self.h, _dp(Xg), _dp(Ug)))
         return Xg.reshape(self.n, ng, 3), Ug.reshape(self.n, ng, 3)
+
+    def ai_points(self):
+        """Adaptive-integration points: (node i, node j (reference ids), xw (P, 4): x, y, z relative
+        to node i and the weight); the points of an entry are contiguous.  None for other modes."""
+        P = int(self.L.pdm_num_ai_points(self.h))
+        if P <= 0:
+            return None
+        ni = np.zeros(P, dtype=np.int32)
+        nj = np.zeros(P, dtype=np.int32)
+        xw = np.zeros(4 * P)
+        self._chk(self.L.pdm_get_ai_points(self.h, ni.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           nj.ctypes.data_as(C.POINTER(C.c_int32)), _dp(xw)))
+        return ni, nj, xw.reshape(-1, 4)
 
     def tangent(self, u: np.ndarray) -> np.ndarray:
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)

@@ -67,24 +67,34 @@ int create_common(const std::string& text, const pdm_options* opt, pdm_handle** 
     }
     const Deck& d = h->deck;
     if (!d.cci_on && !d.fgi_on) {
-        g_create_err = "adaptive integration (ai_on) is not supported by this build";
-        delete h;
-        return PDM_E_MODEL;
-    }
-    if (!d.cci_on) {
+        // adaptive integration: trapezoidal index 1..5 (2 to 17 points per direction), no
+        // error-controlled refinement (its point counts depend on the run-time displacement)
+        if (d.err_on) {
+            g_create_err = "adaptive integration with error_control_on 1 is not built (run-time point refinement)";
+            delete h;
+            return PDM_E_MODEL;
+        }
+        if (d.intg_point < 1 || d.intg_point > 5) {
+            g_create_err = "intg_point must be 1..5 for adaptive integration (2^(n-1)+1 points per direction)";
+            delete h;
+            return PDM_E_PARSE;
+        }
+    } else if (!d.cci_on) {
         const int n = d.intg_point;
         if (n < 1 || n == 7 || n > 8) {
             g_create_err = "intg_point must be 1..6 or 8 (the reference's Gauss rules)";
             delete h;
             return PDM_E_PARSE;
         }
-        if (n > 1 && !(d.rw > 0.0)) {
+    }
+    if (!d.cci_on) {
+        if (d.mls_on && !(d.rw > 0.0)) {
             g_create_err = "support_domain_size must be positive";
             delete h;
             return PDM_E_PARSE;
         }
         if (h->opt.precision != PDM_FP64) {
-            g_create_err = "fixed Gaussian integration runs in PDM_FP64 only";
+            g_create_err = "fixed Gaussian and adaptive integration run in PDM_FP64 only";
             delete h;
             return PDM_E_MODEL;
         }
@@ -260,7 +270,8 @@ int ensure_chunk(pdm_handle* h, int cap)
 int enqueue_steps(pdm_handle* h, int count)
 {
     StageArgs a = stage_args(h);
-    const bool fgi = h->fgi_ng > 0;
+    const bool fgi = !h->deck.cci_on;  // fixed Gaussian or adaptive integration
+    const bool gauss = h->fgi_ng > 0;
     const FgiArgs f = fgi ? fgi_args(h) : FgiArgs{};
     for (int k = 0; k < count; ++k) {
         for (int st = 0; st < 4; ++st) {
@@ -273,7 +284,7 @@ int enqueue_steps(pdm_handle* h, int count)
             // Gauss points follow each stage's new displacements: stages 2-4 read them, and
             // the next step's stage 1 reads those of the updated U (pdDataManager.cpp:2043-2125)
             HCHK(launch_fgi_stage(st, a, f, k, h->stream));
-            HCHK(launch_fgi_gauss(f, st == 3 ? a.uout : f.trial, k, h->stream));
+            if (gauss) HCHK(launch_fgi_gauss(f, st == 3 ? a.uout : f.trial + (st & 1) * f.n, k, h->stream));
         }
     }
     // per-step KE / EE totals, one launch for the chunk
@@ -846,6 +857,7 @@ int pdm_destroy(pdm_handle* h)
     dfree(h->elast); dfree(h->part); dfree(h->energy); dfree(h->steps); dfree(h->tc64); dfree(h->tc32);
     dfree(h->specs_d); dfree(h->mats_d); dfree(h->bcs_d); dfree(h->tile_spec_d); dfree(h->spec_tile_ptr_d);
     free_fgi(h);
+    free_ai(h);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream && !h->shared_stream) (void)hipStreamDestroy(h->stream);
@@ -919,7 +931,7 @@ int pdm_init(pdm_handle* h)
     rc = upload_spec(h);
     if (rc) return rc;
     if (fgi) {
-        rc = build_fgi(h);
+        rc = h->deck.fgi_on ? build_fgi(h) : build_ai(h);
         if (rc) return rc;
     }
     rc = alloc_parts(h);
@@ -998,7 +1010,10 @@ int pdm_compute_forces(pdm_handle* h, const double* u, double* F, double* elast)
     HCHK(hipMalloc(&Fd, sizeof(d4) * n));
     HCHK(hipMalloc(&ed, sizeof(double) * n));
     HCHK(hipMemcpy(ud, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
-    if (h->fgi_ng) {
+    if (h->ai) {
+        // every node's points interpolated from u
+        HCHK(launch_fgi_forces(stage_args(h), fgi_args(h), ud, Fd, ed, h->stream));
+    } else if (h->fgi_ng) {
         // every node's Gauss points interpolated from u, into scratch (the stepping state keeps its own)
         d4* gd = nullptr;
         HCHK(hipMalloc(&gd, sizeof(d4) * n * h->fgi_ng));
@@ -1032,7 +1047,7 @@ int pdm_compute_forces_noalloc(pdm_handle* h, const double* u, double* F, double
     int rc = require_init(h);
     if (rc) return rc;
     if (!u || !F) return fail(h, PDM_E_ARG, "null u or F");
-    if (h->fgi_ng) return fail(h, PDM_E_MODEL, "pdm_compute_forces_noalloc takes CCI decks only");
+    if (!h->deck.cci_on) return fail(h, PDM_E_MODEL, "pdm_compute_forces_noalloc takes CCI decks only");
     if (h->n > kNoAllocMaxNodes)
         return fail(h, PDM_E_LIMIT, std::to_string(h->n) + " nodes exceed the " + std::to_string(kNoAllocMaxNodes) +
                                         " the no-allocation scratch holds; use pdm_compute_forces");
@@ -1049,7 +1064,7 @@ int pdm_areal_force_density(pdm_handle* h, const int64_t* nodes, int64_t count, 
     int rc = require_init(h);
     if (rc) return rc;
     if (count < 0 || (count > 0 && (!nodes || !sigma))) return fail(h, PDM_E_ARG, "null nodes or sigma");
-    if (h->fgi_ng) return fail(h, PDM_E_MODEL, "the areal force density is built for CCI decks only");
+    if (!h->deck.cci_on) return fail(h, PDM_E_MODEL, "the areal force density is built for CCI decks only");
     if (count == 0) return PDM_OK;
     const Deck& d = h->deck;
     const int64_t n = h->n;
@@ -1143,6 +1158,37 @@ int pdm_areal_force_density(pdm_handle* h, const int64_t* nodes, int64_t count, 
     return PDM_OK;
 }
 
+int64_t pdm_num_ai_points(const pdm_handle* h) { return h ? h->ai_np : -1; }
+
+int pdm_get_ai_points(const pdm_handle* h, int32_t* node, int32_t* nbr, double* xw)
+{
+    int rc = require_init(h);
+    if (rc) return rc;
+    if (!h->ai) return fail(h, PDM_E_MODEL, "the deck has no adaptive-integration points");
+    const int64_t n = h->n, P = h->ai_np;
+    std::vector<int64_t> ptr(n + 1);
+    std::vector<int32_t> j(P);
+    std::vector<d4> x(P);
+    HCHK(hipStreamSynchronize(h->stream));
+    HCHK(hipMemcpy(ptr.data(), h->ai_ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost));
+    if (P) {
+        HCHK(hipMemcpy(j.data(), h->ai_j, sizeof(int32_t) * P, hipMemcpyDeviceToHost));
+        HCHK(hipMemcpy(x.data(), h->ai_xw, sizeof(d4) * P, hipMemcpyDeviceToHost));
+    }
+    for (int64_t a = 0; a < n; ++a)
+        for (int64_t q = ptr[a]; q < ptr[a + 1]; ++q) {
+            if (node) node[q] = h->perm[a];
+            if (nbr) nbr[q] = h->perm[j[q]];
+            if (xw) {
+                xw[4 * q] = x[q].x;
+                xw[4 * q + 1] = x[q].y;
+                xw[4 * q + 2] = x[q].z;
+                xw[4 * q + 3] = x[q].w;
+            }
+        }
+    return PDM_OK;
+}
+
 int pdm_num_gauss_points(const pdm_handle* h) { return h ? h->fgi_ng : -1; }
 int64_t pdm_num_live_gauss_pairs(const pdm_handle* h) { return h ? h->fgi_npairs : -1; }
 
@@ -1183,7 +1229,7 @@ int pdm_apply_tangent(pdm_handle* h, const double* u, double* y)
     int rc = require_init(h);
     if (rc) return rc;
     if (!u || !y) return fail(h, PDM_E_ARG, "null u or y");
-    if (h->fgi_ng) return fail(h, PDM_E_MODEL, "the tangent operator is built for CCI decks only");
+    if (!h->deck.cci_on) return fail(h, PDM_E_MODEL, "the tangent operator is built for CCI decks only");
     int64_t n = h->n;
     if (n == 0) return PDM_OK;
     std::vector<d4> buf;
@@ -1301,7 +1347,7 @@ int pdm_get_stage_launch(const pdm_handle* h, int* max_vgprs, int64_t* lds_bytes
 {
     int rc = require_init(h);
     if (rc) return rc;
-    if (h->fgi_ng) return fail(h, PDM_E_MODEL, "fixed Gaussian decks step with their own kernels");
+    if (!h->deck.cci_on) return fail(h, PDM_E_MODEL, "fixed Gaussian and adaptive-integration decks step with their own kernels");
     const size_t lds = stage_lds_bytes(tile_view(h), !h->uniform_mat, true, h->opt.precision != PDM_FP64);
     if (max_vgprs) *max_vgprs = stage_vgpr_cap(tile_view(h), h->opt.precision, lds);
     if (lds_bytes) *lds_bytes = (int64_t)lds;
@@ -1432,7 +1478,7 @@ int pdm_solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_stat
 {
     int rc = require_init(h);
     if (rc) return rc;
-    if (h->fgi_ng) return fail(h, PDM_E_MODEL, "the static solve is built for CCI decks only");
+    if (!h->deck.cci_on) return fail(h, PDM_E_MODEL, "the static solve is built for CCI decks only");
     HCHK(hipSetDevice(h->dev));
     return solve_static(h, rel_tol, max_iterations, stats);
 }

@@ -18,6 +18,7 @@
 #include <cmath>
 #include <vector>
 
+#include "pdm_geom.h"
 #include "pdm_internal.h"
 
 namespace pdm {
@@ -171,16 +172,12 @@ __global__ void k_mls_fill(MlsArgs m)
 
 __device__ __forceinline__ constexpr int tri(int r, int c) { return r * (r + 1) / 2 + c; }
 
-// MLS coefficients of Gauss point g of internal node a (one thread per point).
-__global__ __launch_bounds__(256) void k_mls_coef(MlsArgs m)
+// MLS coefficients at the point xg of internal node a's contributing nodes: coefficient k of
+// the list at out[k * stride].  The basis is taken relative to node a in units of dx.  false if
+// the moment matrix is not positive definite.
+__device__ bool mls_point(const MlsArgs& m, int64_t a, const double* xg, double* out, int64_t stride)
 {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= m.n * m.ng) return;
-    const int64_t a = q / m.ng;
-    const int g = (int)(q - a * m.ng);
     const d4 xa = m.Xi[a];
-    const double* t = m.tab + kFgiTab * g;
-    const double xg[3] = {t[0] + xa.x, t[1] + xa.y, t[2] + xa.z};
     const int64_t k0 = m.cb_ptr[a], k1 = m.cb_ptr[a + 1];
     double A[kBasis * (kBasis + 1) / 2];
 #pragma unroll
@@ -213,10 +210,7 @@ __global__ __launch_bounds__(256) void k_mls_coef(MlsArgs m)
             A[tri(r, j)] = v / ljj;
         }
     }
-    if (!ok) {
-        *m.err = 1;
-        return;
-    }
+    if (!ok) return false;
     double s[kBasis];
     basis((xg[0] - xa.x) / m.dx, (xg[1] - xa.y) / m.dx, (xg[2] - xa.z) / m.dx, s);
 #pragma unroll
@@ -240,8 +234,136 @@ __global__ __launch_bounds__(256) void k_mls_coef(MlsArgs m)
         double v = 0.0;
 #pragma unroll
         for (int r = 0; r < kBasis; ++r) v += p[r] * s[r];
-        m.phi[k * m.ng + g] = weight(xk, xg, m.rw) * v;
+        out[(k - k0) * stride] = weight(xk, xg, m.rw) * v;
     }
+    return true;
+}
+
+// MLS coefficients of Gauss point g of internal node a (one thread per point).
+__global__ __launch_bounds__(256) void k_mls_coef(MlsArgs m)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.n * m.ng) return;
+    const int64_t a = q / m.ng;
+    const int g = (int)(q - a * m.ng);
+    const d4 xa = m.Xi[a];
+    const double* t = m.tab + kFgiTab * g;
+    const double xg[3] = {t[0] + xa.x, t[1] + xa.y, t[2] + xa.z};
+    if (!mls_point(m, a, xg, m.phi + m.cb_ptr[a] * m.ng + g, m.ng)) *m.err = 1;
+}
+
+// ------------------------------------------------------------ adaptive integration (ai_on)
+struct AiBuild {
+    TileView tv;
+    const int32_t* refid;    // internal -> reference id
+    const int64_t* cb_ptr;   // MLS contributing nodes (mls) or null
+    double horizon, dx;
+    int nseg;
+    int64_t* cnt;            // count pass: points, coefficients and entries with points, per node
+    int64_t* cphi;
+    int64_t* ent;
+    const int64_t* ai_ptr;   // fill pass
+    const int64_t* phi_base;
+    d4* xw;
+    uint32_t* eo;
+    int64_t* po;
+    int32_t* pi;             // the point's node i (set-up only)
+    int32_t* pj;             // the point's node j
+};
+
+// The AI points of one entry of internal row i: node j's cell relative to node i, with the bond
+// distances InitBond stores (from the lower reference id to the higher one's cell).
+template <class EMIT>
+__device__ __forceinline__ int ai_entry_points(const AiBuild& b, int64_t i, int32_t j, const d4& xi, const d4& xj,
+                                               EMIT emit)
+{
+    double dmin, dmax;
+    if (b.refid[i] < b.refid[j]) g_dist_to_cube_mm(xi.x, xi.y, xi.z, xj.x, xj.y, xj.z, b.dx, &dmin, &dmax);
+    else g_dist_to_cube_mm(xj.x, xj.y, xj.z, xi.x, xi.y, xi.z, b.dx, &dmin, &dmax);
+    const double c[3] = {xj.x - xi.x, xj.y - xi.y, xj.z - xi.z};
+    return g_ai_cell_points(c, dmin, dmax, b.horizon, b.dx, b.nseg, emit);
+}
+
+// Count (FILL = false) or write (FILL = true) the points of every internal row, in row order,
+// then x, y, z order within an entry.  One workgroup per tile, one wavefront per node, one lane
+// per entry; cut entries and row padding (coefficient 0) have none.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_ai_pts(AiBuild b)
+{
+    const TileView& tv = b.tv;
+    const int64_t k = blockIdx.x;
+    const int32_t tid = tv.tiles[k];
+    const int64_t a0 = tv.tile_start[tid];
+    const int nt = (int)(tv.tile_start[tid + 1] - a0);
+    const int64_t h0 = tv.halo_ptr[k];
+    const int lane = threadIdx.x & 63;
+    for (int loc = threadIdx.x >> 6; loc < nt; loc += 4) {
+        const int64_t i = a0 + loc;
+        const d4 xi = tv.X[i];
+        const int64_t eb = tv.rpi[i], ee = tv.rpi[i + 1];
+        int64_t base = FILL ? b.ai_ptr[i] : 0, pbase = FILL ? b.phi_base[i] : 0;
+        int64_t rc = 0, rp = 0, re = 0;
+        for (int64_t e0 = eb; e0 < ee; e0 += 64) {
+            const int64_t e = e0 + lane;
+            const bool uncut = e < ee && tv.coef64[e] != 0.0;
+            int32_t j = 0;
+            d4 xj = xi;
+            int c = 0;
+            int64_t m = 0;
+            if (uncut) {
+                j = tv.halo[h0 + tv.slot[e]];
+                xj = tv.X[j];
+                c = ai_entry_points(b, i, j, xi, xj, [](int, double, double, double, double) {});
+                m = b.cb_ptr ? b.cb_ptr[j + 1] - b.cb_ptr[j] : 0;
+            }
+            // inclusive prefix of the lanes' point and coefficient counts
+            int64_t ic = c, ip = (int64_t)c * m;
+            for (int off = 1; off < 64; off <<= 1) {
+                const int64_t v = __shfl_up(ic, off, 64), w = __shfl_up(ip, off, 64);
+                if (lane >= off) {
+                    ic += v;
+                    ip += w;
+                }
+            }
+            const int64_t tc = __shfl(ic, 63, 64), tp = __shfl(ip, 63, 64);
+            if (FILL && c > 0) {
+                const int64_t o = base + (ic - c);
+                const int64_t po = pbase + (ip - (int64_t)c * m);
+                const uint32_t off = (uint32_t)(e - eb);
+                ai_entry_points(b, i, j, xi, xj, [&](int q, double x, double y, double z, double w) {
+                    b.xw[o + q] = d4{x, y, z, w};
+                    b.eo[o + q] = off;
+                    b.po[o + q] = po + (int64_t)q * m;
+                    b.pi[o + q] = (int32_t)i;
+                    b.pj[o + q] = j;
+                });
+            }
+            base += tc;
+            pbase += tp;
+            rc += tc;
+            rp += tp;
+            re += __popcll(__ballot(c > 0));
+        }
+        if (!FILL && lane == 0) {
+            b.cnt[i] = rc;
+            b.cphi[i] = rp;
+            b.ent[i] = re;
+        }
+    }
+}
+
+// MLS coefficients of every AI point over its cell owner's (node j's) contributing nodes, at
+// the point's global position X_i + xi (InitMLSCoeff, pdIntgManager2.cpp:1879-1930).
+__global__ __launch_bounds__(256) void k_ai_mls(MlsArgs m, const d4* __restrict__ xw, const int64_t* __restrict__ po,
+                                                const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
+                                                double* __restrict__ phi, int64_t np)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const d4 x = xw[q];
+    const d4 xi = m.Xi[pi[q]];
+    const double xg[3] = {x.x + xi.x, x.y + xi.y, x.z + xi.z};
+    if (!mls_point(m, pj[q], xg, phi + po[q], 1)) *m.err = 1;
 }
 
 // Bytes above which the MLS coefficient table is refused.
@@ -249,6 +371,66 @@ constexpr double kMaxPhiBytes = 96.0 * 1024 * 1024 * 1024;
 constexpr double kMaxPairBytes = 64.0 * 1024 * 1024 * 1024;
 
 }  // namespace
+
+// Device temporaries of the MLS set-up; they live in the caller's frame, registered with the
+// caller's DevTemps, because the coefficient kernels read them after build_contrib returns.
+struct ContribTemps {
+    int32_t* ip = nullptr;
+    int64_t* cnt = nullptr;
+    uint8_t* wide = nullptr;
+    int* err = nullptr;
+};
+
+// MLS contributing nodes of every node (h->cb_ptr, h->cb, h->ncb) and the argument block of
+// the coefficient kernels; the temporaries are released with `temps`.
+int build_contrib(pdm_handle* h, MlsArgs& m, DevTemps& temps, ContribTemps& ct)
+{
+    const Deck& d = h->deck;
+    const int64_t n = h->n;
+    hipStream_t s = h->stream;
+    std::vector<int32_t> iperm(n);
+    for (int64_t a2 = 0; a2 < n; ++a2) iperm[h->perm[a2]] = (int32_t)a2;
+    int32_t*& ip = ct.ip;
+    int64_t*& cnt = ct.cnt;
+    uint8_t*& wide = ct.wide;
+    int*& err = ct.err;
+    temps.add(&ct.ip);
+    temps.add(&ct.cnt);
+    temps.add(&ct.wide);
+    temps.add(&ct.err);
+    HIPCHK(hipMalloc(&ip, sizeof(int32_t) * std::max<int64_t>(n, 1)));
+    HIPCHK(hipMalloc(&cnt, sizeof(int64_t) * std::max<int64_t>(n, 1)));
+    HIPCHK(hipMalloc(&wide, std::max<int64_t>(n, 1)));
+    HIPCHK(hipMalloc(&err, sizeof(int)));
+    HIPCHK(hipMemcpy(ip, iperm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(err, 0, sizeof(int), s));
+    HIPCHK(hipMalloc(&h->cb_ptr, sizeof(int64_t) * (n + 1)));
+    m.Xd = h->Xd;
+    m.rp = h->rp;
+    m.col = h->col;
+    m.iperm = ip;
+    m.Xi = h->Xi;
+    m.n = n;
+    m.rw = d.rw;
+    m.cnt = cnt;
+    m.wide = wide;
+    m.dx = d.dx;
+    m.err = err;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    k_mls_count<<<dim3(nb), dim3(256), 0, s>>>(m);
+    HIPCHK(hipGetLastError());
+    int rc = scan_exclusive(h, cnt, h->cb_ptr, n);
+    if (rc) return rc;
+    int64_t total = 0;
+    HIPCHK(hipMemcpy(&total, h->cb_ptr + n, sizeof(int64_t), hipMemcpyDeviceToHost));
+    h->ncb = total;
+    HIPCHK(hipMalloc(&h->cb, sizeof(int32_t) * std::max<int64_t>(total, 1)));
+    m.cb_ptr = h->cb_ptr;
+    m.cb = h->cb;
+    k_mls_fill<<<dim3(nb), dim3(256), 0, s>>>(m);
+    HIPCHK(hipGetLastError());
+    return PDM_OK;
+}
 
 void free_fgi(pdm_handle* h)
 {
@@ -302,9 +484,11 @@ int build_fgi(pdm_handle* h)
     const size_t gbytes = sizeof(d4) * (size_t)std::max<int64_t>(n, 1) * ng;
     HIPCHK(hipMalloc(&h->gauss, gbytes));
     HIPCHK(hipMemsetAsync(h->gauss, 0, gbytes, s));  // zero initial displacement (pdDataManager2.cpp:95-97)
+    // fgi_trial: two buffers of N (the stages alternate between them), fgi_u0: N
     for (d4** p : {&h->fgi_trial, &h->fgi_u0}) {
-        HIPCHK(hipMalloc(p, sizeof(d4) * std::max<int64_t>(n, 1)));
-        HIPCHK(hipMemsetAsync(*p, 0, sizeof(d4) * std::max<int64_t>(n, 1), s));
+        const size_t bytes = sizeof(d4) * std::max<int64_t>(n, 1) * (p == &h->fgi_trial ? 2 : 1);
+        HIPCHK(hipMalloc(p, bytes));
+        HIPCHK(hipMemsetAsync(*p, 0, bytes, s));
     }
     {
         // live (entry, Gauss point) pairs: the gathers skip the pairs whose beta is 0
@@ -342,65 +526,156 @@ int build_fgi(pdm_handle* h)
         HIPCHK(hipStreamSynchronize(s));
         return PDM_OK;
     }
-    std::vector<int32_t> iperm(n);
-    for (int64_t a2 = 0; a2 < n; ++a2) iperm[h->perm[a2]] = (int32_t)a2;
     MlsArgs m{};
-    int32_t* ip = nullptr;
-    int64_t* cnt = nullptr;
-    uint8_t* wide = nullptr;
-    int* err = nullptr;
-    HIPCHK(hipMalloc(&ip, sizeof(int32_t) * n));
-    HIPCHK(hipMalloc(&cnt, sizeof(int64_t) * n));
-    HIPCHK(hipMalloc(&wide, n));
-    HIPCHK(hipMalloc(&err, sizeof(int)));
-    HIPCHK(hipMemcpy(ip, iperm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemsetAsync(err, 0, sizeof(int), s));
-    HIPCHK(hipMalloc(&h->cb_ptr, sizeof(int64_t) * (n + 1)));
-    m.Xd = h->Xd;
-    m.rp = h->rp;
-    m.col = h->col;
-    m.iperm = ip;
-    m.Xi = h->Xi;
-    m.n = n;
-    m.rw = d.rw;
-    m.cnt = cnt;
-    m.wide = wide;
-    m.tab = h->fgi_tab;
-    m.ng = ng;
-    m.dx = d.dx;
-    m.err = err;
-    const unsigned nb = (unsigned)((n + 255) / 256);
-    k_mls_count<<<dim3(nb), dim3(256), 0, s>>>(m);
-    HIPCHK(hipGetLastError());
-    int rc = scan_exclusive(h, cnt, h->cb_ptr, n);
+    ContribTemps ct;
+    DevTemps temps(s);
+    int rc = build_contrib(h, m, temps, ct);
     if (rc) return rc;
-    int64_t total = 0;
-    HIPCHK(hipMemcpy(&total, h->cb_ptr + n, sizeof(int64_t), hipMemcpyDeviceToHost));
+    const int64_t total = h->ncb;
     if ((double)total * ng * sizeof(double) > kMaxPhiBytes) {
         h->err = "MLS coefficient table exceeds 96 GiB (lower intg_point or support_domain_size)";
         return PDM_E_LIMIT;
     }
-    HIPCHK(hipMalloc(&h->cb, sizeof(int32_t) * std::max<int64_t>(total, 1)));
     HIPCHK(hipMalloc(&h->phi, sizeof(double) * std::max<int64_t>(total, 1) * ng));
-    m.cb_ptr = h->cb_ptr;
-    m.cb = h->cb;
     m.phi = h->phi;
-    k_mls_fill<<<dim3(nb), dim3(256), 0, s>>>(m);
-    HIPCHK(hipGetLastError());
+    m.tab = h->fgi_tab;
+    m.ng = ng;
     const int64_t q = n * ng;
     k_mls_coef<<<dim3((unsigned)((q + 255) / 256)), dim3(256), 0, s>>>(m);
     HIPCHK(hipGetLastError());
     int herr = 0;
     HIPCHK(hipStreamSynchronize(s));
-    HIPCHK(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHK(hipFree(ip));
-    HIPCHK(hipFree(cnt));
-    HIPCHK(hipFree(wide));
-    HIPCHK(hipFree(err));
+    HIPCHK(hipMemcpy(&herr, m.err, sizeof(int), hipMemcpyDeviceToHost));
     if (herr) {
         h->err = "MLS moment matrix is singular at a Gauss point (too few contributing nodes)";
         return PDM_E_MODEL;
     }
+    return PDM_OK;
+}
+
+void free_ai(pdm_handle* h)
+{
+    for (void* p : {(void*)h->ai_ptr, (void*)h->ai_xw, (void*)h->ai_eo, (void*)h->ai_po, (void*)h->ai_phi,
+                    (void*)h->ai_j})
+        if (p) (void)hipFree(p);
+    h->ai_j = nullptr;
+    h->ai_ptr = nullptr;
+    h->ai_xw = nullptr;
+    h->ai_eo = nullptr;
+    h->ai_po = nullptr;
+    h->ai_phi = nullptr;
+    h->ai_np = h->ai_nphi = 0;
+}
+
+// Adaptive integration (ai_on 1) after the tile build: the trapezoidal points of every entry
+// (InitIntegrationPoint's AI branch, pdDataManager2.cpp:257-297, places them with a unit
+// integrand) and, with MLS, their coefficients over node j's contributing nodes.  Without MLS
+// the integrand prescribes each point's relative displacement from node j's DISPLGRAD
+// condition and exits when node j has none (pdIntgManager2.cpp:187-211): such a deck is refused
+// here unless every node carries one.
+int build_ai(pdm_handle* h)
+{
+    const Deck& d = h->deck;
+    const int64_t n = h->n;
+    hipStream_t s = h->stream;
+    h->ai = d.mls_on ? 1 : 2;
+    h->ai_nseg = 1 << (d.intg_point - 1);
+    if (!d.mls_on)
+        for (int64_t i = 0; i < n; ++i) {
+            const int b = h->bc[i];
+            if (b < 0 || d.bcs[b].type != 3) {
+                h->err = "adaptive integration without MLS (ai_on 1 0) needs a DISPLGRAD condition on every node";
+                return PDM_E_MODEL;
+            }
+        }
+    // fgi_trial: two buffers of N (the stages alternate between them), fgi_u0: N
+    for (d4** p : {&h->fgi_trial, &h->fgi_u0}) {
+        const size_t bytes = sizeof(d4) * std::max<int64_t>(n, 1) * (p == &h->fgi_trial ? 2 : 1);
+        HIPCHK(hipMalloc(p, bytes));
+        HIPCHK(hipMemsetAsync(*p, 0, bytes, s));
+    }
+    HIPCHK(hipMalloc(&h->ai_ptr, sizeof(int64_t) * (n + 1)));
+    if (n == 0) {
+        HIPCHK(hipMemsetAsync(h->ai_ptr, 0, sizeof(int64_t), s));
+        HIPCHK(hipStreamSynchronize(s));
+        return PDM_OK;
+    }
+    MlsArgs m{};
+    ContribTemps ct;
+    DevTemps temps(s);
+    if (d.mls_on) {
+        int rc = build_contrib(h, m, temps, ct);
+        if (rc) return rc;
+    }
+    AiBuild b{};
+    b.tv = tile_view(h);
+    b.refid = h->permd;
+    b.cb_ptr = d.mls_on ? h->cb_ptr : nullptr;
+    b.horizon = d.horizon;
+    b.dx = d.dx;
+    b.nseg = h->ai_nseg;
+    int64_t *cnt = nullptr, *cphi = nullptr, *ent = nullptr, *phib = nullptr;
+    int32_t* pi = nullptr;
+    temps.add(&cnt);
+    temps.add(&cphi);
+    temps.add(&ent);
+    temps.add(&phib);
+    temps.add(&pi);
+    HIPCHK(hipMalloc(&cnt, sizeof(int64_t) * n));
+    HIPCHK(hipMalloc(&cphi, sizeof(int64_t) * n));
+    HIPCHK(hipMalloc(&ent, sizeof(int64_t) * n));
+    HIPCHK(hipMalloc(&phib, sizeof(int64_t) * (n + 1)));
+    b.cnt = cnt;
+    b.cphi = cphi;
+    b.ent = ent;
+    k_ai_pts<false><<<dim3((unsigned)h->ntl), dim3(256), 0, s>>>(b);
+    HIPCHK(hipGetLastError());
+    int rc = scan_exclusive(h, cnt, h->ai_ptr, n);
+    if (rc) return rc;
+    rc = scan_exclusive(h, cphi, phib, n);
+    if (rc) return rc;
+    std::vector<int64_t> enth(n);
+    HIPCHK(hipMemcpy(enth.data(), ent, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&h->ai_np, h->ai_ptr + n, sizeof(int64_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&h->ai_nphi, phib + n, sizeof(int64_t), hipMemcpyDeviceToHost));
+    h->Elive = 0;
+    for (int64_t a2 = 0; a2 < n; ++a2) h->Elive += enth[a2];
+    const double bytes = (double)h->ai_np * (sizeof(d4) + sizeof(uint32_t) + sizeof(int64_t) + 2 * sizeof(int32_t)) +
+                         (double)h->ai_nphi * sizeof(double);
+    if (bytes > kMaxPhiBytes) {
+        h->err = "adaptive-integration point tables exceed 96 GiB (lower intg_point or support_domain_size)";
+        return PDM_E_LIMIT;
+    }
+    const int64_t np1 = std::max<int64_t>(h->ai_np, 1);
+    HIPCHK(hipMalloc(&h->ai_xw, sizeof(d4) * np1));
+    HIPCHK(hipMalloc(&h->ai_eo, sizeof(uint32_t) * np1));
+    HIPCHK(hipMalloc(&h->ai_po, sizeof(int64_t) * np1));
+    HIPCHK(hipMalloc(&pi, sizeof(int32_t) * np1));
+    HIPCHK(hipMalloc(&h->ai_j, sizeof(int32_t) * np1));
+    b.ai_ptr = h->ai_ptr;
+    b.phi_base = phib;
+    b.xw = h->ai_xw;
+    b.eo = h->ai_eo;
+    b.po = h->ai_po;
+    b.pi = pi;
+    b.pj = h->ai_j;
+    k_ai_pts<true><<<dim3((unsigned)h->ntl), dim3(256), 0, s>>>(b);
+    HIPCHK(hipGetLastError());
+    if (d.mls_on) {
+        HIPCHK(hipMalloc(&h->ai_phi, sizeof(double) * std::max<int64_t>(h->ai_nphi, 1)));
+        if (h->ai_np)
+            k_ai_mls<<<dim3((unsigned)((h->ai_np + 255) / 256)), dim3(256), 0, s>>>(m, h->ai_xw, h->ai_po, pi, h->ai_j,
+                                                                                 h->ai_phi, h->ai_np);
+        HIPCHK(hipGetLastError());
+        int herr = 0;
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipMemcpy(&herr, m.err, sizeof(int), hipMemcpyDeviceToHost));
+        if (herr) {
+            h->err = "MLS moment matrix is singular at an integration point (too few contributing nodes)";
+            return PDM_E_MODEL;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(s));
     return PDM_OK;
 }
 
@@ -425,6 +700,13 @@ FgiArgs fgi_args(const pdm_handle* h)
     f.u0 = h->fgi_u0;
     f.pair_ptr = h->fgi_pair_ptr;
     f.pairs = h->fgi_pairs;
+    f.ai = h->ai;
+    f.ai_ptr = h->ai_ptr;
+    f.ai_xw = h->ai_xw;
+    f.ai_eo = h->ai_eo;
+    f.ai_po = h->ai_po;
+    f.ai_phi = h->ai_phi;
+    f.ai_vol = h->deck.dx * h->deck.dx * h->deck.dx;
     return f;
 }
 

@@ -79,6 +79,112 @@ PDM_HD double g_dist_to_cube(double x1i, double x2i, double x3i, double x1j, dou
     return m;
 }
 
+// Shortest and longest distances from node i to node j's cell, both as CalcDistanceToCube
+// computes them (pdDataManager3.cpp:637-686): over the faces the line i->j pierces, the distance
+// to the nearest and to the farthest point of the face (CalcDistanceToFace, :1573-1661).
+PDM_HD void g_dist_to_cube_mm(double x1i, double x2i, double x3i, double x1j, double x2j, double x3j, double dx,
+                              double* dmin, double* dmax)
+{
+    double mn = DBL_MAX, mx = DBL_MIN;
+    const double tol = 1e-4 * dx;
+    const double x1lo = x1j - 0.5 * dx, x1hi = x1j + 0.5 * dx;
+    const double x2lo = x2j - 0.5 * dx, x2hi = x2j + 0.5 * dx;
+    const double x3lo = x3j - 0.5 * dx, x3hi = x3j + 0.5 * dx;
+    for (int f = 0; f < 6; ++f) {
+        const int norm = f / 2 + 1;
+        const int n1 = norm == 1, n2 = norm == 2, n3 = norm == 3;
+        const double x_face = (f == 0) ? x1lo : (f == 1) ? x1hi : (f == 2) ? x2lo : (f == 3) ? x2hi : (f == 4) ? x3lo : x3hi;
+        if (fabs((x1i - x1j) * n1 + (x2i - x2j) * n2 + (x3i - x3j) * n3) < tol) continue;
+        double x_i = x1i * n1 + x2i * n2 + x3i * n3;
+        double x_j = x1j * n1 + x2j * n2 + x3j * n3;
+        double fac = (x_face - x_i) / (x_j - x_i);
+        double x1int = x1i + fac * (x1j - x1i);
+        double x2int = x2i + fac * (x2j - x2i);
+        double x3int = x3i + fac * (x3j - x3i);
+        if (!(x1int > x1lo - tol && x1int < x1hi + tol && x2int > x2lo - tol && x2int < x2hi + tol &&
+              x3int > x3lo - tol && x3int < x3hi + tol))
+            continue;
+        double x1f = (1 - n1) * x1i + n1 * x_face;
+        double x2f = (1 - n2) * x2i + n2 * x_face;
+        double x3f = (1 - n3) * x3i + n3 * x_face;
+        double x1s = (1 - n1) * g_shortest(x1f, x1lo, x1hi, tol);
+        double x2s = (1 - n2) * g_shortest(x2f, x2lo, x2hi, tol);
+        double x3s = (1 - n3) * g_shortest(x3f, x3lo, x3hi, tol);
+        double x1l = (1 - n1) * g_longest(x1f, x1lo, x1hi, tol);
+        double x2l = (1 - n2) * g_longest(x2f, x2lo, x2hi, tol);
+        double x3l = (1 - n3) * g_longest(x3f, x3lo, x3hi, tol);
+        double ff_s = sqrt(x1s * x1s + x2s * x2s + x3s * x3s);
+        double ff_l = sqrt(x1l * x1l + x2l * x2l + x3l * x3l);
+        double itf = sqrt((x1f - x1i) * (x1f - x1i) + (x2f - x2i) * (x2f - x2i) + (x3f - x3i) * (x3f - x3i));
+        double a = sqrt(ff_s * ff_s + itf * itf), b = sqrt(ff_l * ff_l + itf * itf);
+        mn = (a < mn) ? a : mn;
+        mx = (b > mx) ? b : mx;
+    }
+    *dmin = mn;
+    *dmax = mx;
+}
+
+// Adaptive-integration points of one family entry (CalcBondForceIntg, pdDataManager2.cpp:
+// 1222-1291, with the nested trapezoidal rules of pdIntgManager.cpp:49-451): node j's cell,
+// centre c = X_j - X_i in family coordinates, clipped to the horizon sphere around node i.
+// Nothing when delta <= dmin; the whole cell when delta >= dmax (CalcConfigurationType3,
+// :3471-3482); otherwise x (outer), y, z (inner) run between the cell faces and the sphere,
+// whichever is nearer at the outer coordinates -- one rule for every configuration the
+// reference enumerates in its 44 cases (:523-3469).  nseg = 2^(intg_point - 1) segments per
+// direction, end points weighted 1/2; an interval of length <= 1e-10 adds nothing.  emit(x, y,
+// z, w) receives each point; returns the count.
+template <class EMIT>
+PDM_HD int g_ai_cell_points(const double* c, double dmin, double dmax, double delta, double dx, int nseg, EMIT emit)
+{
+    if (delta <= dmin) return 0;
+    const bool full = delta >= dmax;
+    const double h = 0.5 * dx;
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = c[k] - h;
+        hi[k] = c[k] + h;
+    }
+    double a1 = lo[0], b1 = hi[0];
+    if (!full) {
+        a1 = fmax(a1, -delta);
+        b1 = fmin(b1, delta);
+    }
+    if (b1 - a1 <= 1.0e-10) return 0;
+    const double H1 = (b1 - a1) / nseg;
+    int cnt = 0;
+    for (int ia = 0; ia <= nseg; ++ia) {
+        const double x = ((nseg - ia) * a1 + ia * b1) / nseg;
+        const double w1 = H1 * ((ia == 0 || ia == nseg) ? 0.5 : 1.0);
+        double a2 = lo[1], b2 = hi[1];
+        if (!full) {
+            const double r1 = sqrt(fmax(delta * delta - x * x, 0.0));
+            a2 = fmax(a2, -r1);
+            b2 = fmin(b2, r1);
+        }
+        if (b2 - a2 <= 1.0e-10) continue;
+        const double H2 = (b2 - a2) / nseg;
+        for (int ib = 0; ib <= nseg; ++ib) {
+            const double y = ((nseg - ib) * a2 + ib * b2) / nseg;
+            const double w2 = H2 * ((ib == 0 || ib == nseg) ? 0.5 : 1.0);
+            double a3 = lo[2], b3 = hi[2];
+            if (!full) {
+                const double r2 = sqrt(fmax(delta * delta - x * x - y * y, 0.0));
+                a3 = fmax(a3, -r2);
+                b3 = fmin(b3, r2);
+            }
+            if (b3 - a3 <= 1.0e-10) continue;
+            const double H3 = (b3 - a3) / nseg;
+            for (int ic = 0; ic <= nseg; ++ic) {
+                const double z = ((nseg - ic) * a3 + ic * b3) / nseg;
+                const double w3 = H3 * ((ic == 0 || ic == nseg) ? 0.5 : 1.0);
+                emit(cnt, x, y, z, w1 * w2 * w3);
+                ++cnt;
+            }
+        }
+    }
+    return cnt;
+}
+
 // InitBond's acceptance (pdDataManager2.cpp:33-45) for the pair (i, j), i the lower id.
 // The coarse prefilter uses correctly rounded squares where the reference calls
 // pow(x, 2.0); it cannot decide a pair, since a skipped pair has dmin > delta + 0.6 dx.

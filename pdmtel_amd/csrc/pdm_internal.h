@@ -172,10 +172,19 @@ struct FgiArgs {
     const BCDev* bcs;
     const StepParam* steps; // the step's time (mode 0)
     int64_t n;
-    d4* trial;              // RK4 trial displacement of every node (stages 1-3 outputs)
+    d4* trial;              // RK4 trial displacement of every node: two buffers of n, stage s writes
+                            // the one at (s & 1) n, which stage s + 1 reads
     d4* u0;                 // U at the start of the step (BC nodes)
     const int64_t* pair_ptr;  // [N + 1] live (entry, Gauss point) pairs of each internal row
     const uint32_t* pairs;    // (entry offset in the row) << 16 | point
+    // adaptive integration (ai_on): trapezoidal points of each internal row's entries
+    int ai;                   // 0 off, 1 MLS point displacements, 2 relative displacement G_j xi
+    const int64_t* ai_ptr;    // [N + 1]
+    const d4* ai_xw;          // [P] position relative to node i (x, y, z) and weight
+    const uint32_t* ai_eo;    // [P] entry offset in node i's internal row
+    const int64_t* ai_po;     // [P] the point's first MLS coefficient in ai_phi
+    const double* ai_phi;     // coefficients over node j's contributing nodes (cb_ptr / cb)
+    double ai_vol;            // dx^3: a point's energy share is its weight / dx^3
 };
 
 // Device temporaries of a set-up routine, released on every return path once the
@@ -313,7 +322,7 @@ struct pdm_handle {
     int fgi_n = 0, fgi_ng = 0;
     double* fgi_tab = nullptr;
     pdm::d4* gauss = nullptr;          // [N][ng] Gauss-point displacements
-    pdm::d4* fgi_trial = nullptr;      // [N] RK4 trial displacements the Gauss points interpolate
+    pdm::d4* fgi_trial = nullptr;      // [2][N] RK4 trial displacements the integration points interpolate
     pdm::d4* fgi_u0 = nullptr;         // [N] U at the start of the step
     int64_t* cb_ptr = nullptr;         // MLS contributing nodes (n > 1)
     int32_t* cb = nullptr;
@@ -323,6 +332,18 @@ struct pdm_handle {
     int64_t* fgi_pair_ptr = nullptr;   // [N + 1] live (entry, Gauss point) pairs per internal row
     uint32_t* fgi_pairs = nullptr;
     int64_t fgi_npairs = 0;
+    int64_t ncb = 0;                   // MLS contributing-list entries (cb)
+
+    // adaptive integration (ai_on): per internal row, the trapezoidal points of its entries
+    int ai = 0;                        // 1 with MLS, 2 without
+    int ai_nseg = 0;                   // trapezoidal segments per direction, 2^(intg_point - 1)
+    int64_t ai_np = 0, ai_nphi = 0;    // points, MLS coefficients
+    int64_t* ai_ptr = nullptr;         // [N + 1]
+    pdm::d4* ai_xw = nullptr;          // [P] x, y, z relative to node i, weight
+    uint32_t* ai_eo = nullptr;         // [P] entry offset in the internal row
+    int64_t* ai_po = nullptr;          // [P] first coefficient in ai_phi
+    int32_t* ai_j = nullptr;           // [P] node j (internal id) of the point's entry
+    double* ai_phi = nullptr;
 
     // graph cache: the captured chunks of RK4 steps, keyed by their step count (a run of
     // nsteps uses at most two: full chunks and the remainder)
@@ -377,6 +398,10 @@ hipError_t launch_fly_corr(const TileView& tv, const d4* u, uint8_t* corr, int* 
 // fixed Gaussian integration (pdm_fgi.hip, kernels in pdm_kernels.hip)
 int build_fgi(pdm_handle* h);
 void free_fgi(pdm_handle* h);
+// adaptive integration (pdm_fgi.hip): the points and MLS coefficients; stages run through the
+// fixed-Gaussian stage kernels with FgiArgs::ai set
+int build_ai(pdm_handle* h);
+void free_ai(pdm_handle* h);
 FgiArgs fgi_args(const pdm_handle* h);
 hipError_t launch_fgi_gauss(const FgiArgs& f, const d4* u, int step_idx, hipStream_t s);
 hipError_t launch_fgi_stage(int stage, const StageArgs& a, const FgiArgs& f, int step_idx, hipStream_t s);

@@ -1392,6 +1392,91 @@ __device__ __forceinline__ void fgi_family(const TileView& tv, const FgiArgs& f,
     acc[3] = wave_sum(el);
 }
 
+// The adaptive-integration branch of CalcBondBasedMotionRK4's family loop (pdDataManager.cpp:
+// 1888-1916): for each uncut entry (i, j), CalcBondBasedForce (pdIntgManager2.cpp:92-310) at
+// every trapezoidal point of node j's cell clipped to node i's horizon, beta = 1, summed with
+// the points' weights.  A point's displacement: f.ai == 2 (no MLS) gives the relative
+// displacement G_j xi from node j's DISPLGRAD condition; otherwise a BC node j's DISPL /
+// DISPLGRAD value at time t in its flagged components (0 in the others) when mode is 0, else the
+// MLS interpolation of the contributing nodes' displacements src.  A point's elastic energy
+// density is scaled by its share of a cell, w / dx^3.  One wavefront per node, lanes stepping
+// through the node's points; returns (fx, fy, fz, energy density) in every lane.
+__device__ __forceinline__ void ai_family(const TileView& tv, const FgiArgs& f, const MatDev* mats, const BCDev* bcs,
+                                          const int32_t* __restrict__ mat, const int32_t* __restrict__ bc, int64_t h0,
+                                          int64_t i, const d4& ui, const d4* __restrict__ src, int mode, double t,
+                                          double acc[4])
+{
+    const int lane = threadIdx.x & 63;
+    const d4 xi = tv.X[i];
+    const MatDev& mi = mats[mat[i]];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0, el = 0.0;
+    const int64_t eb = tv.rpi[i];
+    const int64_t q1 = f.ai_ptr[i + 1];
+    for (int64_t q = f.ai_ptr[i] + lane; q < q1; q += 64) {
+        const d4 w = f.ai_xw[q];
+        const int64_t e = eb + (int64_t)f.ai_eo[q];
+        const double bij = tv.coef64[e];
+        const int32_t j = tv.halo[h0 + tv.slot[e]];
+        const MatDev& mj = mats[mat[j]];
+        const double ecrit = (mj.ecrit < mi.ecrit) ? mj.ecrit : mi.ecrit;
+        const double spring = (mj.spring < mi.spring) ? mj.spring : mi.spring;
+        const int b = bc[j];
+        double e1, e2, e3;
+        if (f.ai == 2) {
+            const double* G = bcs[b].v;
+            e1 = G[0] * w.x + G[1] * w.y + G[2] * w.z;
+            e2 = G[3] * w.x + G[4] * w.y + G[5] * w.z;
+            e3 = G[6] * w.x + G[7] * w.y + G[8] * w.z;
+        } else {
+            double v[3] = {0.0, 0.0, 0.0};
+            if (mode == 0 && b >= 0 && bcs[b].bcnode) {
+                const BCDev& bd = bcs[b];
+                const double fac = time_factor_dev(bd, t);
+                const double xg[3] = {w.x + xi.x, w.y + xi.y, w.z + xi.z};
+                for (int c = 0; c < 3; ++c) {
+                    if (bd.f[c] != 1) continue;
+                    if (bd.type == 1) v[c] = bd.v[c] * fac;
+                    else if (bd.type == 3)
+                        v[c] = (bd.v[3 * c] * xg[0] + bd.v[3 * c + 1] * xg[1] + bd.v[3 * c + 2] * xg[2]) * fac;
+                }
+            } else {
+                const int64_t k0 = f.cb_ptr[j], k1 = f.cb_ptr[j + 1];
+                const double* ph = f.ai_phi + f.ai_po[q] - k0;
+#pragma unroll 4
+                for (int64_t k = k0; k < k1; ++k) {
+                    const double c = ph[k];
+                    const d4 x = src[f.cb[k]];
+                    v[0] += x.x * c;
+                    v[1] += x.y * c;
+                    v[2] += x.z * c;
+                }
+            }
+            e1 = v[0] - ui.x;
+            e2 = v[1] - ui.y;
+            e3 = v[2] - ui.z;
+        }
+        const double a1 = w.x + e1, a2 = w.y + e2, a3 = w.z + e3;
+        const double r = sqrt_rn(w.x * w.x + w.y * w.y + w.z * w.z);
+        const double p = sqrt_rn(a1 * a1 + a2 * a2 + a3 * a3);
+        const double u = p - r;
+        // one reciprocal for the divisions by r and p: 1/r = p / (r p), 1/p = r / (r p)
+        const double inv = 1.0 / (r * p);
+        const double rinv = p * inv;
+        const double st = u * rinv;
+        const bool ok = (u <= 0.0) | (st <= ecrit);
+        const double c = ok ? bij * spring * st : 0.0;
+        const double wf = w.w * (c * (r * inv));
+        f0 = __builtin_fma(wf, a1, f0);
+        f1 = __builtin_fma(wf, a2, f1);
+        f2 = __builtin_fma(wf, a3, f2);
+        el += ok ? bij * 0.5 * (0.5 * (spring * rinv) * u * u) * (w.w / f.ai_vol) : 0.0;
+    }
+    acc[0] = wave_sum(f0);
+    acc[1] = wave_sum(f1);
+    acc[2] = wave_sum(f2);
+    acc[3] = wave_sum(el);
+}
+
 constexpr int kFgiWaves = 4;
 
 // Gauss-point displacements, one thread per (node, point): CalcIntgPointDisplRK4
@@ -1451,7 +1536,7 @@ __host__ __device__ inline size_t fgi_lds_bytes(int tile_max, int ng)
 // One RK4 stage of a fixed-Gaussian specimen for one tile: family sums over the neighbours'
 // Gauss points (a.uin: this stage's displacement of the family owner, f.gin: the Gauss
 // points'), then the same stage update as k_stage.
-template <int STAGE, bool ENERGY>
+template <int STAGE, bool ENERGY, bool AI>
 __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_stage(StageArgs a, FgiArgs f, int step_idx)
 {
     extern __shared__ double L[];
@@ -1476,7 +1561,12 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_stage(StageArgs a, FgiAr
     for (int loc = threadIdx.x >> 6; loc < nt; loc += kFgiWaves) {
         const int64_t i = a0 + loc;
         double acc[4];
-        fgi_family(tv, f, tab, l.mats, a.mat, h0, i, a.uin[i], acc);
+        if (AI)  // the points interpolate this stage's trial displacements (displStep), written by the last stage
+            ai_family(tv, f, l.mats, l.bcs, a.mat, a.bc, h0, i, a.uin[i],
+                      STAGE == 0 ? a.uin : f.trial + ((STAGE + 1) & 1) * f.n, 0,
+                      a.steps[step_idx].t, acc);
+        else
+            fgi_family(tv, f, tab, l.mats, a.mat, h0, i, a.uin[i], acc);
         if ((threadIdx.x & 63) == 0)
             for (int c = 0; c < 4; ++c) l.res[4 * loc + c] = acc[c];
     }
@@ -1489,11 +1579,13 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_stage(StageArgs a, FgiAr
     __syncthreads();
     const double dts = (STAGE < 2) ? 0.5 * a.steps[step_idx].dt : a.steps[step_idx].dt;
     const double c = (STAGE < 2) ? 0.5 : 1.0;
+    // two trial buffers: the next stage reads this one while its own output goes to the other
+    d4* trial = f.trial + (STAGE & 1) * f.n;
     for (int loc = threadIdx.x; loc < nt; loc += blockDim.x) {
         const int64_t i = a0 + loc;
         const EpiRec& ep = l.epi[loc];
         if (ep.b < 0 || !l.bcs[ep.b].bcnode) {
-            f.trial[i] = a.uout[i];
+            trial[i] = a.uout[i];
             continue;
         }
         d4 u1;
@@ -1504,11 +1596,13 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_stage(StageArgs a, FgiAr
             u1 = f.u0[i];
         }
         const d4 vb = a.Vb[i];
-        f.trial[i] = d4{u1.x + c * (vb.x * dts), u1.y + c * (vb.y * dts), u1.z + c * (vb.z * dts), 0.0};
+        trial[i] = d4{u1.x + c * (vb.x * dts), u1.y + c * (vb.y * dts), u1.z + c * (vb.z * dts), 0.0};
     }
 }
 
-// Fixed-Gaussian family sums of every node for the displacement u (f.gin interpolated from u).
+// Fixed-Gaussian family sums of every node for the displacement u (f.gin interpolated from u);
+// with AI, the points interpolate u for every node (no BC values).
+template <bool AI>
 __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_forces(StageArgs a, FgiArgs f, const d4* __restrict__ u,
                                                                d4* __restrict__ F, double* __restrict__ elast)
 {
@@ -1527,7 +1621,8 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_forces(StageArgs a, FgiA
     for (int loc = threadIdx.x >> 6; loc < nt; loc += kFgiWaves) {
         const int64_t i = a0 + loc;
         double acc[4];
-        fgi_family(tv, f, tab, l.mats, a.mat, h0, i, u[i], acc);
+        if (AI) ai_family(tv, f, l.mats, l.bcs, a.mat, a.bc, h0, i, u[i], u, 2, 0.0, acc);
+        else fgi_family(tv, f, tab, l.mats, a.mat, h0, i, u[i], acc);
         if ((threadIdx.x & 63) == 0) {
             F[i] = d4{acc[0], acc[1], acc[2], 0.0};
             if (elast) elast[i] = acc[3];
@@ -1735,19 +1830,23 @@ hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, 
     }
 }
 
+template <int STAGE, bool ENERGY, bool AI>
+hipError_t launch_fgi_stage_e(const StageArgs& a, const FgiArgs& f, int step_idx, size_t lds, hipStream_t s)
+{
+    hipError_t e = set_lds<k_fgi_stage<STAGE, ENERGY, AI>>(lds);
+    if (e != hipSuccess) return e;
+    k_fgi_stage<STAGE, ENERGY, AI><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, step_idx);
+    return hipGetLastError();
+}
+
 template <int STAGE>
 hipError_t launch_fgi_stage_t(const StageArgs& a, const FgiArgs& f, int step_idx, size_t lds, hipStream_t s)
 {
-    if (a.energies) {
-        hipError_t e = set_lds<k_fgi_stage<STAGE, true>>(lds);
-        if (e != hipSuccess) return e;
-        k_fgi_stage<STAGE, true><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, step_idx);
-    } else {
-        hipError_t e = set_lds<k_fgi_stage<STAGE, false>>(lds);
-        if (e != hipSuccess) return e;
-        k_fgi_stage<STAGE, false><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, step_idx);
-    }
-    return hipGetLastError();
+    if (f.ai)
+        return a.energies ? launch_fgi_stage_e<STAGE, true, true>(a, f, step_idx, lds, s)
+                          : launch_fgi_stage_e<STAGE, false, true>(a, f, step_idx, lds, s);
+    return a.energies ? launch_fgi_stage_e<STAGE, true, false>(a, f, step_idx, lds, s)
+                      : launch_fgi_stage_e<STAGE, false, false>(a, f, step_idx, lds, s);
 }
 
 hipError_t launch_fgi_stage(int stage, const StageArgs& a, const FgiArgs& f, int step_idx, hipStream_t s)
@@ -1774,9 +1873,15 @@ hipError_t launch_fgi_forces(const StageArgs& a, const FgiArgs& f, const d4* u, 
 {
     if (a.tv.ntl == 0) return hipSuccess;
     const size_t lds = fgi_lds_bytes(a.tv.tile_max, f.ng);
-    hipError_t e = set_lds<k_fgi_forces>(lds);
+    if (f.ai) {
+        hipError_t e = set_lds<k_fgi_forces<true>>(lds);
+        if (e != hipSuccess) return e;
+        k_fgi_forces<true><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, u, F, elast);
+        return hipGetLastError();
+    }
+    hipError_t e = set_lds<k_fgi_forces<false>>(lds);
     if (e != hipSuccess) return e;
-    k_fgi_forces<<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, u, F, elast);
+    k_fgi_forces<false><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, u, F, elast);
     return hipGetLastError();
 }
 

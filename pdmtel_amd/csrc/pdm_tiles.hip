@@ -751,7 +751,7 @@ int build_tiles(pdm_handle* h)
     ca.dx = d.dx;
     ca.vcube = d.dx * d.dx * d.dx;
     ca.original_cci = d.cci_on && !d.cci_mdf;
-    ca.fgi = !d.cci_on && d.fgi_on;
+    ca.fgi = !d.cci_on;  // fixed Gaussian and adaptive integration
     for (size_t m = 0; m < d.mats.size() && m < (size_t)kMaxMat; ++m) ca.spring[m] = h->spring[m];
     if (n) {
         k_coef<<<dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s>>>(ca);

@@ -17,6 +17,7 @@ Spec keys (all lengths in metres, moduli in Pa):
       [{geo: 1, ref, box: (x1lo,x1hi,x2lo,x2hi,x3lo,x3hi)} |
        {geo: 2, ref, cyl: (r, cx, cy, x3lo, x3hi)}]   (ref is 1-based, as in the deck)
   damping, bdryeff_on, cci: (on, modified) | None, fgi: (on, intg_point) | None,
+  ai: (on, mls_on, intg_point) | None (used when cci and fgi are off),
   support_domain_size (None = 3 dx),
   jitter (fraction of dx), seed, slits: [{type: 1, box: (6)} | {type: 2, c, n, t, half_t, half_s}],
   traction_body (0/1)
@@ -99,10 +100,17 @@ def deck_text(spec: Dict[str, Any], crlf: bool = False) -> str:
     else:
         kw("cci_on", 0, 0)
         fgi = spec.get("fgi")
-        if fgi is None or not fgi[0]:
-            raise ValueError("spec needs cci or fgi")
-        kw("fgi_on", 1)
-        kw("intg_point", int(fgi[1]))
+        ai = spec.get("ai")
+        if fgi is not None and fgi[0]:
+            kw("fgi_on", 1)
+            kw("intg_point", int(fgi[1]))
+        elif ai is not None and ai[0]:
+            kw("fgi_on", 0)
+            kw("ai_on", 1, int(ai[1]))
+            if len(ai) > 2 and ai[2] is not None:
+                kw("intg_point", int(ai[2]))
+        else:
+            raise ValueError("spec needs cci, fgi or ai")
         if spec.get("support_domain_size") is not None:
             kw("support_domain_size", spec["support_domain_size"])
     if spec.get("jitter"):

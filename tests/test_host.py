@@ -122,8 +122,17 @@ def test_error_codes():
     fgi = text.replace("cci_on\n 1 1", "cci_on\n 0 0\nfgi_on\n 1\nintg_point\n 7")
     rc, _ = create(fgi)
     assert rc == -3 and b"intg_point" in L.pdm_last_error(None)
-    rc, _ = create(text.replace("cci_on\n 1 1", "cci_on\n 0 0\nfgi_on\n 0\nai_on\n 1 1"))
-    assert rc == -4 and b"adaptive" in L.pdm_last_error(None)
+    ai = text.replace("cci_on\n 1 1", "cci_on\n 0 0\nfgi_on\n 0\nai_on\n 1 1")
+    rc, h = create(ai)
+    assert rc == 0  # adaptive integration, default trapezoidal index 3
+    L.pdm_destroy(h)
+    rc, _ = create(ai + "error_control_on\n 1\n")
+    assert rc == -4 and b"error_control_on" in L.pdm_last_error(None)
+    rc, _ = create(ai + "intg_point\n 6\n")
+    assert rc == -3 and b"intg_point" in L.pdm_last_error(None)
+    h = C.c_void_p()
+    rc = L.pdm_create_from_text(ai.encode(), C.byref(pm.options(pm.PDM_FP32C)), C.byref(h))
+    assert rc == -4 and b"FP64" in L.pdm_last_error(None)
     o32 = pm.options(pm.PDM_FP32C)
     h = C.c_void_p()
     rc = L.pdm_create_from_text(fgi.replace("intg_point\n 7", "intg_point\n 2").encode(), C.byref(o32), C.byref(h))
