@@ -211,6 +211,67 @@ def run_fgi(args):
     print(json.dumps(line))
 
 
+AI_FLOPS_PER_POINT = 55  # FP64 operations of one point's bond force (sqrt and division as one)
+
+
+def run_ai(args):
+    """Adaptive integration (ai_on 1 1, intg_point n): FP64 RK4 on the bar (20 x 10 x 10 unless
+    dimensions are given), one JSON line.  The stage kernel interpolates every point's
+    displacement from its MLS coefficients, so its bytes are the coefficient stream."""
+    import pdmtel_amd as pm
+
+    n1 = args.intg_point
+    dims = (20, 10, 10) if (args.nx, args.ny, args.nz) == DEFAULT_DIMS else (args.nx, args.ny, args.nz)
+    spec = pm.bar_spec(*dims, jitter=0.01, seed=0, steps=args.warmup + args.steps + 10)
+    spec["cci"] = None
+    spec["ai"] = (1, 1, n1)
+    t0 = time.perf_counter()
+    prob = pm.Problem(pm.deck_text(spec), precision=pm.PDM_FP64, graph_chunk=graph_chunk(args), energies=1)
+    prob.prepare(graph_chunk(args))
+    t_init = time.perf_counter() - t0
+    P, M = prob.ai_counts
+    print(f"# ai n={n1}: init {t_init:.2f} s, {prob.n} nodes, {prob.E} entries, {P} points, {M} coefficients",
+          file=sys.stderr, flush=True)
+    if args.warmup:
+        prob.step(args.warmup)
+    t0 = time.perf_counter()
+    done = prob.step(args.steps)
+    sec = time.perf_counter() - t0
+    assert done == args.steps, f"ran {done} of {args.steps} steps"
+    ms, launches = prob.kernel_timing()
+    n, E = prob.n, prob.E
+    # per launch: the coefficients (8 B) and their contributing-node ids (4 B), each point's
+    # position / weight (32 B), entry offset (4 B) and coefficient offset (8 B), the node bytes
+    nbytes = 12 * M + 44 * P + NODE_BYTES * n
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    line = {
+        "metric": f"bond-updates/s (RK4, adaptive integration, trapezoidal index {n1}, FP64)",
+        "value": 4.0 * E * args.steps / sec,
+        "unit": "bond-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"bar {dims[0]}x{dims[1]}x{dims[2]}, {n} nodes, {E} family entries, delta=3.015dx, "
+                               f"jitter 0.01dx seed 0, ai_on 1 1 intg_point {n1}, RK4 {args.steps} steps",
+                   "points": P, "mls_coefficients": M, "init_seconds": t_init},
+        "point_evaluations_per_s": 4.0 * P * args.steps / sec,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_fgi_stage<AI> (one RK4 stage)", "kernel_avg_ms": ms, "launches": launches,
+                     "bytes_per_launch": nbytes,
+                     "fp64_tflops": P * (AI_FLOPS_PER_POINT + 6 * M / max(P, 1)) / (ms * 1e-3) / 1e12},
+        "cpu_baseline": None,
+    }
+    prob.close()
+    print(json.dumps(line))
+
+
 def run_batch(args):
     """Batched specimens: FP32C and FP64 runs of the same random specimens."""
     import numpy as np
@@ -357,11 +418,12 @@ def main():
     ap.add_argument("--skip-static", action="store_true")
     ap.add_argument("--skip-plain", action="store_true", help="slit workload: leave out the plain-bar figures")
     ap.add_argument("--calib-copy", type=int, default=0, help="run only a copy kernel of this many bytes")
-    ap.add_argument("--workload", choices=["slit", "bar", "batch", "plate", "fgi"], default="slit",
+    ap.add_argument("--workload", choices=["slit", "bar", "batch", "plate", "fgi", "ai"], default="slit",
                     help="slit: 200x40x40 bar with a 30 deg through-section slit (default); bar: the same bar "
                          "without it; batch: random specimens; plate: --nx --ny --nz plate "
                          "with two crossing slits (400 400 50 for the 8M-node plate); fgi: the bar with fixed "
-                         "Gaussian integration (--intg-point), FP64")
+                         "Gaussian integration (--intg-point), FP64; ai: adaptive integration with MLS "
+                         "(--intg-point), FP64, on the 20x10x10 bar unless --nx --ny --nz are given")
     ap.add_argument("--intg-point", type=int, default=2)
     ap.add_argument("--specimens", type=int, default=512)
     ap.add_argument("--layout", default="", help="edge,waves,families: fixed tile layout (default: autotuned)")
@@ -379,6 +441,9 @@ def main():
         return
     if args.workload == "fgi":
         run_fgi(args)
+        return
+    if args.workload == "ai":
+        run_ai(args)
         return
 
     import numpy as np

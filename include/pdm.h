@@ -193,6 +193,9 @@ int64_t pdm_num_live_gauss_pairs(const pdm_handle* h);
  * contiguous, in x (outer), y, z (inner) order; entries are in no particular order.  Any
  * pointer may be NULL. */
 int64_t pdm_num_ai_points(const pdm_handle* h);
+/* Adaptive integration with MLS: the stored interpolation coefficients (one per point and
+ * contributing node of the point's cell owner), 0 otherwise; for throughput accounting. */
+int64_t pdm_num_ai_coefficients(const pdm_handle* h);
 int pdm_get_ai_points(const pdm_handle* h, int32_t* node, int32_t* nbr, double* xw);
 
 /* Set / get the dynamic state U, V (3N each, host; NULL skips). */

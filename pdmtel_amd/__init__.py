@@ -99,6 +99,8 @@ def lib() -> C.CDLL:
     L.pdm_num_ai_points.restype = C.c_int64
     L.pdm_num_ai_points.argtypes = [vp]
     L.pdm_get_ai_points.argtypes = [vp, i32p, i32p, dp]
+    L.pdm_num_ai_coefficients.restype = C.c_int64
+    L.pdm_num_ai_coefficients.argtypes = [vp]
     cpp = C.POINTER(C.c_char_p)
     L.pdm_batch_create.argtypes = [cpp, C.c_int, C.POINTER(Options), C.POINTER(vp)]
     L.pdm_batch_create_from_text.argtypes = [cpp, C.c_int, C.POINTER(Options), C.POINTER(vp)]
@@ -240,6 +242,11 @@ class Problem:
         Ug = np.zeros(3 * ng * self.n)
         self._chk(self.L.pdm_get_gauss(self.h, _dp(Xg), _dp(Ug)))
         return Xg.reshape(self.n, ng, 3), Ug.reshape(self.n, ng, 3)
+
+    @property
+    def ai_counts(self) -> Tuple[int, int]:
+        """(points, MLS coefficients) of adaptive integration (0, 0 for other modes)."""
+        return int(self.L.pdm_num_ai_points(self.h)), int(self.L.pdm_num_ai_coefficients(self.h))
 
     def ai_points(self):
         """Adaptive-integration points: (node i, node j (reference ids), xw (P, 4): x, y, z relative

@@ -283,6 +283,11 @@ int enqueue_steps(pdm_handle* h, int count)
             }
             // Gauss points follow each stage's new displacements: stages 2-4 read them, and
             // the next step's stage 1 reads those of the updated U (pdDataManager.cpp:2043-2125)
+            // adaptive integration: the points interpolate this stage's trial displacements
+            // (displStep: U at stage 1, then the trial the previous stage wrote)
+            if (f.ai)
+                HCHK(launch_ai_family(a, f, a.uin, st == 0 ? a.uin : f.trial + ((st + 1) & 1) * f.n, 0, k, h->ai_res,
+                                      h->stream));
             HCHK(launch_fgi_stage(st, a, f, k, h->stream));
             if (gauss) HCHK(launch_fgi_gauss(f, st == 3 ? a.uout : f.trial + (st & 1) * f.n, k, h->stream));
         }
@@ -1011,8 +1016,14 @@ int pdm_compute_forces(pdm_handle* h, const double* u, double* F, double* elast)
     HCHK(hipMalloc(&ed, sizeof(double) * n));
     HCHK(hipMemcpy(ud, buf.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
     if (h->ai) {
-        // every node's points interpolated from u
-        HCHK(launch_fgi_forces(stage_args(h), fgi_args(h), ud, Fd, ed, h->stream));
+        // every node's points interpolated from u (the stepping state recomputes its own per stage)
+        HCHK(launch_ai_family(stage_args(h), fgi_args(h), ud, ud, 2, 0, Fd, h->stream));
+        HCHK(hipStreamSynchronize(h->stream));
+        std::vector<d4> r(n);
+        HCHK(hipMemcpy(r.data(), Fd, sizeof(d4) * n, hipMemcpyDeviceToHost));
+        std::vector<double> e(n);
+        for (int64_t q = 0; q < n; ++q) e[q] = r[q].w;
+        HCHK(hipMemcpy(ed, e.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     } else if (h->fgi_ng) {
         // every node's Gauss points interpolated from u, into scratch (the stepping state keeps its own)
         d4* gd = nullptr;
@@ -1159,6 +1170,7 @@ int pdm_areal_force_density(pdm_handle* h, const int64_t* nodes, int64_t count, 
 }
 
 int64_t pdm_num_ai_points(const pdm_handle* h) { return h ? h->ai_np : -1; }
+int64_t pdm_num_ai_coefficients(const pdm_handle* h) { return h ? h->ai_nphi : -1; }
 
 int pdm_get_ai_points(const pdm_handle* h, int32_t* node, int32_t* nbr, double* xw)
 {

@@ -266,9 +266,10 @@ struct AiBuild {
     const int64_t* phi_base;
     d4* xw;
     uint32_t* eo;
-    int64_t* po;
-    int32_t* pi;             // the point's node i (set-up only)
-    int32_t* pj;             // the point's node j
+    int64_t* po;             // the point's first coefficient; coefficient k at po + k * ps
+    int32_t* ps;             // points of the point's entry (the coefficient stride)
+    int32_t* pi;             // the point's nodes i and j
+    int32_t* pj;
 };
 
 // The AI points of one entry of internal row i: node j's cell relative to node i, with the bond
@@ -333,7 +334,10 @@ __global__ __launch_bounds__(256) void k_ai_pts(AiBuild b)
                 ai_entry_points(b, i, j, xi, xj, [&](int q, double x, double y, double z, double w) {
                     b.xw[o + q] = d4{x, y, z, w};
                     b.eo[o + q] = off;
-                    b.po[o + q] = po + (int64_t)q * m;
+                    // coefficients of an entry: [contributing node][point], so that consecutive
+                    // points read consecutive words
+                    b.po[o + q] = po + q;
+                    b.ps[o + q] = c;
                     b.pi[o + q] = (int32_t)i;
                     b.pj[o + q] = j;
                 });
@@ -355,15 +359,15 @@ __global__ __launch_bounds__(256) void k_ai_pts(AiBuild b)
 // MLS coefficients of every AI point over its cell owner's (node j's) contributing nodes, at
 // the point's global position X_i + xi (InitMLSCoeff, pdIntgManager2.cpp:1879-1930).
 __global__ __launch_bounds__(256) void k_ai_mls(MlsArgs m, const d4* __restrict__ xw, const int64_t* __restrict__ po,
-                                                const int32_t* __restrict__ pi, const int32_t* __restrict__ pj,
-                                                double* __restrict__ phi, int64_t np)
+                                                const int32_t* __restrict__ ps, const int32_t* __restrict__ pi,
+                                                const int32_t* __restrict__ pj, double* __restrict__ phi, int64_t np)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= np) return;
     const d4 x = xw[q];
     const d4 xi = m.Xi[pi[q]];
     const double xg[3] = {x.x + xi.x, x.y + xi.y, x.z + xi.z};
-    if (!mls_point(m, pj[q], xg, phi + po[q], 1)) *m.err = 1;
+    if (!mls_point(m, pj[q], xg, phi + po[q], ps[q])) *m.err = 1;
 }
 
 // Bytes above which the MLS coefficient table is refused.
@@ -556,9 +560,10 @@ int build_fgi(pdm_handle* h)
 void free_ai(pdm_handle* h)
 {
     for (void* p : {(void*)h->ai_ptr, (void*)h->ai_xw, (void*)h->ai_eo, (void*)h->ai_po, (void*)h->ai_phi,
-                    (void*)h->ai_j})
+                    (void*)h->ai_j, (void*)h->ai_i, (void*)h->ai_ps, (void*)h->ai_res})
         if (p) (void)hipFree(p);
-    h->ai_j = nullptr;
+    h->ai_j = h->ai_i = h->ai_ps = nullptr;
+    h->ai_res = nullptr;
     h->ai_ptr = nullptr;
     h->ai_xw = nullptr;
     h->ai_eo = nullptr;
@@ -615,12 +620,10 @@ int build_ai(pdm_handle* h)
     b.dx = d.dx;
     b.nseg = h->ai_nseg;
     int64_t *cnt = nullptr, *cphi = nullptr, *ent = nullptr, *phib = nullptr;
-    int32_t* pi = nullptr;
     temps.add(&cnt);
     temps.add(&cphi);
     temps.add(&ent);
     temps.add(&phib);
-    temps.add(&pi);
     HIPCHK(hipMalloc(&cnt, sizeof(int64_t) * n));
     HIPCHK(hipMalloc(&cphi, sizeof(int64_t) * n));
     HIPCHK(hipMalloc(&ent, sizeof(int64_t) * n));
@@ -640,7 +643,7 @@ int build_ai(pdm_handle* h)
     HIPCHK(hipMemcpy(&h->ai_nphi, phib + n, sizeof(int64_t), hipMemcpyDeviceToHost));
     h->Elive = 0;
     for (int64_t a2 = 0; a2 < n; ++a2) h->Elive += enth[a2];
-    const double bytes = (double)h->ai_np * (sizeof(d4) + sizeof(uint32_t) + sizeof(int64_t) + 2 * sizeof(int32_t)) +
+    const double bytes = (double)h->ai_np * (sizeof(d4) + sizeof(uint32_t) + sizeof(int64_t) + 3 * sizeof(int32_t)) +
                          (double)h->ai_nphi * sizeof(double);
     if (bytes > kMaxPhiBytes) {
         h->err = "adaptive-integration point tables exceed 96 GiB (lower intg_point or support_domain_size)";
@@ -650,21 +653,25 @@ int build_ai(pdm_handle* h)
     HIPCHK(hipMalloc(&h->ai_xw, sizeof(d4) * np1));
     HIPCHK(hipMalloc(&h->ai_eo, sizeof(uint32_t) * np1));
     HIPCHK(hipMalloc(&h->ai_po, sizeof(int64_t) * np1));
-    HIPCHK(hipMalloc(&pi, sizeof(int32_t) * np1));
+    HIPCHK(hipMalloc(&h->ai_ps, sizeof(int32_t) * np1));
+    HIPCHK(hipMalloc(&h->ai_i, sizeof(int32_t) * np1));
     HIPCHK(hipMalloc(&h->ai_j, sizeof(int32_t) * np1));
+    HIPCHK(hipMalloc(&h->ai_res, sizeof(d4) * n));
+    HIPCHK(hipMemsetAsync(h->ai_res, 0, sizeof(d4) * n, s));
     b.ai_ptr = h->ai_ptr;
     b.phi_base = phib;
     b.xw = h->ai_xw;
     b.eo = h->ai_eo;
     b.po = h->ai_po;
-    b.pi = pi;
+    b.ps = h->ai_ps;
+    b.pi = h->ai_i;
     b.pj = h->ai_j;
     k_ai_pts<true><<<dim3((unsigned)h->ntl), dim3(256), 0, s>>>(b);
     HIPCHK(hipGetLastError());
     if (d.mls_on) {
         HIPCHK(hipMalloc(&h->ai_phi, sizeof(double) * std::max<int64_t>(h->ai_nphi, 1)));
         if (h->ai_np)
-            k_ai_mls<<<dim3((unsigned)((h->ai_np + 255) / 256)), dim3(256), 0, s>>>(m, h->ai_xw, h->ai_po, pi, h->ai_j,
+            k_ai_mls<<<dim3((unsigned)((h->ai_np + 255) / 256)), dim3(256), 0, s>>>(m, h->ai_xw, h->ai_po, h->ai_ps, h->ai_i, h->ai_j,
                                                                                  h->ai_phi, h->ai_np);
         HIPCHK(hipGetLastError());
         int herr = 0;
@@ -705,6 +712,11 @@ FgiArgs fgi_args(const pdm_handle* h)
     f.ai_xw = h->ai_xw;
     f.ai_eo = h->ai_eo;
     f.ai_po = h->ai_po;
+    f.ai_ps = h->ai_ps;
+    f.ai_i = h->ai_i;
+    f.ai_j = h->ai_j;
+    f.ai_res = h->ai_res;
+    f.ai_np = h->ai_np;
     f.ai_phi = h->ai_phi;
     f.ai_vol = h->deck.dx * h->deck.dx * h->deck.dx;
     return f;

@@ -182,8 +182,13 @@ struct FgiArgs {
     const int64_t* ai_ptr;    // [N + 1]
     const d4* ai_xw;          // [P] position relative to node i (x, y, z) and weight
     const uint32_t* ai_eo;    // [P] entry offset in node i's internal row
-    const int64_t* ai_po;     // [P] the point's first MLS coefficient in ai_phi
+    const int64_t* ai_po;     // [P] the point's first MLS coefficient in ai_phi, coefficient k at
+    const int32_t* ai_ps;     //     ai_po + k * ai_ps (an entry's block is [contributing node][point])
     const double* ai_phi;     // coefficients over node j's contributing nodes (cb_ptr / cb)
+    const int32_t* ai_i;      // [P] the point's nodes i and j (internal ids)
+    const int32_t* ai_j;
+    const d4* ai_res;         // [N] family sums of k_ai_family, read by the stage kernel
+    int64_t ai_np;
     double ai_vol;            // dx^3: a point's energy share is its weight / dx^3
 };
 
@@ -342,7 +347,10 @@ struct pdm_handle {
     pdm::d4* ai_xw = nullptr;          // [P] x, y, z relative to node i, weight
     uint32_t* ai_eo = nullptr;         // [P] entry offset in the internal row
     int64_t* ai_po = nullptr;          // [P] first coefficient in ai_phi
-    int32_t* ai_j = nullptr;           // [P] node j (internal id) of the point's entry
+    int32_t* ai_j = nullptr;           // [P] nodes j and i (internal ids) of the point's entry
+    int32_t* ai_i = nullptr;
+    int32_t* ai_ps = nullptr;          // [P] coefficient stride (points of the entry)
+    pdm::d4* ai_res = nullptr;         // [N] family sums (fx, fy, fz, energy density) of a stage
     double* ai_phi = nullptr;
 
     // graph cache: the captured chunks of RK4 steps, keyed by their step count (a run of
@@ -402,6 +410,11 @@ void free_fgi(pdm_handle* h);
 // fixed-Gaussian stage kernels with FgiArgs::ai set
 int build_ai(pdm_handle* h);
 void free_ai(pdm_handle* h);
+// Adaptive-integration family sums of every node into out (fx, fy, fz, energy density): node i's
+// displacement from uin, the points' from the MLS interpolation of src (the stage's trial); mode
+// 0 prescribes BC nodes' points at the time of step step_idx, mode 2 interpolates every point.
+hipError_t launch_ai_family(const StageArgs& a, const FgiArgs& f, const d4* uin, const d4* src, int mode, int step_idx,
+                            d4* out, hipStream_t s);
 FgiArgs fgi_args(const pdm_handle* h);
 hipError_t launch_fgi_gauss(const FgiArgs& f, const d4* u, int step_idx, hipStream_t s);
 hipError_t launch_fgi_stage(int stage, const StageArgs& a, const FgiArgs& f, int step_idx, hipStream_t s);

@@ -1393,30 +1393,41 @@ __device__ __forceinline__ void fgi_family(const TileView& tv, const FgiArgs& f,
 }
 
 // The adaptive-integration branch of CalcBondBasedMotionRK4's family loop (pdDataManager.cpp:
-// 1888-1916): for each uncut entry (i, j), CalcBondBasedForce (pdIntgManager2.cpp:92-310) at
-// every trapezoidal point of node j's cell clipped to node i's horizon, beta = 1, summed with
-// the points' weights.  A point's displacement: f.ai == 2 (no MLS) gives the relative
-// displacement G_j xi from node j's DISPLGRAD condition; otherwise a BC node j's DISPL /
-// DISPLGRAD value at time t in its flagged components (0 in the others) when mode is 0, else the
-// MLS interpolation of the contributing nodes' displacements src.  A point's elastic energy
-// density is scaled by its share of a cell, w / dx^3.  One wavefront per node, lanes stepping
-// through the node's points; returns (fx, fy, fz, energy density) in every lane.
-__device__ __forceinline__ void ai_family(const TileView& tv, const FgiArgs& f, const MatDev* mats, const BCDev* bcs,
-                                          const int32_t* __restrict__ mat, const int32_t* __restrict__ bc, int64_t h0,
-                                          int64_t i, const d4& ui, const d4* __restrict__ src, int mode, double t,
-                                          double acc[4])
+// 1888-1916) for one node i per workgroup: for each uncut entry (i, j), CalcBondBasedForce
+// (pdIntgManager2.cpp:92-310) at every trapezoidal point of node j's cell clipped to node i's
+// horizon, beta = 1, summed with the points' weights.  The point's relative displacement:
+// - f.ai == 2 (no MLS): G_j xi from node j's DISPLGRAD condition;
+// - a BC node j (mode 0): its DISPL / DISPLGRAD value at the step's time in the flagged
+//   components, 0 in the others (pdIntgManager2.cpp:115-172), less u_i;
+// - otherwise the MLS interpolation sum_k phi_k src[cb_k] over node j's contributing nodes
+//   (pdIntgManager2.cpp:173-186), src holding the stage's trial displacements, less u_i.
+// A point's elastic energy density is scaled by its share of a cell, w / dx^3.  The threads step
+// through the node's points; the points of an entry are consecutive threads sharing node j, and
+// their coefficients are stored [contributing node][point], so a wavefront reads them in
+// contiguous runs.  out[i] = (fx, fy, fz, energy density), sums in a fixed order.
+constexpr int kAiThreads = 256;
+
+__global__ __launch_bounds__(kAiThreads) void k_ai_family(TileView tv, FgiArgs f, const int32_t* __restrict__ mat,
+                                                          const int32_t* __restrict__ bc, const MatDev* __restrict__ mats,
+                                                          const BCDev* __restrict__ bcs, const d4* __restrict__ uin,
+                                                          const d4* __restrict__ src, int mode,
+                                                          const StepParam* __restrict__ steps, int step_idx,
+                                                          d4* __restrict__ out)
 {
-    const int lane = threadIdx.x & 63;
+    __shared__ double sh[4][kAiThreads / 64];
+    const int64_t i = blockIdx.x;
+    const double t = (mode == 0) ? steps[step_idx].t : 0.0;
+    const d4 ui = uin[i];
     const d4 xi = tv.X[i];
-    const MatDev& mi = mats[mat[i]];
+    const MatDev mi = mats[mat[i]];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0, el = 0.0;
     const int64_t eb = tv.rpi[i];
     const int64_t q1 = f.ai_ptr[i + 1];
-    for (int64_t q = f.ai_ptr[i] + lane; q < q1; q += 64) {
+    for (int64_t q = f.ai_ptr[i] + threadIdx.x; q < q1; q += kAiThreads) {
         const d4 w = f.ai_xw[q];
         const int64_t e = eb + (int64_t)f.ai_eo[q];
         const double bij = tv.coef64[e];
-        const int32_t j = tv.halo[h0 + tv.slot[e]];
+        const int32_t j = f.ai_j[q];
         const MatDev& mj = mats[mat[j]];
         const double ecrit = (mj.ecrit < mi.ecrit) ? mj.ecrit : mi.ecrit;
         const double spring = (mj.spring < mi.spring) ? mj.spring : mi.spring;
@@ -1441,10 +1452,12 @@ __device__ __forceinline__ void ai_family(const TileView& tv, const FgiArgs& f, 
                 }
             } else {
                 const int64_t k0 = f.cb_ptr[j], k1 = f.cb_ptr[j + 1];
-                const double* ph = f.ai_phi + f.ai_po[q] - k0;
+                const int64_t st = f.ai_ps[q];
+                const double* ph = f.ai_phi + f.ai_po[q];
 #pragma unroll 4
                 for (int64_t k = k0; k < k1; ++k) {
-                    const double c = ph[k];
+                    const double c = *ph;
+                    ph += st;
                     const d4 x = src[f.cb[k]];
                     v[0] += x.x * c;
                     v[1] += x.y * c;
@@ -1462,19 +1475,30 @@ __device__ __forceinline__ void ai_family(const TileView& tv, const FgiArgs& f, 
         // one reciprocal for the divisions by r and p: 1/r = p / (r p), 1/p = r / (r p)
         const double inv = 1.0 / (r * p);
         const double rinv = p * inv;
-        const double st = u * rinv;
-        const bool ok = (u <= 0.0) | (st <= ecrit);
-        const double c = ok ? bij * spring * st : 0.0;
+        const double sr = u * rinv;
+        const bool ok = (u <= 0.0) | (sr <= ecrit);
+        const double c = ok ? bij * spring * sr : 0.0;
         const double wf = w.w * (c * (r * inv));
         f0 = __builtin_fma(wf, a1, f0);
         f1 = __builtin_fma(wf, a2, f1);
         f2 = __builtin_fma(wf, a3, f2);
         el += ok ? bij * 0.5 * (0.5 * (spring * rinv) * u * u) * (w.w / f.ai_vol) : 0.0;
     }
-    acc[0] = wave_sum(f0);
-    acc[1] = wave_sum(f1);
-    acc[2] = wave_sum(f2);
-    acc[3] = wave_sum(el);
+    const double r0 = wave_sum(f0), r1 = wave_sum(f1), r2 = wave_sum(f2), r3 = wave_sum(el);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][wave] = r0;
+        sh[1][wave] = r1;
+        sh[2][wave] = r2;
+        sh[3][wave] = r3;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s4[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < 4; ++c)
+            for (int q = 0; q < kAiThreads / 64; ++q) s4[c] += sh[c][q];
+        out[i] = d4{s4[0], s4[1], s4[2], s4[3]};
+    }
 }
 
 constexpr int kFgiWaves = 4;
@@ -1561,11 +1585,13 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_stage(StageArgs a, FgiAr
     for (int loc = threadIdx.x >> 6; loc < nt; loc += kFgiWaves) {
         const int64_t i = a0 + loc;
         double acc[4];
-        if (AI)  // the points interpolate this stage's trial displacements (displStep), written by the last stage
-            ai_family(tv, f, l.mats, l.bcs, a.mat, a.bc, h0, i, a.uin[i],
-                      STAGE == 0 ? a.uin : f.trial + ((STAGE + 1) & 1) * f.n, 0,
-                      a.steps[step_idx].t, acc);
-        else
+        if (AI) {  // the family sums of k_ai_family
+            const d4 r = f.ai_res[i];
+            acc[0] = r.x;
+            acc[1] = r.y;
+            acc[2] = r.z;
+            acc[3] = r.w;
+        } else
             fgi_family(tv, f, tab, l.mats, a.mat, h0, i, a.uin[i], acc);
         if ((threadIdx.x & 63) == 0)
             for (int c = 0; c < 4; ++c) l.res[4 * loc + c] = acc[c];
@@ -1600,9 +1626,7 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_stage(StageArgs a, FgiAr
     }
 }
 
-// Fixed-Gaussian family sums of every node for the displacement u (f.gin interpolated from u);
-// with AI, the points interpolate u for every node (no BC values).
-template <bool AI>
+// Fixed-Gaussian family sums of every node for the displacement u (f.gin interpolated from u).
 __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_forces(StageArgs a, FgiArgs f, const d4* __restrict__ u,
                                                                d4* __restrict__ F, double* __restrict__ elast)
 {
@@ -1621,8 +1645,7 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_forces(StageArgs a, FgiA
     for (int loc = threadIdx.x >> 6; loc < nt; loc += kFgiWaves) {
         const int64_t i = a0 + loc;
         double acc[4];
-        if (AI) ai_family(tv, f, l.mats, l.bcs, a.mat, a.bc, h0, i, u[i], u, 2, 0.0, acc);
-        else fgi_family(tv, f, tab, l.mats, a.mat, h0, i, u[i], acc);
+        fgi_family(tv, f, tab, l.mats, a.mat, h0, i, u[i], acc);
         if ((threadIdx.x & 63) == 0) {
             F[i] = d4{acc[0], acc[1], acc[2], 0.0};
             if (elast) elast[i] = acc[3];
@@ -1861,6 +1884,15 @@ hipError_t launch_fgi_stage(int stage, const StageArgs& a, const FgiArgs& f, int
     }
 }
 
+hipError_t launch_ai_family(const StageArgs& a, const FgiArgs& f, const d4* uin, const d4* src, int mode, int step_idx,
+                            d4* out, hipStream_t s)
+{
+    if (f.n == 0) return hipSuccess;
+    k_ai_family<<<dim3((unsigned)f.n), dim3(kAiThreads), 0, s>>>(a.tv, f, a.mat, a.bc, a.mats, a.bcs, uin, src, mode,
+                                                                a.steps, step_idx, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_fgi_gauss(const FgiArgs& f, const d4* u, int step_idx, hipStream_t s)
 {
     const int64_t m = f.n * f.ng;
@@ -1873,15 +1905,9 @@ hipError_t launch_fgi_forces(const StageArgs& a, const FgiArgs& f, const d4* u, 
 {
     if (a.tv.ntl == 0) return hipSuccess;
     const size_t lds = fgi_lds_bytes(a.tv.tile_max, f.ng);
-    if (f.ai) {
-        hipError_t e = set_lds<k_fgi_forces<true>>(lds);
-        if (e != hipSuccess) return e;
-        k_fgi_forces<true><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, u, F, elast);
-        return hipGetLastError();
-    }
-    hipError_t e = set_lds<k_fgi_forces<false>>(lds);
+    hipError_t e = set_lds<k_fgi_forces>(lds);
     if (e != hipSuccess) return e;
-    k_fgi_forces<false><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, u, F, elast);
+    k_fgi_forces<<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, u, F, elast);
     return hipGetLastError();
 }
 
