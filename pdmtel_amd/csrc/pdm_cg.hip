@@ -12,7 +12,8 @@
 // force counted once per unbroken bond, as the dynamic path does, plus the
 // traction body force of the extension.
 //
-// Solver: Jacobi-preconditioned CG on the free components (mask m), all vector
+// Solver: block-Jacobi-preconditioned CG on the free components (mask m): each node's 3x3
+// tangent block, restricted to its free components, is inverted once; all vector
 // algebra in FP64, matvecs in the handle's precision (FP64, or FP32 bonds for
 // PDM_FP32C/C2).  After every block of 25 iterations (captured in a hipGraph, CG
 // scalars on the device) the residual is recomputed with the FP64 operator and
@@ -48,6 +49,13 @@ struct Scal {
 };
 
 __device__ __forceinline__ double dot3(const d4& a, const d4& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// z = m (B^-1 r): the node's inverse block, diagonal in ia, off-diagonal (12, 13, 23) in ib.
+__device__ __forceinline__ d4 precond(const d4& r, const d4& ia, const d4& ib, const d4& m)
+{
+    return d4{m.x * (ia.x * r.x + ib.x * r.y + ib.y * r.z), m.y * (ib.x * r.x + ia.y * r.y + ib.z * r.z),
+              m.z * (ib.y * r.x + ib.z * r.y + ia.z * r.z), 0.0};
+}
 
 __device__ __forceinline__ void block_sum2(double a, double b, double* out2)
 {
@@ -90,16 +98,17 @@ __global__ __launch_bounds__(kVB) void k_residual(int64_t n, const d4* __restric
     block_sum2(rr, 0.0, part);
 }
 
-// d = 0, z = m r / diag, p = z; partial r.z and |r|^2.
-__global__ __launch_bounds__(kVB) void k_pcg_start(int64_t n, const d4* __restrict__ r, const d4* __restrict__ dg,
-                                                   const d4* __restrict__ m, d4* __restrict__ z, d4* __restrict__ p,
-                                                   d4* __restrict__ d, double* part)
+// d = 0, z = m B^-1 r, p = z; partial r.z and |r|^2.
+__global__ __launch_bounds__(kVB) void k_pcg_start(int64_t n, const d4* __restrict__ r, const d4* __restrict__ ia,
+                                                   const d4* __restrict__ ib, const d4* __restrict__ m,
+                                                   d4* __restrict__ z, d4* __restrict__ p, d4* __restrict__ d,
+                                                   double* part)
 {
     int64_t i = (int64_t)blockIdx.x * kVB + threadIdx.x;
     double rz = 0.0, rr = 0.0;
     if (i < n) {
-        d4 ri = r[i], g = dg[i], mm = m[i];
-        d4 zi{mm.x * ri.x / g.x, mm.y * ri.y / g.y, mm.z * ri.z / g.z, 0.0};
+        d4 ri = r[i];
+        d4 zi = precond(ri, ia[i], ib[i], m[i]);
         z[i] = zi;
         p[i] = zi;
         d[i] = d4{0.0, 0.0, 0.0, 0.0};
@@ -109,17 +118,18 @@ __global__ __launch_bounds__(kVB) void k_pcg_start(int64_t n, const d4* __restri
     block_sum2(rz, rr, part);
 }
 
-// alpha = rho / pq; d += alpha p; r -= alpha m q; z = m r / diag; partial r.z, |r|^2.
+// alpha = rho / pq; d += alpha p; r -= alpha m q; z = m B^-1 r; partial r.z, |r|^2.
 __global__ __launch_bounds__(kVB) void k_pcg_update(int64_t n, const Scal* __restrict__ sc, const d4* __restrict__ p,
-                                                    const d4* __restrict__ q, const d4* __restrict__ dg,
-                                                    const d4* __restrict__ m, d4* __restrict__ d, d4* __restrict__ r,
-                                                    d4* __restrict__ z, double* part)
+                                                    const d4* __restrict__ q, const d4* __restrict__ ia,
+                                                    const d4* __restrict__ ib, const d4* __restrict__ m,
+                                                    d4* __restrict__ d, d4* __restrict__ r, d4* __restrict__ z,
+                                                    double* part)
 {
     int64_t i = (int64_t)blockIdx.x * kVB + threadIdx.x;
     double rz = 0.0, rr = 0.0;
     const double alpha = (sc->pq != 0.0) ? sc->rho / sc->pq : 0.0;
     if (i < n) {
-        d4 pi = p[i], qi = q[i], mm = m[i], g = dg[i];
+        d4 pi = p[i], qi = q[i], mm = m[i];
         d4 di = d[i], ri = r[i];
         di.x += alpha * pi.x;
         di.y += alpha * pi.y;
@@ -127,7 +137,7 @@ __global__ __launch_bounds__(kVB) void k_pcg_update(int64_t n, const Scal* __res
         ri.x -= alpha * mm.x * qi.x;
         ri.y -= alpha * mm.y * qi.y;
         ri.z -= alpha * mm.z * qi.z;
-        d4 zi{mm.x * ri.x / g.x, mm.y * ri.y / g.y, mm.z * ri.z / g.z, 0.0};
+        d4 zi = precond(ri, ia[i], ib[i], mm);
         d[i] = di;
         r[i] = ri;
         z[i] = zi;
@@ -137,15 +147,16 @@ __global__ __launch_bounds__(kVB) void k_pcg_update(int64_t n, const Scal* __res
     block_sum2(rz, rr, part);
 }
 
-// z = m r / diag after a residual replacement; partial r.z, |r|^2 (p is kept).
-__global__ __launch_bounds__(kVB) void k_pcg_refresh(int64_t n, const d4* __restrict__ r, const d4* __restrict__ dg,
-                                                     const d4* __restrict__ m, d4* __restrict__ z, double* part)
+// z = m B^-1 r after a residual replacement; partial r.z, |r|^2 (p is kept).
+__global__ __launch_bounds__(kVB) void k_pcg_refresh(int64_t n, const d4* __restrict__ r, const d4* __restrict__ ia,
+                                                     const d4* __restrict__ ib, const d4* __restrict__ m,
+                                                     d4* __restrict__ z, double* part)
 {
     int64_t i = (int64_t)blockIdx.x * kVB + threadIdx.x;
     double rz = 0.0, rr = 0.0;
     if (i < n) {
-        d4 ri = r[i], g = dg[i], mm = m[i];
-        d4 zi{mm.x * ri.x / g.x, mm.y * ri.y / g.y, mm.z * ri.z / g.z, 0.0};
+        d4 ri = r[i];
+        d4 zi = precond(ri, ia[i], ib[i], m[i]);
         z[i] = zi;
         rz = dot3(ri, zi);
         rr = dot3(ri, ri);
@@ -218,13 +229,13 @@ __global__ __launch_bounds__(kVB) void k_reduce_scal(const double* __restrict__ 
 }
 
 struct Work {
-    d4 *m = nullptr, *uc = nullptr, *b = nullptr, *dg = nullptr, *rhs = nullptr, *x = nullptr, *r = nullptr,
-       *z = nullptr, *p = nullptr, *q = nullptr, *d = nullptr, *zero = nullptr;
+    d4 *m = nullptr, *uc = nullptr, *b = nullptr, *dg = nullptr, *og = nullptr, *rhs = nullptr, *x = nullptr,
+       *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *d = nullptr, *zero = nullptr;
     double* part = nullptr;
     Scal* sc = nullptr;
     void release()
     {
-        for (d4** v : {&m, &uc, &b, &dg, &rhs, &x, &r, &z, &p, &q, &d, &zero})
+        for (d4** v : {&m, &uc, &b, &dg, &og, &rhs, &x, &r, &z, &p, &q, &d, &zero})
             if (*v) (void)hipFree(*v);
         if (part) (void)hipFree(part);
         if (sc) (void)hipFree(sc);
@@ -282,7 +293,8 @@ int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_s
         hipError_t e_ = (x);                                                                 \
         if (e_ != hipSuccess) return fail(PDM_E_HIP, std::string("HIP error ") + hipGetErrorString(e_) + " at " #x); \
     } while (0)
-    for (d4** v : {&w.m, &w.uc, &w.b, &w.dg, &w.rhs, &w.x, &w.r, &w.z, &w.p, &w.q, &w.d, &w.zero}) WCHK(hipMalloc(v, vb));
+    for (d4** v : {&w.m, &w.uc, &w.b, &w.dg, &w.og, &w.rhs, &w.x, &w.r, &w.z, &w.p, &w.q, &w.d, &w.zero})
+        WCHK(hipMalloc(v, vb));
     const int64_t ntl = h->ntl;
     WCHK(hipMalloc(&w.part, sizeof(double) * 2 * std::max<int64_t>(nblk, std::max<int64_t>(ntl, 1))));
     WCHK(hipMalloc(&w.sc, sizeof(Scal)));
@@ -301,18 +313,46 @@ int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_s
         }
     }
     const dim3 g((unsigned)nblk), bk(kVB);
-    // Jacobi diagonal; a free component with no bonds keeps diagonal 1 (its row is empty)
-    WCHK(launch_tangent_diag(tv, w.zero, w.dg, s));
+    // Block-Jacobi preconditioner: each node's 3x3 tangent block sum_j w xi xi^T restricted to
+    // its free components, inverted.  A free component with no bonds keeps diagonal 1 (its
+    // row is empty); a block that is not safely invertible (all of a node's bonds on one line)
+    // falls back to its diagonal.  w.dg / w.og then hold the inverse's diagonal and
+    // off-diagonal (12, 13, 23) entries.
+    WCHK(launch_tangent_diag(tv, w.zero, w.dg, s, w.og));
     {
-        std::vector<d4> dgh(n);
+        std::vector<d4> dgh(n), ogh(n);
         WCHK(hipMemcpyAsync(dgh.data(), w.dg, vb, hipMemcpyDeviceToHost, s));
+        WCHK(hipMemcpyAsync(ogh.data(), w.og, vb, hipMemcpyDeviceToHost, s));
         WCHK(hipStreamSynchronize(s));
-        for (auto& v : dgh) {
-            if (!(v.x > 0.0)) v.x = 1.0;
-            if (!(v.y > 0.0)) v.y = 1.0;
-            if (!(v.z > 0.0)) v.z = 1.0;
+        for (int64_t a = 0; a < n; ++a) {
+            const d4 mm = mh[a];
+            const double fr[3] = {mm.x, mm.y, mm.z};
+            double B[3][3] = {{dgh[a].x, ogh[a].x, ogh[a].y}, {ogh[a].x, dgh[a].y, ogh[a].z}, {ogh[a].y, ogh[a].z, dgh[a].z}};
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c)
+                    if (r != c && (fr[r] == 0.0 || fr[c] == 0.0)) B[r][c] = 0.0;
+            for (int r = 0; r < 3; ++r)
+                if (fr[r] == 0.0 || !(B[r][r] > 0.0)) {
+                    for (int c = 0; c < 3; ++c) B[r][c] = B[c][r] = 0.0;
+                    B[r][r] = 1.0;
+                }
+            const double c00 = B[1][1] * B[2][2] - B[1][2] * B[2][1];
+            const double c01 = B[0][2] * B[2][1] - B[0][1] * B[2][2];
+            const double c02 = B[0][1] * B[1][2] - B[0][2] * B[1][1];
+            const double det = B[0][0] * c00 + B[1][0] * c01 + B[2][0] * c02;
+            const double scale = B[0][0] * B[1][1] * B[2][2];
+            if (det > 1e-8 * scale) {
+                const double id = 1.0 / det;
+                dgh[a] = d4{c00 * id, (B[0][0] * B[2][2] - B[0][2] * B[2][0]) * id,
+                            (B[0][0] * B[1][1] - B[0][1] * B[1][0]) * id, 0.0};
+                ogh[a] = d4{c01 * id, c02 * id, (B[0][2] * B[1][0] - B[0][0] * B[1][2]) * id, 0.0};
+            } else {
+                dgh[a] = d4{1.0 / B[0][0], 1.0 / B[1][1], 1.0 / B[2][2], 0.0};
+                ogh[a] = d4{0.0, 0.0, 0.0, 0.0};
+            }
         }
         WCHK(hipMemcpyAsync(w.dg, dgh.data(), vb, hipMemcpyHostToDevice, s));
+        WCHK(hipMemcpyAsync(w.og, ogh.data(), vb, hipMemcpyHostToDevice, s));
     }
     // rhs = m (b - K uc)
     WCHK(launch_tangent(PDM_FP64, tv, w.uc, w.q, s));
@@ -335,7 +375,7 @@ int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_s
             hipError_t e = launch_tangent(inner_prec, tv, w.p, w.q, s, w.part);
             if (e != hipSuccess) return e;
             k_reduce_scal<<<1, kVB, 0, s>>>(w.part, ntl, w.sc, 2);
-            k_pcg_update<<<g, bk, 0, s>>>(n, w.sc, w.p, w.q, w.dg, w.m, w.d, w.r, w.z, w.part);
+            k_pcg_update<<<g, bk, 0, s>>>(n, w.sc, w.p, w.q, w.dg, w.og, w.m, w.d, w.r, w.z, w.part);
             k_reduce_scal<<<1, kVB, 0, s>>>(w.part, nblk, w.sc, 3);
             k_pcg_dir<<<g, bk, 0, s>>>(n, w.sc, w.z, w.p);
         }
@@ -349,7 +389,7 @@ int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_s
         // search direction: with FP32 matvecs (PDM_FP32C) the solution still converges to
         // the FP64 tolerance, without the restarts of a refinement loop.
         k_residual<<<g, bk, 0, s>>>(n, w.rhs, nullptr, w.m, w.r, w.part);
-        k_pcg_start<<<g, bk, 0, s>>>(n, w.r, w.dg, w.m, w.z, w.p, w.d, w.part);
+        k_pcg_start<<<g, bk, 0, s>>>(n, w.r, w.dg, w.og, w.m, w.z, w.p, w.d, w.part);
         k_reduce_scal<<<1, kVB, 0, s>>>(w.part, nblk, w.sc, 1);
         WCHK(hipGetLastError());
         hipGraph_t gr;
@@ -375,7 +415,7 @@ int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_s
             if (h->opt.verbose && sweeps % 20 == 0)
                 fprintf(stderr, "pdm_solve_static: %d iterations, relative residual %.3e\n", iters, rel);
             if (!std::isfinite(rel) || rel <= rel_tol) break;
-            k_pcg_refresh<<<g, bk, 0, s>>>(n, w.r, w.dg, w.m, w.z, w.part);
+            k_pcg_refresh<<<g, bk, 0, s>>>(n, w.r, w.dg, w.og, w.m, w.z, w.part);
             k_reduce_scal<<<1, kVB, 0, s>>>(w.part, nblk, w.sc, 1);
             WCHK(hipGetLastError());
         }

@@ -392,7 +392,9 @@ hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, 
 // dot[2 k + 1] = 0), in the layout of the CG reductions' partial pairs.
 hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s,
                           double* dot = nullptr);
-hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s);
+// Diagonal of each node's 3x3 tangent block into y; with off set, its off-diagonal entries
+// (12, 13, 23) into off.
+hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s, d4* off = nullptr);
 // Tangent weights coef / r^3 per internal entry in FP64 (tc64) and rounded to FP32 (tc32, may
 // be null); the tangent operations stream them through the view's coef64 / coef32.
 hipError_t launch_tangent_coef(const TileView& tv, double* tc64, float* tc32, hipStream_t s);

@@ -679,6 +679,20 @@ struct OpD64 {
     }
 };
 
+// The off-diagonal entries of each node's 3x3 tangent block: sum_j w xi_1 xi_2, xi_1 xi_3,
+// xi_2 xi_3 (OpD64 gives the diagonal), for the block-Jacobi preconditioner.
+struct OpD64X : OpD64 {
+    template <bool ENERGY, bool UE>
+    __device__ static void bond(const Lds& l, unsigned s, double c, const Self& me, Acc& a)
+    {
+        const Rec r = l.rec[s];
+        const double x1 = r.x - me.x, x2 = r.y - me.y, x3 = r.z - me.z;
+        a.f0 = __builtin_fma(c * x1, x2, a.f0);
+        a.f1 = __builtin_fma(c * x1, x3, a.f1);
+        a.f2 = __builtin_fma(c * x2, x3, a.f2);
+    }
+};
+
 // Whether an operation reads FP32 records.
 template <class OP>
 constexpr bool kF32Rec = std::is_same<typename OP::SelfT, SelfF>::value;
@@ -1939,10 +1953,13 @@ hipError_t launch_tangent_coef(const TileView& tv, double* tc64, float* tc32, hi
     return hipGetLastError();
 }
 
-hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s)
+hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s, d4* off)
 {
     if (tv.ntl == 0) return hipSuccess;
-    return launch_gather_u<OpD64>(tv, u, y, nullptr, stage_lds_bytes(tv, !tv.uniform_mat, false, false), s);
+    const size_t lds = stage_lds_bytes(tv, !tv.uniform_mat, false, false);
+    hipError_t e = launch_gather_u<OpD64>(tv, u, y, nullptr, lds, s);
+    if (e != hipSuccess || !off) return e;
+    return launch_gather_u<OpD64X>(tv, u, off, nullptr, lds, s);
 }
 
 hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y, hipStream_t s, double* dot)
