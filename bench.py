@@ -159,55 +159,79 @@ def run_static(args):
 
 FGI_FLOPS_PER_POINT = 55  # FP64 operations of one Gauss-point bond force (sqrt and division as one)
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def run_fgi(args):
-    """Fixed Gaussian integration (fgi_on 1, intg_point n): FP64 RK4 on the bar, one JSON line."""
+    """Fixed Gaussian integration (fgi_on 1, intg_point n): RK4 on the bar in FP64 and with FP32
+    bond arithmetic (FP32C, FP32C2; FP64 Gauss-point displacements and state), one JSON line whose
+    value is the FP32C rate (FP64 alone with --precision fp64)."""
+    import numpy as np
     import pdmtel_amd as pm
 
     n1 = args.intg_point
     spec = pm.bar_spec(args.nx, args.ny, args.nz, jitter=0.01, seed=0, steps=args.warmup + args.steps + 10)
     spec["cci"] = None
     spec["fgi"] = (1, n1)
-    t0 = time.perf_counter()
-    prob = pm.Problem(pm.deck_text(spec), precision=pm.PDM_FP64, graph_chunk=graph_chunk(args), energies=1)
-    prob.prepare(graph_chunk(args))
-    t_init = time.perf_counter() - t0
-    print(f"# fgi n={n1}: init {t_init:.2f} s, {prob.n} nodes, {prob.E} entries", file=sys.stderr, flush=True)
-    if args.warmup:
-        prob.step(args.warmup)
-    t0 = time.perf_counter()
-    done = prob.step(args.steps)
-    sec = time.perf_counter() - t0
-    assert done == args.steps, f"ran {done} of {args.steps} steps"
-    ms, launches = prob.kernel_timing()
-    n, E, ng, P = prob.n, prob.E, n1 ** 3, prob.live_gauss_pairs
+    names = ["fp64", "fp32c", "fp32c2"] if args.precision == "all" else [args.precision]
+    res = {}
+    for name in names:
+        t0 = time.perf_counter()
+        prob = pm.Problem(pm.deck_text(spec), precision=precision_code(name), graph_chunk=graph_chunk(args),
+                          energies=1)
+        prob.prepare(graph_chunk(args))
+        t_init = time.perf_counter() - t0
+        print(f"# fgi n={n1} {name}: init {t_init:.2f} s, {prob.n} nodes, {prob.E} entries", file=sys.stderr,
+              flush=True)
+        if args.warmup:
+            prob.step(args.warmup)
+        t0 = time.perf_counter()
+        done = prob.step(args.steps)
+        sec = time.perf_counter() - t0
+        assert done == args.steps, f"ran {done} of {args.steps} steps"
+        ms, launches = prob.kernel_timing()
+        res[name] = dict(sec=sec, ms=ms, launches=launches, n=prob.n, E=prob.E, P=prob.live_gauss_pairs,
+                         U=prob.state()[0], init=t_init)
+        print(f"# fgi n={n1} {name}: {sec:.3f} s for {args.steps} steps, stage {ms:.4f} ms", file=sys.stderr,
+              flush=True)
+        prob.close()
+    head = "fp32c" if "fp32c" in res else names[0]
+    r = res[head]
+    n, E, ng, P = r["n"], r["E"], n1 ** 3, r["P"]
     flops = P * FGI_FLOPS_PER_POINT  # the pairs the kernel evaluates (beta > 0)
-    achieved = flops / (ms * 1e-3) / 1e12
+    achieved = flops / (r["ms"] * 1e-3) / 1e12
     line = {
-        "metric": f"bond-updates/s (RK4, fixed Gaussian integration {n1}x{n1}x{n1}, FP64)",
-        "value": 4.0 * E * args.steps / sec,
+        "metric": f"bond-updates/s (RK4, fixed Gaussian integration {n1}x{n1}x{n1}, {head})",
+        "value": 4.0 * E * args.steps / r["sec"],
         "unit": "bond-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": sec / args.steps * 1e3,
+        "ms_per_step": r["sec"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if head != "fp64" else "f64",
         "data": "synthetic",
         "config": {"workload": f"bar {args.nx}x{args.ny}x{args.nz}, {n} nodes, {E} family entries, delta=3.015dx, "
                                f"jitter 0.01dx seed 0, fgi_on 1 intg_point {n1}, RK4 {args.steps} steps",
-                   "gauss_points_per_node": ng, "evaluated_pairs": P, "pairs": E * ng, "init_seconds": t_init},
-        "gauss_point_evaluations_per_s": 4.0 * E * ng * args.steps / sec,
-        "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_fgi_stage + k_fgi_gauss (one RK4 stage)", "kernel_avg_ms": ms, "launches": launches,
-                     "flops_per_launch": flops},
+                   "precision": head, "gauss_points_per_node": ng, "evaluated_pairs": P, "pairs": E * ng,
+                   "init_seconds": r["init"]},
+        "gauss_point_evaluations_per_s": 4.0 * E * ng * args.steps / r["sec"],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS if head == "fp64" else
+                     FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / (FP64_PEAK_TFLOPS if head == "fp64" else FP32_PEAK_TFLOPS), "traffic": None,
+                     "kernel": "k_fgi_stage + k_fgi_gauss (one RK4 stage)", "kernel_avg_ms": r["ms"],
+                     "launches": r["launches"], "flops_per_launch": flops},
         "cpu_baseline": None,
     }
-    prob.close()
+    for name, q in res.items():
+        line[f"{name}_value"] = 4.0 * E * args.steps / q["sec"]
+        line[f"{name}_stage_ms"] = q["ms"]
+        if name != "fp64" and "fp64" in res:
+            U64 = res["fp64"]["U"]
+            line[f"{name}_over_fp64"] = res["fp64"]["sec"] / q["sec"]
+            line[f"{name}_max_rel_disp_err_vs_fp64"] = float(np.abs(q["U"] - U64).max() / np.abs(U64).max())
     print(json.dumps(line))
 
 

@@ -93,8 +93,8 @@ int create_common(const std::string& text, const pdm_options* opt, pdm_handle** 
             delete h;
             return PDM_E_PARSE;
         }
-        if (h->opt.precision != PDM_FP64) {
-            g_create_err = "fixed Gaussian and adaptive integration run in PDM_FP64 only";
+        if (h->opt.precision != PDM_FP64 && !d.fgi_on) {
+            g_create_err = "adaptive integration runs in PDM_FP64 only";
             delete h;
             return PDM_E_MODEL;
         }
@@ -154,7 +154,8 @@ StageArgs stage_args(pdm_handle* h)
 // specimen constants (the FP32 records depend on both).
 int build_corr(pdm_handle* h)
 {
-    if (h->opt.precision == PDM_FP64 || h->ntl == 0) return PDM_OK;
+    // fixed Gaussian integration forms its FP32 bonds from the FP64 tables: no corrections
+    if (h->opt.precision == PDM_FP64 || h->ntl == 0 || !h->deck.cci_on) return PDM_OK;
     int* cl = nullptr;
     DevTemps temps(h->stream);
     temps.add(&cl);

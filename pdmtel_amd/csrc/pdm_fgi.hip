@@ -707,6 +707,7 @@ FgiArgs fgi_args(const pdm_handle* h)
     f.u0 = h->fgi_u0;
     f.pair_ptr = h->fgi_pair_ptr;
     f.pairs = h->fgi_pairs;
+    f.prec = h->opt.precision;
     f.ai = h->ai;
     f.ai_ptr = h->ai_ptr;
     f.ai_xw = h->ai_xw;

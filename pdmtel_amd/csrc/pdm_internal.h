@@ -177,6 +177,7 @@ struct FgiArgs {
     d4* u0;                 // U at the start of the step (BC nodes)
     const int64_t* pair_ptr;  // [N + 1] live (entry, Gauss point) pairs of each internal row
     const uint32_t* pairs;    // (entry offset in the row) << 16 | point
+    int prec;                 // fixed Gaussian bond arithmetic: PDM_FP64, PDM_FP32C, PDM_FP32C2
     // adaptive integration (ai_on): trapezoidal points of each internal row's entries
     int ai;                   // 0 off, 1 MLS point displacements, 2 relative displacement G_j xi
     const int64_t* ai_ptr;    // [N + 1]

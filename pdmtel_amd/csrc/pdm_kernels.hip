@@ -1406,6 +1406,78 @@ __device__ __forceinline__ void fgi_family(const TileView& tv, const FgiArgs& f,
     acc[3] = wave_sum(el);
 }
 
+// fgi_family in FP32 bond arithmetic (PDM_FP32C, PDM_FP32C2): each pair's xi = (X_j + offset)
+// - X_i and eta = u_g - u_i are formed in FP64 and rounded once to FP32; the stretch is formed
+// without cancellation, s = eta . (2 xi + eta) / ((p + r) r), with r, p from rsq; beta over the
+// point's cell size and the ecrit test in FP32.  Sums per lane in FP32 (TF: two-float TwoSum
+// pairs), then across the wavefront in FP64.  The Gauss-point displacements and the RK4 state
+// stay FP64.
+__device__ __forceinline__ float g_beta32(float dist, float delta, float dx)
+{
+    const float inner = delta - 0.5f * dx;
+    const float outer = delta + 0.5f * dx;
+    if (dist <= inner) return 1.0f;
+    if (dist <= outer) return (outer - dist) / dx;
+    return 0.0f;
+}
+
+template <bool TF>
+__device__ __forceinline__ void fgi_family32(const TileView& tv, const FgiArgs& f, const double* __restrict__ tab,
+                                             const MatDev* mats, const int32_t* __restrict__ mat, int64_t h0, int64_t i,
+                                             const d4& ui, double acc[4])
+{
+    const int lane = threadIdx.x & 63;
+    const d4 xi = tv.X[i];
+    const MatDev& mi = mats[mat[i]];
+    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, l0 = 0.0f, l1 = 0.0f, l2 = 0.0f, el = 0.0f;
+    const float delta = (float)f.horizon;
+    const int ng = f.ng;
+    const int64_t eb = tv.rpi[i];
+    const int64_t p1 = f.pair_ptr[i + 1];
+    for (int64_t idx = f.pair_ptr[i] + lane; idx < p1; idx += 64) {
+        const uint32_t pk = f.pairs[idx];
+        const int64_t e = eb + (int64_t)(pk >> 16);
+        const int g = (int)(pk & 0xFFFFu);
+        const float bij = (float)tv.coef64[e];
+        const int32_t j = tv.halo[h0 + tv.slot[e]];
+        const d4 xj = tv.X[j];
+        const MatDev& mj = mats[mat[j]];
+        const float ecrit = (float)((mj.ecrit < mi.ecrit) ? mj.ecrit : mi.ecrit);
+        const float spring = (float)((mj.spring < mi.spring) ? mj.spring : mi.spring);
+        const double* t = tab + kFgiTab * g;
+        const d4 ug = f.gin[(int64_t)j * ng + g];
+        const float x1 = (float)((t[0] + xj.x) - xi.x), x2 = (float)((t[1] + xj.y) - xi.y),
+                    x3 = (float)((t[2] + xj.z) - xi.z);
+        const float e1 = (float)(ug.x - ui.x), e2 = (float)(ug.y - ui.y), e3 = (float)(ug.z - ui.z);
+        const float a1 = x1 + e1, a2 = x2 + e2, a3 = x3 + e3;
+        const float r2 = x1 * x1 + x2 * x2 + x3 * x3;
+        const float p2 = a1 * a1 + a2 * a2 + a3 * a3;
+        const float rinv = __builtin_amdgcn_rsqf(r2), pinv = __builtin_amdgcn_rsqf(p2);
+        const float r = r2 * rinv, p = p2 * pinv;
+        const float num = e1 * (2.0f * x1 + e1) + e2 * (2.0f * x2 + e2) + e3 * (2.0f * x3 + e3);
+        const float st = num * __builtin_amdgcn_rcpf((p + r) * r);
+        const float beta = g_beta32(r, delta, (float)t[4]);
+        const bool ok = (num <= 0.0f) | (st <= ecrit);
+        const float c = ok ? (float)t[3] * (bij * beta * spring * st) * pinv : 0.0f;
+        if (TF) {
+            float s, q;
+            two_sum(f0, c * a1, s, q); f0 = s; l0 += q;
+            two_sum(f1, c * a2, s, q); f1 = s; l1 += q;
+            two_sum(f2, c * a3, s, q); f2 = s; l2 += q;
+        } else {
+            f0 += c * a1;
+            f1 += c * a2;
+            f2 += c * a3;
+        }
+        const float u = st * r;
+        el += ok ? bij * 0.5f * (0.5f * beta * (spring * rinv) * u * u) * (float)t[5] : 0.0f;
+    }
+    acc[0] = wave_sum((double)f0 + (double)l0);
+    acc[1] = wave_sum((double)f1 + (double)l1);
+    acc[2] = wave_sum((double)f2 + (double)l2);
+    acc[3] = wave_sum((double)el);
+}
+
 // The adaptive-integration branch of CalcBondBasedMotionRK4's family loop (pdDataManager.cpp:
 // 1888-1916) for one node i per workgroup: for each uncut entry (i, j), CalcBondBasedForce
 // (pdIntgManager2.cpp:92-310) at every trapezoidal point of node j's cell clipped to node i's
@@ -1574,7 +1646,9 @@ __host__ __device__ inline size_t fgi_lds_bytes(int tile_max, int ng)
 // One RK4 stage of a fixed-Gaussian specimen for one tile: family sums over the neighbours'
 // Gauss points (a.uin: this stage's displacement of the family owner, f.gin: the Gauss
 // points'), then the same stage update as k_stage.
-template <int STAGE, bool ENERGY, bool AI>
+// FAM: 0 fixed Gaussian FP64, 1 adaptive integration (sums from k_ai_family), 2 fixed Gaussian
+// in FP32 bond arithmetic, 3 the same with two-float sums.
+template <int STAGE, bool ENERGY, int FAM>
 __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_stage(StageArgs a, FgiArgs f, int step_idx)
 {
     extern __shared__ double L[];
@@ -1599,14 +1673,19 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_stage(StageArgs a, FgiAr
     for (int loc = threadIdx.x >> 6; loc < nt; loc += kFgiWaves) {
         const int64_t i = a0 + loc;
         double acc[4];
-        if (AI) {  // the family sums of k_ai_family
+        if (FAM == 1) {  // the family sums of k_ai_family
             const d4 r = f.ai_res[i];
             acc[0] = r.x;
             acc[1] = r.y;
             acc[2] = r.z;
             acc[3] = r.w;
-        } else
+        } else if (FAM == 2) {
+            fgi_family32<false>(tv, f, tab, l.mats, a.mat, h0, i, a.uin[i], acc);
+        } else if (FAM == 3) {
+            fgi_family32<true>(tv, f, tab, l.mats, a.mat, h0, i, a.uin[i], acc);
+        } else {
             fgi_family(tv, f, tab, l.mats, a.mat, h0, i, a.uin[i], acc);
+        }
         if ((threadIdx.x & 63) == 0)
             for (int c = 0; c < 4; ++c) l.res[4 * loc + c] = acc[c];
     }
@@ -1640,7 +1719,9 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_stage(StageArgs a, FgiAr
     }
 }
 
-// Fixed-Gaussian family sums of every node for the displacement u (f.gin interpolated from u).
+// Fixed-Gaussian family sums of every node for the displacement u (f.gin interpolated from u);
+// FAM as for k_fgi_stage (0, 2, 3).
+template <int FAM>
 __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_forces(StageArgs a, FgiArgs f, const d4* __restrict__ u,
                                                                d4* __restrict__ F, double* __restrict__ elast)
 {
@@ -1659,7 +1740,9 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_forces(StageArgs a, FgiA
     for (int loc = threadIdx.x >> 6; loc < nt; loc += kFgiWaves) {
         const int64_t i = a0 + loc;
         double acc[4];
-        fgi_family(tv, f, tab, l.mats, a.mat, h0, i, u[i], acc);
+        if (FAM == 2) fgi_family32<false>(tv, f, tab, l.mats, a.mat, h0, i, u[i], acc);
+        else if (FAM == 3) fgi_family32<true>(tv, f, tab, l.mats, a.mat, h0, i, u[i], acc);
+        else fgi_family(tv, f, tab, l.mats, a.mat, h0, i, u[i], acc);
         if ((threadIdx.x & 63) == 0) {
             F[i] = d4{acc[0], acc[1], acc[2], 0.0};
             if (elast) elast[i] = acc[3];
@@ -1867,23 +1950,38 @@ hipError_t launch_forces(int precision, const TileView& tv, const d4* u, d4* F, 
     }
 }
 
-template <int STAGE, bool ENERGY, bool AI>
+template <int STAGE, bool ENERGY, int FAM>
 hipError_t launch_fgi_stage_e(const StageArgs& a, const FgiArgs& f, int step_idx, size_t lds, hipStream_t s)
 {
-    hipError_t e = set_lds<k_fgi_stage<STAGE, ENERGY, AI>>(lds);
+    hipError_t e = set_lds<k_fgi_stage<STAGE, ENERGY, FAM>>(lds);
     if (e != hipSuccess) return e;
-    k_fgi_stage<STAGE, ENERGY, AI><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, step_idx);
+    k_fgi_stage<STAGE, ENERGY, FAM><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, step_idx);
     return hipGetLastError();
+}
+
+// The family kind of an FgiArgs: adaptive integration, or fixed Gaussian in its precision.
+inline int fgi_family_kind(const FgiArgs& f)
+{
+    if (f.ai) return 1;
+    return f.prec == PDM_FP32C ? 2 : f.prec == PDM_FP32C2 ? 3 : 0;
+}
+
+template <int STAGE, bool ENERGY>
+hipError_t launch_fgi_stage_k(const StageArgs& a, const FgiArgs& f, int step_idx, size_t lds, hipStream_t s)
+{
+    switch (fgi_family_kind(f)) {
+    case 1: return launch_fgi_stage_e<STAGE, ENERGY, 1>(a, f, step_idx, lds, s);
+    case 2: return launch_fgi_stage_e<STAGE, ENERGY, 2>(a, f, step_idx, lds, s);
+    case 3: return launch_fgi_stage_e<STAGE, ENERGY, 3>(a, f, step_idx, lds, s);
+    default: return launch_fgi_stage_e<STAGE, ENERGY, 0>(a, f, step_idx, lds, s);
+    }
 }
 
 template <int STAGE>
 hipError_t launch_fgi_stage_t(const StageArgs& a, const FgiArgs& f, int step_idx, size_t lds, hipStream_t s)
 {
-    if (f.ai)
-        return a.energies ? launch_fgi_stage_e<STAGE, true, true>(a, f, step_idx, lds, s)
-                          : launch_fgi_stage_e<STAGE, false, true>(a, f, step_idx, lds, s);
-    return a.energies ? launch_fgi_stage_e<STAGE, true, false>(a, f, step_idx, lds, s)
-                      : launch_fgi_stage_e<STAGE, false, false>(a, f, step_idx, lds, s);
+    return a.energies ? launch_fgi_stage_k<STAGE, true>(a, f, step_idx, lds, s)
+                      : launch_fgi_stage_k<STAGE, false>(a, f, step_idx, lds, s);
 }
 
 hipError_t launch_fgi_stage(int stage, const StageArgs& a, const FgiArgs& f, int step_idx, hipStream_t s)
@@ -1919,9 +2017,13 @@ hipError_t launch_fgi_forces(const StageArgs& a, const FgiArgs& f, const d4* u, 
 {
     if (a.tv.ntl == 0) return hipSuccess;
     const size_t lds = fgi_lds_bytes(a.tv.tile_max, f.ng);
-    hipError_t e = set_lds<k_fgi_forces>(lds);
+    const int kind = fgi_family_kind(f);
+    hipError_t e = kind == 2 ? set_lds<k_fgi_forces<2>>(lds) : kind == 3 ? set_lds<k_fgi_forces<3>>(lds)
+                                                                         : set_lds<k_fgi_forces<0>>(lds);
     if (e != hipSuccess) return e;
-    k_fgi_forces<<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, u, F, elast);
+    if (kind == 2) k_fgi_forces<2><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, u, F, elast);
+    else if (kind == 3) k_fgi_forces<3><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, u, F, elast);
+    else k_fgi_forces<0><<<dim3((unsigned)a.tv.ntl), dim3(64 * kFgiWaves), lds, s>>>(a, f, u, F, elast);
     return hipGetLastError();
 }
 

@@ -144,12 +144,31 @@ def test_live_entries_hold_a_point_with_nonzero_beta():
     assert prob.live_gauss_pairs == int(live.sum())
 
 
+@pytest.mark.parametrize("prec", [pm.PDM_FP32C, pm.PDM_FP32C2])
+def test_fp32_bond_arithmetic(prec):
+    """FP32 bond forces over the Gauss points with FP64 Gauss-point displacements and RK4
+    state: forces within 1e-6 of the oracle's FP64 forces, and after 200 steps the displacement
+    within 1e-6 of the FP64 path's (max |dU| / max |U|)."""
+    spec = fgi_spec(2, nx=16, ny=6, nz=6, steps=400, jitter=0.01, seed=9)
+    spec["slits"] = [pm.mid_slit(spec, 30.0)]
+    p32 = pm.Problem(pm.deck_text(spec), precision=prec)
+    p64 = pm.Problem(pm.deck_text(spec), precision=pm.PDM_FP64)
+    orc = pyoracle.Oracle(spec, bond_method=1)
+    rng = np.random.default_rng(3)
+    u = rng.uniform(-1e-6, 1e-6, size=(p32.n, 3))
+    F, el = p32.forces(u, with_elast=True)
+    Fo, elo = orc.forces(u, with_elast=True)
+    ef, ee = rel(F, Fo), rel(el, elo)
+    p32.step(200)
+    p64.step(200)
+    eu = rel(p32.state()[0], p64.state()[0])
+    print(f"[fgi fp32] prec={prec} eF={ef:.3e} eE={ee:.3e} eU(200 steps)={eu:.3e}", flush=True)
+    assert ef <= 1e-6 and ee <= 1e-6 and eu <= 1e-6
+
+
 def test_unsupported_combinations_fail_loudly():
     spec = fgi_spec(2)
     text = pm.deck_text(spec)
-    with pytest.raises(pm.PdmError) as e:
-        pm.Problem(text, precision=pm.PDM_FP32C)
-    assert e.value.code == pm.PDM_E_MODEL
     prob = pm.Problem(text)
     with pytest.raises(pm.PdmError) as e:
         prob.solve_static()

@@ -136,7 +136,8 @@ def test_error_codes():
     o32 = pm.options(pm.PDM_FP32C)
     h = C.c_void_p()
     rc = L.pdm_create_from_text(fgi.replace("intg_point\n 7", "intg_point\n 2").encode(), C.byref(o32), C.byref(h))
-    assert rc == -4 and b"FP64" in L.pdm_last_error(None)
+    assert rc == 0  # fixed Gaussian integration with FP32 bond arithmetic
+    L.pdm_destroy(h)
     h = C.c_void_p()
     o = pm.options()
     assert L.pdm_create(b"/nonexistent/pds_input.txt", C.byref(o), C.byref(h)) == -2
