@@ -204,8 +204,13 @@ struct alignas(16) Rec {
 };
 
 // Halo record of the FP32 operations, scaled by S = SpecDev::fscale (2^30 over the
-// largest reach of a halo): the position relative to the tile centre as int32 fixed
-// point, so xi = X_j - X_i is an exact integer difference rounded once to FP32, and the
+// largest reach of a halo): the position as int32 fixed point, rint(X S) of the node
+// less the tile reference (an integer in the same units), so xi = X_j - X_i is an exact
+// integer difference rounded once to FP32.  Each node is rounded once, whichever tile
+// stages it, so a pair's xi is the same integer in both directions and f_ij = -f_ji
+// holds to the bit (rounding (X - tile centre) S per tile instead makes the two
+// directions differ by a unit in some pairs: the bond forces then no longer cancel in
+// pairs, and lightly damped oscillations drift in phase by ~1e-6 over 1,000 steps).  The
 // displacement u S in FP64, so eta = u_j - u_i is formed exactly before its rounding
 // to FP32 (an FP32 displacement, even relative to a tile reference, rounds eta at the
 // scale of the displacement field across the halo instead of eta's own).  One 48-byte
@@ -296,9 +301,10 @@ constexpr unsigned kFarFix = 1u << 30;  // fixed-point coordinate of the padding
 __device__ __forceinline__ void frec_store(const Lds& l, int q, const d4& x, const d4& v, const BK& b, const d4& xr,
                                            double S)
 {
+    // rint(X S) and the reference are integers below 2^53: their difference is exact
     RecF r;
-    r.a = u32x4{(unsigned)__double2int_rn((x.x - xr.x) * S), (unsigned)__double2int_rn((x.y - xr.y) * S),
-                (unsigned)__double2int_rn((x.z - xr.z) * S), __float_as_uint((float)x.w)};
+    r.a = u32x4{(unsigned)__double2int_rn(rint(x.x * S) - xr.x), (unsigned)__double2int_rn(rint(x.y * S) - xr.y),
+                (unsigned)__double2int_rn(rint(x.z * S) - xr.z), __float_as_uint((float)x.w)};
     r.u = f64x2{v.x * S, v.y * S};
     r.w = f64x2{v.z * S, __hiloint2double(__float_as_int(b.kc), __float_as_int(b.be))};
     l.recf[q] = r;

@@ -504,6 +504,12 @@ int build_tiles(pdm_handle* h)
     double reach_c = 0.0;
     for (int c = 0; c < 3; ++c) reach_c = std::max(reach_c, (g.R[c] + 0.5) * edge[c]);
     h->fscale = std::ldexp(1.0, 30) / (reach_c + d.dx);
+    // the FP32 records round X fscale to an integer in FP64: it must stay below 2^52
+    for (int c = 0; c < 3; ++c)
+        if (std::max(std::fabs(lo[c]), std::fabs(hi[c])) * h->fscale > std::ldexp(1.0, 52)) {
+            h->err = "node coordinates too far from the origin for the fixed-point records";
+            return PDM_E_LIMIT;
+        }
     if ((double)g.nt[0] * (double)g.nt[1] * (double)g.nt[2] > 2.0e9) {
         h->err = "tile grid too large";
         return PDM_E_LIMIT;
@@ -604,13 +610,14 @@ int build_tiles(pdm_handle* h)
     HIPCHK(hipMemcpyAsync(h->rpi, rpih.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(h->tile_start, tstart.data(), sizeof(int64_t) * (ntiles + 1), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(h->tiles, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice, s));
-    // tile centres: the reference point of the FP32 halo records
+    // tile centres in fixed-point units, rint(c fscale): the reference the FP32 halo
+    // records subtract from each node's rint(X fscale)
     std::vector<d4> trefh(tiles.size());
     for (size_t k = 0; k < tiles.size(); ++k) {
         const int64_t t = tiles[k];
         const int64_t q[3] = {t % g.nt[0], (t / g.nt[0]) % g.nt[1], t / (g.nt[0] * g.nt[1])};
         double c[3];
-        for (int a = 0; a < 3; ++a) c[a] = lo[a] + ((double)q[a] + 0.5) * edge[a];
+        for (int a = 0; a < 3; ++a) c[a] = std::nearbyint((lo[a] + ((double)q[a] + 0.5) * edge[a]) * h->fscale);
         trefh[k] = d4{c[0], c[1], c[2], 0.0};
     }
     HIPCHK(hipMemcpy(h->tref, trefh.data(), sizeof(d4) * trefh.size(), hipMemcpyHostToDevice));

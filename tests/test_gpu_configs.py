@@ -6,7 +6,7 @@ count log-uniform in [1e3, 1e5] (every side at least 5 nodes), aspect ratios uni
 in [50, 200] GPa; one launch per RK4 stage over all of them, graph-captured chunks, tile
 layout from the autotuner; 1,000 steps in FP64 and in FP32C.
   - every specimen's FP32C state against its FP64 state, max |U32 - U64| / max |U64| at the
-    last step (the per-specimen count over 1e-6 is printed and bounded);
+    last step, within 1e-6 for every specimen;
   - sampled specimens (the thinnest delta = 4.015 dx ones among them) bit-identical to the
     same deck run alone with the batch's layout;
   - the three smallest specimens' FP64 trajectories against the oracle over all 1,000 steps.
@@ -67,7 +67,7 @@ def test_batch_512_specimens():
           f"p99 {np.quantile(errs, 0.99):.3e} max {errs.max():.3e}; over 1e-6: {len(over)} of {len(specs)} "
           f"{[(int(k), specs[k]['grid_dimension'], f'{errs[k]:.2e}') for k in over]}", flush=True)
     assert np.median(errs) <= 1e-7
-    assert len(over) <= len(specs) // 50 and errs.max() <= 1e-5
+    assert len(over) == 0 and errs.max() <= 1e-6
 
     # sampled specimens bit-identical to their decks run alone with the batch layout
     lay = b32.tuning()
