@@ -41,7 +41,8 @@ extern "C" {
 #define PDM_FP64 0          /* reference arithmetic in FP64 */
 #define PDM_FP32C 1         /* FP32 bond force (cancellation-free stretch), FP32 family sums,
                                FP64 state */
-#define PDM_FP32C2 2        /* as PDM_FP32C with two-float (hi, lo) FP32 family sums */
+#define PDM_FP32C2 2        /* as PDM_FP32C with two-float (hi, lo) FP32 family sums (TwoSum of
+                               FP32 partials of 4 bond forces) */
 
 typedef struct pdm_handle pdm_handle;
 

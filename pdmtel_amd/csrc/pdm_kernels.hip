@@ -28,8 +28,9 @@
 //               coefficient formed from the two records and the entry's correction byte;
 //               each lane's bond forces summed in FP32 and the lane sums of a family
 //               reduced in FP32, then widened to FP64 for the stage update.
-//   PDM_FP32C2: the same bond force with two-float (hi, lo) FP32 family sums, also across
-//               lanes.  The stage update is FP64 in every mode.
+//   PDM_FP32C2: the same bond force with two-float (hi, lo) FP32 family sums: FP32 partials
+//               of 4 bonds added to each lane's pair with TwoSum, the lanes' hi + lo summed
+//               in FP64.  The stage update is FP64 in every mode.
 #include <cmath>
 #include <cstddef>
 #include <type_traits>
@@ -157,36 +158,6 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
     s = a + b;
     float bp = s - a;
     e = (a - (s - bp)) + (b - bp);
-}
-
-// Two-float group sum: (h, l) pairs combined with TwoSum at every butterfly step.
-template <int CTRL>
-__device__ __forceinline__ void tf_step_dpp(float& h, float& l)
-{
-    float oh = dppf<CTRL>(h), ol = dppf<CTRL>(l), t, e;
-    two_sum(h, oh, t, e);
-    h = t;
-    l = l + ol + e;
-}
-
-template <bool SWAP16>
-__device__ __forceinline__ void tf_step_perm(float& h, float& l)
-{
-    float oh = perm_xorf<SWAP16>(h), ol = perm_xorf<SWAP16>(l), t, e;
-    two_sum(h, oh, t, e);
-    h = t;
-    l = l + ol + e;
-}
-
-template <int LPN>
-__device__ __forceinline__ void group_sum_tf(float& h, float& l)
-{
-    tf_step_dpp<kDppXor1>(h, l);
-    tf_step_dpp<kDppXor2>(h, l);
-    tf_step_dpp<kDppHalfMirror>(h, l);
-    tf_step_dpp<kDppMirror>(h, l);
-    if (LPN >= 32) tf_step_perm<true>(h, l);
-    if (LPN >= 64) tf_step_perm<false>(h, l);
 }
 
 __device__ __forceinline__ double wave_sum(double v) { return group_sum<64>(v); }
@@ -576,36 +547,53 @@ struct OpF32 {  // PDM_FP32C: FP32 bond force, FP32 sums per lane and across the
     }
 };
 
-struct OpF32TF {  // PDM_FP32C2: FP32 bond force, two-float family sums
+// PDM_FP32C2: FP32 bond forces, two-float (hi, lo) family sums.  Each lane sums blocks of
+// kBlockN bond forces in FP32 (the block partial p) and adds each block partial to its
+// (hi, lo) pair with TwoSum, so the only roundings besides those of the bond forces
+// themselves are the (kBlockN - 1) additions inside a block, each on a partial of kBlockN
+// terms.  Each lane's hi + lo is exact in FP64, and the lane group sums those in FP64 (16
+// to 64 terms, DPP and permlane steps on doubles).  A TwoSum per bond and TwoSum at every
+// butterfly step (the first version) rounded nothing less that matters -- every term is
+// already an FP32 bond force -- and cost 35 % of the stage time against 17 % for this.
+struct OpF32TF {
     using Coef = float;
     static constexpr bool kCoef = false;
     using SelfT = SelfF;
     struct Acc {
+        float p0, p1, p2;  // FP32 partials of the current block
         float h0, h1, h2, l0, l1, l2, el;
+        double s0, s1, s2;  // the family sums, after reduce
     };
     template <bool ENERGY, bool UE>
     __device__ static void bond(const Lds& l, unsigned s, float c, const SelfF& me, Acc& a)
     {
         const Bond32 b = bond32<ENERGY, UE>(l, s, c, me);
-        float t, err;
-        two_sum(a.h0, b.g * b.a1, t, err); a.h0 = t; a.l0 += err;
-        two_sum(a.h1, b.g * b.a2, t, err); a.h1 = t; a.l1 += err;
-        two_sum(a.h2, b.g * b.a3, t, err); a.h2 = t; a.l2 += err;
+        a.p0 = __builtin_fmaf(b.g, b.a1, a.p0);
+        a.p1 = __builtin_fmaf(b.g, b.a2, a.p1);
+        a.p2 = __builtin_fmaf(b.g, b.a3, a.p2);
         if (ENERGY) a.el = __builtin_fmaf(b.cs, b.e, a.el);
+    }
+    __device__ static void flush(Acc& a)
+    {
+        float t, err;
+        two_sum(a.h0, a.p0, t, err); a.h0 = t; a.l0 += err;
+        two_sum(a.h1, a.p1, t, err); a.h1 = t; a.l1 += err;
+        two_sum(a.h2, a.p2, t, err); a.h2 = t; a.l2 += err;
+        a.p0 = a.p1 = a.p2 = 0.0f;
     }
     template <int LPN>
     __device__ static void reduce(Acc& a)
     {
-        group_sum_tf<LPN>(a.h0, a.l0);
-        group_sum_tf<LPN>(a.h1, a.l1);
-        group_sum_tf<LPN>(a.h2, a.l2);
+        a.s0 = group_sum<LPN>((double)a.h0 + (double)a.l0);
+        a.s1 = group_sum<LPN>((double)a.h1 + (double)a.l1);
+        a.s2 = group_sum<LPN>((double)a.h2 + (double)a.l2);
         a.el = group_sumf<LPN>(a.el);
     }
     __device__ static void store(const Acc& a, double ivc4, double* out)
     {
-        out[0] = (double)a.h0 + (double)a.l0;
-        out[1] = (double)a.h1 + (double)a.l1;
-        out[2] = (double)a.h2 + (double)a.l2;
+        out[0] = a.s0;
+        out[1] = a.s1;
+        out[2] = a.s2;
         out[3] = (double)a.el * ivc4;
     }
 };
@@ -833,11 +821,11 @@ __device__ __forceinline__ void load4(const TileView& tv, int64_t e, int64_t e1,
 // the one before is computed.  FP32 operations: the first block is 8 entries at e0 + 8 l
 // plus 4 at e0 + 8 L + 4 l (lane l of L), so with 16 lanes it covers 192 entries, an
 // interior CCI family at delta = 3.015 dx (178 entries with beta > 0) whole; the FP64
-// operations keep 4 (fewer registers).  The later blocks of 4 lie at e0 + kBlock0 L + 4 l,
-// then 4 L further each time.  Each load instruction of a lane group covers contiguous
-// memory.
+// operations keep 4 and the two-float sums 8 at e0 + 8 l (fewer registers: their
+// accumulators take six more).  The later blocks of 4 lie at e0 + kBlock0 L + 4 l, then
+// 4 L further each time.  Each load instruction of a lane group covers contiguous memory.
 template <class OP>
-constexpr int kBlock0 = (sizeof(typename OP::Coef) == 8) ? 4 : 12;
+constexpr int kBlock0 = (sizeof(typename OP::Coef) == 8) ? 4 : HasFlush<OP>::value ? 8 : 12;
 constexpr int kBlockN = 4;
 
 template <class OP, int LPN>
@@ -848,6 +836,8 @@ __device__ __forceinline__ void load_first(const TileView& tv, int64_t e0, int s
     if constexpr (kBlock0<OP> == 12) {
         load8<OP, 12>(tv, e0 + 8 * sl, e1, pad2, ch, 0);
         load4<OP, 12>(tv, e0 + 8 * LPN + 4 * sl, e1, pad2, ch, 8);
+    } else if constexpr (kBlock0<OP> == 8) {
+        load8<OP, 8>(tv, e0 + 8 * sl, e1, pad2, ch, 0);
     } else {
         load4<OP, 4>(tv, e0 + 4 * sl, e1, pad2, ch, 0);
     }
@@ -870,8 +860,10 @@ __device__ __forceinline__ void family_round(const TileView& tv, const SpecDev& 
     Chunk<OP, kBlockN> nx;
     if (base < e1) load4<OP, kBlockN>(tv, base + kBlockN * sl, e1, pad2, nx, 0);
 #pragma unroll
-    for (int k = 0; k < B0; ++k) OP::template bond<ENERGY, UE>(l, cur.slot(k), cur.coef(k), me, acc);
-    flush<OP>(acc);
+    for (int k = 0; k < B0; ++k) {
+        OP::template bond<ENERGY, UE>(l, cur.slot(k), cur.coef(k), me, acc);
+        if (k % kBlockN == kBlockN - 1) flush<OP>(acc);  // the first block in blocks of kBlockN
+    }
     while (base < e1) {
         const Chunk<OP, kBlockN> c = nx;
         base += LPN * kBlockN;
