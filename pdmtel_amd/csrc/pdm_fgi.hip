@@ -516,13 +516,13 @@ int build_fgi(pdm_handle* h)
             return PDM_E_HIP;
         }
         if (rc) return rc;
-        if ((double)h->fgi_npairs * sizeof(uint32_t) > kMaxPairBytes) {
+        if ((double)h->fgi_npairs * sizeof(uint2) > kMaxPairBytes) {
             h->err = "Gauss-point pair lists exceed 64 GiB (lower intg_point)";
             return PDM_E_LIMIT;
         }
         h->Elive = 0;
         for (int64_t a2 = 0; a2 < n; ++a2) h->Elive += enth[a2];
-        HIPCHK(hipMalloc(&h->fgi_pairs, sizeof(uint32_t) * std::max<int64_t>(h->fgi_npairs, 1)));
+        HIPCHK(hipMalloc(&h->fgi_pairs, sizeof(uint2) * std::max<int64_t>(h->fgi_npairs, 1)));
         f = fgi_args(h);
         HIPCHK(launch_fgi_pairs(tv, f, true, nullptr, nullptr, h->fgi_pairs, s));
     }

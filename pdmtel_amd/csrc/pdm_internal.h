@@ -176,7 +176,7 @@ struct FgiArgs {
                             // the one at (s & 1) n, which stage s + 1 reads
     d4* u0;                 // U at the start of the step (BC nodes)
     const int64_t* pair_ptr;  // [N + 1] live (entry, Gauss point) pairs of each internal row
-    const uint32_t* pairs;    // (entry offset in the row) << 16 | point
+    const uint2* pairs;       // ((entry offset in the row) << 16 | point, node j's internal id)
     int prec;                 // fixed Gaussian bond arithmetic: PDM_FP64, PDM_FP32C, PDM_FP32C2
     // adaptive integration (ai_on): trapezoidal points of each internal row's entries
     int ai;                   // 0 off, 1 MLS point displacements, 2 relative displacement G_j xi
@@ -336,7 +336,7 @@ struct pdm_handle {
     double* tc64 = nullptr;            // tangent weights coef / r^3 per internal entry (static solve)
     float* tc32 = nullptr;             // the same rounded to FP32 (FP32 precisions' CG matvecs)
     int64_t* fgi_pair_ptr = nullptr;   // [N + 1] live (entry, Gauss point) pairs per internal row
-    uint32_t* fgi_pairs = nullptr;
+    uint2* fgi_pairs = nullptr;        // (entry offset << 16 | point, j) per live pair
     int64_t fgi_npairs = 0;
     int64_t ncb = 0;                   // MLS contributing-list entries (cb)
 
@@ -425,7 +425,7 @@ hipError_t launch_fgi_forces(const StageArgs& a, const FgiArgs& f, const d4* u, 
 // Live-pair lists: fill = false counts each row's live pairs (cnt) and the entries that hold
 // one (ent); fill = true writes them at f.pair_ptr.
 hipError_t launch_fgi_pairs(const TileView& tv, const FgiArgs& f, bool fill, int64_t* cnt, int64_t* ent,
-                            uint32_t* pairs, hipStream_t s);
+                            uint2* pairs, hipStream_t s);
 
 // force sums with no allocation in the call (pdm_noalloc.hip): the per-device scratch
 // address is looked up once per device at set-up; noalloc_forces enqueues on h->stream

@@ -1356,48 +1356,84 @@ __global__ __launch_bounds__(256) void k_fly_corr(TileView tv, const d4* __restr
 // node; its lanes step through the row's live (entry, point) pairs (f.pairs: uncut entries,
 // beta > 0), so that consecutive lanes read consecutive Gauss points; a pair left out adds
 // exactly 0 to every sum.  Returns (fx, fy, fz, energy density) in every lane.
+// One live pair's inputs: the entry's coefficient, node j's position and the Gauss point's
+// displacement, and the pair's ecrit and spring constant.
+struct FgiPairIn {
+    double bij, ecrit, spring;
+    d4 xj, ug;
+    int g;
+};
+
+__device__ __forceinline__ FgiPairIn fgi_pair_in(const TileView& tv, const FgiArgs& f, const MatDev* mats,
+                                                 const int32_t* __restrict__ mat, const MatDev& mi, int64_t eb,
+                                                 int64_t idx)
+{
+    FgiPairIn q;
+    const uint2 pr = f.pairs[idx];
+    const int32_t j = (int32_t)pr.y;
+    q.g = (int)(pr.x & 0xFFFFu);
+    q.bij = tv.coef64[eb + (int64_t)(pr.x >> 16)];
+    q.xj = tv.X[j];
+    q.ug = f.gin[(int64_t)j * f.ng + q.g];
+    if (tv.uniform_mat) {
+        q.ecrit = mi.ecrit;
+        q.spring = mi.spring;
+    } else {
+        const MatDev& mj = mats[mat[j]];
+        q.ecrit = (mj.ecrit < mi.ecrit) ? mj.ecrit : mi.ecrit;
+        q.spring = (mj.spring < mi.spring) ? mj.spring : mi.spring;
+    }
+    return q;
+}
+
+// The FP64 bond force of one pair (CalcBondForceCCI's expression, volume outside) times the
+// point's weight, added to the lane's sums.
+__device__ __forceinline__ void fgi_pair_add(const FgiArgs& f, const double* __restrict__ tab, const d4& xi,
+                                             const d4& ui, const FgiPairIn& q, double& f0, double& f1, double& f2,
+                                             double& el)
+{
+    const double* t = tab + kFgiTab * q.g;
+    const double xi1 = (t[0] + q.xj.x) - xi.x, xi2 = (t[1] + q.xj.y) - xi.y, xi3 = (t[2] + q.xj.z) - xi.z;
+    const double a1 = xi1 + (q.ug.x - ui.x), a2 = xi2 + (q.ug.y - ui.y), a3 = xi3 + (q.ug.z - ui.z);
+    const double r = sqrt_rn(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
+    const double p = sqrt_rn(a1 * a1 + a2 * a2 + a3 * a3);
+    const double beta = g_beta(r, f.horizon, t[4], false);
+    const double u = p - r;
+    // one reciprocal for the divisions by r and p: 1/r = p / (r p), 1/p = r / (r p)
+    const double inv = 1.0 / (r * p);
+    const double rinv = p * inv;
+    const double st = u * rinv;
+    // u <= 0 || (u > 0 && st <= ecrit), without branches
+    const bool ok = (u <= 0.0) | (st <= q.ecrit);
+    const double c = ok ? q.bij * beta * q.spring * st : 0.0;
+    const double w = t[3] * (c * (r * inv));
+    f0 = __builtin_fma(w, a1, f0);
+    f1 = __builtin_fma(w, a2, f1);
+    f2 = __builtin_fma(w, a3, f2);
+    el += ok ? q.bij * 0.5 * (0.5 * beta * (q.spring * rinv) * u * u) * t[5] : 0.0;
+}
+
 __device__ __forceinline__ void fgi_family(const TileView& tv, const FgiArgs& f, const double* __restrict__ tab,
                                            const MatDev* mats, const int32_t* __restrict__ mat, int64_t h0, int64_t i,
                                            const d4& ui, double acc[4])
 {
+    (void)h0;
     const int lane = threadIdx.x & 63;
     const d4 xi = tv.X[i];
     const MatDev& mi = mats[mat[i]];
     double f0 = 0.0, f1 = 0.0, f2 = 0.0, el = 0.0;
-    const int ng = f.ng;
     const int64_t eb = tv.rpi[i];
     const int64_t p1 = f.pair_ptr[i + 1];
-    for (int64_t idx = f.pair_ptr[i] + lane; idx < p1; idx += 64) {
-        const uint32_t pk = f.pairs[idx];
-        const int64_t e = eb + (int64_t)(pk >> 16);
-        const int g = (int)(pk & 0xFFFFu);
-        const double bij = tv.coef64[e];
-        const int32_t j = tv.halo[h0 + tv.slot[e]];
-        const d4 xj = tv.X[j];
-        const MatDev& mj = mats[mat[j]];
-        const double ecrit = (mj.ecrit < mi.ecrit) ? mj.ecrit : mi.ecrit;
-        const double spring = (mj.spring < mi.spring) ? mj.spring : mi.spring;
-        const double* t = tab + kFgiTab * g;
-        const d4 ug = f.gin[(int64_t)j * ng + g];
-        const double xi1 = (t[0] + xj.x) - xi.x, xi2 = (t[1] + xj.y) - xi.y, xi3 = (t[2] + xj.z) - xi.z;
-        const double a1 = xi1 + (ug.x - ui.x), a2 = xi2 + (ug.y - ui.y), a3 = xi3 + (ug.z - ui.z);
-        const double r = sqrt_rn(xi1 * xi1 + xi2 * xi2 + xi3 * xi3);
-        const double p = sqrt_rn(a1 * a1 + a2 * a2 + a3 * a3);
-        const double beta = g_beta(r, f.horizon, t[4], false);
-        const double u = p - r;
-        // one reciprocal for the divisions by r and p: 1/r = p / (r p), 1/p = r / (r p)
-        const double inv = 1.0 / (r * p);
-        const double rinv = p * inv;
-        const double st = u * rinv;
-        // u <= 0 || (u > 0 && st <= ecrit), without branches
-        const bool ok = (u <= 0.0) | (st <= ecrit);
-        const double c = ok ? bij * beta * spring * st : 0.0;
-        const double w = t[3] * (c * (r * inv));
-        f0 = __builtin_fma(w, a1, f0);
-        f1 = __builtin_fma(w, a2, f1);
-        f2 = __builtin_fma(w, a3, f2);
-        el += ok ? bij * 0.5 * (0.5 * beta * (spring * rinv) * u * u) * t[5] : 0.0;
+    // two pairs per lane per pass (idx and idx + 64: the lane's order of summation is unchanged),
+    // so that each lane has two chains of dependent loads in flight
+    int64_t idx = f.pair_ptr[i] + lane;
+    for (; idx + 64 < p1; idx += 128) {
+        const FgiPairIn qa = fgi_pair_in(tv, f, mats, mat, mi, eb, idx);
+        const FgiPairIn qb = fgi_pair_in(tv, f, mats, mat, mi, eb, idx + 64);
+        fgi_pair_add(f, tab, xi, ui, qa, f0, f1, f2, el);
+        fgi_pair_add(f, tab, xi, ui, qb, f0, f1, f2, el);
     }
+    if (idx < p1) fgi_pair_add(f, tab, xi, ui, fgi_pair_in(tv, f, mats, mat, mi, eb, idx), f0, f1, f2, el);
     acc[0] = wave_sum(f0);
     acc[1] = wave_sum(f1);
     acc[2] = wave_sum(f2);
@@ -1420,56 +1456,62 @@ __device__ __forceinline__ float g_beta32(float dist, float delta, float dx)
 }
 
 template <bool TF>
+__device__ __forceinline__ void fgi_pair_add32(const double* __restrict__ tab, float delta, const d4& xi,
+                                               const d4& ui, const FgiPairIn& q, float& f0, float& f1, float& f2,
+                                               float& l0, float& l1, float& l2, float& el)
+{
+    const double* t = tab + kFgiTab * q.g;
+    const float bij = (float)q.bij, ecrit = (float)q.ecrit, spring = (float)q.spring;
+    const float x1 = (float)((t[0] + q.xj.x) - xi.x), x2 = (float)((t[1] + q.xj.y) - xi.y),
+                x3 = (float)((t[2] + q.xj.z) - xi.z);
+    const float e1 = (float)(q.ug.x - ui.x), e2 = (float)(q.ug.y - ui.y), e3 = (float)(q.ug.z - ui.z);
+    const float a1 = x1 + e1, a2 = x2 + e2, a3 = x3 + e3;
+    const float r2 = x1 * x1 + x2 * x2 + x3 * x3;
+    const float p2 = a1 * a1 + a2 * a2 + a3 * a3;
+    const float rinv = __builtin_amdgcn_rsqf(r2), pinv = __builtin_amdgcn_rsqf(p2);
+    const float r = r2 * rinv, p = p2 * pinv;
+    const float num = e1 * (2.0f * x1 + e1) + e2 * (2.0f * x2 + e2) + e3 * (2.0f * x3 + e3);
+    const float st = num * __builtin_amdgcn_rcpf((p + r) * r);
+    const float beta = g_beta32(r, delta, (float)t[4]);
+    const bool ok = (num <= 0.0f) | (st <= ecrit);
+    const float c = ok ? (float)t[3] * (bij * beta * spring * st) * pinv : 0.0f;
+    if (TF) {
+        float sum, err;
+        two_sum(f0, c * a1, sum, err); f0 = sum; l0 += err;
+        two_sum(f1, c * a2, sum, err); f1 = sum; l1 += err;
+        two_sum(f2, c * a3, sum, err); f2 = sum; l2 += err;
+    } else {
+        f0 += c * a1;
+        f1 += c * a2;
+        f2 += c * a3;
+    }
+    const float u = st * r;
+    el += ok ? bij * 0.5f * (0.5f * beta * (spring * rinv) * u * u) * (float)t[5] : 0.0f;
+}
+
+template <bool TF>
 __device__ __forceinline__ void fgi_family32(const TileView& tv, const FgiArgs& f, const double* __restrict__ tab,
                                              const MatDev* mats, const int32_t* __restrict__ mat, int64_t h0, int64_t i,
                                              const d4& ui, double acc[4])
 {
+    (void)h0;
     const int lane = threadIdx.x & 63;
     const d4 xi = tv.X[i];
     const MatDev& mi = mats[mat[i]];
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, l0 = 0.0f, l1 = 0.0f, l2 = 0.0f, el = 0.0f;
     const float delta = (float)f.horizon;
-    const int ng = f.ng;
     const int64_t eb = tv.rpi[i];
     const int64_t p1 = f.pair_ptr[i + 1];
-    for (int64_t idx = f.pair_ptr[i] + lane; idx < p1; idx += 64) {
-        const uint32_t pk = f.pairs[idx];
-        const int64_t e = eb + (int64_t)(pk >> 16);
-        const int g = (int)(pk & 0xFFFFu);
-        const float bij = (float)tv.coef64[e];
-        const int32_t j = tv.halo[h0 + tv.slot[e]];
-        const d4 xj = tv.X[j];
-        const MatDev& mj = mats[mat[j]];
-        const float ecrit = (float)((mj.ecrit < mi.ecrit) ? mj.ecrit : mi.ecrit);
-        const float spring = (float)((mj.spring < mi.spring) ? mj.spring : mi.spring);
-        const double* t = tab + kFgiTab * g;
-        const d4 ug = f.gin[(int64_t)j * ng + g];
-        const float x1 = (float)((t[0] + xj.x) - xi.x), x2 = (float)((t[1] + xj.y) - xi.y),
-                    x3 = (float)((t[2] + xj.z) - xi.z);
-        const float e1 = (float)(ug.x - ui.x), e2 = (float)(ug.y - ui.y), e3 = (float)(ug.z - ui.z);
-        const float a1 = x1 + e1, a2 = x2 + e2, a3 = x3 + e3;
-        const float r2 = x1 * x1 + x2 * x2 + x3 * x3;
-        const float p2 = a1 * a1 + a2 * a2 + a3 * a3;
-        const float rinv = __builtin_amdgcn_rsqf(r2), pinv = __builtin_amdgcn_rsqf(p2);
-        const float r = r2 * rinv, p = p2 * pinv;
-        const float num = e1 * (2.0f * x1 + e1) + e2 * (2.0f * x2 + e2) + e3 * (2.0f * x3 + e3);
-        const float st = num * __builtin_amdgcn_rcpf((p + r) * r);
-        const float beta = g_beta32(r, delta, (float)t[4]);
-        const bool ok = (num <= 0.0f) | (st <= ecrit);
-        const float c = ok ? (float)t[3] * (bij * beta * spring * st) * pinv : 0.0f;
-        if (TF) {
-            float s, q;
-            two_sum(f0, c * a1, s, q); f0 = s; l0 += q;
-            two_sum(f1, c * a2, s, q); f1 = s; l1 += q;
-            two_sum(f2, c * a3, s, q); f2 = s; l2 += q;
-        } else {
-            f0 += c * a1;
-            f1 += c * a2;
-            f2 += c * a3;
-        }
-        const float u = st * r;
-        el += ok ? bij * 0.5f * (0.5f * beta * (spring * rinv) * u * u) * (float)t[5] : 0.0f;
+    int64_t idx = f.pair_ptr[i] + lane;
+    for (; idx + 64 < p1; idx += 128) {  // two pairs per lane per pass, as fgi_family
+        const FgiPairIn qa = fgi_pair_in(tv, f, mats, mat, mi, eb, idx);
+        const FgiPairIn qb = fgi_pair_in(tv, f, mats, mat, mi, eb, idx + 64);
+        fgi_pair_add32<TF>(tab, delta, xi, ui, qa, f0, f1, f2, l0, l1, l2, el);
+        fgi_pair_add32<TF>(tab, delta, xi, ui, qb, f0, f1, f2, l0, l1, l2, el);
     }
+    if (idx < p1)
+        fgi_pair_add32<TF>(tab, delta, xi, ui, fgi_pair_in(tv, f, mats, mat, mi, eb, idx), f0, f1, f2, l0, l1, l2,
+                           el);
     acc[0] = wave_sum((double)f0 + (double)l0);
     acc[1] = wave_sum((double)f1 + (double)l1);
     acc[2] = wave_sum((double)f2 + (double)l2);
@@ -1751,7 +1793,9 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_forces(StageArgs a, FgiA
 // Live (entry, Gauss point) pairs of each node's row, for fgi_family: an uncut entry
 // (coefficient != 0) and a point whose beta is nonzero, with r formed exactly as fgi_family
 // forms it.  One workgroup per tile, one wavefront per node, one lane per entry; the pairs
-// are written in (entry, point) order.
+// are written in (entry, point) order, each with node j's internal id, so that the gathers
+// reach node j's position and Gauss points in one dependent load instead of three (slot,
+// halo, node).
 __device__ __forceinline__ bool fgi_point_live(const FgiArgs& f, int g, const d4& xi, const d4& xj)
 {
     const double* t = f.tab + kFgiTab * g;
@@ -1762,7 +1806,7 @@ __device__ __forceinline__ bool fgi_point_live(const FgiArgs& f, int g, const d4
 
 template <bool FILL>
 __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_pairs(TileView tv, FgiArgs f, int64_t* __restrict__ cnt,
-                                                              int64_t* __restrict__ ent, uint32_t* __restrict__ pairs)
+                                                              int64_t* __restrict__ ent, uint2* __restrict__ pairs)
 {
     const int64_t k = blockIdx.x;
     const int32_t tid = tv.tiles[k];
@@ -1781,7 +1825,11 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_pairs(TileView tv, FgiAr
             const int64_t e = e0 + lane;
             const bool uncut = e < ee && tv.coef64[e] != 0.0;
             d4 xj = xi;
-            if (uncut) xj = tv.X[tv.halo[h0 + tv.slot[e]]];  // cut and padding entries hold the padding slot
+            int32_t j = 0;
+            if (uncut) {  // cut and padding entries hold the padding slot
+                j = tv.halo[h0 + tv.slot[e]];
+                xj = tv.X[j];
+            }
             int c = 0;
             if (uncut)
                 for (int g = 0; g < ng; ++g) c += fgi_point_live(f, g, xi, xj) ? 1 : 0;
@@ -1796,7 +1844,7 @@ __global__ __launch_bounds__(64 * kFgiWaves) void k_fgi_pairs(TileView tv, FgiAr
                 int64_t o = base + (incl - c);
                 const uint32_t eo = (uint32_t)(e - eb) << 16;
                 for (int g = 0; g < ng; ++g)
-                    if (fgi_point_live(f, g, xi, xj)) pairs[o++] = eo | (uint32_t)g;
+                    if (fgi_point_live(f, g, xi, xj)) pairs[o++] = make_uint2(eo | (uint32_t)g, (uint32_t)j);
             }
             base += total;
             rc += total;
@@ -2026,7 +2074,7 @@ hipError_t launch_fgi_forces(const StageArgs& a, const FgiArgs& f, const d4* u, 
 }
 
 hipError_t launch_fgi_pairs(const TileView& tv, const FgiArgs& f, bool fill, int64_t* cnt, int64_t* ent,
-                            uint32_t* pairs, hipStream_t s)
+                            uint2* pairs, hipStream_t s)
 {
     if (tv.ntl == 0) return hipSuccess;
     if (fill)
