@@ -66,10 +66,12 @@ constexpr int kDppXor2 = 0x4E;
 constexpr int kDppHalfMirror = 0x141;
 constexpr int kDppMirror = 0x140;
 
+// (update_dpp with old 0 and bound_ctrl: every lane of these permutations is written, and the
+// form lets the compiler fold the move into the add that consumes it, v_add_f32_dpp)
 template <int CTRL>
 __device__ __forceinline__ float dppf(float v)
 {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
 template <int CTRL>
@@ -440,9 +442,11 @@ struct Bond32 {
 // coef = bdryEff_ij dx^3 beta(r) c_ij is formed from the two records as
 // (be_i + be_j) min(kc_i, kc_j) (beta_32(r) + (corr - 128) 2^-26), kc = dx^3 c / 2, where
 // beta_32 is beta evaluated in FP32 from the record geometry and corr (uint8 per entry)
-// takes it to the FP64 coefficient within 2^-27 (k_fly_corr): one byte per entry instead
-// of an FP32 coefficient, and the correction also absorbs the FP32 rounding of the
-// constants and of be, kc.  The -128 2^-26 offset is folded into beta_32's constant and
+// takes beta_32 + (corr - 128) 2^-26 to coef64 / ((be_i + be_j) min(kc_i, kc_j)) within
+// 2^-27 (k_fly_corr); the FP32 rounding of that sum and of the product then leave the
+// formed coefficient within about one FP32 ulp of the FP64 one.  One byte per entry
+// instead of an FP32 coefficient, and the correction also absorbs the FP32 rounding of
+// the constants and of be, kc.  The -128 2^-26 offset is folded into beta_32's constant and
 // clamp bounds.  CCI rows hold no entry with coefficient 0 (cut bonds and beta 0 are
 // left out when the rows are built); row padding reads the padding record (be = kc = 0).
 constexpr float kCorrStep = 0x1p-26f;
