@@ -41,8 +41,9 @@ coefficients) plus 264 B per node (trial u, X, Vbuf: 72 B, and the fused stage u
 correction byte from which the FP32 kernels form the coefficient: 3 B per entry; uint16
 slot + FP64 coefficient: 10 B); `format_bytes` gives that figure.  `traffic` is
 FETCH_SIZE + WRITE_SIZE per launch from rocprofv3 counter runs on this workload, scaled
-by a plain copy kernel's readings, as recorded under profiles/ (null for any other
-workload), and `measured_gbs` is that traffic over the launch time.
+by a plain copy kernel's readings, as recorded under profiles/ for this workload (the
+200 x 40 x 40 bar with and without the slit, the 400 x 400 x 50 plate; null for any
+other), and `measured_gbs` is that traffic over the launch time.
 
 cpu_baseline: the oracle (a serial C restatement of the reference's RK4 loop; the
 reference itself does not build here, so the figure is not calibrated against it) on the
@@ -67,8 +68,9 @@ ENTRY_BYTES = {"fp32c": 8, "fp64": 12}        # int32 neighbour + coefficient
 FORMAT_ENTRY_BYTES = {"fp32c": 3, "fp64": 10}  # uint16 slot + correction byte / FP64 coefficient (this library's format)
 NODE_BYTES = 72 + 8 * 24                       # trial u, X, Vbuf + fused stage update (8 x 24 B)
 DEFAULT_DIMS = (200, 40, 40)
-TRAFFIC_FILES = {"slit": os.path.join(ROOT, "profiles", "k_stage_slitbar200x40x40_traffic.txt"),
-                 "bar": os.path.join(ROOT, "profiles", "k_stage_bar200x40x40_traffic.txt")}
+TRAFFIC_FILES = {("slit", DEFAULT_DIMS): os.path.join(ROOT, "profiles", "k_stage_slitbar200x40x40_traffic.txt"),
+                 ("bar", DEFAULT_DIMS): os.path.join(ROOT, "profiles", "k_stage_bar200x40x40_traffic.txt"),
+                 ("plate", (400, 400, 50)): os.path.join(ROOT, "profiles", "k_stage_plate400x400x50_traffic.txt")}
 PRECISIONS = ("fp64", "fp32c", "fp32c2")
 
 
@@ -388,8 +390,8 @@ def cpu_baseline(spec, steps):
 
 def recorded_traffic(args, prec, workload=None):
     """FETCH_SIZE + WRITE_SIZE bytes per stage launch recorded for this workload, else None."""
-    path = TRAFFIC_FILES.get(workload or args.workload)
-    if path is None or (args.nx, args.ny, args.nz) != DEFAULT_DIMS or not os.path.exists(path):
+    path = TRAFFIC_FILES.get((workload or args.workload, (args.nx, args.ny, args.nz)))
+    if path is None or not os.path.exists(path):
         return None
     for line in open(path):
         m = re.match(r"^traffic_bytes_per_launch\s+(\w+)\s+([0-9.eE+]+)", line)
