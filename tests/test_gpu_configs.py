@@ -4,9 +4,9 @@ Batch (pdmtel_amd.deck.specimen_specs(512, seed=0)): 512 bars with an edge crack
 count log-uniform in [1e3, 1e5] (every side at least 5 nodes), aspect ratios uniform in
 [1, 5], delta / dx from {3.015, 4.015}, crack normal uniform in [0, 90] degrees, E uniform
 in [50, 200] GPa; one launch per RK4 stage over all of them, graph-captured chunks, tile
-layout from the autotuner; 1,000 steps in FP64 and in FP32C.
-  - every specimen's FP32C state against its FP64 state, max |U32 - U64| / max |U64| at the
-    last step, within 1e-6 for every specimen;
+layout from the autotuner; 1,000 steps in FP64, FP32C and FP32C2.
+  - every specimen's FP32C and FP32C2 state against its FP64 state, max |U32 - U64| /
+    max |U64| at the last step, within 1e-6 for every specimen;
   - sampled specimens (the thinnest delta = 4.015 dx ones among them) bit-identical to the
     same deck run alone with the batch's layout;
   - the three smallest specimens' FP64 trajectories against the oracle over all 1,000 steps.
@@ -68,6 +68,16 @@ def test_batch_512_specimens():
           f"{[(int(k), specs[k]['grid_dimension'], f'{errs[k]:.2e}') for k in over]}", flush=True)
     assert np.median(errs) <= 1e-7
     assert len(over) == 0 and errs.max() <= 1e-6
+
+    # the two-float (compensated) family sums on the same batch
+    print("[batch512] fp32c2: init", flush=True)
+    b2 = pm.Batch(decks, precision=pm.PDM_FP32C2)
+    assert b2.step(BATCH_STEPS) == BATCH_STEPS
+    errs2 = np.array([rel(b2.state(k)[0], b64.state(k)[0]) for k in range(len(specs))])
+    b2.close()
+    print(f"[batch512] fp32c2 vs fp64 after {BATCH_STEPS} steps: median {np.median(errs2):.3e} "
+          f"max {errs2.max():.3e}; over 1e-6: {int((errs2 > 1e-6).sum())}", flush=True)
+    assert np.median(errs2) <= 1e-7 and errs2.max() <= 1e-6
 
     # sampled specimens bit-identical to their decks run alone with the batch layout
     lay = b32.tuning()
