@@ -311,7 +311,7 @@ def run_batch(args):
     for name, prec in (("fp32c", pm.PDM_FP32C), ("fp32c2", pm.PDM_FP32C2), ("fp64", pm.PDM_FP64)):
         print(f"# batch {name}: building {len(decks)} specimens", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        b = pm.Batch(decks, precision=prec, graph_chunk=graph_chunk(args), energies=1)
+        b = pm.Batch(decks, precision=prec, graph_chunk=graph_chunk(args), energies=1, verbose=1)
         b.prepare(graph_chunk(args))
         t_init = time.perf_counter() - t0
         if args.warmup:

@@ -294,8 +294,15 @@ int pdm_batch_get_state(const pdm_batch* b, int spec, double* U, double* V);
 int pdm_batch_set_state(pdm_batch* b, int spec, const double* U, const double* V);
 int pdm_batch_get_clock(const pdm_batch* b, int spec, double* t, int* step, double* dt_next);
 int pdm_batch_get_energy(const pdm_batch* b, int spec, double* ke, double* ee);
+/* The batch's layout: the tile edge, and the waves and families of the tile class with the
+ * most tiles (tiles are launched in classes by how many fit a CU's LDS, each with its own
+ * layout); stage_ms: the autotuned stage time summed over the classes. */
 int pdm_batch_get_tuning(const pdm_batch* b, int* tile_edge, int* waves_per_tile, int* families_per_wave,
                          double* stage_ms);
+/* The layout specimen spec's tiles run with (its class's).  Batches only: no counterpart in
+ * the reference. */
+int pdm_batch_get_spec_tuning(const pdm_batch* b, int spec, int* tile_edge, int* waves_per_tile,
+                              int* families_per_wave);
 int pdm_batch_last_kernel_timing(const pdm_batch* b, double* avg_ms, int64_t* launches);
 
 #ifdef __cplusplus

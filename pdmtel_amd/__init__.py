@@ -119,6 +119,7 @@ def lib() -> C.CDLL:
     L.pdm_batch_get_clock.argtypes = [vp, C.c_int, dp, ip, dp]
     L.pdm_batch_get_energy.argtypes = [vp, C.c_int, dp, dp]
     L.pdm_batch_get_tuning.argtypes = [vp, ip, ip, ip, dp]
+    L.pdm_batch_get_spec_tuning.argtypes = [vp, C.c_int, ip, ip, ip]
     L.pdm_batch_last_kernel_timing.argtypes = [vp, dp, i64p]
     _lib = L
     return L
@@ -394,8 +395,13 @@ class Batch:
         self._chk(self.L.pdm_batch_get_energy(self.h, spec, C.byref(ke), C.byref(ee)))
         return ke.value, ee.value
 
-    def tuning(self) -> dict:
+    def tuning(self, spec: int = -1) -> dict:
+        """The batch's layout (that of its largest tile class), or with spec >= 0 the layout
+        that specimen's tiles run with."""
         t, w, f, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        if spec >= 0:
+            self._chk(self.L.pdm_batch_get_spec_tuning(self.h, spec, C.byref(t), C.byref(w), C.byref(f)))
+            return dict(tile_edge=t.value, waves_per_tile=w.value, families_per_wave=f.value)
         self._chk(self.L.pdm_batch_get_tuning(self.h, C.byref(t), C.byref(w), C.byref(f), C.byref(ms)))
         return dict(tile_edge=t.value, waves_per_tile=w.value, families_per_wave=f.value, tuned_stage_ms=ms.value)
 

@@ -267,19 +267,46 @@ int ensure_chunk(pdm_handle* h, int cap)
     return h->tiles ? alloc_parts(h) : PDM_OK;  // one row of tile partials per step
 }
 
-// Enqueue `count` steps whose parameters sit in h->steps[0..count).
-int enqueue_steps(pdm_handle* h, int count)
+// The stage arguments of a batch's tile class: its workgroup map, LDS sizes and layout.
+StageArgs class_args(const StageArgs& a, const TileClass& c)
+{
+    StageArgs b = a;
+    b.tv.lmap = c.map;
+    b.tv.nmap = c.n;
+    b.tv.halo_max = c.halo_max;
+    b.tv.tile_max = c.tile_max;
+    b.tv.waves = c.waves;
+    b.tv.fam_per_wave = c.fams;
+    return b;
+}
+
+// Enqueue `count` steps whose parameters sit in h->steps[0..count).  A batch with tile
+// classes launches each stage once per class (or, with only >= 0, for that class alone,
+// which the autotuner times).
+int enqueue_steps(pdm_handle* h, int count, int only = -1)
 {
     StageArgs a = stage_args(h);
     const bool fgi = !h->deck.cci_on;  // fixed Gaussian or adaptive integration
     const bool gauss = h->fgi_ng > 0;
     const FgiArgs f = fgi ? fgi_args(h) : FgiArgs{};
+    std::vector<StageArgs> ca;
+    if (!fgi)
+        for (size_t c = 0; c < h->tclass.size(); ++c)
+            if (only < 0 || (int)c == only) ca.push_back(class_args(a, h->tclass[c]));
     for (int k = 0; k < count; ++k) {
         for (int st = 0; st < 4; ++st) {
             a.uin = h->u[st & 1];
             a.uout = h->u[(st + 1) & 1];
             if (!fgi) {
-                HCHK(launch_stage(st, h->opt.precision, a, k, h->stream));
+                if (h->tclass.empty()) {
+                    HCHK(launch_stage(st, h->opt.precision, a, k, h->stream));
+                    continue;
+                }
+                for (StageArgs& b : ca) {
+                    b.uin = a.uin;
+                    b.uout = a.uout;
+                    HCHK(launch_stage(st, h->opt.precision, b, k, h->stream));
+                }
                 continue;
             }
             // Gauss points follow each stage's new displacements: stages 2-4 read them, and
@@ -386,8 +413,16 @@ void plan_steps(pdm_handle* h, int want, std::vector<StepParam>& sp)
 constexpr int64_t kTuneMinEntries = 4000000;  // smaller specimens take the default layout
 
 // Release the tile layout (everything build_tiles allocates), so it can be rebuilt.
+void free_classes(pdm_handle* h)
+{
+    for (TileClass& c : h->tclass) dfree(c.map);
+    h->tclass.clear();
+    h->spec_class.clear();
+}
+
 void free_tiles(pdm_handle* h)
 {
+    free_classes(h);
     dfree(h->permd); dfree(h->Xi); dfree(h->u[0]); dfree(h->u[1]); dfree(h->rpi); dfree(h->slot);
     dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->bk); dfree(h->corr); dfree(h->mati);
     dfree(h->bci); dfree(h->nb); dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr);
@@ -671,6 +706,37 @@ int batch_concat(pdm_handle* B)
     }
     B->spec_node_ptr[S] = no;
     stp[S] = to;
+    // tile classes: specimens whose tiles fit three or more per CU (LDS), the rest
+    free_classes(B);
+    std::vector<int> cls(S);
+    int ncls[2] = {0, 0};
+    for (int k = 0; k < S; ++k) {
+        TileView v{};
+        v.halo_max = B->members[k]->halo_max;
+        v.tile_max = B->members[k]->tile_max;
+        const size_t need = stage_lds_bytes(v, !ue, true, !f64) + sizeof(double) * 2 * 8;
+        cls[k] = (B->members[k]->ntl == 0 || kLdsBytes / need >= 3) ? 0 : 1;
+        if (B->members[k]->ntl) ncls[cls[k]]++;
+    }
+    if (ncls[0] && ncls[1]) {
+        B->tclass.resize(2);
+        B->spec_class = cls;
+        for (int c = 0; c < 2; ++c) {
+            TileClass& tc = B->tclass[c];
+            std::vector<int32_t> map;
+            for (int k = 0; k < S; ++k) {
+                if (cls[k] != c) continue;
+                for (int64_t q = stp[k]; q < stp[k + 1]; ++q) map.push_back((int32_t)q);
+                tc.halo_max = std::max(tc.halo_max, B->members[k]->halo_max);
+                tc.tile_max = std::max(tc.tile_max, B->members[k]->tile_max);
+            }
+            tc.n = (int64_t)map.size();
+            tc.waves = B->waves;
+            tc.fams = B->fam_per_wave;
+            HCHK(hipMalloc(&tc.map, sizeof(int32_t) * std::max<int64_t>(tc.n, 1)));
+            HCHK(hipMemcpy(tc.map, map.data(), sizeof(int32_t) * tc.n, hipMemcpyHostToDevice));
+        }
+    }
     HCHK(hipMemcpyAsync(B->rpi + N, &Ei, sizeof(int64_t), hipMemcpyHostToDevice, s));
     HCHK(hipMemcpyAsync(B->tile_start + ntiles, &N, sizeof(int64_t), hipMemcpyHostToDevice, s));
     HCHK(hipMemcpyAsync(B->halo_ptr + ntl, &halo, sizeof(int64_t), hipMemcpyHostToDevice, s));
@@ -733,7 +799,9 @@ int batch_timing_steps(pdm_handle* B)
     return PDM_OK;
 }
 
-// Autotuner over the batch: tile edges 4 and 5, waves per tile 8 and 4, families per wave 2 and 4.
+// Autotuner over the batch: tile edges 4 and 5, waves per tile 8 and 4, families per wave 2
+// and 4; with tile classes each class's layout is timed on its own launches, and an edge
+// scores the sum of its classes' best stage times.
 int batch_autotune(pdm_handle* B)
 {
     pdm_handle* const h = B;  // for HCHK
@@ -741,7 +809,9 @@ int batch_autotune(pdm_handle* B)
     static const int kWaves[] = {8, 4};
     static const int kFams[] = {2, 4};
     double best = 1e300;
-    int bt = 0, bw = 8, bf = 2;
+    int bt = 0;
+    std::vector<int> bw, bf;
+    std::vector<double> bms;
     int rc = ensure_chunk(B, kTuneSteps);
     if (rc) return rc;
     for (int t : kEdges) {
@@ -753,37 +823,69 @@ int batch_autotune(pdm_handle* B)
         if (rc) return rc;
         rc = batch_timing_steps(B);
         if (rc) return rc;
-        for (int w : kWaves)
-            for (int f : kFams) {
-                B->waves = w;
-                B->fam_per_wave = f;
-                rc = enqueue_steps(B, kTuneWarmSteps);
-                if (rc) return rc;
-                HCHK(hipEventRecord(B->ev0, B->stream));
-                rc = enqueue_steps(B, kTuneSteps);
-                if (rc) return rc;
-                HCHK(hipEventRecord(B->ev1, B->stream));
-                HCHK(hipEventSynchronize(B->ev1));
-                float ms = 0.f;
-                HCHK(hipEventElapsedTime(&ms, B->ev0, B->ev1));
-                const double per = ms / (4.0 * kTuneSteps);
-                if (B->opt.verbose)
-                    fprintf(stderr, "pdm_batch_init: tune edge %d waves %d families %d: %.4f ms per stage\n", t, w, f,
-                            per);
-                if (per < best) {
-                    best = per;
-                    bt = t;
-                    bw = w;
-                    bf = f;
+        const int nc = std::max<int>(1, (int)B->tclass.size());
+        std::vector<int> cw(nc, 8), cf(nc, 2);
+        std::vector<double> cms(nc, 1e300);
+        double total = 0.0;
+        for (int c = 0; c < nc; ++c) {
+            const int only = B->tclass.empty() ? -1 : c;
+            for (int w : kWaves)
+                for (int f : kFams) {
+                    if (B->tclass.empty()) {
+                        B->waves = w;
+                        B->fam_per_wave = f;
+                    } else {
+                        B->tclass[c].waves = w;
+                        B->tclass[c].fams = f;
+                    }
+                    rc = enqueue_steps(B, kTuneWarmSteps, only);
+                    if (rc) return rc;
+                    HCHK(hipEventRecord(B->ev0, B->stream));
+                    rc = enqueue_steps(B, kTuneSteps, only);
+                    if (rc) return rc;
+                    HCHK(hipEventRecord(B->ev1, B->stream));
+                    HCHK(hipEventSynchronize(B->ev1));
+                    float ms = 0.f;
+                    HCHK(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+                    const double per = ms / (4.0 * kTuneSteps);
+                    if (B->opt.verbose)
+                        fprintf(stderr, "pdm_batch_init: tune edge %d class %d waves %d families %d: %.4f ms per stage\n",
+                                t, only, w, f, per);
+                    if (per < cms[c]) {
+                        cms[c] = per;
+                        cw[c] = w;
+                        cf[c] = f;
+                    }
                 }
-            }
+            total += cms[c];
+        }
+        if (total < best) {
+            best = total;
+            bt = t;
+            bw = cw;
+            bf = cf;
+            bms = cms;
+        }
     }
     if (!bt) return fail(B, PDM_E_LIMIT, "no tile edge fits these specimens");
-    B->waves = bw;
-    B->fam_per_wave = bf;
+    if (B->tile_edge[0] != bt) {
+        rc = batch_layout(B, bt);
+        if (rc) return rc;
+    }
+    if (B->tclass.size() != (bw.size() > 1 ? bw.size() : 0)) return fail(B, PDM_E_STATE, "tile classes changed");
+    // the reported layout: that of the class with the most tiles
+    size_t big = 0;
+    for (size_t c = 0; c < B->tclass.size(); ++c) {
+        B->tclass[c].waves = bw[c];
+        B->tclass[c].fams = bf[c];
+        B->tclass[c].tune_ms = bms[c];
+        if (B->tclass[c].n > B->tclass[big].n) big = c;
+    }
+    B->waves = bw[big];
+    B->fam_per_wave = bf[big];
     B->tuned = 1;
     B->tune_ms = best;
-    return (B->tile_edge[0] == bt) ? PDM_OK : batch_layout(B, bt);
+    return PDM_OK;
 }
 
 }  // namespace
@@ -862,6 +964,7 @@ int pdm_destroy(pdm_handle* h)
     dfree(h->halo); dfree(h->U); dfree(h->V); dfree(h->Vb); dfree(h->k1); dfree(h->k23);
     dfree(h->elast); dfree(h->part); dfree(h->energy); dfree(h->steps); dfree(h->tc64); dfree(h->tc32);
     dfree(h->specs_d); dfree(h->mats_d); dfree(h->bcs_d); dfree(h->tile_spec_d); dfree(h->spec_tile_ptr_d);
+    free_classes(h);
     free_fgi(h);
     free_ai(h);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -1649,11 +1752,15 @@ int pdm_batch_init(pdm_batch* b)
     if (rc) return rc;
     B->last_kernel_ms = 0.0;
     B->last_kernel_launches = 0;
-    if (B->opt.verbose)
+    if (B->opt.verbose) {
         fprintf(stderr, "pdm_batch_init: %d specimens, nodes=%lld entries=%lld tiles=%lld edge %d waves %d "
                         "families %d halo_max=%d\n",
                 B->nspec, (long long)B->n, (long long)B->E, (long long)B->ntl, B->tile_edge[0], B->waves,
                 B->fam_per_wave, B->halo_max);
+        for (size_t c = 0; c < B->tclass.size(); ++c)
+            fprintf(stderr, "pdm_batch_init: tile class %zu: %lld tiles, halo_max=%d, waves %d families %d\n", c,
+                    (long long)B->tclass[c].n, B->tclass[c].halo_max, B->tclass[c].waves, B->tclass[c].fams);
+    }
     B->inited = true;
     return PDM_OK;
 }
@@ -1795,6 +1902,24 @@ int pdm_batch_get_tuning(const pdm_batch* b, int* tile_edge, int* waves_per_tile
 {
     if (!b || !b->h) return PDM_E_ARG;
     return pdm_get_tuning(b->h, tile_edge, waves_per_tile, families_per_wave, stage_ms);
+}
+
+int pdm_batch_get_spec_tuning(const pdm_batch* b, int spec, int* tile_edge, int* waves_per_tile,
+                              int* families_per_wave)
+{
+    if (!b || !b->h) return PDM_E_ARG;
+    const pdm_handle* B = b->h;
+    if (spec < 0 || spec >= B->nspec) return PDM_E_ARG;
+    int w = B->waves, f = B->fam_per_wave;
+    if (!B->tclass.empty()) {
+        const TileClass& c = B->tclass[B->spec_class[spec]];
+        w = c.waves;
+        f = c.fams;
+    }
+    if (tile_edge) *tile_edge = B->tile_edge[0];
+    if (waves_per_tile) *waves_per_tile = w;
+    if (families_per_wave) *families_per_wave = f;
+    return PDM_OK;
 }
 
 int pdm_batch_last_kernel_timing(const pdm_batch* b, double* avg_ms, int64_t* launches)

@@ -127,6 +127,19 @@ struct TileView {
                                // no per-neighbour ecrit in LDS, and kc0 scales the FP32 family sums
     const SpecDev* specs;      // per-specimen constants
     const int32_t* tile_spec;  // launch index -> specimen (nullptr: one specimen)
+    const int32_t* lmap;       // stage launch over a class of tiles: workgroup -> launch index
+    int64_t nmap;              // workgroups of such a launch (lmap nullptr: every tile, ntl)
+};
+
+// A batch's tiles whose LDS footprint allows the same number of tiles per CU, launched
+// together (one stage launch per class): specimens with small halos then run at their own
+// occupancy instead of that of the batch's largest halo.
+struct TileClass {
+    int32_t* map = nullptr;    // class workgroup -> launch index (device)
+    int64_t n = 0;             // tiles in the class
+    int halo_max = 0, tile_max = 0;
+    int waves = 8, fams = 2;   // layout of the class's launches
+    double tune_ms = 0.0;      // stage time of that layout when autotuned
 };
 
 // Arguments of the RK4 stage kernels.
@@ -291,6 +304,8 @@ struct pdm_handle {
     pdm::MatDev* mats_d = nullptr;      // [specimen][kMaxMat]
     pdm::BCDev* bcs_d = nullptr;        // [specimen][kMaxBC]
     int32_t* tile_spec_d = nullptr;     // launch index -> specimen (batches only)
+    std::vector<pdm::TileClass> tclass; // batches: stage launches per class (empty: one launch)
+    std::vector<int> spec_class;        // batches with classes: specimen -> class
     int64_t* spec_tile_ptr_d = nullptr; // specimen -> first launch index (nspec + 1)
     int nspec = 1;
     std::vector<pdm_handle*> members; // batch: the specimens (parsed decks, nodes, bonds, clocks)

@@ -1169,8 +1169,9 @@ __global__ __launch_bounds__(64 * NW, (OCC6 ? 6 : 4)) void k_stage(StageArgs a, 
     extern __shared__ double L[];
     __shared__ double sh_e[2][NW];
     const TileView& tv = a.tv;
-    const int64_t k = launch_index(tv.xcd_remap);
-    if (k >= tv.ntl) return;
+    int64_t k = launch_index(tv.xcd_remap);
+    if (k >= (tv.lmap ? tv.nmap : tv.ntl)) return;
+    if (tv.lmap) k = tv.lmap[k];
     const int32_t tid = tv.tiles[k];
     const int64_t a0 = tv.tile_start[tid];
     const int nt = (int)(tv.tile_start[tid + 1] - a0);
@@ -1882,7 +1883,7 @@ hipError_t launch_stage_t(const StageArgs& a, int step_idx, size_t lds, hipStrea
     hipError_t e = set_lds<k_stage<STAGE, PREC, ENERGY, UE, NW, NPW, OCC6>>(lds);
     if (e != hipSuccess) return e;
     k_stage<STAGE, PREC, ENERGY, UE, NW, NPW, OCC6>
-        <<<dim3((unsigned)padded_grid(a.tv.ntl)), dim3(64 * NW), lds, s>>>(a, step_idx);
+        <<<dim3((unsigned)padded_grid(a.tv.lmap ? a.tv.nmap : a.tv.ntl)), dim3(64 * NW), lds, s>>>(a, step_idx);
     return hipGetLastError();
 }
 
@@ -1972,7 +1973,7 @@ size_t stage_lds_bytes(const TileView& tv, bool with_ecrit, bool with_epi, bool 
 
 hipError_t launch_stage(int stage, int precision, const StageArgs& a, int step_idx, hipStream_t s)
 {
-    if (a.tv.ntl == 0) return hipSuccess;
+    if (a.tv.ntl == 0 || (a.tv.lmap && a.tv.nmap == 0)) return hipSuccess;
     size_t lds = stage_lds_bytes(a.tv, !a.tv.uniform_mat, true, precision != PDM_FP64);
     switch (precision) {
     case PDM_FP64: return launch_stage_p<PDM_FP64>(stage, a, step_idx, lds, s);

@@ -79,15 +79,14 @@ def test_batch_512_specimens():
           f"max {errs2.max():.3e}; over 1e-6: {int((errs2 > 1e-6).sum())}", flush=True)
     assert np.median(errs2) <= 1e-7 and errs2.max() <= 1e-6
 
-    # sampled specimens bit-identical to their decks run alone with the batch layout
-    lay = b32.tuning()
-    layout = dict(tile_edge=lay["tile_edge"], waves_per_tile=lay["waves_per_tile"],
-                  families_per_wave=lay["families_per_wave"])
+    # sampled specimens bit-identical to their decks run alone with the layout of their tile
+    # class in the batch (both horizons, so both classes)
     thin = sorted(range(len(specs)), key=lambda k: min(specs[k]["grid_dimension"]))
     thin = [k for k in thin if specs[k]["horizon"] > 4e-3][:3]
     sample = sorted(set(thin) | {0, 255, 511} | {int(k) for k in over[:2]})
+    assert len({specs[k]["horizon"] for k in sample}) == 2
     for k in sample:
-        p = pm.Problem(decks[k], precision=pm.PDM_FP32C, **layout)
+        p = pm.Problem(decks[k], precision=pm.PDM_FP32C, **b32.tuning(k))
         assert p.step(BATCH_STEPS) == BATCH_STEPS
         U, V = p.state()
         Ub, Vb = b32.state(k)

@@ -232,6 +232,33 @@ def test_forces_every_layout(layout, precision):
         pyoracle.set_square_mode(False)
 
 
+@pytest.mark.parametrize("layout", [(4, 8, 4), (3, 4, 4), (5, 8, 2), (4, 4, 1)])
+@pytest.mark.parametrize("precision", [pm.PDM_FP64, pm.PDM_FP32C, pm.PDM_FP32C2])
+def test_rk4_every_layout(layout, precision):
+    """The stage kernel in each lane mapping (16 to 64 lanes per family, 4 or 8 wavefronts
+    per tile) steps the slit bar as the oracle does: 1e-11 in FP64, 1e-6 in the FP32 modes,
+    after 12 steps."""
+    edge, waves, fams = layout
+    spec = make("bar_delta4", steps=40)
+    spec["slits"] = [pm.mid_slit(spec, 30.0)]
+    prob, orc = pair(spec, precision=precision, use_graph=0, tile_edge=edge, waves_per_tile=waves,
+                     families_per_wave=fams)
+    assert prob.tuning()["families_per_wave"] == fams
+    pyoracle.set_square_mode(True)
+    try:
+        prob.step(12)
+        orc.run(12)
+        U, V = prob.state()
+        Uo, Vo = orc.state()
+        eu = np.abs(U - Uo).max() / np.abs(Uo).max()
+        ev = np.abs(V - Vo).max() / np.abs(Vo).max()
+        print(f"[rk4 layout] {layout} precision={precision} eU={eu:.3e} eV={ev:.3e}", flush=True)
+        tol = 1e-11 if precision == pm.PDM_FP64 else 1e-6
+        assert eu <= tol and ev <= tol
+    finally:
+        pyoracle.set_square_mode(False)
+
+
 def test_rk4_fp32c_error():
     spec = make("bar_jitter", steps=40)
     p64 = pm.Problem(pm.deck_text(spec), precision=pm.PDM_FP64)
