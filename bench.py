@@ -70,7 +70,8 @@ NODE_BYTES = 72 + 8 * 24                       # trial u, X, Vbuf + fused stage 
 DEFAULT_DIMS = (200, 40, 40)
 TRAFFIC_FILES = {("slit", DEFAULT_DIMS): os.path.join(ROOT, "profiles", "k_stage_slitbar200x40x40_traffic.txt"),
                  ("bar", DEFAULT_DIMS): os.path.join(ROOT, "profiles", "k_stage_bar200x40x40_traffic.txt"),
-                 ("plate", (400, 400, 50)): os.path.join(ROOT, "profiles", "k_stage_plate400x400x50_traffic.txt")}
+                 ("plate", (400, 400, 50)): os.path.join(ROOT, "profiles", "k_stage_plate400x400x50_traffic.txt"),
+                 ("batch", 512): os.path.join(ROOT, "profiles", "k_stage_batch512_traffic.txt")}
 PRECISIONS = ("fp64", "fp32c", "fp32c2")
 
 
@@ -307,11 +308,17 @@ def run_batch(args):
 
     specs = specimen_specs(args.specimens, seed=0, steps=args.steps + args.warmup + 10)
     decks = [pm.deck_text(s) for s in specs]
+    layout = {}
+    if args.layout:  # "edge,waves,families": every tile class takes this layout
+        e, w, f = (int(v) for v in args.layout.split(","))
+        layout = dict(tile_edge=e, waves_per_tile=w, families_per_wave=f)
     res = {}
     for name, prec in (("fp32c", pm.PDM_FP32C), ("fp32c2", pm.PDM_FP32C2), ("fp64", pm.PDM_FP64)):
+        if args.precision not in ("all", name):
+            continue
         print(f"# batch {name}: building {len(decks)} specimens", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        b = pm.Batch(decks, precision=prec, graph_chunk=graph_chunk(args), energies=1, verbose=1)
+        b = pm.Batch(decks, precision=prec, graph_chunk=graph_chunk(args), energies=1, verbose=1, **layout)
         b.prepare(graph_chunk(args))
         t_init = time.perf_counter() - t0
         if args.warmup:
@@ -327,6 +334,25 @@ def run_batch(args):
         print(f"# batch {name}: {args.specimens} specimens, {sum(b.n)} nodes, {sum(b.E)} entries, init {t_init:.1f} s, "
               f"{t1 - t0:.3f} s for {args.steps} steps, stage {ms:.4f} ms", file=sys.stderr, flush=True)
         b.close()
+    if len(res) < 3:  # one precision (profiling runs): its rate and stage time only
+        name, r = next(iter(res.items()))
+        ks = r["kernel_ms"] * 1e-3
+        bytes_launch = r["E_live"] * ENTRY_BYTES["fp64" if name == "fp64" else "fp32c"] + r["n"] * NODE_BYTES
+        traffic = recorded_traffic(args, name)
+        print(json.dumps({
+            "metric": "bond-updates/s (RK4)", "value": 4.0 * r["E"] * args.steps / r["seconds"],
+            "unit": "bond-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": r["seconds"] / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64" if name == "fp64" else "f32+f64", "data": "synthetic",
+            "config": {"workload": f"batch of {args.specimens} random specimens (seed 0)", "precision": name,
+                       "layout": r["tuning"]},
+            "roofline": {"bound": "hbm", "achieved": bytes_launch / ks / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": bytes_launch / ks / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
+                         "measured_gbs": traffic / ks / 1e9 if traffic else None,
+                         "kernel": "k_stage, the tile classes' launches of one stage",
+                         "kernel_avg_ms": r["kernel_ms"], "bytes_per_launch": bytes_launch},
+            "cpu_baseline": None}))
+        return
     E, n, E_live = res["fp32c"]["E"], res["fp32c"]["n"], res["fp32c"]["E_live"]
     errs = np.array([float(np.abs(a - c).max() / max(np.abs(c).max(), 1e-300))
                      for a, c in zip(res["fp32c"]["U"], res["fp64"]["U"])])
@@ -360,8 +386,11 @@ def run_batch(args):
         "fp32c2_rel_err_per_specimen": {"median": float(np.median(errs2)), "p99": float(np.quantile(errs2, 0.99)),
                                         "over_1e-6": int((errs2 > 1e-6).sum())},
         "roofline": {"bound": "hbm", "achieved": bytes_launch / ks / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": bytes_launch / ks / 1e9 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_stage (fused bond-force gather + RK4 stage), all specimens per launch",
+                     "frac": bytes_launch / ks / 1e9 / HBM_PEAK_GBS, "traffic": recorded_traffic(args, "fp32c"),
+                     "measured_gbs": (recorded_traffic(args, "fp32c") / ks / 1e9
+                                      if recorded_traffic(args, "fp32c") else None),
+                     "kernel": "k_stage (fused bond-force gather + RK4 stage), the tile classes' launches of "
+                               "one stage over all specimens",
                      "kernel_avg_ms": res["fp32c"]["kernel_ms"], "bytes_per_launch": bytes_launch},
         "init_seconds": {"fp32c": res["fp32c"]["init"], "fp32c2": res["fp32c2"]["init"], "fp64": res["fp64"]["init"]},
         "cpu_baseline": None,
@@ -389,8 +418,10 @@ def cpu_baseline(spec, steps):
 
 
 def recorded_traffic(args, prec, workload=None):
-    """FETCH_SIZE + WRITE_SIZE bytes per stage launch recorded for this workload, else None."""
-    path = TRAFFIC_FILES.get((workload or args.workload, (args.nx, args.ny, args.nz)))
+    """FETCH_SIZE + WRITE_SIZE bytes per stage launch (a batch: per stage, its tile classes'
+    launches together) recorded for this workload, else None."""
+    wl = workload or args.workload
+    path = TRAFFIC_FILES.get((wl, args.specimens if wl == "batch" else (args.nx, args.ny, args.nz)))
     if path is None or not os.path.exists(path):
         return None
     for line in open(path):
