@@ -825,22 +825,23 @@ __device__ __forceinline__ void load4(const TileView& tv, int64_t e, int64_t e1,
 // the one before is computed.  FP32 operations: the first block is 8 entries at e0 + 8 l
 // plus 4 at e0 + 8 L + 4 l (lane l of L), so with 16 lanes it covers 192 entries, an
 // interior CCI family at delta = 3.015 dx (178 entries with beta > 0) whole; the FP64
-// operations keep 4 and the two-float sums 8 at e0 + 8 l (fewer registers: their
-// accumulators take six more).  The later blocks of 4 lie at e0 + kBlock0 L + 4 l, then
-// 4 L further each time.  Each load instruction of a lane group covers contiguous memory.
-template <class OP>
-constexpr int kBlock0 = (sizeof(typename OP::Coef) == 8) ? 4 : HasFlush<OP>::value ? 8 : 12;
+// operations keep 4, and the two-float sums 8 at e0 + 8 l in the 80-VGPR kernel (their
+// accumulators take six more registers; 12 would spill there).  The later blocks of 4 lie
+// at e0 + kBlock0 L + 4 l, then 4 L further each time.  Each load instruction of a lane
+// group covers contiguous memory.
+template <class OP, bool OCC6 = false>
+constexpr int kBlock0 = (sizeof(typename OP::Coef) == 8) ? 4 : (HasFlush<OP>::value && OCC6) ? 8 : 12;
 constexpr int kBlockN = 4;
 
-template <class OP, int LPN>
+template <class OP, int LPN, int B0>
 __device__ __forceinline__ void load_first(const TileView& tv, int64_t e0, int sl, int64_t e1, int pad,
-                                           Chunk<OP, kBlock0<OP>>& ch)
+                                           Chunk<OP, B0>& ch)
 {
     const unsigned pad2 = (unsigned)pad * 0x10001u;
-    if constexpr (kBlock0<OP> == 12) {
+    if constexpr (B0 == 12) {
         load8<OP, 12>(tv, e0 + 8 * sl, e1, pad2, ch, 0);
         load4<OP, 12>(tv, e0 + 8 * LPN + 4 * sl, e1, pad2, ch, 8);
-    } else if constexpr (kBlock0<OP> == 8) {
+    } else if constexpr (B0 == 8) {
         load8<OP, 8>(tv, e0 + 8 * sl, e1, pad2, ch, 0);
     } else {
         load4<OP, 4>(tv, e0 + 4 * sl, e1, pad2, ch, 0);
@@ -850,12 +851,10 @@ __device__ __forceinline__ void load_first(const TileView& tv, int64_t e0, int s
 // One round of a lane group: the family of tile node loc (first block from `cur`, which the
 // caller loaded), with the next round's first block loaded into the caller's other buffer
 // beforehand.
-template <class OP, bool ENERGY, bool UE, int LPN>
+template <class OP, bool ENERGY, bool UE, int LPN, int B0>
 __device__ __forceinline__ void family_round(const TileView& tv, const SpecDev& sp, const Lds& l, int nt, int loc,
-                                             int64_t e0, int64_t e1, int sl, const Chunk<OP, kBlock0<OP>>& cur,
-                                             double ivc4)
+                                             int64_t e0, int64_t e1, int sl, const Chunk<OP, B0>& cur, double ivc4)
 {
-    constexpr int B0 = kBlock0<OP>;
     const int ss = (loc < nt) ? l.self[loc] : l.pad;
     const typename OP::SelfT me = OP::SelfT::make(l, sp, ss, UE);
     typename OP::Acc acc{};
@@ -889,11 +888,11 @@ __device__ __forceinline__ void family_round(const TileView& tv, const SpecDev& 
 // (r NW + w) NPW + g for lane groups g = 0..NPW-1; the first block of round r + 1
 // is loaded before round r is computed.  Rounds alternate between two block
 // buffers, so no registers are copied between rounds.
-template <class OP, bool ENERGY, bool UE, int NW, int NPW>
+template <class OP, bool ENERGY, bool UE, int NW, int NPW, bool OCC6 = false>
 __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev& sp, const Lds& l, int nt)
 {
     constexpr int LPN = 64 / NPW;
-    constexpr int B = kBlock0<OP>;
+    constexpr int B = kBlock0<OP, OCC6>;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane / LPN, sl = lane % LPN;
     // FP32 records are scaled by S: the energy sum coef s (s r S) carries one factor S
@@ -904,7 +903,7 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev&
     int loc = first + g;
     int64_t e0 = (loc < nt) ? l.row[loc] : 0, e1 = (loc < nt) ? l.row[loc + 1] : 0;
     Chunk<OP, B> c0, c1;
-    load_first<OP, LPN>(tv, e0, sl, e1, l.pad, c0);
+    load_first<OP, LPN, B>(tv, e0, sl, e1, l.pad, c0);
     if constexpr (!kF32Rec<OP>) {
         // FP64 operations: one block buffer and a copy per round (fewer registers)
         for (;;) {
@@ -916,9 +915,9 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev&
                     n0 = l.row[nloc];
                     n1 = l.row[nloc + 1];
                 }
-                load_first<OP, LPN>(tv, n0, sl, n1, l.pad, c1);
+                load_first<OP, LPN, B>(tv, n0, sl, n1, l.pad, c1);
             }
-            family_round<OP, ENERGY, UE, LPN>(tv, sp, l, nt, loc, e0, e1, sl, c0, ivc4);
+            family_round<OP, ENERGY, UE, LPN, B>(tv, sp, l, nt, loc, e0, e1, sl, c0, ivc4);
             if (nfirst >= nt) return;
             first = nfirst;
             loc = nloc;
@@ -937,9 +936,9 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev&
                 n0 = l.row[nloc];
                 n1 = l.row[nloc + 1];
             }
-            load_first<OP, LPN>(tv, n0, sl, n1, l.pad, c1);
+            load_first<OP, LPN, B>(tv, n0, sl, n1, l.pad, c1);
         }
-        family_round<OP, ENERGY, UE, LPN>(tv, sp, l, nt, loc, e0, e1, sl, c0, ivc4);
+        family_round<OP, ENERGY, UE, LPN, B>(tv, sp, l, nt, loc, e0, e1, sl, c0, ivc4);
         if (nfirst >= nt) break;
         first = nfirst;
         loc = nloc;
@@ -955,9 +954,9 @@ __device__ __forceinline__ void tile_families(const TileView& tv, const SpecDev&
                 n0 = l.row[nloc];
                 n1 = l.row[nloc + 1];
             }
-            load_first<OP, LPN>(tv, n0, sl, n1, l.pad, c0);
+            load_first<OP, LPN, B>(tv, n0, sl, n1, l.pad, c0);
         }
-        family_round<OP, ENERGY, UE, LPN>(tv, sp, l, nt, loc, e0, e1, sl, c1, ivc4);
+        family_round<OP, ENERGY, UE, LPN, B>(tv, sp, l, nt, loc, e0, e1, sl, c1, ivc4);
         if (nfirst >= nt) break;
         first = nfirst;
         loc = nloc;
@@ -1194,7 +1193,7 @@ __global__ __launch_bounds__(64 * NW, (OCC6 ? 6 : 4)) void k_stage(StageArgs a, 
         more.store(l.epi[loc]);
     }
     __syncthreads();
-    tile_families<typename OpFor<PREC>::Force, ENERGY, UE, NW, NPW>(tv, sp, l, nt);
+    tile_families<typename OpFor<PREC>::Force, ENERGY, UE, NW, NPW, OCC6>(tv, sp, l, nt);
     __syncthreads();
     tile_update<STAGE, ENERGY, NW>(a, sp, l, k, a0, nt, step_idx, spec, sh_e);
 }
