@@ -145,7 +145,7 @@ def run_static(args):
         spec["slits"] = [pm.mid_slit(spec, 30.0, edge=True)]
     text = pm.deck_text(spec)
     out = {}
-    for name, prec in (("fp32c", pm.PDM_FP32C), ("fp64", pm.PDM_FP64)):
+    for name, prec in (("fp32c", pm.PDM_FP32C), ("fp32c2", pm.PDM_FP32C2), ("fp64", pm.PDM_FP64)):
         prob = pm.Problem(text, precision=prec, verbose=1)  # progress lines on stderr
         t0 = time.perf_counter()
         st = prob.solve_static(rel_tol=1e-10, max_iterations=200000)
@@ -156,7 +156,8 @@ def run_static(args):
         print(f"# static {name}: {wall:.3f} s, {st.iterations} iterations, {st.refinements} refinements, "
               f"residual {st.rel_residual:.2e}", file=sys.stderr, flush=True)
         prob.close()
-    err = float(np.abs(out["fp32c"]["U"] - out["fp64"]["U"]).max() / np.abs(out["fp64"]["U"]).max())
+    err = {k: float(np.abs(out[k]["U"] - out["fp64"]["U"]).max() / np.abs(out["fp64"]["U"]).max())
+           for k in ("fp32c", "fp32c2")}
     return out, err
 
 
@@ -593,7 +594,9 @@ def main():
             "wall_s": st["fp32c"]["wall"], "iterations": st["fp32c"]["iterations"],
             "refinements": st["fp32c"]["refinements"], "rel_residual": st["fp32c"]["rel_residual"],
             "fp64_wall_s": st["fp64"]["wall"], "fp64_iterations": st["fp64"]["iterations"],
-            "max_rel_disp_err_vs_fp64": serr,
+            "max_rel_disp_err_vs_fp64": serr["fp32c"],
+            "fp32c2_wall_s": st["fp32c2"]["wall"], "fp32c2_iterations": st["fp32c2"]["iterations"],
+            "fp32c2_max_rel_disp_err_vs_fp64": serr["fp32c2"],
         }
     if not args.skip_cpu:
         cb = cpu_baseline(spec, steps=args.cpu_steps)

@@ -426,7 +426,7 @@ void free_tiles(pdm_handle* h)
     dfree(h->permd); dfree(h->Xi); dfree(h->u[0]); dfree(h->u[1]); dfree(h->rpi); dfree(h->slot);
     dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->bk); dfree(h->corr); dfree(h->mati);
     dfree(h->bci); dfree(h->nb); dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr);
-    dfree(h->halo); dfree(h->part); dfree(h->tc64); dfree(h->tc32);
+    dfree(h->halo); dfree(h->part); dfree(h->tc64); dfree(h->tc32); dfree(h->tc16);
     h->part_cap = 0;
     h->ntl = 0;
     h->ntiles = 0;
@@ -897,14 +897,18 @@ int tangent_view(pdm_handle* h, TileView& tv)
     tv = tile_view(h);
     if (h->ntl == 0) return PDM_OK;
     const bool f32 = h->opt.precision != PDM_FP64;
-    if (!h->tc64 || (f32 && !h->tc32)) {
+    const bool lo = h->opt.precision == PDM_FP32C2;
+    if (!h->tc64 || (f32 && !h->tc32) || (lo && !h->tc16)) {
         if (!h->coef64) return fail(h, PDM_E_STATE, "the tangent needs the FP64 force coefficients");
-        if (!h->tc64) HCHK(hipMalloc(&h->tc64, sizeof(double) * std::max<int64_t>(h->Ei, 1)));
-        if (f32 && !h->tc32) HCHK(hipMalloc(&h->tc32, sizeof(float) * std::max<int64_t>(h->Ei, 1)));
-        HCHK(launch_tangent_coef(tv, h->tc64, f32 ? h->tc32 : nullptr, h->stream));
+        const int64_t ne = std::max<int64_t>(h->Ei, 1);
+        if (!h->tc64) HCHK(hipMalloc(&h->tc64, sizeof(double) * ne));
+        if (f32 && !h->tc32) HCHK(hipMalloc(&h->tc32, sizeof(float) * ne));
+        if (lo && !h->tc16) HCHK(hipMalloc(&h->tc16, sizeof(int16_t) * ne));
+        HCHK(launch_tangent_coef(tv, h->tc64, f32 ? h->tc32 : nullptr, lo ? h->tc16 : nullptr, h->stream));
     }
     tv.coef64 = h->tc64;
     tv.coef32 = h->tc32;
+    tv.wlo = h->tc16;
     return PDM_OK;
 }
 
@@ -962,7 +966,7 @@ int pdm_destroy(pdm_handle* h)
     dfree(h->self_slot); dfree(h->coef64); dfree(h->coef32); dfree(h->bk); dfree(h->corr); dfree(h->mati);
     dfree(h->bci); dfree(h->nb); dfree(h->tiles); dfree(h->tref); dfree(h->tile_start); dfree(h->halo_ptr);
     dfree(h->halo); dfree(h->U); dfree(h->V); dfree(h->Vb); dfree(h->k1); dfree(h->k23);
-    dfree(h->elast); dfree(h->part); dfree(h->energy); dfree(h->steps); dfree(h->tc64); dfree(h->tc32);
+    dfree(h->elast); dfree(h->part); dfree(h->energy); dfree(h->steps); dfree(h->tc64); dfree(h->tc32); dfree(h->tc16);
     dfree(h->specs_d); dfree(h->mats_d); dfree(h->bcs_d); dfree(h->tile_spec_d); dfree(h->spec_tile_ptr_d);
     free_classes(h);
     free_fgi(h);

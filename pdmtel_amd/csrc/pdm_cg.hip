@@ -364,7 +364,7 @@ int solve_static(pdm_handle* h, double rel_tol, int max_iterations, pdm_static_s
     WCHK(hipStreamSynchronize(s));
     const double bnorm = std::sqrt(sh.rr0);
     const bool mixed = h->opt.precision != PDM_FP64;
-    const int inner_prec = mixed ? PDM_FP32C : PDM_FP64;
+    const int inner_prec = h->opt.precision;  // PDM_FP32C: FP32 weights; PDM_FP32C2: FP32 + 16-bit correction
     int iters = 0, sweeps = 0;
     double rel = 0.0;
     hipGraphExec_t gexec = nullptr;

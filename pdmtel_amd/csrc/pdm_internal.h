@@ -127,6 +127,7 @@ struct TileView {
                                // no per-neighbour ecrit in LDS, and kc0 scales the FP32 family sums
     const SpecDev* specs;      // per-specimen constants
     const int32_t* tile_spec;  // launch index -> specimen (nullptr: one specimen)
+    const int16_t* wlo;        // PDM_FP32C2 tangent weights: 16-bit correction of each coef32
     const int32_t* lmap;       // stage launch over a class of tiles: workgroup -> launch index
     int64_t nmap;              // workgroups of such a launch (lmap nullptr: every tile, ntl)
 };
@@ -350,6 +351,8 @@ struct pdm_handle {
     double* phi = nullptr;
     double* tc64 = nullptr;            // tangent weights coef / r^3 per internal entry (static solve)
     float* tc32 = nullptr;             // the same rounded to FP32 (FP32 precisions' CG matvecs)
+    int16_t* tc16 = nullptr;           // PDM_FP32C2: the 16 significand bits of tc64 below tc32
+                                       // (then tc32 is tc64 rounded toward zero)
     int64_t* fgi_pair_ptr = nullptr;   // [N + 1] live (entry, Gauss point) pairs per internal row
     uint2* fgi_pairs = nullptr;        // (entry offset << 16 | point, j) per live pair
     int64_t fgi_npairs = 0;
@@ -413,7 +416,7 @@ hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y,
 hipError_t launch_tangent_diag(const TileView& tv, const d4* u, d4* y, hipStream_t s, d4* off = nullptr);
 // Tangent weights coef / r^3 per internal entry in FP64 (tc64) and rounded to FP32 (tc32, may
 // be null); the tangent operations stream them through the view's coef64 / coef32.
-hipError_t launch_tangent_coef(const TileView& tv, double* tc64, float* tc32, hipStream_t s);
+hipError_t launch_tangent_coef(const TileView& tv, double* tc64, float* tc32, int16_t* tc16, hipStream_t s);
 // The tile view of the tangent operator: coef64 / coef32 replaced by the tangent weights,
 // built on first use and kept with the tile layout.
 int tangent_view(pdm_handle* h, TileView& tv);

@@ -645,6 +645,16 @@ struct OpTangent {
 };
 using OpT64 = OpTangent<double>;
 using OpT32 = OpTangent<float>;
+// PDM_FP32C2's operator: each weight rounded to nearest at 39 significant bits (relative
+// error at most 2^-40) and streamed as the FP32 truncation of that plus its next 16
+// significand bits (tc16), rebuilt exactly when used by placing those bits below the
+// converted FP32 significand: 4 + 2 bytes per entry besides the slot.  The FP32 rounding of
+// the weights alone perturbs the smooth, slowly converging modes of the tangent by ~6e-8
+// (L/dx)^2 relative (their bond terms cancel), which costs CG a third more iterations; near
+// a 1e-10 residual even a 2^-38 truncation bias costs 15 %.
+struct OpT32C : OpTangent<double> {
+    static constexpr bool kWeightLo = true;
+};
 
 // Diagonal of the tangent operator (Jacobi preconditioner): d_c = sum_j w_ij xi_c^2, w the
 // tangent weight coef / r^3.
@@ -708,6 +718,23 @@ struct CoefStream<OP, std::enable_if_t<!OP::kCoef>> {
 template <class OP>
 constexpr bool kCoefStream = CoefStream<OP>::value;
 
+// Whether an operation streams FP32 weights with a 16-bit correction each (OpT32C).
+template <class OP, class = void>
+struct WeightLo {
+    static constexpr bool value = false;
+};
+template <class OP>
+struct WeightLo<OP, std::enable_if_t<OP::kWeightLo>> {
+    static constexpr bool value = true;
+};
+template <class OP>
+constexpr bool kWeightLo = WeightLo<OP>::value;
+
+// What a block holds per entry besides its slot: 0 the streamed coefficient, 1 the
+// correction byte of the FP32 force operations, 2 an FP32 weight and its 16-bit correction.
+template <class OP>
+constexpr int kChunkKind = kWeightLo<OP> ? 2 : kCoefStream<OP> ? 0 : 1;
+
 // Operations without FP32 partial sums have nothing to flush.
 template <class OP, class = void>
 struct HasFlush {
@@ -725,7 +752,7 @@ __device__ __forceinline__ void flush(typename OP::Acc& a)
 
 // One lane's block: B consecutive family entries, slots packed two per word, and the
 // coefficients or, for the FP32 force operations, the correction bytes packed four per word.
-template <class OP, int B, bool STREAM = kCoefStream<OP>>
+template <class OP, int B, int KIND = kChunkKind<OP>>
 struct Chunk {
     unsigned sp[B / 2];
     typename OP::Coef c[B];
@@ -733,7 +760,21 @@ struct Chunk {
     __device__ __forceinline__ typename OP::Coef coef(int k) const { return c[k]; }
 };
 template <class OP, int B>
-struct Chunk<OP, B, false> {
+struct Chunk<OP, B, 2> {
+    unsigned sp[B / 2];
+    float w[B];
+    unsigned lo[B / 2];  // two 16-bit corrections per word
+    __device__ __forceinline__ unsigned slot(int k) const { return (sp[k >> 1] >> ((k & 1) * 16)) & 0xFFFFu; }
+    __device__ __forceinline__ double coef(int k) const
+    {
+        // FP64 significand bits 28..13 (an FP32 significand fills 51..29 after conversion)
+        const unsigned x = (k & 1) ? (lo[k >> 1] >> 3) : (lo[k >> 1] << 13);
+        const double d = (double)w[k];
+        return __hiloint2double(__double2hiint(d), __double2loint(d) | (int)(x & 0x1FFFE000u));
+    }
+};
+template <class OP, int B>
+struct Chunk<OP, B, 1> {
     unsigned sp[B / 2];
     unsigned cb[B / 4];
     __device__ __forceinline__ unsigned slot(int k) const { return (sp[k >> 1] >> ((k & 1) * 16)) & 0xFFFFu; }
@@ -751,7 +792,17 @@ __device__ __forceinline__ void load8(const TileView& tv, int64_t e, int64_t e1,
         const u32x4 sv = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(tv.slot + e));
 #pragma unroll
         for (int k = 0; k < 4; ++k) ch.sp[o / 2 + k] = sv[k];
-        if constexpr (!kCoefStream<OP>) {
+        if constexpr (kWeightLo<OP>) {
+            const f32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e));
+            const f32x4 c1 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e + 4));
+            const u32x4 lv = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(tv.wlo + e));
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                ch.w[o + k] = c0[k];
+                ch.w[o + 4 + k] = c1[k];
+                ch.lo[o / 2 + k] = lv[k];
+            }
+        } else if constexpr (!kCoefStream<OP>) {
             const u32x2 cv = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(tv.corr + e));
             ch.cb[o / 4] = cv[0];
             ch.cb[o / 4 + 1] = cv[1];
@@ -774,7 +825,12 @@ __device__ __forceinline__ void load8(const TileView& tv, int64_t e, int64_t e1,
     } else {
 #pragma unroll
         for (int k = 0; k < 4; ++k) ch.sp[o / 2 + k] = pad2;
-        if constexpr (kCoefStream<OP>) {
+        if constexpr (kWeightLo<OP>) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) ch.w[o + k] = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) ch.lo[o / 2 + k] = 0;
+        } else if constexpr (kCoefStream<OP>) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) ch.c[o + k] = 0;
         } else {
@@ -795,7 +851,14 @@ __device__ __forceinline__ void load4(const TileView& tv, int64_t e, int64_t e1,
         const u32x2 sv = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(tv.slot + e));
         ch.sp[o / 2] = sv[0];
         ch.sp[o / 2 + 1] = sv[1];
-        if constexpr (!kCoefStream<OP>) {
+        if constexpr (kWeightLo<OP>) {
+            const f32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e));
+            const u32x2 lv = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(tv.wlo + e));
+#pragma unroll
+            for (int k = 0; k < 4; ++k) ch.w[o + k] = c0[k];
+            ch.lo[o / 2] = lv[0];
+            ch.lo[o / 2 + 1] = lv[1];
+        } else if constexpr (!kCoefStream<OP>) {
             ch.cb[o / 4] = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(tv.corr + e));
         } else if constexpr (sizeof(typename OP::Coef) == 4) {
             const f32x4 c0 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(tv.coef32 + e));
@@ -811,7 +874,11 @@ __device__ __forceinline__ void load4(const TileView& tv, int64_t e, int64_t e1,
         }
     } else {
         ch.sp[o / 2] = ch.sp[o / 2 + 1] = pad2;
-        if constexpr (kCoefStream<OP>) {
+        if constexpr (kWeightLo<OP>) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) ch.w[o + k] = 0.0f;
+            ch.lo[o / 2] = ch.lo[o / 2 + 1] = 0;
+        } else if constexpr (kCoefStream<OP>) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) ch.c[o + k] = 0;
         } else {
@@ -1280,7 +1347,7 @@ __global__ __launch_bounds__(64 * NW) void k_gather(TileView tv, const d4* __res
 // workgroup per tile: tc64 in FP64, tc32 (may be null) rounded to FP32.  Entries with
 // coefficient 0 (row padding) get 0.
 __global__ __launch_bounds__(256) void k_tangent_coef(TileView tv, double* __restrict__ tc64,
-                                                      float* __restrict__ tc32)
+                                                      float* __restrict__ tc32, int16_t* __restrict__ tc16)
 {
     const int64_t k = blockIdx.x;
     const int32_t tid = tv.tiles[k];
@@ -1306,7 +1373,18 @@ __global__ __launch_bounds__(256) void k_tangent_coef(TileView tv, double* __res
             w = c / (r2 * sqrt(r2));
         }
         tc64[e] = w;
-        if (tc32) tc32[e] = (float)w;
+        if (tc16) {
+            // w rounded to nearest at 39 significant bits (an integer add on the magnitude
+            // bits), then split into its FP32 truncation and the next 16 significand bits
+            // (none for FP32 subnormals, which no weight reaches)
+            const double w39 = __longlong_as_double((__double_as_longlong(w) + 0x1000LL) & ~0x1FFFLL);
+            const float wt = __double2float_rz(w39);
+            const bool normal = fabsf(wt) >= 0x1p-126f;
+            tc32[e] = wt;
+            tc16[e] = normal ? (int16_t)(uint16_t)((unsigned)__double2loint(w39) >> 13) : (int16_t)0;
+        } else if (tc32) {
+            tc32[e] = (float)w;
+        }
     }
 }
 
@@ -2098,10 +2176,10 @@ hipError_t launch_fly_corr(const TileView& tv, const d4* u, uint8_t* corr, int* 
     return hipGetLastError();
 }
 
-hipError_t launch_tangent_coef(const TileView& tv, double* tc64, float* tc32, hipStream_t s)
+hipError_t launch_tangent_coef(const TileView& tv, double* tc64, float* tc32, int16_t* tc16, hipStream_t s)
 {
     if (tv.ntl == 0) return hipSuccess;
-    k_tangent_coef<<<dim3((unsigned)tv.ntl), dim3(256), 0, s>>>(tv, tc64, tc32);
+    k_tangent_coef<<<dim3((unsigned)tv.ntl), dim3(256), 0, s>>>(tv, tc64, tc32, tc16);
     return hipGetLastError();
 }
 
@@ -2119,6 +2197,7 @@ hipError_t launch_tangent(int precision, const TileView& tv, const d4* u, d4* y,
     if (tv.ntl == 0) return hipSuccess;
     size_t lds = stage_lds_bytes(tv, !tv.uniform_mat, false, false);  // FP64 records in every precision
     if (precision == PDM_FP64) return launch_gather_u<OpT64>(tv, u, y, nullptr, lds, s, dot);
+    if (precision == PDM_FP32C2 && tv.wlo) return launch_gather_u<OpT32C>(tv, u, y, nullptr, lds, s, dot);
     return launch_gather_u<OpT32>(tv, u, y, nullptr, lds, s, dot);
 }
 

@@ -272,10 +272,10 @@ def test_rk4_fp32c_error():
     assert err <= 1e-6
 
 
-@pytest.mark.parametrize("precision", [pm.PDM_FP64, pm.PDM_FP32C])
+@pytest.mark.parametrize("precision", [pm.PDM_FP64, pm.PDM_FP32C, pm.PDM_FP32C2])
 def test_static_solve_matches_dense(precision):
-    """Matrix-free PCG (FP64, or FP32 matvecs with FP64 refinement) vs a dense LAPACK
-    solve of the same masked tangent system assembled from the oracle."""
+    """Matrix-free PCG (FP64, or FP32 / FP32 + 16-bit weights with FP64 residual replacement)
+    vs a dense LAPACK solve of the same masked tangent system assembled from the oracle."""
     spec = make("bar_jitter")
     prob, orc = pair(spec, precision=precision)
     st = prob.solve_static(rel_tol=1e-10)
@@ -303,10 +303,11 @@ def test_static_solve_edge_crack_fp32c():
     assert st.rel_residual <= 1e-10 and err <= 1e-7
 
 
-@pytest.mark.parametrize("precision", [pm.PDM_FP64, pm.PDM_FP32C])
+@pytest.mark.parametrize("precision", [pm.PDM_FP64, pm.PDM_FP32C, pm.PDM_FP32C2])
 def test_tangent_operator(precision):
     """pdm_apply_tangent (per-entry weights coef / r^3 precomputed once) against the oracle's
-    tangent, on a jittered, slit bar: FP64 within 1e-12, FP32 weights within 1e-6; the operator
+    tangent, on a jittered, slit bar: FP64 within 1e-12, FP32 weights within 1e-6, FP32 weights
+    with their 16-bit corrections (within 2^-39 of the FP64 weights) within 1e-10; the operator
     is symmetric."""
     spec = make("bar_jitter")
     spec["slits"] = [pm.mid_slit(spec, 30.0, edge=True)]
@@ -319,5 +320,5 @@ def test_tangent_operator(precision):
     err = np.abs(Ku - Kuo).max() / np.abs(Kuo).max()
     sym = abs(np.vdot(v, Ku) - np.vdot(u, prob.tangent(v))) / abs(np.vdot(v, Ku))
     print(f"[tangent] precision={precision} err={err:.3e} symmetry={sym:.3e}", flush=True)
-    assert err <= (1e-12 if precision == pm.PDM_FP64 else 1e-6)
-    assert sym <= (1e-12 if precision == pm.PDM_FP64 else 1e-6)
+    tol = {pm.PDM_FP64: 1e-12, pm.PDM_FP32C: 1e-6, pm.PDM_FP32C2: 1e-10}[precision]
+    assert err <= tol and sym <= tol
