@@ -40,9 +40,10 @@ extern "C" {
 /* precision of the bond-force gather and of CG matvecs */
 #define PDM_FP64 0          /* reference arithmetic in FP64 */
 #define PDM_FP32C 1         /* FP32 bond force (cancellation-free stretch), FP32 family sums,
-                               FP64 state */
+                               FP64 state; CG matvecs on FP32 weights */
 #define PDM_FP32C2 2        /* as PDM_FP32C with two-float (hi, lo) FP32 family sums (TwoSum of
-                               FP32 partials of 4 bond forces) */
+                               FP32 partials of 4 bond forces); CG matvecs on FP32 weights with
+                               a 16-bit significand extension (39-bit weights) */
 
 typedef struct pdm_handle pdm_handle;
 
