@@ -135,7 +135,7 @@ def run_precision(args, spec, precision):
     return res
 
 
-def run_static(args):
+def run_static(args, names=("fp32c", "fp32c2", "fp64")):
     import numpy as np
 
     import pdmtel_amd as pm
@@ -146,6 +146,8 @@ def run_static(args):
     text = pm.deck_text(spec)
     out = {}
     for name, prec in (("fp32c", pm.PDM_FP32C), ("fp32c2", pm.PDM_FP32C2), ("fp64", pm.PDM_FP64)):
+        if name not in names:
+            continue
         prob = pm.Problem(text, precision=prec, verbose=1)  # progress lines on stderr
         t0 = time.perf_counter()
         st = prob.solve_static(rel_tol=1e-10, max_iterations=200000)
@@ -157,7 +159,7 @@ def run_static(args):
               f"residual {st.rel_residual:.2e}", file=sys.stderr, flush=True)
         prob.close()
     err = {k: float(np.abs(out[k]["U"] - out["fp64"]["U"]).max() / np.abs(out["fp64"]["U"]).max())
-           for k in ("fp32c", "fp32c2")}
+           for k in ("fp32c", "fp32c2") if k in out and "fp64" in out}
     return out, err
 
 
@@ -471,6 +473,8 @@ def main():
     ap.add_argument("--intg-point", type=int, default=2)
     ap.add_argument("--specimens", type=int, default=512)
     ap.add_argument("--layout", default="", help="edge,waves,families: fixed tile layout (default: autotuned)")
+    ap.add_argument("--static-only", action="store_true",
+                    help="only the static solve of the edge-cracked bar in --precision (one handle)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -488,6 +492,16 @@ def main():
         return
     if args.workload == "ai":
         run_ai(args)
+        return
+    if args.static_only:  # one precision's static solve, one handle in the process (profiling)
+        name = "fp32c" if args.precision == "all" else args.precision
+        st, _ = run_static(args, (name,))
+        r = st[name]
+        print(json.dumps({"metric": "static-solve wall s", "value": r["wall"], "unit": "s", "n_gpus": 1,
+                          "higher_is_better": False, "dtype": "f64" if name == "fp64" else "f32+f64",
+                          "data": "synthetic", "config": {"workload": "static solve of the edge-cracked bar",
+                                                          "precision": name},
+                          "iterations": r["iterations"], "rel_residual": r["rel_residual"]}))
         return
 
     import numpy as np
