@@ -499,8 +499,10 @@ def main():
         r = st[name]
         print(json.dumps({"metric": "static-solve wall s", "value": r["wall"], "unit": "s", "n_gpus": 1,
                           "higher_is_better": False, "dtype": "f64" if name == "fp64" else "f32+f64",
-                          "data": "synthetic", "config": {"workload": "static solve of the edge-cracked bar",
-                                                          "precision": name},
+                          "data": "synthetic",
+                          "config": {"workload": ("static solve of the edge-cracked bar" if args.workload in
+                                                  ("bar", "slit") else f"static solve of the {args.workload}"),
+                                     "precision": name},
                           "iterations": r["iterations"], "rel_residual": r["rel_residual"]}))
         return
 
